@@ -1,0 +1,311 @@
+/*
+ * wfpt.h - C ABI of the MI355X wavefront path tracer.
+ *
+ * Every entry point is stream-ordered on the hipStream_t passed in (NULL = the
+ * default stream) and returns an int status (WF_OK = 0, negative = error; see
+ * wf_status_string).  Buffers are device pointers owned by the caller unless a
+ * function says otherwise.  When a `float* ms` out-parameter is non-NULL the
+ * call brackets its launches with hipEvents, waits for them and writes the GPU
+ * time in milliseconds, which is what the reference's ComputePass::launchTimed
+ * returns (projects/WaveFrontPathTracer/Compute/ComputePass.cpp:85-110); with
+ * ms == NULL the call only enqueues work.
+ *
+ * Record layouts are byte-compatible with the reference's
+ * Ray / RayResult (projects/WaveFrontPathTracer/Ray/Ray.h:32-44).
+ */
+#ifndef WFPT_H
+#define WFPT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* wf_stream_t; /* == hipStream_t */
+
+/* ---------------------------------------------------------------- records */
+
+/* Ray.h:32-37 : {origin.xyz, tmin, direction.xyz, tmax}, 32 bytes. */
+typedef struct wf_ray {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} wf_ray;
+
+/* Ray.h:39-44 : {primitiveId, geometryId, t, packUnorm2x16(bary)}, 16 bytes.
+ * A miss is {-1, -1, -1.0f, 0} (shaders/glsl/WaveFrontPathTracer/trace.comp:78-81). */
+typedef struct wf_hit {
+    int32_t prim;
+    int32_t geom;
+    float t;
+    uint32_t bary;
+} wf_hit;
+
+/* --------------------------------------------------------------- statuses */
+enum {
+    WF_OK = 0,
+    WF_ERR_INVALID_ARG = -1,  /* null pointer, bad size, unknown mode      */
+    WF_ERR_HIP = -2,          /* a HIP runtime call failed                  */
+    WF_ERR_OUT_OF_MEMORY = -3,
+    WF_ERR_SCRATCH_TOO_SMALL = -4,
+    WF_ERR_TOO_LARGE = -5,    /* element count beyond the kernels' limits   */
+    WF_ERR_PARSE = -6,        /* malformed env.json                         */
+    WF_ERR_IO = -7,           /* file cannot be opened                      */
+    WF_ERR_STATE = -8         /* object used before it was set up           */
+};
+const char* wf_status_string(int status);
+/* Last error message of the calling thread (empty string if none). */
+const char* wf_last_error(void);
+/* Library version string. */
+const char* wf_version(void);
+
+/* Largest element count accepted by the sort / compaction kernels (2^30-1):
+ * look-back words carry a 30-bit count. */
+#define WF_MAX_ELEMENTS ((int64_t)((1u << 30) - 1u))
+
+/* ------------------------------------------------------------ Morton keys */
+/* Replaces Tracer::computeMortonCodes (Tracer/Tracer.cpp:22-37) and
+ * computeMortonCodesAila32.comp:24-96.
+ *   WF_KEY_AILA32          : the reference's 32-bit key (7 origin bits/axis
+ *                            interleaved with 4 direction bits/axis), u32.
+ *   WF_KEY_ORIGIN30        : 30-bit origin Morton code (10 bits/axis), u32.
+ *   WF_KEY_ORIGIN30_OCT30  : 60-bit key = origin30 << 30 | 2-D Morton code
+ *                            of the 15+15-bit octahedral direction, u64.
+ *   WF_KEY_SCENE6_ORIGIN30_OCT28 : batched key = scene_id << 58 |
+ *                            origin30 << 28 | 14+14-bit octahedral, u64.
+ * keys receive u32 (AILA32, ORIGIN30) or u64 (others); indices[i] = i. */
+enum {
+    WF_KEY_AILA32 = 0,
+    WF_KEY_ORIGIN30 = 1,
+    WF_KEY_ORIGIN30_OCT30 = 2,
+    WF_KEY_SCENE6_ORIGIN30_OCT28 = 3
+};
+int wf_key_bytes(int mode);  /* 4 or 8, or WF_ERR_INVALID_ARG */
+int wf_key_bits(int mode);   /* significant bits: 32, 30, 60, 64 */
+int wf_morton_keys(const wf_ray* rays, void* keys, uint32_t* indices, int64_t n,
+                   int mode, const float scene_min[3], const float scene_max[3],
+                   wf_stream_t stream, float* ms);
+/* Batched variant: scene_of_ray[i] selects scene_min/max[3*scene] and the
+ * 6-bit prefix (scene < 64). */
+int wf_morton_keys_batched(const wf_ray* rays, const uint8_t* scene_of_ray,
+                           void* keys, uint32_t* indices, int64_t n,
+                           const float* scene_min, const float* scene_max,
+                           int num_scenes, wf_stream_t stream, float* ms);
+
+/* -------------------------------------------------------------- radix sort */
+/* Replaces vrdxCmdSortKeyValue (external/vulkan_radix_sort/vk_radix_sort.h:
+ * 2579-2585, gpuSort :2598-2755) as called by Tracer::radixSort
+ * (Tracer/Tracer.cpp:39-78): stable ascending LSD sort of keys[n] (u32 when
+ * key_bytes == 4, u64 when 8) carrying vals[n] (u32), result in place.
+ * Only the low key_bits bits take part; passes = ceil(key_bits/digit_bits),
+ * digit_bits in {4, 8, 11}.  scratch must hold wf_sort_scratch_bytes(...). */
+int64_t wf_sort_scratch_bytes(int64_t n, int key_bytes, int key_bits, int digit_bits);
+int wf_sort_pairs(void* keys, uint32_t* vals, int64_t n, int key_bytes, int key_bits,
+                  int digit_bits, void* scratch, int64_t scratch_bytes,
+                  wf_stream_t stream, float* ms);
+
+/* ----------------------------------------------------------------- reorder */
+/* Replaces Tracer::reorderRays (Tracer/Tracer.cpp:80-98), reorderRays.comp:
+ * out_rays[i] = in_rays[idx[i]]; out_s2i[i] = in_s2i[idx[i]];
+ * out_i2s[out_s2i[i]] = i.  out_i2s must hold max(in_s2i)+1 entries. */
+int wf_reorder(const uint32_t* idx, const wf_ray* in_rays, wf_ray* out_rays,
+               const int32_t* in_s2i, int32_t* out_s2i, int32_t* out_i2s, int64_t n,
+               wf_stream_t stream, float* ms);
+
+/* ------------------------------------------------------------------- scene */
+/* Replaces the BLAS/TLAS build (WaveFrontPathTracer.cpp:202-350, :355-457)
+ * and Tracer::setAccererationStructure (Tracer/Tracer.cpp:166-183) with an
+ * LBVH.  Input (device): vertex records of vertex_stride bytes holding
+ * position float3 at byte 0 and normal float3 at byte 12 (vkglTF::Vertex,
+ * base/VulkanglTFModel.h:221-228 uses stride 96), and a u32 index buffer.
+ * geom_first_tri (host, num_geoms+1 entries, ascending, geom_first_tri[0] = 0)
+ * splits the triangles into geometries: a hit reports
+ * prim = tri - geom_first_tri[geom] and geom, as the reference's
+ * gl_PrimitiveID / gl_GeometryIndex (trace.comp:71-72).
+ * tri_material (device, optional) gives each triangle a material index
+ * (default 0).  The scene keeps its own copies; inputs may be freed after. */
+typedef struct wf_scene_s* wf_scene;
+int wf_scene_create(const void* vertices, int64_t num_vertices, int vertex_stride,
+                    const uint32_t* indices, int64_t num_tris,
+                    const int32_t* geom_first_tri, int num_geoms,
+                    const int32_t* tri_material, wf_stream_t stream, wf_scene* out);
+int wf_scene_destroy(wf_scene scene);
+/* Per-material albedo (rgb) and emission (rgb), device or host pointers not
+ * accepted: host arrays of 3*num_materials floats, copied. Default one
+ * material: albedo 0.8 (raygenPath.comp:138), emission 0. */
+int wf_scene_set_materials(wf_scene scene, const float* albedo_rgb,
+                           const float* emission_rgb, int num_materials,
+                           wf_stream_t stream);
+/* Scene AABB (getSceneDimensions, base/VulkanglTFModel.cpp:1478-1488). */
+int wf_scene_bounds(wf_scene scene, float out_min[3], float out_max[3]);
+/* Sizes: triangles, BVH nodes, leaves. */
+int wf_scene_info(wf_scene scene, int64_t* num_tris, int64_t* num_nodes,
+                  int64_t* num_leaves, int* num_geoms);
+/* Copy the built hierarchy to host memory (for inspection / tests):
+ * nodes as 16 floats each (child0 lo/hi, child1 lo/hi, child refs as int
+ * bit patterns, pad), tri_order[num_tris] = original triangle of each slot. */
+int wf_scene_download(wf_scene scene, float* nodes_host, int32_t* tri_order_host);
+
+/* ------------------------------------------------------------------- trace */
+/* Replaces Tracer::trace (Tracer/Tracer.cpp:185-207) / trace.comp:27-88.
+ * For i < n: r = ray_index ? ray_index[i] : i; hits[r] = closest (closest != 0)
+ * or any (closest == 0, terminate on first hit) intersection of rays[r] with
+ * the scene over [tmin, tmax]; opaque, no face culling.  Closest-hit ties on t
+ * resolve to the smaller global triangle index, so results do not depend on
+ * traversal order.  tmax < tmin always misses. */
+int wf_trace(wf_scene scene, const wf_ray* rays, const uint32_t* ray_index,
+             wf_hit* hits, int64_t n, int closest, wf_stream_t stream, float* ms);
+
+/* -------------------------------------------------------------- ray buffer */
+/* Mirror of RayBuffer (Ray/RayBuffer.h:22-70): double-buffered rays and LUTs.
+ * All pointers are device pointers owned by the caller (the C++ host class
+ * wfpt::RayBuffer manages them).  keys hold key_bytes per element. */
+typedef struct wf_ray_buffer {
+    wf_ray* rays[2];
+    int32_t* index_to_slot[2];
+    int32_t* slot_to_index[2];
+    wf_hit* results;
+    void* keys;
+    uint32_t* indices;
+    void* sort_scratch;
+    int64_t sort_scratch_bytes;
+    int64_t size;
+    int64_t capacity;
+    int32_t swap;        /* RayBuffer::swapBuffers                       */
+    int32_t closest_hit; /* RayBuffer::closestHit                         */
+} wf_ray_buffer;
+
+/* Tracer::traceSort (Tracer/Tracer.cpp:209-225): keys -> sort -> [reorder,
+ * swap] -> trace (indirect through the sorted indices when reorder == 0).
+ * times (optional, 4 floats) = {keys, sort, reorder, trace} ms. */
+int wf_trace_sort(wf_scene scene, wf_ray_buffer* rb, int key_mode, int digit_bits,
+                  const float scene_min[3], const float scene_max[3], int reorder,
+                  wf_stream_t stream, float times[4]);
+
+/* ---------------------------------------------------------- ray generation */
+/* initSeeds.comp:31-42: seeds[i] = tea(i, frame_index, 16 rounds). */
+int wf_init_seeds(uint32_t* seeds, int64_t n, int frame_index, wf_stream_t stream, float* ms);
+
+/* raygenPrimary.comp:44-87 with RayGen::primary (Ray/RayGen.cpp:80-105).
+ * screen_to_world: 16 floats, column-major (glm::mat4 layout), the inverse
+ * of P*V.  index_to_pixel: PixelTable LUT (n = width*height entries). */
+int wf_raygen_primary(const float screen_to_world[16], const float origin[3],
+                      const int32_t* index_to_pixel, wf_ray* rays, int width, int height,
+                      int sample_index, float max_dist, wf_stream_t stream, float* ms);
+
+/* raygenShadow.comp:68-139 with RayGen::shadow (Ray/RayGen.cpp:107-133).
+ * For each input slot j in [batch_begin, batch_end): S rays toward light +
+ * radius * (Sobol/Hammersley point + Cranley-Patterson offset).  Seeds are
+ * indexed by the path's pixel (in_s2i[j]) so the random numbers follow the
+ * path, not its slot. */
+int wf_raygen_shadow(const uint32_t* seeds, const wf_ray* in_rays, const wf_hit* in_hits,
+                     const int32_t* in_s2i, wf_ray* out_rays, int32_t* out_s2i,
+                     int32_t* out_i2s, int64_t batch_begin, int64_t batch_end,
+                     int num_samples, const float light[3], float light_radius,
+                     wf_stream_t stream, float* ms);
+
+/* raygenPath.comp:78-170 with RayGen::path (Ray/RayGen.cpp:135-186):
+ * shade + extend.  Surviving paths are compacted in input order (prefix-sum
+ * compaction, deterministic) into out_rays / out_s2i; *d_count (device u32)
+ * receives the survivor count and, if host_count != NULL, it is also copied
+ * to *host_count (synchronising).  Seeds are indexed by pixel.  decreases
+ * (float4 per pixel) *= albedo of the hit material.  With russian_roulette
+ * and bounce >= rr_start a path survives with probability
+ * q = clamp(max(decrease.rgb), 0.05, 1) and is divided by q. */
+int wf_shade_extend(wf_scene scene, uint32_t* seeds, const wf_ray* in_rays,
+                    const wf_hit* in_hits, const int32_t* in_s2i, int64_t n,
+                    wf_ray* out_rays, int32_t* out_s2i, float* decreases,
+                    uint32_t* d_count, int russian_roulette, int bounce, int rr_start,
+                    void* scratch, int64_t scratch_bytes, uint32_t* host_count,
+                    wf_stream_t stream, float* ms);
+int64_t wf_compact_scratch_bytes(int64_t n);
+
+/* ---------------------------------------------------------- reconstruction */
+/* Renderer::initDecreases (Renderer/Renderer.cpp:93-104): float4 = 1.0. */
+int wf_fill_f32(float* buf, int64_t count, float value, wf_stream_t stream, float* ms);
+/* reconstructSmooth.comp:34-97.  pixels[s2i[i]].rgb += decrease * (miss ?
+ * background : emission + max(dot(ffN, L), 0.1)) / spp. */
+int wf_reconstruct_smooth(wf_scene scene, const wf_ray* rays, const wf_hit* hits,
+                          const int32_t* s2i, float* pixels, const float* decreases,
+                          int64_t n, const float light[3], const float background[3],
+                          int num_samples, wf_stream_t stream, float* ms);
+/* reconstructShadow.comp:30-62. */
+int wf_reconstruct_shadow(const wf_hit* shadow_hits, const int32_t* in_s2i,
+                          const int32_t* shadow_i2s, const float* in_pixels,
+                          float* out_pixels, int64_t batch_begin, int64_t batch_end,
+                          int num_samples, int replace, wf_stream_t stream, float* ms);
+/* interpolateColors.comp:71-100 (progressive average + Reinhard). */
+int wf_interpolate_colors(float* frame_pixels, float* pixels, int64_t num_pixels,
+                          int frame_index, float key_value, float white_point,
+                          wf_stream_t stream, float* ms);
+/* countRayHits.comp:22-51: *d_count += number of hits with prim >= 0. */
+int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t stream,
+                  float* ms);
+
+/* ------------------------------------------------------------- host logic */
+/* PixelTable::recalculate (Ray/PixelTable.cpp:21-126): 8x8 Z-order blocks
+ * plus stripe tails.  Host arrays of width*height ints. */
+int wf_pixel_table(int width, int height, int32_t* index_to_pixel, int32_t* pixel_to_index);
+
+/* Camera (base/camera.hpp:23-45, :108-120) for a first-person camera:
+ * writes view, perspective (Y flipped, depth zero-to-one) and
+ * screen_to_world = inverse(P*V), each 16 floats column-major. */
+int wf_camera_matrices(const float position[3], const float rotation_deg[3], float fov_deg,
+                       float aspect, float znear, float zfar, float view[16],
+                       float perspective[16], float screen_to_world[16]);
+
+/* -------------------------------------------------------------- renderer */
+/* Mirror of Renderer (Renderer/Renderer.h:212-276) driven by an env.json
+ * (Environment/AppEnvironment.cpp:10-43 keys and defaults, plus the keys
+ * listed in the DESIGN notes of this library). */
+typedef struct wf_renderer_s* wf_renderer;
+int wf_renderer_create(const char* env_json_path, wf_renderer* out);
+int wf_renderer_create_from_string(const char* env_json_text, wf_renderer* out);
+int wf_renderer_destroy(wf_renderer r);
+/* Builds the procedural scene the env names (Scene.procedural) or the one
+ * given by wf_renderer_set_scene. */
+int wf_renderer_build_scene(wf_renderer r);
+int wf_renderer_set_scene(wf_renderer r, wf_scene scene, const float scene_min[3],
+                          const float scene_max[3]);
+int wf_renderer_get_scene(wf_renderer r, wf_scene* out);
+/* Renders one frame (Renderer::render, Renderer.cpp:552-654). Returns the
+ * summed stage GPU ms in *ms_out if non-NULL. */
+int wf_renderer_render(wf_renderer r, float* ms_out);
+int wf_renderer_reset_frame_index(wf_renderer r);
+/* Copies width*height float4 of the accumulated (linear) or tone-mapped
+ * framebuffer to host memory. */
+int wf_renderer_read_pixels(wf_renderer r, int tone_mapped, float* host_rgba);
+int wf_renderer_size(wf_renderer r, int* width, int* height);
+/* Stats of the last frame (Renderer.cpp:660-706):
+ * counts[3] = primary, shadow, path rays; times[3] = trace ms per type;
+ * stage_ms[8] = {keys, sort, reorder, trace_sorted, trace_unsorted,
+ *                shade, raygen, reconstruct} summed over the frame for path rays. */
+int wf_renderer_stats(wf_renderer r, uint64_t counts[3], float times[3], float stage_ms[8],
+                      uint64_t* sorted_rays);
+/* Environment lookups (Environment::get*Value). */
+int wf_env_get_int(wf_renderer r, const char* name, int* value);
+int wf_env_get_float(wf_renderer r, const char* name, float* value);
+int wf_env_get_string(wf_renderer r, const char* name, char* buf, int buf_len);
+int wf_env_get_vector(wf_renderer r, const char* name, float value[3]);
+/* Capture: when enabled, each path bounce's pre-sort wavefront is copied into
+ * renderer-owned buffers; wf_renderer_captured returns bounce b's rays. */
+int wf_renderer_set_capture(wf_renderer r, int enable);
+int wf_renderer_captured(wf_renderer r, int bounce, const wf_ray** rays, int64_t* n);
+
+/* ------------------------------------------------------- procedural scenes */
+/* Host-side generators for the five workloads (seeded SplitMix64).  They
+ * write host arrays; call first with NULL arrays to get the sizes.
+ * Vertex records are {pos.xyz, normal.xyz} (stride 24 bytes). */
+int wf_procedural_scene(int kind, uint64_t seed, int64_t* num_vertices, int64_t* num_tris,
+                        int* num_geoms, int* num_materials, float* vertices,
+                        uint32_t* indices, int32_t* geom_first_tri, int32_t* tri_material,
+                        float* albedo, float* emission, float scene_min[3],
+                        float scene_max[3]);
+enum { WF_SCENE_BOXES = 0, WF_SCENE_INTERIOR = 1, WF_SCENE_CITY = 2 };
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WFPT_H */

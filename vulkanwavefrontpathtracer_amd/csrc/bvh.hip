@@ -1,0 +1,523 @@
+// LBVH build: replaces the driver BLAS/TLAS of the reference
+// (projects/WaveFrontPathTracer/WaveFrontPathTracer.cpp:202-350 builds one
+// BLAS with a geometry per glTF primitive, :355-457 one identity TLAS
+// instance; Tracer::setAccererationStructure binds it, Tracer/Tracer.cpp:
+// 166-183).
+//
+// Build (all on the GPU, stream ordered):
+//   1. tri_prep   : per-triangle AABB and centroid, scene / centroid bounds
+//                   (wave reductions + one ordered-int atomic per wave),
+//                   shading records in original order.
+//   2. centroid Morton codes, 21 bits per axis (63-bit keys), sorted with
+//      this library's radix sort (tri index payload).
+//   3. Karras 2012 binary radix tree over the sorted codes (equal codes are
+//      split by index), parent links.
+//   4. bottom-up AABB refit (second child to arrive at a node merges).
+//   5. emit 64-byte BVH2 nodes holding both child boxes; subtrees of at most
+//      BVH_LEAF_MAX triangles collapse into one leaf (contiguous slots);
+//      child boxes are enlarged by 1e-5 x scene diagonal so that fp32 slab
+//      tests never cull a box whose triangles the intersection test hits.
+#include "scene.h"
+
+#include <cstring>
+#include <vector>
+
+namespace wfpt {
+
+// Monotone map float -> uint for atomicMin/atomicMax on floats.
+__device__ __forceinline__ uint32_t fkey(float f) {
+    uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float funkey(uint32_t k) {
+    uint32_t b = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(b);
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// bounds[0..5]: scene min xyz (keys), scene max xyz; [6..11] centroid min/max.
+__global__ __launch_bounds__(256) void tri_prep_kernel(
+    const char* __restrict__ verts, int64_t nverts, int stride, const uint32_t* __restrict__ idx,
+    int64_t ntri, const int32_t* __restrict__ geom_first, int ngeom,
+    const int32_t* __restrict__ tri_mat, float4* __restrict__ tri_box /* 2 per tri */,
+    float4* __restrict__ shade, int32_t* __restrict__ tri_geom, uint32_t* __restrict__ bounds,
+    uint32_t* __restrict__ bad_index) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool ok = t < ntri;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    float c[3] = {0, 0, 0};
+    if (ok) {
+        float p[3][3], nrm[3][3];
+        for (int k = 0; k < 3; ++k) {
+            uint32_t vi = idx[3 * t + k];
+            if ((int64_t)vi >= nverts) {
+                atomicOr(bad_index, 1u);
+                vi = 0;
+            }
+            const float* v = reinterpret_cast<const float*>(verts + (int64_t)vi * stride);
+            for (int a = 0; a < 3; ++a) {
+                p[k][a] = v[a];
+                nrm[k][a] = v[3 + a];
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = fminf(fminf(p[0][a], p[1][a]), p[2][a]);
+            hi[a] = fmaxf(fmaxf(p[0][a], p[1][a]), p[2][a]);
+            c[a] = (lo[a] + hi[a]) * 0.5f;
+        }
+        tri_box[2 * t] = make_float4(lo[0], lo[1], lo[2], 0.f);
+        tri_box[2 * t + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
+        int mat = tri_mat ? tri_mat[t] : 0;
+        shade[3 * t + 0] = make_float4(nrm[0][0], nrm[0][1], nrm[0][2], __int_as_float(mat));
+        shade[3 * t + 1] = make_float4(nrm[1][0], nrm[1][1], nrm[1][2], 0.f);
+        shade[3 * t + 2] = make_float4(nrm[2][0], nrm[2][1], nrm[2][2], 0.f);
+        // geometry of this triangle: last g with geom_first[g] <= t
+        int lo_g = 0, hi_g = ngeom - 1;
+        while (lo_g < hi_g) {
+            int mid = (lo_g + hi_g + 1) >> 1;
+            if (geom_first[mid] <= t) lo_g = mid;
+            else hi_g = mid - 1;
+        }
+        tri_geom[t] = lo_g;
+    }
+    float cl[3], ch[3];
+    for (int a = 0; a < 3; ++a) {
+        cl[a] = ok ? c[a] : INFINITY;
+        ch[a] = ok ? c[a] : -INFINITY;
+    }
+    for (int a = 0; a < 3; ++a) {
+        float v0 = wave_min(lo[a]), v1 = wave_max(hi[a]), v2 = wave_min(cl[a]), v3 = wave_max(ch[a]);
+        if (lane_id() == 0) {
+            if (v0 <= v1) {
+                atomicMin(&bounds[a], fkey(v0));
+                atomicMax(&bounds[3 + a], fkey(v1));
+            }
+            if (v2 <= v3) {
+                atomicMin(&bounds[6 + a], fkey(v2));
+                atomicMax(&bounds[9 + a], fkey(v3));
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ uint64_t spread3_21(uint64_t v) {
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x001f00000000ffffull;
+    v = (v | (v << 16)) & 0x001f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void centroid_keys_kernel(const float4* __restrict__ tri_box,
+                                                            int64_t ntri,
+                                                            const uint32_t* __restrict__ bounds,
+                                                            uint64_t* __restrict__ keys,
+                                                            uint32_t* __restrict__ vals) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntri) return;
+    float4 lo = tri_box[2 * t], hi = tri_box[2 * t + 1];
+    float c[3] = {(lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f};
+    uint64_t q[3];
+    for (int a = 0; a < 3; ++a) {
+        float mn = funkey(bounds[6 + a]), mx = funkey(bounds[9 + a]);
+        float ext = mx - mn;
+        float u = ext > 0.f ? (c[a] - mn) / ext : 0.f;
+        u = fminf(fmaxf(u, 0.f), 1.f);
+        q[a] = (uint64_t)(u * 2097151.0f);
+    }
+    keys[t] = (spread3_21(q[0]) << 2) | (spread3_21(q[1]) << 1) | spread3_21(q[2]);
+    vals[t] = (uint32_t)t;
+}
+
+__device__ __forceinline__ int lbvh_delta(const uint64_t* __restrict__ k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint64_t a = k[i], b = k[j];
+    if (a != b) return __clzll(a ^ b);
+    return 64 + __clz(i ^ j);
+}
+
+// Internal nodes 0..n-2; combined index space: internal i -> i, leaf s -> n-1+s.
+__global__ __launch_bounds__(256) void karras_kernel(const uint64_t* __restrict__ k, int n,
+                                                     int2* __restrict__ child,
+                                                     int2* __restrict__ range,
+                                                     int32_t* __restrict__ parent) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (lbvh_delta(k, n, i, i + 1) - lbvh_delta(k, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = lbvh_delta(k, n, i, i - d);
+    int lmax = 2;
+    while (lbvh_delta(k, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (lbvh_delta(k, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = lbvh_delta(k, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (lbvh_delta(k, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + min(d, 0);
+    int first = min(i, j), last = max(i, j);
+    int left = (first == gamma) ? (n - 1 + gamma) : gamma;
+    int right = (last == gamma + 1) ? (n - 1 + gamma + 1) : gamma + 1;
+    child[i] = make_int2(left, right);
+    range[i] = make_int2(first, last);
+    parent[left] = i;
+    parent[right] = i;
+    if (i == 0) parent[0] = -1;
+}
+
+__global__ __launch_bounds__(256) void refit_kernel(const float4* __restrict__ tri_box,
+                                                    const uint32_t* __restrict__ sorted_tri, int n,
+                                                    const int2* __restrict__ child,
+                                                    const int32_t* __restrict__ parent,
+                                                    float4* __restrict__ box /* 2 per node */,
+                                                    uint32_t* __restrict__ flags,
+                                                    uint32_t* __restrict__ max_depth) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    uint32_t t = sorted_tri[s];
+    int leaf = n - 1 + s;
+    float4 lo = tri_box[2 * (int64_t)t], hi = tri_box[2 * (int64_t)t + 1];
+    box[2 * leaf] = lo;
+    box[2 * leaf + 1] = hi;
+    // depth of this leaf (number of internal ancestors)
+    uint32_t depth = 0;
+    for (int p = parent[leaf]; p >= 0; p = parent[p]) ++depth;
+    atomicMax(max_depth, depth);
+    int p = parent[leaf];
+    while (p >= 0) {
+        __threadfence();
+        uint32_t old = atomicAdd(&flags[p], 1u);
+        if (old == 0) return;  // the sibling subtree finishes this node
+        __threadfence();
+        int2 c = child[p];
+        float4 al = box[2 * c.x], ah = box[2 * c.x + 1];
+        float4 bl = box[2 * c.y], bh = box[2 * c.y + 1];
+        box[2 * p] = make_float4(fminf(al.x, bl.x), fminf(al.y, bl.y), fminf(al.z, bl.z), 0.f);
+        box[2 * p + 1] = make_float4(fmaxf(ah.x, bh.x), fmaxf(ah.y, bh.y), fmaxf(ah.z, bh.z), 0.f);
+        p = parent[p];
+    }
+}
+
+__device__ __forceinline__ void child_ref(int c, int n, const int2* __restrict__ range,
+                                          int& ref) {
+    if (c >= n - 1) {
+        ref = leaf_ref(c - (n - 1), 1);
+    } else {
+        int2 r = range[c];
+        int size = r.y - r.x + 1;
+        ref = size <= BVH_LEAF_MAX ? leaf_ref(r.x, size) : c;
+    }
+}
+
+__global__ __launch_bounds__(256) void emit_nodes_kernel(int n, const int2* __restrict__ child,
+                                                         const int2* __restrict__ range,
+                                                         const float4* __restrict__ box,
+                                                         const uint32_t* __restrict__ bounds,
+                                                         float4* __restrict__ nodes,
+                                                         uint32_t* __restrict__ leaves) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    float dx = funkey(bounds[3]) - funkey(bounds[0]);
+    float dy = funkey(bounds[4]) - funkey(bounds[1]);
+    float dz = funkey(bounds[5]) - funkey(bounds[2]);
+    float margin = 1e-5f * sqrtf(dx * dx + dy * dy + dz * dz) + 1e-30f;
+    int2 c = child[i];
+    int r0, r1;
+    child_ref(c.x, n, range, r0);
+    child_ref(c.y, n, range, r1);
+    float4 l0 = box[2 * c.x], h0 = box[2 * c.x + 1], l1 = box[2 * c.y], h1 = box[2 * c.y + 1];
+    l0.x -= margin; l0.y -= margin; l0.z -= margin; h0.x += margin; h0.y += margin; h0.z += margin;
+    l1.x -= margin; l1.y -= margin; l1.z -= margin; h1.x += margin; h1.y += margin; h1.z += margin;
+    float4* o = nodes + 4 * (int64_t)i;
+    o[0] = make_float4(l0.x, l0.y, l0.z, h0.x);
+    o[1] = make_float4(h0.y, h0.z, l1.x, l1.y);
+    o[2] = make_float4(l1.z, h1.x, h1.y, h1.z);
+    o[3] = make_float4(__int_as_float(r0), __int_as_float(r1), 0.f, 0.f);
+    if (r0 < 0) atomicAdd(leaves, 1u);
+    if (r1 < 0) atomicAdd(leaves, 1u);
+}
+
+// Single-triangle scene: one node with the leaf and an empty box.
+__global__ void emit_single_kernel(const float4* __restrict__ tri_box, float4* __restrict__ nodes) {
+    float4 lo = tri_box[0], hi = tri_box[1];
+    float m = 1e-5f * sqrtf((hi.x - lo.x) * (hi.x - lo.x) + (hi.y - lo.y) * (hi.y - lo.y) +
+                            (hi.z - lo.z) * (hi.z - lo.z)) + 1e-30f;
+    nodes[0] = make_float4(lo.x - m, lo.y - m, lo.z - m, hi.x + m);
+    nodes[1] = make_float4(hi.y + m, hi.z + m, INFINITY, INFINITY);
+    nodes[2] = make_float4(INFINITY, -INFINITY, -INFINITY, -INFINITY);
+    nodes[3] = make_float4(__int_as_float(leaf_ref(0, 1)), __int_as_float(leaf_ref(0, 1)), 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(256) void emit_tris_kernel(const char* __restrict__ verts, int stride,
+                                                        const uint32_t* __restrict__ idx,
+                                                        const uint32_t* __restrict__ sorted_tri,
+                                                        const int32_t* __restrict__ tri_geom,
+                                                        const int32_t* __restrict__ geom_first,
+                                                        int64_t ntri, float4* __restrict__ tris) {
+    int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ntri) return;
+    uint32_t t = sorted_tri[s];
+    int g = tri_geom[t];
+    int prim = (int)t - geom_first[g];
+    float w[3] = {__int_as_float((int)t), __int_as_float(prim), __int_as_float(g)};
+    for (int k = 0; k < 3; ++k) {
+        const float* v = reinterpret_cast<const float*>(verts + (int64_t)idx[3 * (int64_t)t + k] * stride);
+        tris[3 * s + k] = make_float4(v[0], v[1], v[2], w[k]);
+    }
+}
+
+// ------------------------------------------------------------------ host
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+static int dev_alloc(DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) return fail(WF_ERR_OUT_OF_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return WF_OK;
+}
+
+static void free_scene(wf_scene_s* s) {
+    if (!s) return;
+    if (s->tris) (void)hipFree(s->tris);
+    if (s->nodes) (void)hipFree(s->nodes);
+    if (s->shade) (void)hipFree(s->shade);
+    if (s->geom_first) (void)hipFree(s->geom_first);
+    if (s->albedo) (void)hipFree(s->albedo);
+    if (s->emission) (void)hipFree(s->emission);
+    delete s;
+}
+
+}  // namespace wfpt
+
+using namespace wfpt;
+
+extern "C" int wf_scene_destroy(wf_scene scene) {
+    free_scene(scene);
+    return WF_OK;
+}
+
+extern "C" int wf_scene_set_materials(wf_scene scene, const float* albedo_rgb,
+                                      const float* emission_rgb, int num_materials,
+                                      wf_stream_t stream) {
+    WF_CHECK_ARG(scene, "wf_scene_set_materials: null scene");
+    WF_CHECK_ARG(num_materials >= 1 && albedo_rgb && emission_rgb,
+                 "wf_scene_set_materials: need >= 1 material and host albedo/emission arrays");
+    hipStream_t s = as_stream(stream);
+    float *a = nullptr, *e = nullptr;
+    WF_HIP(hipMalloc((void**)&a, sizeof(float) * 3 * num_materials));
+    WF_HIP(hipMalloc((void**)&e, sizeof(float) * 3 * num_materials));
+    WF_HIP(hipMemcpyAsync(a, albedo_rgb, sizeof(float) * 3 * num_materials, hipMemcpyHostToDevice, s));
+    WF_HIP(hipMemcpyAsync(e, emission_rgb, sizeof(float) * 3 * num_materials, hipMemcpyHostToDevice, s));
+    WF_HIP(hipStreamSynchronize(s));
+    if (scene->albedo) (void)hipFree(scene->albedo);
+    if (scene->emission) (void)hipFree(scene->emission);
+    scene->albedo = a;
+    scene->emission = e;
+    scene->num_materials = num_materials;
+    return WF_OK;
+}
+
+extern "C" int wf_scene_create(const void* vertices, int64_t num_vertices, int vertex_stride,
+                               const uint32_t* indices, int64_t num_tris,
+                               const int32_t* geom_first_tri, int num_geoms,
+                               const int32_t* tri_material, wf_stream_t stream, wf_scene* out) {
+    WF_CHECK_ARG(out, "wf_scene_create: null out");
+    *out = nullptr;
+    WF_CHECK_ARG(num_tris >= 0 && num_vertices >= 0, "wf_scene_create: negative size");
+    WF_CHECK_ARG(vertex_stride >= 24 && vertex_stride % 4 == 0,
+                 "wf_scene_create: vertex_stride must be >= 24 and a multiple of 4");
+    WF_CHECK_ARG(num_tris <= (int64_t)(1 << 28), "wf_scene_create: more than 2^28 triangles");
+    WF_CHECK_ARG(num_geoms >= 1 && geom_first_tri, "wf_scene_create: need >= 1 geometry");
+    WF_CHECK_ARG(geom_first_tri[0] == 0 && geom_first_tri[num_geoms] == num_tris,
+                 "wf_scene_create: geom_first_tri must start at 0 and end at num_tris");
+    for (int g = 0; g < num_geoms; ++g)
+        WF_CHECK_ARG(geom_first_tri[g] <= geom_first_tri[g + 1],
+                     "wf_scene_create: geom_first_tri must be ascending");
+    if (num_tris > 0) WF_CHECK_ARG(vertices && indices, "wf_scene_create: null vertex/index buffer");
+
+    hipStream_t s = as_stream(stream);
+    wf_scene_s* sc = new wf_scene_s();
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    sc->device = dev;
+    sc->num_tris = num_tris;
+    sc->num_geoms = num_geoms;
+    struct Guard {
+        wf_scene_s*& p;
+        ~Guard() {
+            if (p) free_scene(p);
+        }
+    };
+    wf_scene_s* owned = sc;
+    Guard guard{owned};
+
+    const int64_t n = num_tris;
+    WF_HIP(hipMalloc((void**)&sc->geom_first, sizeof(int32_t) * (num_geoms + 1)));
+    WF_HIP(hipMemcpyAsync(sc->geom_first, geom_first_tri, sizeof(int32_t) * (num_geoms + 1),
+                          hipMemcpyHostToDevice, s));
+    {
+        float a[3] = {0.8f, 0.8f, 0.8f}, e[3] = {0.f, 0.f, 0.f};
+        WF_TRY(wf_scene_set_materials(sc, a, e, 1, stream));
+    }
+    WF_HIP(hipMalloc((void**)&sc->tris, sizeof(float4) * 3 * (n > 0 ? n : 1)));
+    WF_HIP(hipMalloc((void**)&sc->shade, sizeof(float4) * 3 * (n > 0 ? n : 1)));
+    sc->num_nodes = n >= 2 ? n - 1 : 1;
+    WF_HIP(hipMalloc((void**)&sc->nodes, sizeof(float4) * 4 * sc->num_nodes));
+
+    if (n == 0) {
+        for (int a = 0; a < 3; ++a) sc->bmin[a] = sc->bmax[a] = 0.f;
+        sc->num_leaves = 0;
+        *out = sc;
+        owned = nullptr;
+        return WF_OK;
+    }
+
+    DevBuf tri_box, tri_geom, bounds, keys, vals, scratch, child, range, parent, box, flags, misc;
+    WF_TRY(dev_alloc(tri_box, sizeof(float4) * 2 * n));
+    WF_TRY(dev_alloc(tri_geom, sizeof(int32_t) * n));
+    WF_TRY(dev_alloc(bounds, sizeof(uint32_t) * 16));
+    WF_TRY(dev_alloc(keys, sizeof(uint64_t) * n));
+    WF_TRY(dev_alloc(vals, sizeof(uint32_t) * n));
+    int64_t sb = wf_sort_scratch_bytes(n, 8, 63, 8);
+    if (sb < 0) return (int)sb;
+    WF_TRY(dev_alloc(scratch, (size_t)sb));
+    WF_TRY(dev_alloc(child, sizeof(int2) * (n > 1 ? n - 1 : 1)));
+    WF_TRY(dev_alloc(range, sizeof(int2) * (n > 1 ? n - 1 : 1)));
+    WF_TRY(dev_alloc(parent, sizeof(int32_t) * (2 * n - 1)));
+    WF_TRY(dev_alloc(box, sizeof(float4) * 2 * (2 * n - 1)));
+    WF_TRY(dev_alloc(flags, sizeof(uint32_t) * (n > 1 ? n - 1 : 1)));
+    WF_TRY(dev_alloc(misc, sizeof(uint32_t) * 4));  // [0] bad index, [1] depth, [2] leaves
+
+    uint32_t* b = (uint32_t*)bounds.p;
+    // min slots start at key(+inf) > everything, max slots at key(-inf) = small
+    {
+        std::vector<uint32_t> init(16, 0u);
+        auto key_host = [](float f) {
+            uint32_t u;
+            memcpy(&u, &f, 4);
+            return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+        };
+        for (int a = 0; a < 3; ++a) {
+            init[a] = key_host(INFINITY);
+            init[3 + a] = key_host(-INFINITY);
+            init[6 + a] = key_host(INFINITY);
+            init[9 + a] = key_host(-INFINITY);
+        }
+        WF_HIP(hipMemcpyAsync(b, init.data(), sizeof(uint32_t) * 16, hipMemcpyHostToDevice, s));
+        WF_HIP(hipStreamSynchronize(s));
+    }
+    WF_TRY(fill_u32((uint32_t*)misc.p, 4, 0u, s));
+    if (n > 1) WF_TRY(fill_u32((uint32_t*)flags.p, n - 1, 0u, s));
+
+    tri_prep_kernel<<<grid_for(n, 256), 256, 0, s>>>(
+        (const char*)vertices, num_vertices, vertex_stride, indices, n, sc->geom_first, num_geoms,
+        tri_material, (float4*)tri_box.p, sc->shade, (int32_t*)tri_geom.p, b, (uint32_t*)misc.p);
+    WF_TRY(launch_check("tri_prep_kernel"));
+
+    if (n == 1) {
+        WF_TRY(fill_u32((uint32_t*)vals.p, 1, 0u, s));
+        emit_single_kernel<<<1, 1, 0, s>>>((const float4*)tri_box.p, sc->nodes);
+        WF_TRY(launch_check("emit_single_kernel"));
+        sc->num_leaves = 1;
+    } else {
+        centroid_keys_kernel<<<grid_for(n, 256), 256, 0, s>>>((const float4*)tri_box.p, n, b,
+                                                              (uint64_t*)keys.p, (uint32_t*)vals.p);
+        WF_TRY(launch_check("centroid_keys_kernel"));
+        WF_TRY(wf_sort_pairs(keys.p, (uint32_t*)vals.p, n, 8, 63, 8, scratch.p, sb, stream, nullptr));
+        karras_kernel<<<grid_for(n - 1, 256), 256, 0, s>>>((const uint64_t*)keys.p, (int)n,
+                                                           (int2*)child.p, (int2*)range.p,
+                                                           (int32_t*)parent.p);
+        WF_TRY(launch_check("karras_kernel"));
+        refit_kernel<<<grid_for(n, 256), 256, 0, s>>>(
+            (const float4*)tri_box.p, (const uint32_t*)vals.p, (int)n, (const int2*)child.p,
+            (const int32_t*)parent.p, (float4*)box.p, (uint32_t*)flags.p, (uint32_t*)misc.p + 1);
+        WF_TRY(launch_check("refit_kernel"));
+        emit_nodes_kernel<<<grid_for(n - 1, 256), 256, 0, s>>>(
+            (int)n, (const int2*)child.p, (const int2*)range.p, (const float4*)box.p, b, sc->nodes,
+            (uint32_t*)misc.p + 2);
+        WF_TRY(launch_check("emit_nodes_kernel"));
+    }
+    emit_tris_kernel<<<grid_for(n, 256), 256, 0, s>>>(
+        (const char*)vertices, vertex_stride, indices, (const uint32_t*)vals.p,
+        (const int32_t*)tri_geom.p, sc->geom_first, n, sc->tris);
+    WF_TRY(launch_check("emit_tris_kernel"));
+
+    uint32_t hb[16], hm[4];
+    WF_HIP(hipMemcpyAsync(hb, b, sizeof(hb), hipMemcpyDeviceToHost, s));
+    WF_HIP(hipMemcpyAsync(hm, misc.p, sizeof(hm), hipMemcpyDeviceToHost, s));
+    WF_HIP(hipStreamSynchronize(s));
+    if (hm[0]) return fail(WF_ERR_INVALID_ARG, "wf_scene_create: index >= num_vertices");
+    auto unkey = [](uint32_t k) {
+        uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+        float f;
+        memcpy(&f, &u, 4);
+        return f;
+    };
+    for (int a = 0; a < 3; ++a) {
+        sc->bmin[a] = unkey(hb[a]);
+        sc->bmax[a] = unkey(hb[3 + a]);
+    }
+    sc->max_depth = (int)hm[1];
+    if (n > 1) sc->num_leaves = hm[2];
+    if (sc->max_depth >= TRACE_STACK)
+        return fail(WF_ERR_TOO_LARGE, "wf_scene_create: BVH deeper than the traversal stack");
+    *out = sc;
+    owned = nullptr;
+    return WF_OK;
+}
+
+extern "C" int wf_scene_bounds(wf_scene scene, float out_min[3], float out_max[3]) {
+    WF_CHECK_ARG(scene && out_min && out_max, "wf_scene_bounds: null argument");
+    for (int a = 0; a < 3; ++a) {
+        out_min[a] = scene->bmin[a];
+        out_max[a] = scene->bmax[a];
+    }
+    return WF_OK;
+}
+
+extern "C" int wf_scene_info(wf_scene scene, int64_t* num_tris, int64_t* num_nodes,
+                             int64_t* num_leaves, int* num_geoms) {
+    WF_CHECK_ARG(scene, "wf_scene_info: null scene");
+    if (num_tris) *num_tris = scene->num_tris;
+    if (num_nodes) *num_nodes = scene->num_tris >= 1 ? scene->num_nodes : 0;
+    if (num_leaves) *num_leaves = scene->num_leaves;
+    if (num_geoms) *num_geoms = scene->num_geoms;
+    return WF_OK;
+}
+
+extern "C" int wf_scene_download(wf_scene scene, float* nodes_host, int32_t* tri_order_host) {
+    WF_CHECK_ARG(scene, "wf_scene_download: null scene");
+    if (scene->num_tris == 0) return WF_OK;
+    if (nodes_host)
+        WF_HIP(hipMemcpy(nodes_host, scene->nodes, sizeof(float4) * 4 * scene->num_nodes,
+                         hipMemcpyDeviceToHost));
+    if (tri_order_host) {
+        std::vector<float4> t(3 * scene->num_tris);
+        WF_HIP(hipMemcpy(t.data(), scene->tris, sizeof(float4) * 3 * scene->num_tris,
+                         hipMemcpyDeviceToHost));
+        for (int64_t s = 0; s < scene->num_tris; ++s) {
+            float w = t[3 * s].w;
+            memcpy(&tri_order_host[s], &w, 4);
+        }
+    }
+    return WF_OK;
+}

@@ -1,0 +1,130 @@
+// Internal helpers shared by the HIP translation units of libwfpt.
+//
+// All device code is compiled with -ffp-contract=off: every multiply-add that
+// should fuse is written as an explicit fmaf(), so the CPU restatement can
+// reproduce the arithmetic bit for bit.  fp32 division and sqrt are correctly
+// rounded (clang's HIP default).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/wfpt.h"
+
+namespace wfpt {
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string& msg);
+int fail(int status, const std::string& msg);
+
+#define WF_HIP(call)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return ::wfpt::fail(WF_ERR_HIP, std::string(#call) + ": " +              \
+                                                hipGetErrorString(e_));              \
+    } while (0)
+
+#define WF_CHECK_ARG(cond, msg)                                                      \
+    do {                                                                             \
+        if (!(cond)) return ::wfpt::fail(WF_ERR_INVALID_ARG, msg);                   \
+    } while (0)
+
+#define WF_TRY(expr)                                                                 \
+    do {                                                                             \
+        int s_ = (expr);                                                             \
+        if (s_ != WF_OK) return s_;                                                  \
+    } while (0)
+
+inline hipStream_t as_stream(wf_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Brackets the launches of one entry point with hipEvents when the caller
+// asked for a time (ms != nullptr), like ComputePass::launchTimed.
+class StageTimer {
+  public:
+    StageTimer(hipStream_t s, float* ms) : s_(s), ms_(ms) {
+        if (ms_) {
+            if (hipEventCreate(&a_) != hipSuccess || hipEventCreate(&b_) != hipSuccess) {
+                ms_ = nullptr;
+                return;
+            }
+            (void)hipEventRecord(a_, s_);
+        }
+    }
+    int finish() {
+        if (!ms_) return WF_OK;
+        WF_HIP(hipEventRecord(b_, s_));
+        WF_HIP(hipEventSynchronize(b_));
+        float t = 0.f;
+        WF_HIP(hipEventElapsedTime(&t, a_, b_));
+        *ms_ = t;
+        return WF_OK;
+    }
+    ~StageTimer() {
+        if (a_) (void)hipEventDestroy(a_);
+        if (b_) (void)hipEventDestroy(b_);
+    }
+
+  private:
+    hipStream_t s_;
+    float* ms_;
+    hipEvent_t a_ = nullptr, b_ = nullptr;
+};
+
+inline int launch_check(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(WF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return WF_OK;
+}
+
+inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+// ---------------------------------------------------------- device helpers
+struct float3_ {
+    float x, y, z;
+};
+
+__device__ __forceinline__ float4 ld_f4(const float* p) {
+    return *reinterpret_cast<const float4*>(p);
+}
+
+// Load one 32-byte Ray record as two 16-byte loads.
+__device__ __forceinline__ void load_ray(const wf_ray* rays, int64_t i, float4& a, float4& b) {
+    const float4* p = reinterpret_cast<const float4*>(rays + i);
+    a = p[0];
+    b = p[1];
+}
+__device__ __forceinline__ void store_ray(wf_ray* rays, int64_t i, float4 a, float4 b) {
+    float4* p = reinterpret_cast<float4*>(rays + i);
+    p[0] = a;
+    p[1] = b;
+}
+
+// Wave64 primitives.
+__device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+// Number of set bits of `mask` below the calling lane.
+__device__ __forceinline__ unsigned popc_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+// Look-back status words: 2 flag bits + 30-bit count.
+constexpr uint32_t LB_FLAG_AGG = 1u << 30;
+constexpr uint32_t LB_FLAG_INC = 2u << 30;
+constexpr uint32_t LB_MASK = (1u << 30) - 1u;
+
+__device__ __forceinline__ uint32_t lb_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void lb_store(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Fill kernel used instead of hipMemsetAsync so that the clears stay valid
+// inside captured graphs.
+__global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v);
+int fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
+
+}  // namespace wfpt

@@ -1,0 +1,67 @@
+// Status strings, per-thread error messages and the graph-safe fill kernels.
+#include "common.h"
+
+#include <cstring>
+
+namespace wfpt {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int status, const std::string& msg) {
+    g_last_error = msg;
+    return status;
+}
+
+__global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+int fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
+    if (n <= 0) return WF_OK;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    fill_u32_kernel<<<(unsigned)blocks, 256, 0, s>>>(p, n, v);
+    return launch_check("fill_u32_kernel");
+}
+
+}  // namespace wfpt
+
+using namespace wfpt;
+
+extern "C" const char* wf_status_string(int status) {
+    switch (status) {
+        case WF_OK: return "ok";
+        case WF_ERR_INVALID_ARG: return "invalid argument";
+        case WF_ERR_HIP: return "HIP runtime error";
+        case WF_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case WF_ERR_SCRATCH_TOO_SMALL: return "scratch buffer too small";
+        case WF_ERR_TOO_LARGE: return "element count too large";
+        case WF_ERR_PARSE: return "parse error";
+        case WF_ERR_IO: return "I/O error";
+        case WF_ERR_STATE: return "invalid state";
+        default: return "unknown status";
+    }
+}
+
+extern "C" const char* wf_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" const char* wf_version(void) { return "wfpt-mi355x 0.1.0 (gfx950)"; }
+
+extern "C" int wf_fill_f32(float* buf, int64_t count, float value, wf_stream_t stream, float* ms) {
+    WF_CHECK_ARG(count >= 0, "wf_fill_f32: count < 0");
+    if (count == 0) {
+        if (ms) *ms = 0.f;
+        return WF_OK;
+    }
+    WF_CHECK_ARG(buf, "wf_fill_f32: null buffer");
+    uint32_t bits;
+    std::memcpy(&bits, &value, 4);
+    hipStream_t s = as_stream(stream);
+    StageTimer t(s, ms);
+    WF_TRY(fill_u32(reinterpret_cast<uint32_t*>(buf), count, bits, s));
+    return t.finish();
+}

@@ -1,0 +1,228 @@
+// Ray sort keys.
+//
+// WF_KEY_AILA32 restates computeMortonCodesAila32.comp:39-88 (7 origin bits
+// and 4 direction bits per axis, the direction's z bit 0 dropped).  The other
+// modes are this library's own (origin-only 30-bit key, and 60-/64-bit keys
+// with an octahedral direction code after Cigolle et al. 2014).
+//
+// Arithmetic conventions (mirrored exactly by the CPU restatement):
+//   a    = clamp01((o - scene_min) / scene_size)   per component, IEEE div
+//   norm = d * (1 / sqrt(dx*dx + dy*dy + dz*dz))    no fused multiply-add
+//   clamp01(x) = fminf(fmaxf(x, 0), 1)              (NaN -> 0)
+//   quantisation truncates: uint(x * scale)
+#include "common.h"
+
+namespace wfpt {
+
+__device__ __forceinline__ float clamp01(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+
+// Spread the low 10 bits of v so that bit k lands on bit 3k.
+__device__ __forceinline__ uint32_t spread3_10(uint32_t v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+// Spread the low 16 bits of v so that bit k lands on bit 2k.
+__device__ __forceinline__ uint32_t spread2_16(uint32_t v) {
+    v &= 0xffffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+struct KeyFrame {
+    float minx, miny, minz, sx, sy, sz;
+};
+
+__device__ __forceinline__ void origin_unit(const KeyFrame& f, float4 o, float& ax, float& ay,
+                                            float& az) {
+    ax = clamp01((o.x - f.minx) / f.sx);
+    ay = clamp01((o.y - f.miny) / f.sy);
+    az = clamp01((o.z - f.minz) / f.sz);
+}
+
+__device__ __forceinline__ uint32_t origin30(const KeyFrame& f, float4 o) {
+    float ax, ay, az;
+    origin_unit(f, o, ax, ay, az);
+    uint32_t ix = (uint32_t)(ax * 1023.0f), iy = (uint32_t)(ay * 1023.0f),
+             iz = (uint32_t)(az * 1023.0f);
+    return (spread3_10(ix) << 2) | (spread3_10(iy) << 1) | spread3_10(iz);
+}
+
+// Octahedral direction code with `bits` bits per axis, u-major 2-D Morton.
+__device__ __forceinline__ uint32_t oct_code(float4 d, int bits) {
+    float l1 = (fabsf(d.x) + fabsf(d.y)) + fabsf(d.z);
+    float px = d.x / l1, py = d.y / l1;
+    if (d.z < 0.0f) {
+        float nx = (1.0f - fabsf(py)) * (px >= 0.0f ? 1.0f : -1.0f);
+        float ny = (1.0f - fabsf(px)) * (py >= 0.0f ? 1.0f : -1.0f);
+        px = nx;
+        py = ny;
+    }
+    float u = px * 0.5f + 0.5f, v = py * 0.5f + 0.5f;
+    float scale = (float)((1u << bits) - 1u);
+    uint32_t qu = (uint32_t)(clamp01(u) * scale), qv = (uint32_t)(clamp01(v) * scale);
+    return (spread2_16(qu) << 1) | spread2_16(qv);
+}
+
+__device__ __forceinline__ uint32_t aila32(const KeyFrame& f, float4 o, float4 d) {
+    float ax, ay, az;
+    origin_unit(f, o, ax, ay, az);
+    float inv = 1.0f / sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+    float bx = clamp01((d.x * inv + 1.0f) * 0.5f);
+    float by = clamp01((d.y * inv + 1.0f) * 0.5f);
+    float bz = clamp01((d.z * inv + 1.0f) * 0.5f);
+    uint32_t ia[3] = {(uint32_t)(ax * 127.0f), (uint32_t)(ay * 127.0f), (uint32_t)(az * 127.0f)};
+    uint32_t ib[3] = {(uint32_t)(bx * 15.0f), (uint32_t)(by * 15.0f), (uint32_t)(bz * 15.0f)};
+    // Triples of (x,y,z) bits from the most significant end: origin bits
+    // 6,5,4,3, direction bit 3, origin 2, direction 2, origin 1, direction 1,
+    // origin 0, then the x and y direction bits 0.
+    auto tri = [](const uint32_t* v, int k) {
+        return (((v[0] >> k) & 1u) << 2) | (((v[1] >> k) & 1u) << 1) | ((v[2] >> k) & 1u);
+    };
+    uint32_t key = 0;
+    key |= tri(ia, 6) << 29;
+    key |= tri(ia, 5) << 26;
+    key |= tri(ia, 4) << 23;
+    key |= tri(ia, 3) << 20;
+    key |= tri(ib, 3) << 17;
+    key |= tri(ia, 2) << 14;
+    key |= tri(ib, 2) << 11;
+    key |= tri(ia, 1) << 8;
+    key |= tri(ib, 1) << 5;
+    key |= tri(ia, 0) << 2;
+    key |= ((ib[0] & 1u) << 1) | (ib[1] & 1u);
+    return key;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void keys_kernel(const wf_ray* __restrict__ rays,
+                                                   void* __restrict__ keys,
+                                                   uint32_t* __restrict__ indices, int64_t n,
+                                                   KeyFrame f) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 o, d;
+    load_ray(rays, i, o, d);
+    if (MODE == WF_KEY_AILA32) {
+        static_cast<uint32_t*>(keys)[i] = aila32(f, o, d);
+    } else if (MODE == WF_KEY_ORIGIN30) {
+        static_cast<uint32_t*>(keys)[i] = origin30(f, o);
+    } else {
+        uint64_t k = ((uint64_t)origin30(f, o) << 30) | (uint64_t)oct_code(d, 15);
+        static_cast<uint64_t*>(keys)[i] = k;
+    }
+    indices[i] = (uint32_t)i;
+}
+
+struct SceneFrames {
+    float f[6 * 64];  // per scene: min.xyz, size.xyz
+};
+
+__global__ __launch_bounds__(256) void keys_batched_kernel(
+    const wf_ray* __restrict__ rays, const uint8_t* __restrict__ scene_of_ray,
+    uint64_t* __restrict__ keys, uint32_t* __restrict__ indices, int64_t n,
+    const SceneFrames frames) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 o, d;
+    load_ray(rays, i, o, d);
+    uint32_t s = scene_of_ray[i];
+    const float* fr = frames.f + 6 * (s & 63u);
+    KeyFrame f{fr[0], fr[1], fr[2], fr[3], fr[4], fr[5]};
+    uint64_t k = ((uint64_t)(s & 63u) << 58) | ((uint64_t)origin30(f, o) << 28) |
+                 (uint64_t)oct_code(d, 14);
+    keys[i] = k;
+    indices[i] = (uint32_t)i;
+}
+
+}  // namespace wfpt
+
+using namespace wfpt;
+
+extern "C" int wf_key_bytes(int mode) {
+    switch (mode) {
+        case WF_KEY_AILA32:
+        case WF_KEY_ORIGIN30: return 4;
+        case WF_KEY_ORIGIN30_OCT30:
+        case WF_KEY_SCENE6_ORIGIN30_OCT28: return 8;
+        default: return WF_ERR_INVALID_ARG;
+    }
+}
+
+extern "C" int wf_key_bits(int mode) {
+    switch (mode) {
+        case WF_KEY_AILA32: return 32;
+        case WF_KEY_ORIGIN30: return 30;
+        case WF_KEY_ORIGIN30_OCT30: return 60;
+        case WF_KEY_SCENE6_ORIGIN30_OCT28: return 64;
+        default: return WF_ERR_INVALID_ARG;
+    }
+}
+
+extern "C" int wf_morton_keys(const wf_ray* rays, void* keys, uint32_t* indices, int64_t n,
+                              int mode, const float scene_min[3], const float scene_max[3],
+                              wf_stream_t stream, float* ms) {
+    WF_CHECK_ARG(n >= 0, "wf_morton_keys: n < 0");
+    WF_CHECK_ARG(mode >= WF_KEY_AILA32 && mode <= WF_KEY_ORIGIN30_OCT30,
+                 "wf_morton_keys: unknown key mode (use wf_morton_keys_batched for scene keys)");
+    WF_CHECK_ARG(scene_min && scene_max, "wf_morton_keys: null scene bounds");
+    if (n == 0) {
+        if (ms) *ms = 0.f;
+        return WF_OK;
+    }
+    WF_CHECK_ARG(rays && keys && indices, "wf_morton_keys: null buffer");
+    WF_CHECK_ARG(n <= (int64_t)UINT32_MAX, "wf_morton_keys: n exceeds u32 indices");
+    // Tracer::computeMortonCodes passes sceneSize = max - min (Tracer.cpp:29).
+    KeyFrame f{scene_min[0], scene_min[1], scene_min[2], scene_max[0] - scene_min[0],
+               scene_max[1] - scene_min[1], scene_max[2] - scene_min[2]};
+    hipStream_t s = as_stream(stream);
+    StageTimer t(s, ms);
+    unsigned g = grid_for(n, 256);
+    switch (mode) {
+        case WF_KEY_AILA32:
+            keys_kernel<WF_KEY_AILA32><<<g, 256, 0, s>>>(rays, keys, indices, n, f);
+            break;
+        case WF_KEY_ORIGIN30:
+            keys_kernel<WF_KEY_ORIGIN30><<<g, 256, 0, s>>>(rays, keys, indices, n, f);
+            break;
+        default:
+            keys_kernel<WF_KEY_ORIGIN30_OCT30><<<g, 256, 0, s>>>(rays, keys, indices, n, f);
+            break;
+    }
+    WF_TRY(launch_check("keys_kernel"));
+    return t.finish();
+}
+
+extern "C" int wf_morton_keys_batched(const wf_ray* rays, const uint8_t* scene_of_ray,
+                                      void* keys, uint32_t* indices, int64_t n,
+                                      const float* scene_min, const float* scene_max,
+                                      int num_scenes, wf_stream_t stream, float* ms) {
+    WF_CHECK_ARG(n >= 0, "wf_morton_keys_batched: n < 0");
+    WF_CHECK_ARG(num_scenes >= 1 && num_scenes <= 64,
+                 "wf_morton_keys_batched: num_scenes must be in [1, 64]");
+    WF_CHECK_ARG(scene_min && scene_max, "wf_morton_keys_batched: null scene bounds");
+    if (n == 0) {
+        if (ms) *ms = 0.f;
+        return WF_OK;
+    }
+    WF_CHECK_ARG(rays && keys && indices && scene_of_ray, "wf_morton_keys_batched: null buffer");
+    WF_CHECK_ARG(n <= (int64_t)UINT32_MAX, "wf_morton_keys_batched: n exceeds u32 indices");
+    SceneFrames frames{};
+    for (int k = 0; k < num_scenes; ++k)
+        for (int a = 0; a < 3; ++a) {
+            frames.f[6 * k + a] = scene_min[3 * k + a];
+            frames.f[6 * k + 3 + a] = scene_max[3 * k + a] - scene_min[3 * k + a];
+        }
+    hipStream_t s = as_stream(stream);
+    StageTimer t(s, ms);
+    keys_batched_kernel<<<grid_for(n, 256), 256, 0, s>>>(
+        rays, scene_of_ray, static_cast<uint64_t*>(keys), indices, n, frames);
+    WF_TRY(launch_check("keys_batched_kernel"));
+    return t.finish();
+}

@@ -1,0 +1,49 @@
+// Scene = triangles + LBVH + materials, owned by the library.
+//
+// Device layout (all 16-byte aligned):
+//   tris    [3 * num_tris] float4, in BVH leaf order ("slots"):
+//             {v0.xyz, original triangle index}, {v1.xyz, prim in geometry},
+//             {v2.xyz, geometry index}   (the .w fields hold int bits)
+//   nodes   [max(num_tris - 1, 1)] x 4 float4 (64 bytes):
+//             {c0.lo.xyz, c0.hi.x}, {c0.hi.yz, c1.lo.xy}, {c1.lo.z, c1.hi.xyz},
+//             {ref0, ref1, 0, 0} (int bits).  ref >= 0: internal node index;
+//             ref < 0: leaf ~(first_slot << 3 | (count - 1)), count <= 8.
+//   shade   [3 * num_tris] float4 in ORIGINAL triangle order:
+//             {n0.xyz, material}, {n1.xyz, 0}, {n2.xyz, 0}
+//   geom_first [num_geoms + 1] int: first original triangle of each geometry.
+//   albedo / emission [3 * num_materials] float.
+#pragma once
+
+#include "common.h"
+
+struct wf_scene_s {
+    int64_t num_tris = 0;
+    int64_t num_nodes = 0;
+    int64_t num_leaves = 0;
+    int num_geoms = 0;
+    int num_materials = 0;
+    int max_depth = 0;
+    float bmin[3] = {0, 0, 0};
+    float bmax[3] = {0, 0, 0};
+    float4* tris = nullptr;
+    float4* nodes = nullptr;
+    float4* shade = nullptr;
+    int32_t* geom_first = nullptr;
+    float* albedo = nullptr;
+    float* emission = nullptr;
+    int device = 0;
+};
+
+namespace wfpt {
+
+constexpr int BVH_LEAF_MAX = 4;        // triangles per collapsed leaf (<= 8)
+constexpr int TRACE_STACK = 64;        // traversal stack entries per ray
+
+__device__ __forceinline__ int leaf_ref(int first, int count) { return ~((first << 3) | (count - 1)); }
+__device__ __forceinline__ int leaf_first(int ref) { return (~ref) >> 3; }
+__device__ __forceinline__ int leaf_count(int ref) { return ((~ref) & 7) + 1; }
+
+// Material lookup for shading; nmat >= 1.
+__device__ __forceinline__ int clamp_material(int m, int nmat) { return (m >= 0 && m < nmat) ? m : 0; }
+
+}  // namespace wfpt

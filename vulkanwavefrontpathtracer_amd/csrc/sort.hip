@@ -1,0 +1,434 @@
+// Stable LSD radix sort of (key, u32 value) pairs, in place.
+//
+// Replaces the vulkan_radix_sort dependency called by Tracer::radixSort
+// (projects/WaveFrontPathTracer/Tracer/Tracer.cpp:39-78 ->
+// external/vulkan_radix_sort/vk_radix_sort.h:2579-2585).  The reference runs
+// 4 x (upsweep, spine, downsweep) with 8-bit digits on u32 keys; this is a
+// onesweep-style design for CDNA4:
+//   1. one histogram kernel reads every key once and counts the digits of
+//      all passes (LDS histograms, one global atomic per non-empty bin);
+//   2. one tiny scan kernel turns them into per-pass digit start offsets;
+//   3. one binning kernel per pass: each 256-thread workgroup (4 wave64)
+//      takes a 4096-element tile in arrival order (atomic tile counter),
+//      ranks its keys stably with wave64 ballot matching, publishes its
+//      per-digit counts and resolves the exclusive prefix of earlier tiles by
+//      decoupled look-back, stages the tile in LDS in digit order and writes
+//      each digit run contiguously.
+// HBM traffic per element: key read once for the histograms plus, per pass,
+// one read and one write of key and value.
+#include "common.h"
+
+namespace wfpt {
+
+constexpr int SORT_BLOCK = 256;
+constexpr int SORT_WAVES = SORT_BLOCK / 64;
+constexpr int SORT_ITEMS = 16;
+constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
+constexpr int SORT_MAX_PASSES = 16;
+constexpr int HIST_BLOCK = 256;
+constexpr int HIST_ITEMS = 32;
+
+inline int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+struct SortPlan {
+    int passes = 0;
+    int radix = 0;
+    int64_t tiles = 0;
+    // byte offsets inside scratch
+    int64_t off_counters = 0, off_hist = 0, off_offsets = 0, off_status = 0;
+    int64_t off_keys_a = 0, off_vals_a = 0, off_keys_b = 0, off_vals_b = 0;
+    int64_t total = 0;
+};
+
+static bool make_plan(int64_t n, int key_bytes, int key_bits, int digit_bits, SortPlan& p) {
+    if (!(key_bytes == 4 || key_bytes == 8)) return false;
+    if (key_bits < 1 || key_bits > key_bytes * 8) return false;
+    if (!(digit_bits == 4 || digit_bits == 8 || digit_bits == 11)) return false;
+    if (n < 0) return false;
+    p.passes = (key_bits + digit_bits - 1) / digit_bits;
+    if (p.passes > SORT_MAX_PASSES) return false;
+    p.radix = 1 << digit_bits;
+    p.tiles = (n + SORT_TILE - 1) / SORT_TILE;
+    int64_t o = 0;
+    p.off_counters = o;
+    o += align256(sizeof(uint32_t) * SORT_MAX_PASSES);
+    p.off_hist = o;
+    o += align256(sizeof(uint32_t) * (int64_t)p.passes * p.radix);
+    p.off_offsets = o;
+    o += align256(sizeof(uint32_t) * (int64_t)p.passes * p.radix);
+    p.off_status = o;
+    o += align256(sizeof(uint32_t) * (int64_t)p.passes * p.tiles * p.radix);
+    p.off_keys_a = o;
+    o += align256((int64_t)key_bytes * n);
+    p.off_vals_a = o;
+    o += align256(sizeof(uint32_t) * n);
+    p.off_keys_b = o;
+    o += align256((int64_t)key_bytes * n);
+    p.off_vals_b = o;
+    o += align256(sizeof(uint32_t) * n);
+    p.total = o;
+    return true;
+}
+
+template <typename K>
+__device__ __forceinline__ uint32_t digit_of(K key, int shift, uint32_t mask) {
+    return (uint32_t)(key >> shift) & mask;
+}
+
+// ------------------------------------------------------------- histograms
+template <typename K>
+__global__ __launch_bounds__(HIST_BLOCK) void sort_hist_kernel(const K* __restrict__ keys,
+                                                               int64_t n, int passes,
+                                                               int digit_bits, int key_bits,
+                                                               uint32_t* __restrict__ hist) {
+    extern __shared__ uint32_t s_hist[];  // passes * radix
+    const int radix = 1 << digit_bits;
+    const int total = passes * radix;
+    for (int i = threadIdx.x; i < total; i += HIST_BLOCK) s_hist[i] = 0;
+    __syncthreads();
+    const int64_t chunk = (int64_t)HIST_BLOCK * HIST_ITEMS;
+    for (int64_t base = (int64_t)blockIdx.x * chunk; base < n; base += (int64_t)gridDim.x * chunk) {
+        K kk[HIST_ITEMS];
+        bool ok[HIST_ITEMS];
+#pragma unroll
+        for (int it = 0; it < HIST_ITEMS; ++it) {
+            int64_t idx = base + (int64_t)it * HIST_BLOCK + threadIdx.x;
+            ok[it] = idx < n;
+            kk[it] = ok[it] ? keys[idx] : K(0);
+        }
+        for (int p = 0; p < passes; ++p) {
+            const int shift = p * digit_bits;
+            const int width = min(digit_bits, key_bits - shift);
+            const uint32_t mask = (1u << width) - 1u;
+#pragma unroll
+            for (int it = 0; it < HIST_ITEMS; ++it)
+                if (ok[it]) atomicAdd(&s_hist[p * radix + digit_of(kk[it], shift, mask)], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < total; i += HIST_BLOCK) {
+        uint32_t c = s_hist[i];
+        if (c) atomicAdd(&hist[i], c);
+    }
+}
+
+// Exclusive scan of each pass's histogram (one workgroup per pass).
+__global__ __launch_bounds__(256) void sort_scan_kernel(const uint32_t* __restrict__ hist,
+                                                        uint32_t* __restrict__ offsets,
+                                                        int radix) {
+    __shared__ uint32_t s_part[256];
+    const int p = blockIdx.x;
+    const int per = (radix + 255) / 256;  // digits per thread, contiguous
+    const int d0 = threadIdx.x * per;
+    uint32_t local = 0;
+    for (int k = 0; k < per; ++k)
+        if (d0 + k < radix) local += hist[p * radix + d0 + k];
+    s_part[threadIdx.x] = local;
+    __syncthreads();
+    // Hillis-Steele over 256 partial sums.
+    for (int off = 1; off < 256; off <<= 1) {
+        uint32_t v = threadIdx.x >= off ? s_part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s_part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = s_part[threadIdx.x] - local;  // exclusive
+    for (int k = 0; k < per; ++k)
+        if (d0 + k < radix) {
+            offsets[p * radix + d0 + k] = run;
+            run += hist[p * radix + d0 + k];
+        }
+}
+
+// ---------------------------------------------------------------- binning
+// Block-wide exclusive scan of one value per thread; returns the exclusive
+// prefix and writes the block total to *total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
+    const unsigned lane = lane_id();
+    const unsigned wave = threadIdx.x >> 6;
+    // inclusive wave scan
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= (unsigned)off) x += y;
+    }
+    if (lane == 63) s_tmp[wave] = x;
+    __syncthreads();
+    uint32_t wbase = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < SORT_WAVES; ++w) {
+        uint32_t t = s_tmp[w];
+        if (w < (int)wave) wbase += t;
+        all += t;
+    }
+    __syncthreads();
+    *total = all;
+    return wbase + x - v;
+}
+
+template <typename K, int DBITS>
+__global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
+    const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    K* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n, int shift, int width,
+    uint32_t* __restrict__ tile_counter, const uint32_t* __restrict__ digit_offsets,
+    uint32_t* __restrict__ status) {
+    constexpr int RADIX = 1 << DBITS;
+    constexpr int DPT = RADIX >= SORT_BLOCK ? RADIX / SORT_BLOCK : 1;  // digits per thread
+    __shared__ uint32_t s_wave[SORT_WAVES][RADIX];  // per-wave counts -> wave prefixes
+    __shared__ uint32_t s_lstart[RADIX];            // tile-local start of each digit
+    __shared__ uint32_t s_goff[RADIX];              // global base - local start
+    __shared__ K s_keys[SORT_TILE];
+    __shared__ uint32_t s_vals[SORT_TILE];
+    __shared__ uint32_t s_tmp[SORT_WAVES];
+    __shared__ uint32_t s_tile;
+
+    const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const uint32_t mask = (1u << width) - 1u;
+
+    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+    for (int i = tid; i < SORT_WAVES * RADIX; i += SORT_BLOCK) (&s_wave[0][0])[i] = 0;
+    __syncthreads();
+    const int64_t tile = s_tile;
+    const int64_t base = tile * SORT_TILE;
+
+    // Wave-striped load: element index = base + wave*64*ITEMS + it*64 + lane.
+    K key[SORT_ITEMS];
+    uint32_t val[SORT_ITEMS];
+    uint32_t off[SORT_ITEMS];
+    const int64_t wbase = base + (int64_t)wave * 64 * SORT_ITEMS;
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        int64_t idx = wbase + it * 64 + lane;
+        bool ok = idx < n;
+        key[it] = ok ? keys_in[idx] : K(0);
+        val[it] = ok ? vals_in[idx] : 0u;
+    }
+
+    // Stable within-wave ranking by ballot matching, item by item.
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        int64_t idx = wbase + it * 64 + lane;
+        bool ok = idx < n;
+        uint32_t d = digit_of(key[it], shift, mask);
+        uint64_t peers = ballot64(ok);
+#pragma unroll
+        for (int b = 0; b < DBITS; ++b) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bal = ballot64(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        uint32_t rank = popc_below(peers);
+        uint32_t cnt = (uint32_t)__popcll(peers);
+        uint32_t pre = 0;
+        if (ok) pre = s_wave[wave][d];
+        off[it] = pre + rank;
+        if (ok && rank == 0) s_wave[wave][d] = pre + cnt;
+    }
+    __syncthreads();
+
+    // Per digit: exclusive prefix over waves and tile total.
+    uint32_t cnt[DPT];
+    uint32_t my_total = 0;
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        int d = tid * DPT + k;
+        uint32_t sum = 0;
+        if (d < RADIX) {
+#pragma unroll
+            for (int w = 0; w < SORT_WAVES; ++w) {
+                uint32_t c = s_wave[w][d];
+                s_wave[w][d] = sum;
+                sum += c;
+            }
+            // Publish the tile's aggregate (tile 0 publishes its inclusive value).
+            lb_store(&status[tile * RADIX + d], (tile == 0 ? LB_FLAG_INC : LB_FLAG_AGG) | sum);
+        }
+        cnt[k] = sum;
+        my_total += sum;
+    }
+    // Tile-local digit starts (block scan over digits in thread order).
+    uint32_t tile_total;
+    uint32_t lstart = block_excl_scan(my_total, s_tmp, &tile_total);
+    (void)tile_total;
+
+    // Decoupled look-back: exclusive prefix of each digit over earlier tiles.
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        int d = tid * DPT + k;
+        if (d < RADIX) {
+            uint32_t excl = 0;
+            if (tile > 0) {
+                int64_t j = tile - 1;
+                while (true) {
+                    uint32_t v;
+                    do {
+                        v = lb_load(&status[j * RADIX + d]);
+                    } while ((v & ~LB_MASK) == 0u);
+                    excl += v & LB_MASK;
+                    if (v & LB_FLAG_INC) break;
+                    --j;
+                }
+                lb_store(&status[tile * RADIX + d], LB_FLAG_INC | (excl + cnt[k]));
+            }
+            s_lstart[d] = lstart;
+            s_goff[d] = digit_offsets[d] + excl - lstart;
+            lstart += cnt[k];
+        }
+    }
+    __syncthreads();
+
+    // Stage the tile in LDS in digit order.
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        int64_t idx = wbase + it * 64 + lane;
+        if (idx < n) {
+            uint32_t d = digit_of(key[it], shift, mask);
+            uint32_t pos = s_lstart[d] + s_wave[wave][d] + off[it];
+            s_keys[pos] = key[it];
+            s_vals[pos] = val[it];
+        }
+    }
+    __syncthreads();
+
+    // Contiguous runs per digit go out to their global positions.
+    const int count = (int)min((int64_t)SORT_TILE, n - base);
+    for (int j = tid; j < count; j += SORT_BLOCK) {
+        K k = s_keys[j];
+        uint32_t d = digit_of(k, shift, mask);
+        uint32_t g = s_goff[d] + (uint32_t)j;
+        keys_out[g] = k;
+        vals_out[g] = s_vals[j];
+    }
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void copy_pairs_kernel(const K* __restrict__ ks,
+                                                         const uint32_t* __restrict__ vs,
+                                                         K* __restrict__ kd,
+                                                         uint32_t* __restrict__ vd, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        kd[i] = ks[i];
+        vd[i] = vs[i];
+    }
+}
+
+template <typename K, int DBITS>
+static void launch_bin(const void* ki, const uint32_t* vi, void* ko, uint32_t* vo, int64_t n,
+                       int shift, int width, uint32_t* ctr, const uint32_t* offs,
+                       uint32_t* status, unsigned tiles, hipStream_t s) {
+    sort_bin_kernel<K, DBITS><<<tiles, SORT_BLOCK, 0, s>>>(
+        static_cast<const K*>(ki), vi, static_cast<K*>(ko), vo, n, shift, width, ctr, offs,
+        status);
+}
+
+template <typename K>
+static int sort_impl(void* keys, uint32_t* vals, int64_t n, int key_bits, int digit_bits,
+                     char* scratch, const SortPlan& p, hipStream_t s) {
+    uint32_t* counters = reinterpret_cast<uint32_t*>(scratch + p.off_counters);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(scratch + p.off_hist);
+    uint32_t* offsets = reinterpret_cast<uint32_t*>(scratch + p.off_offsets);
+    uint32_t* status = reinterpret_cast<uint32_t*>(scratch + p.off_status);
+    void* ka = scratch + p.off_keys_a;
+    uint32_t* va = reinterpret_cast<uint32_t*>(scratch + p.off_vals_a);
+    void* kb = scratch + p.off_keys_b;
+    uint32_t* vb = reinterpret_cast<uint32_t*>(scratch + p.off_vals_b);
+
+    // Counters, histograms and look-back words start at zero.
+    const int64_t zero_words = (p.off_keys_a - p.off_counters) / 4;
+    WF_TRY(fill_u32(counters, zero_words, 0u, s));
+
+    {
+        const int64_t chunk = (int64_t)HIST_BLOCK * HIST_ITEMS;
+        int64_t blocks = (n + chunk - 1) / chunk;
+        if (blocks > 2048) blocks = 2048;
+        size_t lds = sizeof(uint32_t) * (size_t)p.passes * p.radix;
+        sort_hist_kernel<K><<<(unsigned)blocks, HIST_BLOCK, lds, s>>>(
+            static_cast<const K*>(keys), n, p.passes, digit_bits, key_bits, hist);
+        WF_TRY(launch_check("sort_hist_kernel"));
+        sort_scan_kernel<<<p.passes, 256, 0, s>>>(hist, offsets, p.radix);
+        WF_TRY(launch_check("sort_scan_kernel"));
+    }
+
+    // Buffer sequence: starts and ends in the caller's buffers, never reads
+    // and writes the same buffer in one pass.
+    struct Buf {
+        void* k;
+        uint32_t* v;
+    };
+    Buf C{keys, vals}, A{ka, va}, B{kb, vb};
+    Buf seq[SORT_MAX_PASSES + 2];
+    int len = 0;
+    seq[len++] = C;
+    if (p.passes == 1) {
+        seq[len++] = A;
+    } else if (p.passes % 2 == 0) {
+        for (int i = 0; i < p.passes; ++i) seq[len++] = (i % 2 == 0) ? A : C;
+    } else {
+        seq[len++] = A;
+        seq[len++] = B;
+        seq[len++] = C;
+        for (int i = 3; i < p.passes; ++i) seq[len++] = (i % 2 == 1) ? A : C;
+    }
+
+    for (int pass = 0; pass < p.passes; ++pass) {
+        const int shift = pass * digit_bits;
+        const int width = std::min(digit_bits, key_bits - shift);
+        uint32_t* ctr = counters + pass;
+        const uint32_t* offs = offsets + (int64_t)pass * p.radix;
+        uint32_t* st = status + (int64_t)pass * p.tiles * p.radix;
+        const Buf& in = seq[pass];
+        const Buf& out = seq[pass + 1];
+        unsigned tiles = (unsigned)p.tiles;
+        switch (digit_bits) {
+            case 4: launch_bin<K, 4>(in.k, in.v, out.k, out.v, n, shift, width, ctr, offs, st, tiles, s); break;
+            case 8: launch_bin<K, 8>(in.k, in.v, out.k, out.v, n, shift, width, ctr, offs, st, tiles, s); break;
+            default: launch_bin<K, 11>(in.k, in.v, out.k, out.v, n, shift, width, ctr, offs, st, tiles, s); break;
+        }
+        WF_TRY(launch_check("sort_bin_kernel"));
+    }
+    if (p.passes == 1) {
+        copy_pairs_kernel<K><<<grid_for(n, 256), 256, 0, s>>>(
+            static_cast<const K*>(ka), va, static_cast<K*>(keys), vals, n);
+        WF_TRY(launch_check("copy_pairs_kernel"));
+    }
+    return WF_OK;
+}
+
+}  // namespace wfpt
+
+using namespace wfpt;
+
+extern "C" int64_t wf_sort_scratch_bytes(int64_t n, int key_bytes, int key_bits, int digit_bits) {
+    SortPlan p;
+    if (!make_plan(n, key_bytes, key_bits, digit_bits, p)) {
+        set_error("wf_sort_scratch_bytes: invalid key_bytes / key_bits / digit_bits / n");
+        return WF_ERR_INVALID_ARG;
+    }
+    return p.total;
+}
+
+extern "C" int wf_sort_pairs(void* keys, uint32_t* vals, int64_t n, int key_bytes, int key_bits,
+                             int digit_bits, void* scratch, int64_t scratch_bytes,
+                             wf_stream_t stream, float* ms) {
+    SortPlan p;
+    WF_CHECK_ARG(make_plan(n, key_bytes, key_bits, digit_bits, p),
+                 "wf_sort_pairs: key_bytes must be 4 or 8, 1 <= key_bits <= 8*key_bytes, "
+                 "digit_bits in {4, 8, 11}, n >= 0");
+    if (n > WF_MAX_ELEMENTS) return fail(WF_ERR_TOO_LARGE, "wf_sort_pairs: n exceeds 2^30-1");
+    if (n <= 1) {
+        if (ms) *ms = 0.f;
+        return WF_OK;
+    }
+    WF_CHECK_ARG(keys && vals && scratch, "wf_sort_pairs: null buffer");
+    if (scratch_bytes < p.total)
+        return fail(WF_ERR_SCRATCH_TOO_SMALL, "wf_sort_pairs: scratch smaller than wf_sort_scratch_bytes");
+    hipStream_t s = as_stream(stream);
+    StageTimer t(s, ms);
+    int st = key_bytes == 4
+                 ? sort_impl<uint32_t>(keys, vals, n, key_bits, digit_bits, (char*)scratch, p, s)
+                 : sort_impl<uint64_t>(keys, vals, n, key_bits, digit_bits, (char*)scratch, p, s);
+    if (st != WF_OK) return st;
+    return t.finish();
+}
