@@ -1,0 +1,320 @@
+"""Benchmark: Mrays/s (trace+sort+reorder) of the wavefront bounce loop.
+
+Workload (one GPU): the medium interior scene (263,250 triangles), 1920x1080,
+1 spp, 8 bounces, Russian roulette after bounce 3, 60-bit keys (30 origin
+bits + 30 octahedral direction bits) sorted as u64 keys with a u32 payload,
+8-bit digits.  One untimed frame of the C++ frame loop captures every
+bounce's path-ray wavefront (rays + slot-to-pixel LUT) in HBM.  A timed step
+is one pass of the hot path over all captured bounces of that frame:
+    keys -> sort -> reorder -> closest-hit trace          (Tracer::traceSort)
+value = rays through the step / wall time of the timed steps.
+
+Also reported: per-stage GPU times (hipEvents on the stream the kernels run
+on), the roofline of every stage and of the dominant one, the unsorted-trace
+rate of the same wavefronts (the reference's sort ablation, Renderer.cpp:
+280-283), the reference-style full-frame figures of the C++ frame loop, and a
+CPU baseline (the oracle's port of the same stages on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def d2d(dst: int, src: int, nbytes: int) -> None:
+    """Device-to-device copy out of renderer-owned buffers (hipMemcpy)."""
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    rc = hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), 3)
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy failed: {rc}")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--env", default="config1.json")
+    ap.add_argument("--digit-bits", type=int, default=8)
+    ap.add_argument("--frames", type=int, default=3, help="reference-style full frames timed")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one "
+              "GPU); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from vulkanwavefrontpathtracer_amd import stages
+    from vulkanwavefrontpathtracer_amd.renderer import Renderer, env_path
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    env = env_path(args.env)
+    cfg = json.loads(env.read_text())
+    log(f"building scene for {args.env}")
+    r = Renderer(env, {"Renderer.digitBits": args.digit_bits})
+    r.build_scene()
+    scene_h = r.scene_handle()
+    r.set_capture(True)
+    r.render()                       # untimed: captures the bounce wavefronts
+    frame_stats = r.stats()
+    mode = {"aila32": 0, "origin30": 1, "origin30_oct30": 2}[cfg["Renderer"]["keyMode"]]
+    kbytes = 4 if mode < 2 else 8
+    kbits = stages.key_bits(mode)
+    smin, smax = (C.c_float * 3)(), (C.c_float * 3)()
+    stages.call("wf_scene_bounds", scene_h, smin, smax)
+
+    # Resident per-bounce buffers (captured rays are copied so the renderer can be reused).
+    bounces = []
+    b = 0
+    while True:
+        rp, sp, n = r.captured(b)
+        if n == 0:
+            break
+        rays = torch.empty((n, 8), dtype=torch.float32, device=dev)
+        s2i = torch.empty(n, dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        d2d(rays.data_ptr(), rp, 32 * n)
+        d2d(s2i.data_ptr(), sp, 4 * n)
+        bounces.append(dict(
+            n=n, rays=rays, s2i=s2i,
+            keys=torch.empty(n, dtype=torch.int64 if kbytes == 8 else torch.int32, device=dev),
+            idx=torch.empty(n, dtype=torch.int32, device=dev),
+            out_rays=torch.empty_like(rays), out_s2i=torch.empty_like(s2i),
+            out_i2s=torch.empty(r.size[0] * r.size[1], dtype=torch.int32, device=dev),
+            hits=torch.empty((n, 4), dtype=torch.int32, device=dev),
+            hits_u=torch.empty((n, 4), dtype=torch.int32, device=dev),
+            sorter=stages.Sorter()))
+        b += 1
+    total_rays = sum(x["n"] for x in bounces)
+    log(f"captured {len(bounces)} bounces, {total_rays} path rays: {[x['n'] for x in bounces]}")
+
+    class SceneRef:
+        handle = scene_h
+
+    sc = SceneRef()
+    stream = torch.cuda.current_stream()
+    s_ptr = C.c_void_p(stream.cuda_stream)
+    lib = stages._capi.load()
+    mn3, mx3 = (C.c_float * 3)(*smin), (C.c_float * 3)(*smax)
+    # pre-size the sort scratch
+    for x in bounces:
+        need = lib.wf_sort_scratch_bytes(x["n"], kbytes, kbits, args.digit_bits)
+        x["sorter"].scratch = torch.empty(need, dtype=torch.uint8, device=dev)
+
+    names = ["keys", "sort", "reorder", "trace"]
+    ev = {k: [] for k in names}
+
+    def step(record=False):
+        for x in bounces:
+            n = x["n"]
+            marks = []
+            if record:
+                marks.append(torch.cuda.Event(enable_timing=True))
+                marks[-1].record(stream)
+            stages._capi.check("wf_morton_keys", lib.wf_morton_keys(
+                C.c_void_p(x["rays"].data_ptr()), C.c_void_p(x["keys"].data_ptr()),
+                C.c_void_p(x["idx"].data_ptr()), n, mode, mn3, mx3, s_ptr, None))
+            if record:
+                marks.append(torch.cuda.Event(enable_timing=True))
+                marks[-1].record(stream)
+            stages._capi.check("wf_sort_pairs", lib.wf_sort_pairs(
+                C.c_void_p(x["keys"].data_ptr()), C.c_void_p(x["idx"].data_ptr()), n, kbytes, kbits,
+                args.digit_bits, C.c_void_p(x["sorter"].scratch.data_ptr()),
+                x["sorter"].scratch.numel(), s_ptr, None))
+            if record:
+                marks.append(torch.cuda.Event(enable_timing=True))
+                marks[-1].record(stream)
+            stages._capi.check("wf_reorder", lib.wf_reorder(
+                C.c_void_p(x["idx"].data_ptr()), C.c_void_p(x["rays"].data_ptr()),
+                C.c_void_p(x["out_rays"].data_ptr()), C.c_void_p(x["s2i"].data_ptr()),
+                C.c_void_p(x["out_s2i"].data_ptr()), C.c_void_p(x["out_i2s"].data_ptr()), n, s_ptr,
+                None))
+            if record:
+                marks.append(torch.cuda.Event(enable_timing=True))
+                marks[-1].record(stream)
+            stages._capi.check("wf_trace", lib.wf_trace(
+                sc.handle, C.c_void_p(x["out_rays"].data_ptr()), None,
+                C.c_void_p(x["hits"].data_ptr()), n, 1, s_ptr, None))
+            if record:
+                marks.append(torch.cuda.Event(enable_timing=True))
+                marks[-1].record(stream)
+                for k, name in enumerate(names):
+                    ev[name].append((marks[k], marks[k + 1], n))
+
+    log(f"warmup {args.warmup} steps")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    value = total_rays * args.steps / wall / 1e6
+    log(f"{value:.1f} Mrays/s ({ms_per_step:.3f} ms/step)")
+
+    # Per-stage times on the kernels' stream (separate, event-bracketed steps).
+    for _ in range(3):
+        step(record=True)
+    torch.cuda.synchronize()
+    stage_ms = {k: sum(a.elapsed_time(b_) for a, b_, _ in ev[k]) / 3 for k in names}
+    # Algorithmic bytes per element: keys 32+K+4, sort P*2*(K+4)+K,
+    # reorder 80, trace 32 read + 16 write.
+    passes = -(-kbits // args.digit_bits)
+    bpe = {"keys": 32 + kbytes + 4, "sort": passes * 2 * (kbytes + 4) + kbytes, "reorder": 80,
+           "trace": 48}
+    stages_roof = {}
+    for k in names:
+        t = stage_ms[k] / 1e3
+        gbs = total_rays * bpe[k] / t / 1e9 if t > 0 else 0.0
+        stages_roof[k] = {"ms_per_frame": round(stage_ms[k], 4), "bytes_per_ray": bpe[k],
+                          "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    dominant = max(names, key=lambda k: stage_ms[k])
+    dom = stages_roof[dominant]
+    # per-launch average duration of the dominant kernel at the largest bounce
+    big = max(range(len(bounces)), key=lambda i: bounces[i]["n"])
+    recs = [e for e in ev[dominant]]
+    launch_ms = [a.elapsed_time(b_) for (a, b_, n) in recs if n == bounces[big]["n"]]
+    avg_launch = sum(launch_ms) / len(launch_ms)
+    launch_bytes = bounces[big]["n"] * bpe[dominant]
+    achieved = launch_bytes / (avg_launch / 1e3) / 1e9
+
+    # Unsorted trace of the same wavefronts (sort ablation).
+    ts = []
+    for _ in range(3):
+        for x in bounces:
+            a = torch.cuda.Event(enable_timing=True)
+            b_ = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            stages._capi.check("wf_trace", lib.wf_trace(sc.handle, C.c_void_p(x["rays"].data_ptr()), None,
+                                                  C.c_void_p(x["hits_u"].data_ptr()), x["n"], 1,
+                                                  s_ptr, None))
+            b_.record(stream)
+            ts.append((a, b_))
+    torch.cuda.synchronize()
+    unsorted_ms = sum(a.elapsed_time(b_) for a, b_ in ts) / 3
+    unsorted_mrays = total_rays / (unsorted_ms / 1e3) / 1e6
+    sorted_ms = sum(stage_ms.values())
+    sorted_mrays = total_rays / (sorted_ms / 1e3) / 1e6
+    trace_only_sorted = total_rays / (stage_ms["trace"] / 1e3) / 1e6
+    # parity of the sorted trace with the unsorted one (same hits per ray)
+    same = all(torch.equal(x["hits"][torch.argsort(x["out_s2i"])], x["hits_u"][torch.argsort(x["s2i"])])
+               for x in bounces)
+
+    # Reference-style full frames (Benchmark::run): path Mrays/s includes shade.
+    frame_walls, fstats = [], []
+    for _ in range(args.frames):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        r.render()
+        frame_walls.append(time.perf_counter() - t1)
+        fstats.append(r.stats())
+    fs = fstats[-1]
+    path_rays = fs["path_rays"]
+    ref_style = {
+        "path_rays": path_rays,
+        "path_mrays_s": round(path_rays / (fs["path_ms"] / 1e3) / 1e6, 1) if fs["path_ms"] > 0 else None,
+        "frame_ms_wall": round(1e3 * sum(frame_walls) / len(frame_walls), 2),
+        "rays_per_frame": fs["primary_rays"] + fs["shadow_rays"] + path_rays,
+        "stage_ms": {k: round(v, 3) for k, v in fs["stage_ms"].items()},
+    }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(bounces[0], scene_h, smin, smax, mode, kbits, args.cpu_sample, r)
+
+    out = {
+        "metric": "Mrays/s (trace+sort+reorder), plain vs batched & adjoint, at RMSE≤1e-3 vs FP64",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural interior scene, captured bounce wavefronts)",
+        "config": {
+            "workload": "medium plain: interior 263,250 tris, 1920x1080, 1 spp, 8 bounces, RR after "
+                        "bounce 3, 60-bit keys (30 origin + 30 octahedral), u64 key sort + u32 payload",
+            "rays_per_step": total_rays, "bounces": len(bounces),
+            "rays_per_bounce": [x["n"] for x in bounces], "digit_bits": args.digit_bits,
+            "parallelism": "1 GPU",
+        },
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "launch_ms": round(avg_launch, 4),
+                     "launch_elems": bounces[big]["n"], "bytes_per_elem": bpe[dominant]},
+        "stages": stages_roof,
+        "sorted_pipeline_mrays_s": round(sorted_mrays, 1),
+        "trace_only_sorted_mrays_s": round(trace_only_sorted, 1),
+        "unsorted_trace_mrays_s": round(unsorted_mrays, 1),
+        "sort_speedup_trace": round(unsorted_ms / stage_ms["trace"], 3),
+        "sorted_vs_unsorted_hits_equal": bool(same),
+        "reference_style_frame": ref_style,
+        "untimed_capture_frame": {"path_rays": frame_stats["path_rays"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_baseline(b0, scene_h, smin, smax, mode, kbits, sample, renderer):
+    """The oracle's port of keys -> sort -> reorder -> trace on the host cores,
+    over the first `sample` rays of the largest bounce (bounded CPU work)."""
+    import numpy as np
+
+    from oracle import oracle
+    from vulkanwavefrontpathtracer_amd import stages
+
+    n = min(sample, b0["n"])
+    rays = b0["rays"][:n].cpu().numpy()
+    s2i = b0["s2i"][:n].cpu().numpy()
+    env = json.loads((ROOT / "vulkanwavefrontpathtracer_amd" / "env" / "config1.json").read_text())
+    hs = stages.HostScene(1, env["Scene"]["seed"])
+    tv = hs.tri_verts()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log(f"cpu baseline: {n} rays, {threads} threads")
+    t0 = time.perf_counter()
+    keys = oracle.keys_oct60(rays, list(smin), list(smax)) if mode == 2 else \
+        oracle.keys_aila32(rays, list(smin), list(smax))
+    t1 = time.perf_counter()
+    k, v = oracle.sort_pairs(keys, np.arange(n, dtype=np.uint32), kbits)
+    t2 = time.perf_counter()
+    out_r, out_s, out_i = oracle.reorder(v, rays, s2i, int(s2i.max()) + 1)
+    t3 = time.perf_counter()
+    t_, tri = oracle.trace_bvh_f64(out_r, tv, closest=True)
+    t4 = time.perf_counter()
+    total = t4 - t0
+    return {"value": round(n / total / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} path rays of bounce 0 of the same frame; oracle keys/sort/reorder "
+                      f"single-threaded ({t1 - t0:.2f}+{t2 - t1:.2f}+{t3 - t2:.2f} s), fp64 BVH "
+                      f"trace on {threads} OpenMP threads ({t4 - t3:.2f} s, BVH build included)"}
+
+
+if __name__ == "__main__":
+    sys.exit(main())

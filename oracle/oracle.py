@@ -88,3 +88,196 @@ def reorder(idx, in_rays, in_s2i, i2s_size):
     out_i2s = np.full(i2s_size, -1, np.int32)
     load().or_reorder(_p(idx), _p(r), _p(out_r), _p(s2i), _p(out_s2i), _p(out_i2s), C.c_int64(n))
     return out_r, out_s2i, out_i2s
+
+
+# ------------------------------------------------------------------- trace
+def trace_f32(rays, tri_verts, closest=True):
+    """Brute-force fp32 watertight trace; int32 [n, 4] = {tri, t bits, bary, 0}."""
+    r = _rays(rays)
+    tv = np.ascontiguousarray(tri_verts, np.float32).reshape(-1, 9)
+    out = np.zeros((len(r), 4), np.int32)
+    load().or_trace_f32(_p(r), C.c_int64(len(r)), _p(tv), C.c_int64(len(tv)), C.c_int(int(closest)),
+                        _p(out))
+    return out
+
+
+def trace_f64(rays, tri_verts):
+    """Brute-force fp64 closest hit: (t, tri, edge distance)."""
+    r = _rays(rays)
+    tv = np.ascontiguousarray(tri_verts, np.float32).reshape(-1, 9)
+    t = np.zeros(len(r), np.float64)
+    tri = np.zeros(len(r), np.int32)
+    edge = np.zeros(len(r), np.float64)
+    load().or_trace_f64(_p(r), C.c_int64(len(r)), _p(tv), C.c_int64(len(tv)), _p(t), _p(tri), _p(edge))
+    return t, tri, edge
+
+
+def trace_bvh_f64(rays, tri_verts, closest=True):
+    r = _rays(rays)
+    tv = np.ascontiguousarray(tri_verts, np.float32).reshape(-1, 9)
+    t = np.zeros(len(r), np.float64)
+    tri = np.zeros(len(r), np.int32)
+    load().or_trace_bvh_f64(_p(r), C.c_int64(len(r)), _p(tv), C.c_int64(len(tv)),
+                            C.c_int(int(closest)), _p(t), _p(tri))
+    return t, tri
+
+
+def edge_distance_f64(ray, tri9):
+    f = load().or_edge_distance_f64
+    f.restype = C.c_double
+    r = np.ascontiguousarray(ray, np.float32)
+    t = np.ascontiguousarray(tri9, np.float32)
+    return f(_p(r), _p(t))
+
+
+def hits_to_prim_geom(hits_tri, geom_first):
+    """Global triangle ids -> (prim in geometry, geometry) as trace.comp reports them."""
+    tri = np.asarray(hits_tri)
+    g = np.searchsorted(np.asarray(geom_first), tri, side="right") - 1
+    prim = tri - np.asarray(geom_first)[np.maximum(g, 0)]
+    g = np.where(tri < 0, -1, g)
+    prim = np.where(tri < 0, -1, prim)
+    return prim.astype(np.int32), g.astype(np.int32)
+
+
+# ---------------------------------------------------------- ray generation
+def tea(a, b, rounds=16):
+    f = load().or_tea
+    f.restype = C.c_uint32
+    return int(f(C.c_uint32(a), C.c_uint32(b), C.c_int(rounds)))
+
+
+def init_seeds(n, frame):
+    out = np.zeros(n, np.uint32)
+    load().or_init_seeds(_p(out), C.c_int64(n), C.c_int(frame))
+    return out
+
+
+def halton(i, base):
+    f = load().or_halton
+    f.restype = C.c_float
+    return float(f(C.c_int(i), C.c_int(base)))
+
+
+def sobol2(i):
+    xy = np.zeros(2, np.float32)
+    load().or_sobol2(C.c_uint32(i), _p(xy))
+    return xy
+
+
+def raygen_primary(s2w_colmajor, origin, i2p, w, h, sample_index, max_dist):
+    m = np.ascontiguousarray(s2w_colmajor, np.float32).reshape(16)
+    o = np.ascontiguousarray(origin, np.float32)
+    lut = np.ascontiguousarray(i2p, np.int32)
+    out = np.zeros((w * h, 8), np.float32)
+    load().or_raygen_primary(_p(m), _p(o), _p(lut), C.c_int(w), C.c_int(h), C.c_int(sample_index),
+                             C.c_float(max_dist), _p(out))
+    return out
+
+
+def raygen_shadow(seeds, in_rays, in_hits, in_s2i, begin, end, S, light, radius):
+    n = (end - begin) * S
+    out = np.zeros((n, 8), np.float32)
+    s2i = np.zeros(n, np.int32)
+    i2s = np.zeros(n, np.int32)
+    sd = np.ascontiguousarray(seeds, np.uint32)
+    r = _rays(in_rays)
+    h = np.ascontiguousarray(in_hits, np.int32)
+    s = np.ascontiguousarray(in_s2i, np.int32)
+    lt = np.ascontiguousarray(light, np.float32)
+    load().or_raygen_shadow(_p(sd), _p(r), _p(h), _p(s), C.c_int64(begin), C.c_int64(end), C.c_int(S),
+                            _p(lt), C.c_float(radius), _p(out), _p(s2i), _p(i2s))
+    return out, s2i, i2s
+
+
+def tri_normals(vertices, indices):
+    v = np.asarray(vertices, np.float32)
+    return v[:, 3:6][np.asarray(indices).reshape(-1)].reshape(-1, 9).copy()
+
+
+def shade_extend(geom_first, tri_n, tri_mat, albedo, seeds, in_rays, in_hits, in_s2i, decreases, rr):
+    gf = np.ascontiguousarray(geom_first, np.int32)
+    tn = np.ascontiguousarray(tri_n, np.float32)
+    tm = np.ascontiguousarray(tri_mat, np.int32)
+    al = np.ascontiguousarray(albedo, np.float32)
+    sd = np.array(seeds, dtype=np.uint32, copy=True)
+    r = _rays(in_rays)
+    h = np.ascontiguousarray(in_hits, np.int32)
+    s = np.ascontiguousarray(in_s2i, np.int32)
+    dec = np.array(decreases, dtype=np.float32, copy=True)
+    n = len(r)
+    out = np.zeros((max(n, 1), 8), np.float32)
+    s2i = np.zeros(max(n, 1), np.int32)
+    f = load().or_shade_extend
+    f.restype = C.c_int64
+    k = f(_p(gf), _p(tn), _p(tm), _p(al), C.c_int(len(al)), _p(sd), _p(r), _p(h), _p(s), C.c_int64(n),
+          _p(out), _p(s2i), _p(dec), C.c_int(int(rr)))
+    return out[:k], s2i[:k], sd, dec
+
+
+def reconstruct_smooth(geom_first, tri_n, tri_mat, emission, rays, hits, s2i, pixels, decreases,
+                       light, bg, spp):
+    px = np.array(pixels, dtype=np.float32, copy=True)
+    args = [np.ascontiguousarray(a, dt) for a, dt in
+            [(geom_first, np.int32), (tri_n, np.float32), (tri_mat, np.int32), (emission, np.float32)]]
+    r = _rays(rays)
+    h = np.ascontiguousarray(hits, np.int32)
+    s = np.ascontiguousarray(s2i, np.int32)
+    dec = np.ascontiguousarray(decreases, np.float32)
+    lt = np.ascontiguousarray(light, np.float32)
+    b = np.ascontiguousarray(bg, np.float32)
+    load().or_reconstruct_smooth(_p(args[0]), _p(args[1]), _p(args[2]), _p(args[3]),
+                                 C.c_int(len(args[3]) // 3 if args[3].ndim == 1 else len(args[3])),
+                                 _p(r), _p(h), _p(s), _p(px), _p(dec), C.c_int64(len(r)), _p(lt), _p(b),
+                                 C.c_int(spp))
+    return px
+
+
+def reconstruct_shadow(shadow_hits, in_s2i, shadow_i2s, in_px, out_px, begin, end, S, replace):
+    o = np.array(out_px, dtype=np.float32, copy=True)
+    sh = np.ascontiguousarray(shadow_hits, np.int32)
+    s = np.ascontiguousarray(in_s2i, np.int32)
+    i2s = np.ascontiguousarray(shadow_i2s, np.int32)
+    ip = np.ascontiguousarray(in_px, np.float32)
+    load().or_reconstruct_shadow(_p(sh), _p(s), _p(i2s), _p(ip), _p(o), C.c_int64(begin),
+                                 C.c_int64(end), C.c_int(S), C.c_int(int(replace)))
+    return o
+
+
+def tonemap(rgb, key, white):
+    i = np.ascontiguousarray(rgb, np.float32)
+    o = np.zeros(3, np.float32)
+    load().or_tonemap(_p(i), C.c_float(key), C.c_float(white), _p(o))
+    return o
+
+
+def interpolate(frame, pixels, frame_index, key, white):
+    f = np.array(frame, dtype=np.float32, copy=True)
+    p = np.array(pixels, dtype=np.float32, copy=True)
+    load().or_interpolate(_p(f), _p(p), C.c_int64(len(f)), C.c_int(frame_index), C.c_float(key),
+                          C.c_float(white))
+    return f, p
+
+
+class FrameCfg(C.Structure):
+    _fields_ = [("w", C.c_int), ("h", C.c_int), ("spp", C.c_int), ("depth", C.c_int), ("S", C.c_int),
+                ("frame", C.c_int), ("rr", C.c_int), ("rr_start", C.c_int),
+                ("light", C.c_float * 3), ("bg", C.c_float * 3), ("origin", C.c_float * 3),
+                ("radius", C.c_float), ("max_dist", C.c_float), ("s2w", C.c_float * 16)]
+
+
+def render_frame_f64(tri_verts, tri_n, tri_mat, albedo, emission, *, w, h, spp, depth, S, frame,
+                     light, bg, origin, radius, max_dist, s2w_colmajor, rr=False, rr_start=3):
+    """Frame 1 of Renderer::renderPath in fp64; returns [w*h, 4] float32 linear RGBA."""
+    cfg = FrameCfg(w, h, spp, depth, S, frame, int(rr), rr_start, (C.c_float * 3)(*light),
+                   (C.c_float * 3)(*bg), (C.c_float * 3)(*origin), radius, max_dist,
+                   (C.c_float * 16)(*np.asarray(s2w_colmajor, np.float32).reshape(16)))
+    tv = np.ascontiguousarray(tri_verts, np.float32)
+    tn = np.ascontiguousarray(tri_n, np.float32)
+    tm = np.ascontiguousarray(tri_mat, np.int32)
+    al = np.ascontiguousarray(albedo, np.float32)
+    em = np.ascontiguousarray(emission, np.float32)
+    out = np.zeros((w * h, 4), np.float32)
+    load().or_render_frame_f64(_p(tv), _p(tn), _p(tm), C.c_int64(len(tv)), _p(al), _p(em),
+                               C.c_int(len(al)), C.byref(cfg), _p(out))
+    return out

@@ -341,3 +341,573 @@ OR_API double or_edge_distance_f64(const float* ray, const float* tv) {
     double v = (d[0] * q[0] + d[1] * q[1] + d[2] * q[2]) * inv;
     return fabs(fmin(fmin(u, v), 1.0 - u - v));
 }
+
+/* ------------------------------------------------------- ray generation */
+/* initSeeds.comp:7-19 (TEA) */
+static uint32_t tea(uint32_t v0, uint32_t v1, int rounds) {
+    uint32_t s0 = 0u;
+    for (int n = 0; n < rounds; n++) {
+        s0 += 0x9e3779b9u;
+        v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + s0) ^ ((v1 >> 5) + 0xc8013ea4u);
+        v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + s0) ^ ((v0 >> 5) + 0x7e95761eu);
+    }
+    return v0;
+}
+OR_API uint32_t or_tea(uint32_t a, uint32_t b, int rounds) { return tea(a, b, rounds); }
+
+OR_API void or_init_seeds(uint32_t* seeds, int64_t n, int frame) {
+    for (int64_t i = 0; i < n; ++i) seeds[i] = tea((uint32_t)i, (uint32_t)frame, 16);
+}
+
+/* raygenPath.comp:16-26 */
+static float randf(uint32_t* seed) {
+    *seed = 1103515245u * *seed + 12345u;
+    return (float)(*seed & 0x00FFFFFFu) / 16777216.0f;
+}
+
+/* raygenPrimary.comp:11-22 */
+static float halton(int i, int base) {
+    float r = 0.0f, f = 1.0f / (float)base;
+    int j = i;
+    while (j > 0) { r += f * (float)(j % base); j /= base; f /= (float)base; }
+    return r;
+}
+OR_API float or_halton(int i, int base) { return halton(i, base); }
+
+/* raygenPrimary.comp:57-84; s2w column-major */
+OR_API void or_raygen_primary(const float* m, const float* origin, const int32_t* i2p, int w, int h,
+                              int sample_index, float max_dist, float* rays) {
+    float hx = halton(sample_index, 2), hy = halton(sample_index, 3);
+    for (int64_t i = 0; i < (int64_t)w * h; ++i) {
+        int pixel = i2p[i];
+        float x = (float)(pixel % w) + hx, y = (float)(pixel / w) + hy;
+        float nx = 2.0f * x / (float)w - 1.0f, ny = 2.0f * y / (float)h - 1.0f;
+        float wx = ((m[0] * nx + m[4] * ny) + m[8] * 0.0f) + m[12] * 1.0f;
+        float wy = ((m[1] * nx + m[5] * ny) + m[9] * 0.0f) + m[13] * 1.0f;
+        float wz = ((m[2] * nx + m[6] * ny) + m[10] * 0.0f) + m[14] * 1.0f;
+        float ww = ((m[3] * nx + m[7] * ny) + m[11] * 0.0f) + m[15] * 1.0f;
+        float px = wx / ww, py = wy / ww, pz = wz / ww;
+        float dx = px - origin[0], dy = py - origin[1], dz = pz - origin[2];
+        float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+        float* r = rays + 8 * i;
+        r[0] = origin[0]; r[1] = origin[1]; r[2] = origin[2]; r[3] = 0.0f;
+        r[4] = dx * inv; r[5] = dy * inv; r[6] = dz * inv; r[7] = max_dist;
+    }
+}
+
+/* raygenShadow.comp:10-20 */
+static void jenkins_mix(uint32_t* a, uint32_t* b, uint32_t* c) {
+    *a -= *b; *a -= *c; *a ^= (*c >> 13);
+    *b -= *c; *b -= *a; *b ^= (*a << 8);
+    *c -= *a; *c -= *b; *c ^= (*b >> 13);
+    *a -= *b; *a -= *c; *a ^= (*c >> 12);
+    *b -= *c; *b -= *a; *b ^= (*a << 16);
+    *c -= *a; *c -= *b; *c ^= (*b >> 5);
+    *a -= *b; *a -= *c; *a ^= (*c >> 3);
+    *b -= *c; *b -= *a; *b ^= (*a << 10);
+    *c -= *a; *c -= *b; *c ^= (*b >> 15);
+}
+/* raygenShadow.comp:22-40 */
+static void sobol2(uint32_t i, float* x, float* y) {
+    uint32_t r1 = 0, r2 = 0;
+    for (uint32_t v1 = 1u << 31, v2 = 3u << 30; i > 0u; i >>= 1) {
+        if (i & 1u) { r1 ^= v1; r2 ^= v2 << 1; }
+        v1 |= v1 >> 1;
+        v2 ^= v2 >> 1;
+    }
+    *x = (float)r1 / 4294967296.0f;
+    *y = (float)r2 / 4294967296.0f;
+}
+OR_API void or_sobol2(uint32_t i, float* xy) { sobol2(i, xy, xy + 1); }
+OR_API void or_jenkins(uint32_t* abc) { jenkins_mix(abc, abc + 1, abc + 2); }
+
+/* Shadow sample point in [-1,1]^3 for sample i of S with the pixel's
+ * Cranley-Patterson offset (raygenShadow.comp:91-118). */
+static void shadow_point(uint32_t seed, int pixel, int i, int S, float p[3]) {
+    uint32_t ha = seed + (uint32_t)pixel, hb = 0x9e3779b9u, hc = 0x9e3779b9u;
+    jenkins_mix(&ha, &hb, &hc);
+    jenkins_mix(&ha, &hb, &hc);
+    const float two_m32 = 2.3283064365386963e-10f;
+    float off[3] = {(float)ha * two_m32, (float)hb * two_m32, (float)hc * two_m32};
+    sobol2((uint32_t)i, &p[0], &p[1]);
+    p[2] = ((float)i + 0.5f) / (float)S;
+    for (int k = 0; k < 3; ++k) {
+        p[k] += off[k];
+        if (p[k] >= 1.0f) p[k] -= 1.0f;
+        p[k] = 2.0f * p[k] - 1.0f;
+    }
+}
+
+OR_API void or_raygen_shadow(const uint32_t* seeds, const float* in_rays, const int32_t* in_hits,
+                             const int32_t* in_s2i, int64_t begin, int64_t end, int S,
+                             const float* light, float radius, float* out_rays, int32_t* out_s2i,
+                             int32_t* out_i2s) {
+    for (int64_t j = 0; j < end - begin; ++j) {
+        int64_t src = begin + j;
+        int pixel = in_s2i[src];
+        uint32_t seed = seeds[pixel];
+        const float* r = in_rays + 8 * src;
+        int tri = in_hits[4 * src];
+        float t;
+        memcpy(&t, &in_hits[4 * src + 2], 4);
+        float adv = fmaxf(t - 1.0e-5f, 0.0f);
+        float o[3] = {r[0] + r[4] * adv, r[1] + r[5] * adv, r[2] + r[6] * adv};
+        if (tri == -1) o[0] = o[1] = o[2] = 0.0f;
+        for (int i = 0; i < S; ++i) {
+            int64_t out = (int64_t)S * j + i;
+            float p[3];
+            shadow_point(seed, pixel, i, S, p);
+            float tg[3] = {light[0] + radius * p[0], light[1] + radius * p[1], light[2] + radius * p[2]};
+            float d[3] = {tg[0] - o[0], tg[1] - o[1], tg[2] - o[2]};
+            float dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            float len = sqrtf(dd);
+            float inv = 1.0f / sqrtf(dd);
+            float* q = out_rays + 8 * out;
+            q[0] = o[0]; q[1] = o[1]; q[2] = o[2]; q[3] = 1.0e-5f;
+            if (tri == -1) { q[4] = 1.0f; q[5] = 0.0f; q[6] = 0.0f; q[7] = -1.0f; }
+            else { q[4] = d[0] * inv; q[5] = d[1] * inv; q[6] = d[2] * inv; q[7] = len; }
+            out_s2i[out] = (int32_t)out;
+            out_i2s[out] = (int32_t)out;
+        }
+    }
+}
+
+/* geometrytypes.glsl:18-47 with the packed barycentrics of the hit */
+static int shading_normal(const float* tri_n, const int32_t* tri_mat, int64_t tri, uint32_t bary,
+                          float n[3]) {
+    float bu = (float)(bary & 0xFFFFu) / 65535.0f, bv = (float)(bary >> 16) / 65535.0f;
+    float bw = 1.0f - bu - bv;
+    const float* a = tri_n + 9 * tri;
+    float s[3];
+    for (int k = 0; k < 3; ++k) s[k] = a[k] * bw + a[3 + k] * bu + a[6 + k] * bv;
+    float inv = 1.0f / sqrtf(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    for (int k = 0; k < 3; ++k) n[k] = s[k] * inv;
+    return tri_mat[tri];
+}
+
+/* raygenPath.comp:28-49 (fp32; sinf/cosf of the host libm) */
+static void cosine_dir(float r0, float r1, const float n[3], float out[3]) {
+    const float two_pi = 2.0f * 3.14159265358979323846f;
+    float theta = two_pi * r1, radius = sqrtf(r0);
+    float x = radius * sinf(theta), z = radius * cosf(theta), y = sqrtf(1.0f - r0);
+    float s = n[2] >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (s + n[2]), b = n[0] * n[1] * a;
+    float b1[3] = {1.0f + s * n[0] * n[0] * a, s * b, -s * n[0]};
+    float b2[3] = {b, s + n[1] * n[1] * a, -n[1]};
+    for (int k = 0; k < 3; ++k) out[k] = x * b1[k] + z * b2[k] + y * n[k];
+}
+
+/* raygenPath.comp:78-170 with the deterministic (input-order) compaction
+ * and pixel-keyed seeds; returns the survivor count.  Not bit-exact against
+ * the device (sinf/cosf differ in the last ulps): compared with a
+ * tolerance. */
+OR_API int64_t or_shade_extend(const int32_t* geom_first, const float* tri_n,
+                               const int32_t* tri_mat, const float* albedo, int nmat,
+                               uint32_t* seeds, const float* in_rays, const int32_t* in_hits,
+                               const int32_t* in_s2i, int64_t n, float* out_rays,
+                               int32_t* out_s2i, float* decreases, int rr) {
+    int64_t k = 0;
+    for (int64_t j = 0; j < n; ++j) {
+        const int32_t* h = in_hits + 4 * j;
+        if (h[0] < 0) continue;
+        int pixel = in_s2i[j];
+        uint32_t seed = seeds[pixel];
+        const float* r = in_rays + 8 * j;
+        int64_t tri = (int64_t)geom_first[h[1]] + h[0];
+        float nrm[3];
+        int m = shading_normal(tri_n, tri_mat, tri, (uint32_t)h[3], nrm);
+        if (m < 0 || m >= nmat) m = 0;
+        float dn = nrm[0] * r[4] + nrm[1] * r[5] + nrm[2] * r[6];
+        if (!(dn < 0.0f)) { nrm[0] = -nrm[0]; nrm[1] = -nrm[1]; nrm[2] = -nrm[2]; }
+        float r0 = randf(&seed), r1 = randf(&seed);
+        float t;
+        memcpy(&t, &h[2], 4);
+        float p[3] = {r[0] + r[4] * t, r[1] + r[5] * t, r[2] + r[6] * t};
+        float d[3];
+        cosine_dir(r0, r1, nrm, d);
+        float w[3] = {albedo[3 * m], albedo[3 * m + 1], albedo[3 * m + 2]};
+        int alive = 1;
+        float* dec = decreases + 4 * (int64_t)pixel;
+        if (rr) {
+            float q = fmaxf(fmaxf(dec[0] * w[0], dec[1] * w[1]), dec[2] * w[2]);
+            q = fminf(fmaxf(q, 0.05f), 1.0f);
+            float r2 = randf(&seed);
+            if (r2 >= q) alive = 0;
+            else { w[0] /= q; w[1] /= q; w[2] /= q; }
+        }
+        seeds[pixel] = seed;
+        if (!alive) continue;
+        float* o = out_rays + 8 * k;
+        const float eps = 1.0e-4f;
+        o[0] = p[0] + d[0] * eps; o[1] = p[1] + d[1] * eps; o[2] = p[2] + d[2] * eps; o[3] = eps;
+        o[4] = d[0]; o[5] = d[1]; o[6] = d[2]; o[7] = 3.402823466e+38f;
+        out_s2i[k] = pixel;
+        dec[0] *= w[0]; dec[1] *= w[1]; dec[2] *= w[2];
+        ++k;
+    }
+    return k;
+}
+
+/* ------------------------------------------------------- reconstruction */
+/* reconstructSmooth.comp:44-93 (+ emission of the hit material) */
+OR_API void or_reconstruct_smooth(const int32_t* geom_first, const float* tri_n,
+                                  const int32_t* tri_mat, const float* emission, int nmat,
+                                  const float* rays, const int32_t* hits, const int32_t* s2i,
+                                  float* pixels, const float* decreases, int64_t n,
+                                  const float* light, const float* bg, int spp) {
+    float fs = (float)spp;
+    for (int64_t i = 0; i < n; ++i) {
+        int pixel = s2i[i];
+        float* c = pixels + 4 * (int64_t)pixel;
+        const float* dec = decreases + 4 * (int64_t)pixel;
+        const int32_t* h = hits + 4 * i;
+        float col[3] = {c[0], c[1], c[2]};
+        if (h[0] == -1) {
+            for (int k = 0; k < 3; ++k) col[k] += dec[k] * bg[k] / fs;
+        } else {
+            const float* r = rays + 8 * i;
+            float nrm[3];
+            int64_t tri = (int64_t)geom_first[h[1]] + h[0];
+            int m = shading_normal(tri_n, tri_mat, tri, (uint32_t)h[3], nrm);
+            if (m < 0 || m >= nmat) m = 0;
+            float t;
+            memcpy(&t, &h[2], 4);
+            float p[3] = {r[0] + r[4] * t, r[1] + r[5] * t, r[2] + r[6] * t};
+            float L[3] = {light[0] - p[0], light[1] - p[1], light[2] - p[2]};
+            float inv = 1.0f / sqrtf(L[0] * L[0] + L[1] * L[1] + L[2] * L[2]);
+            for (int k = 0; k < 3; ++k) L[k] *= inv;
+            float dn = nrm[0] * r[4] + nrm[1] * r[5] + nrm[2] * r[6];
+            if (!(dn < 0.0f)) { nrm[0] = -nrm[0]; nrm[1] = -nrm[1]; nrm[2] = -nrm[2]; }
+            float alpha = fmaxf(nrm[0] * L[0] + nrm[1] * L[1] + nrm[2] * L[2], 0.1f);
+            for (int k = 0; k < 3; ++k) col[k] += dec[k] * (alpha + emission[3 * m + k]) / fs;
+        }
+        c[0] = col[0]; c[1] = col[1]; c[2] = col[2]; c[3] = 1.0f;
+    }
+}
+
+/* reconstructShadow.comp:39-59 */
+OR_API void or_reconstruct_shadow(const int32_t* shadow_hits, const int32_t* in_s2i,
+                                  const int32_t* shadow_i2s, const float* in_px, float* out_px,
+                                  int64_t begin, int64_t end, int S, int replace) {
+    for (int64_t j = 0; j < end - begin; ++j) {
+        int pixel = in_s2i[begin + j];
+        const float* c = in_px + 4 * (int64_t)pixel;
+        float sh = 0.0f;
+        for (int i = 0; i < S; ++i)
+            if (shadow_hits[4 * (int64_t)shadow_i2s[j * S + i]] == -1) sh += 1.0f;
+        sh /= (float)S;
+        float* o = out_px + 4 * (int64_t)pixel;
+        float v[4] = {c[0] * sh, c[1] * sh, c[2] * sh, 1.0f};
+        if (replace) for (int k = 0; k < 4; ++k) o[k] = v[k];
+        else for (int k = 0; k < 4; ++k) o[k] += v[k];
+    }
+}
+
+/* interpolateColors.comp:10-57.  mat3(a..i) fills columns, so M*v uses the
+ * written values column-wise: x = c0*v.x + c3*v.y + c6*v.z. */
+static void mat3_cols(const float* c, const float v[3], float o[3]) {
+    o[0] = (c[0] * v[0] + c[3] * v[1]) + c[6] * v[2];
+    o[1] = (c[1] * v[0] + c[4] * v[1]) + c[7] * v[2];
+    o[2] = (c[2] * v[0] + c[5] * v[1]) + c[8] * v[2];
+}
+static void tonemap(const float in[3], float key, float white, float out[3]) {
+    static const float rgb2xyz[9] = {0.5141364f, 0.3238786f, 0.16036376f, 0.265068f, 0.67023428f,
+                                     0.06409157f, 0.0241188f, 0.1228178f, 0.84442666f};
+    static const float xyz2rgb[9] = {2.5651f, -1.1665f, -0.3986f, -1.0217f, 1.9777f,
+                                     0.0439f, 0.0753f, -0.2543f, 1.1892f};
+    float XYZ[3];
+    mat3_cols(rgb2xyz, in, XYZ);
+    float sum = (XYZ[0] + XYZ[1]) + XYZ[2];
+    float Y = XYZ[1], x = XYZ[0] / sum, y = XYZ[1] / sum;
+    float Lp = Y * key / 0.5f;
+    Y = (Lp * (1.0f + Lp / (white * white))) / (1.0f + Lp);
+    float X2[3] = {Y * x / y, Y, Y * (1.0f - x - y) / y};
+    mat3_cols(xyz2rgb, X2, out);
+}
+OR_API void or_tonemap(const float* in, float key, float white, float* out) { tonemap(in, key, white, out); }
+
+/* interpolateColors.comp:78-97 */
+OR_API void or_interpolate(float* frame, float* pixels, int64_t n, int fi, float key, float white) {
+    for (int64_t i = 0; i < n; ++i) {
+        float* f = frame + 4 * i;
+        float* p = pixels + 4 * i;
+        float c[3];
+        if (fi > 1) {
+            float a = 1.0f / (float)fi, b = a * ((float)fi - 1.0f);
+            for (int k = 0; k < 3; ++k) c[k] = a * f[k] + b * p[k];
+            p[0] = c[0]; p[1] = c[1]; p[2] = c[2]; p[3] = 1.0f;
+        } else {
+            for (int k = 0; k < 4; ++k) p[k] = f[k];
+            c[0] = f[0]; c[1] = f[1]; c[2] = f[2];
+        }
+        float t[3];
+        tonemap(c, key, white, t);
+        f[0] = t[0]; f[1] = t[1]; f[2] = t[2]; f[3] = 1.0f;
+    }
+}
+
+/* =================================================================
+ * FP64 frame oracle: the whole Renderer::renderPath frame
+ * (Renderer.cpp:175-199, :552-654) evaluated per pixel in double
+ * precision with the same random-number streams (pixel-keyed TEA seed,
+ * LCG draws, Sobol/Hammersley shadow points with the Jenkins offset), an
+ * fp64 Moller-Trumbore intersector and a median-split BVH built here.
+ * Output: the accumulated linear framebuffer (frame 1) as float32 RGBA.
+ * ================================================================= */
+
+typedef struct { double lo[3], hi[3]; int left, right, first, count; } bnode;
+typedef struct { bnode* nodes; int nn; int* order; const float* tv; } cbvh;
+
+static void tri_bounds(const float* tv, int t, double lo[3], double hi[3]) {
+    for (int a = 0; a < 3; ++a) {
+        double v0 = tv[9 * t + a], v1 = tv[9 * t + 3 + a], v2 = tv[9 * t + 6 + a];
+        lo[a] = fmin(v0, fmin(v1, v2));
+        hi[a] = fmax(v0, fmax(v1, v2));
+    }
+}
+
+static int cb_build(cbvh* b, int first, int count) {
+    int id = b->nn++;
+    bnode* nd = &b->nodes[id];
+    for (int a = 0; a < 3; ++a) { nd->lo[a] = INFINITY; nd->hi[a] = -INFINITY; }
+    double clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = first; k < first + count; ++k) {
+        double lo[3], hi[3];
+        tri_bounds(b->tv, b->order[k], lo, hi);
+        for (int a = 0; a < 3; ++a) {
+            nd->lo[a] = fmin(nd->lo[a], lo[a]); nd->hi[a] = fmax(nd->hi[a], hi[a]);
+            double c = 0.5 * (lo[a] + hi[a]);
+            clo[a] = fmin(clo[a], c); chi[a] = fmax(chi[a], c);
+        }
+    }
+    if (count <= 4) { nd->left = nd->right = -1; nd->first = first; nd->count = count; return id; }
+    int ax = 0;
+    for (int a = 1; a < 3; ++a) if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
+    double mid = 0.5 * (clo[ax] + chi[ax]);
+    int i = first, j = first + count - 1;
+    while (i <= j) {
+        double lo[3], hi[3];
+        tri_bounds(b->tv, b->order[i], lo, hi);
+        if (0.5 * (lo[ax] + hi[ax]) < mid) ++i;
+        else { int t = b->order[i]; b->order[i] = b->order[j]; b->order[j] = t; --j; }
+    }
+    int nl = i - first;
+    if (nl == 0 || nl == count) nl = count / 2;
+    int l = cb_build(b, first, nl);
+    int r = cb_build(b, first + nl, count - nl);
+    nd = &b->nodes[id];
+    nd->left = l; nd->right = r; nd->first = 0; nd->count = 0;
+    return id;
+}
+
+static double mt_hit(const float* tv, const double o[3], const double d[3], double* u_, double* v_) {
+    double v0[3] = {tv[0], tv[1], tv[2]};
+    double e1[3] = {tv[3] - v0[0], tv[4] - v0[1], tv[5] - v0[2]};
+    double e2[3] = {tv[6] - v0[0], tv[7] - v0[1], tv[8] - v0[2]};
+    double p[3] = {d[1] * e2[2] - d[2] * e2[1], d[2] * e2[0] - d[0] * e2[2], d[0] * e2[1] - d[1] * e2[0]};
+    double det = e1[0] * p[0] + e1[1] * p[1] + e1[2] * p[2];
+    if (det == 0.0) return -1.0;
+    double inv = 1.0 / det;
+    double s[3] = {o[0] - v0[0], o[1] - v0[1], o[2] - v0[2]};
+    double u = (s[0] * p[0] + s[1] * p[1] + s[2] * p[2]) * inv;
+    if (u < 0.0 || u > 1.0) return -1.0;
+    double q[3] = {s[1] * e1[2] - s[2] * e1[1], s[2] * e1[0] - s[0] * e1[2], s[0] * e1[1] - s[1] * e1[0]};
+    double v = (d[0] * q[0] + d[1] * q[1] + d[2] * q[2]) * inv;
+    if (v < 0.0 || u + v > 1.0) return -1.0;
+    *u_ = u; *v_ = v;
+    return (e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2]) * inv;
+}
+
+/* closest (any = 0) or any hit in [tmin, tmax]; returns triangle or -1 */
+static int cb_trace(const cbvh* b, const double o[3], const double d[3], double tmin, double tmax,
+                    int any, double* t_out, double* u_out, double* v_out) {
+    if (!(tmax >= tmin)) return -1;
+    int stack[128], sp = 0, node = 0, best = -1;
+    double best_t = tmax, bu = 0, bv = 0;
+    double inv[3];
+    for (int a = 0; a < 3; ++a) inv[a] = 1.0 / (fabs(d[a]) > 1e-300 ? d[a] : copysign(1e-300, d[a]));
+    stack[sp++] = 0;
+    while (sp) {
+        node = stack[--sp];
+        const bnode* nd = &b->nodes[node];
+        double tn = tmin, tf = best_t;
+        int miss = 0;
+        for (int a = 0; a < 3; ++a) {
+            double t0 = (nd->lo[a] - o[a]) * inv[a], t1 = (nd->hi[a] - o[a]) * inv[a];
+            double lo = fmin(t0, t1), hi = fmax(t0, t1);
+            tn = fmax(tn, lo - 1e-9 * fabs(lo) - 1e-12);
+            tf = fmin(tf, hi + 1e-9 * fabs(hi) + 1e-12);
+            if (tn > tf) { miss = 1; break; }
+        }
+        if (miss) continue;
+        if (nd->left < 0) {
+            for (int k = nd->first; k < nd->first + nd->count; ++k) {
+                int t = b->order[k];
+                double u, v;
+                double th = mt_hit(b->tv + 9 * t, o, d, &u, &v);
+                if (!(th >= tmin && th <= best_t)) continue;
+                if (th == best_t && best >= 0 && t > best) continue;
+                best = t; best_t = th; bu = u; bv = v;
+                if (any) { *t_out = th; *u_out = u; *v_out = v; return t; }
+            }
+        } else {
+            stack[sp++] = nd->right;
+            stack[sp++] = nd->left;
+        }
+    }
+    if (best >= 0) { *t_out = best_t; *u_out = bu; *v_out = bv; }
+    return best;
+}
+
+typedef struct {
+    int w, h, spp, depth, S, frame, rr, rr_start;
+    float light[3], bg[3], origin[3], radius, max_dist, s2w[16];
+} or_frame_cfg;
+
+static void dnorm(double v[3]) {
+    double l = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    v[0] /= l; v[1] /= l; v[2] /= l;
+}
+
+/* shading normal (exact barycentrics), faceforwarded against d */
+static int d_normal(const float* tri_n, const int32_t* tri_mat, int t, double u, double v,
+                    const double d[3], double n[3]) {
+    const float* a = tri_n + 9 * (int64_t)t;
+    double w = 1.0 - u - v;
+    for (int k = 0; k < 3; ++k) n[k] = a[k] * w + a[3 + k] * u + a[6 + k] * v;
+    dnorm(n);
+    if (!(n[0] * d[0] + n[1] * d[1] + n[2] * d[2] < 0.0)) { n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2]; }
+    return tri_mat[t];
+}
+
+/* visibility fraction of the S shadow rays from the hit of (o, d, t) */
+static double d_shadow(const cbvh* b, const or_frame_cfg* c, int hit, const double o[3],
+                       const double d[3], double t, uint32_t seed, int pixel) {
+    if (hit < 0) return 1.0;
+    double adv = fmax(t - 1.0e-5, 0.0);
+    double p[3] = {o[0] + d[0] * adv, o[1] + d[1] * adv, o[2] + d[2] * adv};
+    int vis = 0;
+    for (int i = 0; i < c->S; ++i) {
+        float q[3];
+        shadow_point(seed, pixel, i, c->S, q);
+        double dir[3];
+        for (int k = 0; k < 3; ++k) dir[k] = (double)c->light[k] + (double)c->radius * q[k] - p[k];
+        double len = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+        for (int k = 0; k < 3; ++k) dir[k] /= len;
+        double tt, uu, vv;
+        if (cb_trace(b, p, dir, 1.0e-5, len, 1, &tt, &uu, &vv) < 0) ++vis;
+    }
+    return (double)vis / c->S;
+}
+
+static void d_smooth(const cbvh* b, const or_frame_cfg* c, const float* tri_n, const int32_t* tri_mat,
+                     const float* emission, int nmat, int hit, const double o[3], const double d[3],
+                     double t, double u, double v, const double dec[3], double out[3]) {
+    (void)b;
+    if (hit < 0) {
+        for (int k = 0; k < 3; ++k) out[k] = dec[k] * c->bg[k] / c->spp;
+        return;
+    }
+    double n[3];
+    int m = d_normal(tri_n, tri_mat, hit, u, v, d, n);
+    if (m < 0 || m >= nmat) m = 0;
+    double p[3] = {o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t};
+    double L[3] = {c->light[0] - p[0], c->light[1] - p[1], c->light[2] - p[2]};
+    dnorm(L);
+    double alpha = fmax(n[0] * L[0] + n[1] * L[1] + n[2] * L[2], 0.1);
+    for (int k = 0; k < 3; ++k) out[k] = dec[k] * (alpha + emission[3 * m + k]) / c->spp;
+}
+
+OR_API void or_render_frame_f64(const float* tv, const float* tri_n, const int32_t* tri_mat,
+                                int64_t ntri, const float* albedo, const float* emission, int nmat,
+                                const or_frame_cfg* c, float* out_rgba) {
+    cbvh b;
+    b.tv = tv;
+    b.nodes = (bnode*)malloc(sizeof(bnode) * (2 * ntri + 1));
+    b.order = (int*)malloc(sizeof(int) * (ntri + 1));
+    b.nn = 0;
+    for (int64_t i = 0; i < ntri; ++i) b.order[i] = (int)i;
+    if (ntri > 0) cb_build(&b, 0, (int)ntri);
+    const int64_t np = (int64_t)c->w * c->h;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t pixel = 0; pixel < np; ++pixel) {
+        uint32_t seed = tea((uint32_t)pixel, (uint32_t)c->frame, 16);
+        double acc[3] = {0, 0, 0};
+        for (int pass = 0; pass < c->spp; ++pass) {
+            int si = pass + c->spp * (c->frame - 1);
+            float hx = halton(si, 2), hy = halton(si, 3);
+            double x = (double)(pixel % c->w) + hx, y = (double)(pixel / c->w) + hy;
+            double nx = 2.0 * x / c->w - 1.0, ny = 2.0 * y / c->h - 1.0;
+            const float* m = c->s2w;
+            double wx = m[0] * nx + m[4] * ny + m[12], wy = m[1] * nx + m[5] * ny + m[13];
+            double wz = m[2] * nx + m[6] * ny + m[14], ww = m[3] * nx + m[7] * ny + m[15];
+            double o[3] = {c->origin[0], c->origin[1], c->origin[2]};
+            double d[3] = {wx / ww - o[0], wy / ww - o[1], wz / ww - o[2]};
+            dnorm(d);
+            double dec[3] = {1, 1, 1};
+            double t = 0, u = 0, v = 0;
+            int hit = ntri > 0 ? cb_trace(&b, o, d, 0.0, c->max_dist, 0, &t, &u, &v) : -1;
+            double aux[3];
+            d_smooth(&b, c, tri_n, tri_mat, emission, nmat, hit, o, d, t, u, v, dec, aux);
+            double vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel);
+            for (int k = 0; k < 3; ++k) acc[k] += aux[k] * vis;
+            for (int bounce = 0; bounce < c->depth; ++bounce) {
+                if (hit < 0) break;
+                double n[3];
+                int mt = d_normal(tri_n, tri_mat, hit, u, v, d, n);
+                if (mt < 0 || mt >= nmat) mt = 0;
+                double r0 = randf(&seed), r1 = randf(&seed);
+                double p[3] = {o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t};
+                double theta = 2.0 * 3.14159265358979323846 * r1, rad = sqrt(r0);
+                double xx = rad * sin(theta), zz = rad * cos(theta), yy = sqrt(1.0 - r0);
+                double s = n[2] >= 0.0 ? 1.0 : -1.0;
+                double a = -1.0 / (s + n[2]), bb = n[0] * n[1] * a;
+                double b1[3] = {1.0 + s * n[0] * n[0] * a, s * bb, -s * n[0]};
+                double b2[3] = {bb, s + n[1] * n[1] * a, -n[1]};
+                double nd[3];
+                for (int k = 0; k < 3; ++k) nd[k] = xx * b1[k] + zz * b2[k] + yy * n[k];
+                double w[3] = {albedo[3 * mt], albedo[3 * mt + 1], albedo[3 * mt + 2]};
+                if (c->rr && bounce >= c->rr_start) {
+                    double q = fmax(fmax(dec[0] * w[0], dec[1] * w[1]), dec[2] * w[2]);
+                    q = fmin(fmax(q, 0.05), 1.0);
+                    double r2 = randf(&seed);
+                    if (r2 >= q) break;
+                    for (int k = 0; k < 3; ++k) w[k] /= q;
+                }
+                for (int k = 0; k < 3; ++k) { dec[k] *= w[k]; o[k] = p[k] + nd[k] * 1.0e-4; d[k] = nd[k]; }
+                hit = cb_trace(&b, o, d, 1.0e-4, 3.402823466e+38, 0, &t, &u, &v);
+                d_smooth(&b, c, tri_n, tri_mat, emission, nmat, hit, o, d, t, u, v, dec, aux);
+                vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel);
+                for (int k = 0; k < 3; ++k) acc[k] += aux[k] * vis;
+            }
+        }
+        float* px = out_rgba + 4 * pixel;
+        px[0] = (float)acc[0]; px[1] = (float)acc[1]; px[2] = (float)acc[2]; px[3] = 1.0f;
+    }
+    free(b.nodes);
+    free(b.order);
+}
+
+/* fp64 BVH trace for large scenes (closest or any), same output layout as
+ * or_trace_f64. */
+OR_API void or_trace_bvh_f64(const float* rays, int64_t n, const float* tri_verts, int64_t ntri,
+                             int closest, double* out_t, int32_t* out_tri) {
+    cbvh b;
+    b.tv = tri_verts;
+    b.nodes = (bnode*)malloc(sizeof(bnode) * (2 * ntri + 1));
+    b.order = (int*)malloc(sizeof(int) * (ntri + 1));
+    b.nn = 0;
+    for (int64_t i = 0; i < ntri; ++i) b.order[i] = (int)i;
+    if (ntri > 0) cb_build(&b, 0, (int)ntri);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; ++i) {
+        const float* r = rays + 8 * i;
+        double o[3] = {r[0], r[1], r[2]}, d[3] = {r[4], r[5], r[6]};
+        double t = -1, u, v;
+        int hit = ntri > 0 ? cb_trace(&b, o, d, r[3], r[7], !closest, &t, &u, &v) : -1;
+        out_t[i] = hit >= 0 ? t : -1.0;
+        out_tri[i] = hit;
+    }
+    free(b.nodes);
+    free(b.order);
+}

@@ -1,0 +1,221 @@
+"""Host logic on the CPU: the C-ABI exports, env.json handling, the camera,
+the primary-ray pixel order and the procedural scenes (no GPU calls)."""
+import ctypes as C
+import json
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_library_exports_every_header_symbol():
+    from vulkanwavefrontpathtracer_amd import _capi
+
+    lib = _capi.load()
+    header = (ROOT / "include" / "wfpt.h").read_text()
+    declared = set(re.findall(r"\b(wf_[a-z0-9_]+)\s*\(", header))
+    declared -= {"wf_scene_s", "wf_renderer_s"}
+    missing = sorted(n for n in declared if getattr(lib, n, None) is None)
+    assert not missing, f"declared in wfpt.h but not exported: {missing}"
+    assert set(_capi.SIGNATURES) <= declared | {"wf_last_error"}
+    assert _capi.missing_symbols() == []
+
+
+def test_status_strings():
+    from vulkanwavefrontpathtracer_amd import _capi
+
+    lib = _capi.load()
+    assert lib.wf_status_string(0) == b"ok"
+    assert lib.wf_status_string(-4) == b"scratch buffer too small"
+    assert b"gfx950" in lib.wf_version()
+
+
+def test_sort_scratch_sizes():
+    from vulkanwavefrontpathtracer_amd import _capi
+
+    lib = _capi.load()
+    a = lib.wf_sort_scratch_bytes(1 << 20, 4, 32, 8)
+    b = lib.wf_sort_scratch_bytes(1 << 21, 4, 32, 8)
+    assert 0 < a < b
+    assert lib.wf_sort_scratch_bytes(10, 4, 32, 5) < 0        # bad digit width
+    assert lib.wf_sort_scratch_bytes(10, 4, 40, 8) < 0        # more bits than the key holds
+    assert lib.wf_sort_scratch_bytes(10, 8, 64, 4) > 0
+    assert lib.wf_sort_scratch_bytes(0, 8, 64, 8) > 0
+
+
+# ----------------------------------------------------------------- env.json
+def renderer_from(obj):
+    from vulkanwavefrontpathtracer_amd.renderer import Renderer
+
+    return Renderer(obj)
+
+
+def test_env_defaults_match_app_environment():
+    r = renderer_from({})
+    # AppEnvironment.cpp:12-41
+    assert r.env_string("Application.mode") == "interactive"
+    assert r.env_int("Resolution.width") == 1024 and r.env_int("Resolution.height") == 768
+    assert r.env_vector("Scene.light") == [5.0, -2.0, 1.0]
+    assert r.env_int("Renderer.numberOfShadowSamples") == 2
+    assert r.env_float("Renderer.shadowRadius") == pytest.approx(0.001)
+    assert r.env_string("Renderer.rayType") == "primary"
+    assert r.env_float("Camera.fieldOfView") == pytest.approx(45.0)
+    assert r.size == (1024, 768)
+
+
+def test_env_nested_override_and_types():
+    r = renderer_from({
+        "Resolution": {"width": 320, "height": 200},
+        "Renderer": {"rayType": "path", "recursionDepth": 4, "sortPathRays": True,
+                     "keyValue": 0.5},
+        "Scene": {"light": "1 2 3", "unknown": 7},
+        "Unknown": {"x": 1},
+    })
+    assert r.size == (320, 200)
+    assert r.env_string("Renderer.rayType") == "path"
+    assert r.env_int("Renderer.recursionDepth") == 4
+    assert r.env_int("Renderer.sortPathRays") == 1
+    assert r.env_float("Renderer.keyValue") == pytest.approx(0.5)
+    assert r.env_vector("Scene.light") == [1.0, 2.0, 3.0]
+
+
+def test_env_reference_file_format(tmp_path):
+    # the shape of the reference's env/env.json (Application/Benchmark/...)
+    text = {
+        "Application": {"mode": "benchmark"}, "Benchmark": {"printSortLogs": True},
+        "Resolution": {"width": 64, "height": 32},
+        "Scene": {"filename": "models/x.gltf", "light": "0.0 10.0 0.0",
+                  "backgroundcolor": "0.529 0.808 0.98", "headlight": True},
+        "Renderer": {"numberOfPrimarySamples": 1, "numberOfShadowSamples": 2,
+                     "shadowRadius": 1.0, "rayType": "path", "recursionDepth": 4},
+        "Camera": {"position": "8.287917 3.804885 34.659367",
+                   "rotation": "1.075014 -303.599152 0.000000", "nearPlane": 0.1,
+                   "farPlane": 512.0, "fieldOfView": 60.0},
+    }
+    p = tmp_path / "env.json"
+    p.write_text(json.dumps(text))
+    from vulkanwavefrontpathtracer_amd.renderer import Renderer
+
+    r = Renderer(p)
+    assert r.env_string("Scene.filename") == "models/x.gltf"
+    assert r.env_float("Camera.farPlane") == pytest.approx(512.0)
+    assert r.env_vector("Camera.position") == pytest.approx([8.287917, 3.804885, 34.659367])
+
+
+def test_env_errors(tmp_path):
+    from vulkanwavefrontpathtracer_amd import _capi
+    from vulkanwavefrontpathtracer_amd.renderer import Renderer
+
+    p = tmp_path / "bad.json"
+    p.write_text('{"Resolution": {"width": 3,, }')
+    with pytest.raises(_capi.WfError) as e:
+        Renderer(p)
+    assert e.value.status == -6
+    with pytest.raises(_capi.WfError) as e:
+        Renderer(tmp_path / "missing.json")
+    assert e.value.status == -7
+    with pytest.raises(_capi.WfError):
+        Renderer({"Resolution": {"width": 0, "height": 10}})
+    with pytest.raises(_capi.WfError):
+        Renderer({"Renderer": {"keyMode": "nonsense"}})
+
+
+def test_bundled_configs_parse():
+    from vulkanwavefrontpathtracer_amd.renderer import Renderer, env_path
+
+    sizes = {"config0.json": (256, 256), "config1.json": (1920, 1080),
+             "config2.json": (1920, 1080), "config4.json": (512, 512)}
+    for name, size in sizes.items():
+        assert Renderer(env_path(name)).size == size
+
+
+# ------------------------------------------------------------------- camera
+def test_camera_forward_and_inverse():
+    from vulkanwavefrontpathtracer_amd import stages
+
+    V, P, S = stages.camera_matrices([1, 2, 3], [0, 0, 0], 60.0, 1.5, 0.1, 100.0)
+    assert np.allclose(S @ (P @ V), np.eye(4), atol=1e-4)
+    # glm::perspective RH zero-to-one with Y flipped (camera.hpp:108-120)
+    th = np.tan(np.radians(30.0))
+    assert P[0, 0] == pytest.approx(1 / (1.5 * th), rel=1e-6)
+    assert P[1, 1] == pytest.approx(-1 / th, rel=1e-6)
+    assert P[2, 2] == pytest.approx(100.0 / (0.1 - 100.0), rel=1e-6)
+    assert P[3, 2] == -1.0
+    # the view translates the camera to the origin
+    assert np.allclose(V @ np.array([1, 2, 3, 1.0]), [0, 0, 0, 1], atol=1e-6)
+
+
+@pytest.mark.parametrize("rot,forward", [((0, 0, 0), (0, 0, -1)), ((0, 90, 0), (-1, 0, 0)),
+                                         ((0, -90, 0), (1, 0, 0)), ((30, 0, 0), (0, 0.5, -0.866025))])
+def test_camera_rotation_conventions(rot, forward):
+    from vulkanwavefrontpathtracer_amd import stages
+
+    V, P, S = stages.camera_matrices([0, 0, 0], rot, 60.0, 1.0, 0.1, 100.0)
+    # the world-space view direction is -z of the camera frame = -(third row of V)
+    f = -V[2, :3]
+    assert np.allclose(f, forward, atol=1e-5)
+
+
+# ------------------------------------------------------------- pixel table
+def test_pixel_table_small_known_answer():
+    from vulkanwavefrontpathtracer_amd import stages
+
+    i2p, p2i = stages.pixel_table(16, 8)
+    # first block (8x8 at the origin) is Z-swizzled: inner 0..3 -> (0,0) (1,0) (0,1) (1,1)
+    assert list(i2p[:4]) == [0, 1, 16, 17]
+    assert list(i2p[4:8]) == [2, 3, 18, 19]
+    # the second 8x8 block (x in 8..15) follows the first
+    assert i2p[64] == 8
+    assert np.array_equal(np.sort(i2p), np.arange(128))
+    assert np.array_equal(p2i[i2p], np.arange(128))
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (7, 5), (13, 21), (256, 256), (1920, 1080), (100, 37)])
+def test_pixel_table_bijection_with_stripes(w, h):
+    from vulkanwavefrontpathtracer_amd import stages
+
+    i2p, p2i = stages.pixel_table(w, h)
+    assert np.array_equal(np.sort(i2p), np.arange(w * h))
+    assert np.array_equal(p2i[i2p], np.arange(w * h))
+    bw, bh = w & ~7, h & ~7
+    nb = bw * bh
+    # bulk first: every pixel inside the 8-aligned block region comes before the stripes
+    xs, ys = i2p % w, i2p // w
+    assert np.all(xs[:nb] < bw) and np.all(ys[:nb] < bh)
+    if nb < w * h:
+        assert np.all((xs[nb:] >= bw) | (ys[nb:] >= bh))
+
+
+# --------------------------------------------------------- procedural scenes
+def test_boxes_scene():
+    from vulkanwavefrontpathtracer_amd import stages
+
+    s = stages.HostScene(stages.SCENE_BOXES, 1000)
+    assert s.num_tris == 64 * 192 + 2 + 2 == 12292
+    assert len(s.geom_first) == 64 + 2 + 1 + 0 and s.geom_first[-1] == s.num_tris
+    assert np.all(np.diff(s.geom_first) > 0)
+    assert s.albedo.shape == (66, 3)
+    assert np.all((s.albedo[:64] >= 0.1) & (s.albedo[:64] <= 0.9))
+    assert np.allclose(s.emission[65], 10.0)
+    # unit normals, and bounds contain every vertex
+    n = np.linalg.norm(s.vertices[:, 3:], axis=1)
+    assert np.allclose(n, 1.0, atol=1e-5)
+    v = s.vertices[:, :3]
+    assert np.all(v >= s.bmin - 1e-6) and np.all(v <= s.bmax + 1e-6)
+    s2 = stages.HostScene(stages.SCENE_BOXES, 1000)
+    assert np.array_equal(s.vertices, s2.vertices)          # seeded, deterministic
+    s3 = stages.HostScene(stages.SCENE_BOXES, 1001)
+    assert not np.array_equal(s.vertices, s3.vertices)
+
+
+def test_interior_scene_size():
+    from vulkanwavefrontpathtracer_amd import stages
+
+    s = stages.HostScene(stages.SCENE_INTERIOR, 1001)
+    assert abs(s.num_tris - 262_144) / 262_144 < 0.01
+    assert s.num_tris == 12 + 32 * 4096 + 24 * 3840 + 40_000 + 6
+    assert s.albedo.shape[0] == 9 and np.allclose(s.emission[8], 10.0)
+    assert int(s.tri_material.max()) <= 8 and int(s.tri_material.min()) >= 0

@@ -119,3 +119,175 @@ def test_reorder_invariants(orc):
     # i2s inverts s2i on the written entries
     assert np.array_equal(out_i2s[out_s2i], np.arange(n, dtype=np.int32))
     assert np.sum(out_i2s >= 0) == n
+
+
+# ------------------------------------------------------------ random numbers
+def test_tea_known_answers(orc):
+    assert orc.tea(123, 456, 0) == 123                  # zero rounds: v0 unchanged
+    # one round with v0 = v1 = 0, worked by hand from initSeeds.comp:12-16:
+    # v0 = 0xa341316c ^ 0x9e3779b9 ^ 0xc8013ea4
+    assert orc.tea(0, 0, 1) == 0xF5777671
+    seeds = orc.init_seeds(1000, 3)
+    assert len(set(seeds.tolist())) == 1000              # distinct per pixel
+    assert seeds[17] == orc.tea(17, 3, 16)
+
+
+def test_sobol_halton_known_answers(orc):
+    # the first points of the 2-D Sobol sequence
+    assert np.allclose([orc.sobol2(i) for i in range(4)],
+                       [[0, 0], [0.5, 0.5], [0.75, 0.25], [0.25, 0.75]])
+    assert [orc.halton(i, 2) for i in range(4)] == [0.0, 0.5, 0.25, 0.75]
+    assert orc.halton(1, 3) == pytest.approx(1 / 3, abs=1e-7)
+    assert orc.halton(3, 3) == pytest.approx(1 / 9, abs=1e-7)
+
+
+def test_tonemap_preserves_chromaticity(orc):
+    # GLSL mat3(...) fills columns, so the effective RGB->XYZ matrix is the
+    # transpose of the literal rows written in interpolateColors.comp:17-19.
+    M = np.array([[0.5141364, 0.3238786, 0.16036376], [0.265068, 0.67023428, 0.06409157],
+                  [0.0241188, 0.1228178, 0.84442666]]).T
+    for rgb in ([0.2, 0.5, 0.9], [1.0, 1.0, 1.0], [3.0, 0.1, 0.4]):
+        out = orc.tonemap(np.array(rgb, np.float32), 0.6, 2.0)
+        xi, xo = M @ np.array(rgb), M @ out.astype(np.float64)
+        # chromaticity (x, y) kept; luminance mapped by Reinhard's curve
+        assert np.allclose(xi[:2] / xi.sum(), xo[:2] / xo.sum(), atol=2e-3)
+        Lp = xi[1] * 0.6 / 0.5
+        assert xo[1] == pytest.approx(Lp * (1 + Lp / 4.0) / (1 + Lp), rel=2e-3)
+
+
+def test_interpolate_frame_average(orc):
+    frame = np.array([[1, 2, 3, 1], [0.5, 0.5, 0.5, 1]], np.float32)
+    pixels = np.array([[3, 2, 1, 1], [0.5, 0.5, 0.5, 1]], np.float32)
+    f, p = orc.interpolate(frame, pixels, 4, 0.6, 2.0)
+    assert np.allclose(p[0, :3], [0.25 * 1 + 0.75 * 3, 2.0, 0.25 * 3 + 0.75 * 1])
+    f1, p1 = orc.interpolate(frame, pixels, 1, 0.6, 2.0)
+    assert np.array_equal(p1, frame)                      # first frame: copy
+
+
+# ----------------------------------------------------------------- trace
+def random_soup(n_tri, seed, spread=5.0, size=1.0):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-spread, spread, (n_tri, 1, 3))
+    return (c + rng.uniform(-size, size, (n_tri, 3, 3))).reshape(n_tri, 9).astype(np.float32)
+
+
+def random_rays(n, seed, spread=6.0):
+    rng = np.random.default_rng(seed)
+    r = np.zeros((n, 8), np.float32)
+    r[:, :3] = rng.uniform(-spread, spread, (n, 3))
+    d = rng.standard_normal((n, 3))
+    r[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    r[:, 7] = 1e30
+    return r
+
+
+def classify(orc, rays, tv, f32, f64):
+    """Returns indices where the fp32 watertight result disagrees with fp64
+    beyond the tie rule (|dt| <= 1e-6 relative) and is not a grazing-edge
+    case (fp64 edge distance <= 1e-5 for the triangle either side picked)."""
+    t64, tri64, edge64 = f64
+    bad = []
+    for i in range(len(rays)):
+        a, b = int(f32[i, 0]), int(tri64[i])
+        if a == b:
+            continue
+        t32 = f32[i, 1:2].view(np.float32)[0]
+        if a >= 0 and b >= 0 and abs(t32 - t64[i]) <= 1e-6 * abs(t64[i]):
+            continue                       # tie within 1e-6 relative
+        e = [edge64[i] if b >= 0 else 1.0]
+        if a >= 0:
+            e.append(orc.edge_distance_f64(rays[i], tv[a]))
+        if min(e) <= 1e-5:
+            continue                       # ray grazes an edge
+        bad.append(i)
+    return bad
+
+
+def test_trace_f32_against_f64(orc):
+    tv = random_soup(400, 1)
+    rays = random_rays(3000, 2)
+    f32 = orc.trace_f32(rays, tv, closest=True)
+    f64 = orc.trace_f64(rays, tv)
+    assert np.mean(f32[:, 0] >= 0) > 0.2                  # the test exercises hits
+    assert classify(orc, rays, tv, f32, f64) == []
+    same = f32[:, 0] == f64[1]
+    t32 = f32[:, 1].view(np.float32)
+    hit = same & (f32[:, 0] >= 0)
+    assert np.allclose(t32[hit], f64[0][hit], rtol=1e-5)
+
+
+def test_trace_f32_watertight_shared_edge(orc):
+    # two triangles sharing the edge x = 0 of a unit quad in the z = 0 plane
+    tv = np.array([[-1, -1, 0, 0, -1, 0, 0, 1, 0], [0, -1, 0, 1, -1, 0, 0, 1, 0]], np.float32)
+    ys = np.linspace(-0.99, 0.99, 401, dtype=np.float32)
+    rays = np.array([[0.0, y, 5.0, 0.0, 0.0, 0.0, -1.0, 100.0] for y in ys], np.float32)
+    f32 = orc.trace_f32(rays, tv, closest=True)
+    assert np.all(f32[:, 0] >= 0)                         # no ray slips through the edge
+    # ties on t break to the smaller triangle index
+    assert np.all(f32[:, 0] == 0)
+    # a slanted ray through the shared vertex also hits
+    r = np.array([[0.3, 0.9, 3.0, 0.0, -0.3, 0.1, -3.0, 100.0]], np.float32)
+    assert orc.trace_f32(r, tv)[0, 0] >= 0
+
+
+def test_trace_f32_interval_and_any_hit(orc):
+    tv = random_soup(200, 7)
+    rays = random_rays(1000, 8)
+    rays[:, 3] = 0.5
+    rays[:, 7] = 4.0
+    c = orc.trace_f32(rays, tv, closest=True)
+    a = orc.trace_f32(rays, tv, closest=False)
+    assert np.array_equal(c[:, 0] >= 0, a[:, 0] >= 0)    # any-hit finds a hit iff one exists
+    t = c[:, 1].view(np.float32)
+    assert np.all((t[c[:, 0] >= 0] >= 0.5) & (t[c[:, 0] >= 0] <= 4.0))
+    dead = rays.copy()
+    dead[:, 7] = -1.0                                     # tmax < tmin: miss
+    assert np.all(orc.trace_f32(dead, tv)[:, 0] == -1)
+
+
+def test_bvh_f64_matches_brute_force(orc):
+    tv = random_soup(3000, 3, spread=8.0, size=0.6)
+    rays = random_rays(2000, 4, spread=9.0)
+    t_b, tri_b, _ = orc.trace_f64(rays, tv)
+    t_v, tri_v = orc.trace_bvh_f64(rays, tv)
+    assert np.array_equal(tri_b, tri_v)
+    assert np.allclose(t_b, t_v, rtol=0, atol=0)
+
+
+# ------------------------------------------------------------ frame oracle
+def frame_cfg(w=32, h=24, depth=0, S=2, radius=1e-3, light=(0, 0, 10), bg=(0.2, 0.4, 0.6)):
+    from vulkanwavefrontpathtracer_amd import stages
+
+    V, P, S2W = stages.camera_matrices([0, 0, 10], [0, 0, 0], 60.0, w / h, 0.1, 100.0)
+    return dict(w=w, h=h, spp=1, depth=depth, S=S, frame=1, light=light, bg=bg, origin=(0, 0, 10),
+                radius=radius, max_dist=100.0, s2w_colmajor=S2W.T.reshape(-1)), S2W
+
+
+def test_frame_f64_empty_scene_is_background(orc):
+    cfg, _ = frame_cfg(depth=3)
+    img = orc.render_frame_f64(np.zeros((0, 9), np.float32), np.zeros((0, 9), np.float32),
+                               np.zeros(0, np.int32), np.full((1, 3), 0.8, np.float32),
+                               np.zeros((1, 3), np.float32), **cfg)
+    assert np.allclose(img[:, :3], [0.2, 0.4, 0.6])
+
+
+def test_frame_f64_plane_closed_form(orc):
+    # a large quad at z = 0 facing the camera at z = 10, headlight at the
+    # camera, no bounces: pixel = max(N.L, 0.1) with L towards the camera.
+    tv = np.array([[-100, -100, 0, 100, -100, 0, 100, 100, 0],
+                   [-100, -100, 0, 100, 100, 0, -100, 100, 0]], np.float32)
+    tn = np.tile(np.array([0, 0, 1], np.float32), (2, 3))
+    cfg, S2W = frame_cfg(depth=0)
+    img = orc.render_frame_f64(tv, tn, np.zeros(2, np.int32), np.full((1, 3), 0.8, np.float32),
+                               np.zeros((1, 3), np.float32), **cfg)
+    w, h = cfg["w"], cfg["h"]
+    px = np.arange(w * h)
+    nx, ny = 2 * (px % w) / w - 1, 2 * (px // w) / h - 1            # halton(0) = 0
+    world = S2W.astype(np.float64) @ np.stack([nx, ny, np.zeros_like(nx), np.ones_like(nx)])
+    d = world[:3] / world[3] - np.array([[0], [0], [10.0]])
+    d /= np.linalg.norm(d, axis=0)
+    p = np.array([[0], [0], [10.0]]) + d * (-10.0 / d[2])             # hit on z = 0
+    L = np.array([[0], [0], [10.0]]) - p
+    L /= np.linalg.norm(L, axis=0)
+    expect = np.maximum(L[2], 0.1)
+    assert np.allclose(img[:, 0], expect, rtol=1e-6)

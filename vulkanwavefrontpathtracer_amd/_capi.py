@@ -93,7 +93,7 @@ SIGNATURES: dict[str, tuple] = {
     "wf_env_get_string": (i32, [vp, cs, C.c_char_p, i32]),
     "wf_env_get_vector": (i32, [vp, cs, pf]),
     "wf_renderer_set_capture": (i32, [vp, i32]),
-    "wf_renderer_captured": (i32, [vp, i32, C.POINTER(vp), pi64]),
+    "wf_renderer_captured": (i32, [vp, i32, C.POINTER(vp), C.POINTER(vp), pi64]),
     "wf_procedural_scene": (i32, [i32, u64, pi64, pi64, C.POINTER(i32), C.POINTER(i32), pf, pu32,
                                   pi32, pi32, pf, pf, pf, pf]),
 }
