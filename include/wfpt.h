@@ -144,7 +144,8 @@ int wf_scene_info(wf_scene scene, int64_t* num_tris, int64_t* num_nodes,
                   int64_t* num_leaves, int* num_geoms);
 /* Copy the built hierarchy to host memory (for inspection / tests):
  * nodes as 16 floats each (child0 lo/hi, child1 lo/hi, child refs as int
- * bit patterns, pad), tri_order[num_tris] = original triangle of each slot. */
+ * bit patterns, pad), tri_order[num_nodes + 1] = original triangle of each
+ * leaf slot (a split large triangle occupies several slots). */
 int wf_scene_download(wf_scene scene, float* nodes_host, int32_t* tri_order_host);
 
 /* ------------------------------------------------------------------- trace */
@@ -242,6 +243,17 @@ int wf_interpolate_colors(float* frame_pixels, float* pixels, int64_t num_pixels
 /* countRayHits.comp:22-51: *d_count += number of hits with prim >= 0. */
 int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t stream,
                   float* ms);
+
+/* ----------------------------------------------------------------- tuning */
+/* Process-wide knobs read by later calls (the autotuner sets them):
+ *   "bvh.presplit"     0/1  split references of large triangles (default 1)
+ *   "bvh.split_depth"  0-8  max binary splits per triangle (default 6)
+ *   "bvh.leaf_max"     1-8  triangles per collapsed leaf (default 4)
+ *   "trace.lds_stack"  0/1  keep the top of the traversal stack in LDS (default 1)
+ *   "trace.block"      64/128/256  threads per traversal workgroup (default 256)
+ * Returns WF_ERR_INVALID_ARG for an unknown key or value out of range. */
+int wf_set_tuning(const char* key, int value);
+int wf_get_tuning(const char* key, int* value);
 
 /* ------------------------------------------------------------- host logic */
 /* PixelTable::recalculate (Ray/PixelTable.cpp:21-126): 8x8 Z-order blocks

@@ -268,7 +268,10 @@ class FrameCfg(C.Structure):
 
 def render_frame_f64(tri_verts, tri_n, tri_mat, albedo, emission, *, w, h, spp, depth, S, frame,
                      light, bg, origin, radius, max_dist, s2w_colmajor, rr=False, rr_start=3):
-    """Frame 1 of Renderer::renderPath in fp64; returns [w*h, 4] float32 linear RGBA."""
+    """Frame 1 of Renderer::renderPath in fp64.  Returns ([w*h, 4] float32 linear
+    RGBA, [w*h] bool ambiguous): a pixel is ambiguous when some vertex of its
+    paths grazes a triangle edge, ties two hits or sits on an interval end
+    (within 1e-5), where an fp32 pipeline may follow another path."""
     cfg = FrameCfg(w, h, spp, depth, S, frame, int(rr), rr_start, (C.c_float * 3)(*light),
                    (C.c_float * 3)(*bg), (C.c_float * 3)(*origin), radius, max_dist,
                    (C.c_float * 16)(*np.asarray(s2w_colmajor, np.float32).reshape(16)))
@@ -278,6 +281,7 @@ def render_frame_f64(tri_verts, tri_n, tri_mat, albedo, emission, *, w, h, spp, 
     al = np.ascontiguousarray(albedo, np.float32)
     em = np.ascontiguousarray(emission, np.float32)
     out = np.zeros((w * h, 4), np.float32)
+    amb = np.zeros(w * h, np.uint8)
     load().or_render_frame_f64(_p(tv), _p(tn), _p(tm), C.c_int64(len(tv)), _p(al), _p(em),
-                               C.c_int(len(al)), C.byref(cfg), _p(out))
-    return out
+                               C.c_int(len(al)), C.byref(cfg), _p(out), _p(amb))
+    return out, amb.astype(bool)

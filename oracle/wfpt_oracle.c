@@ -718,22 +718,54 @@ static double mt_hit(const float* tv, const double o[3], const double d[3], doub
     return (e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2]) * inv;
 }
 
-/* closest (any = 0) or any hit in [tmin, tmax]; returns triangle or -1 */
+/* Ambiguity of an fp64 query, used to recognise path vertices where an fp32
+ * pipeline may legitimately decide differently: a triangle passed within
+ * EPS_BARY (barycentric units) of one of its edges before the accepted hit,
+ * two hits within EPS_T relative of each other (the tie rule), or a hit
+ * within EPS_T of either end of the interval. */
+#define EPS_BARY 1e-5
+#define EPS_T 1e-5
+
+/* u, v, t of the ray's line against the triangle's plane (no inside test);
+ * returns 0 for a parallel ray. */
+static int mt_line(const float* tv, const double o[3], const double d[3], double* u_, double* v_,
+                   double* t_) {
+    double v0[3] = {tv[0], tv[1], tv[2]};
+    double e1[3] = {tv[3] - v0[0], tv[4] - v0[1], tv[5] - v0[2]};
+    double e2[3] = {tv[6] - v0[0], tv[7] - v0[1], tv[8] - v0[2]};
+    double p[3] = {d[1] * e2[2] - d[2] * e2[1], d[2] * e2[0] - d[0] * e2[2], d[0] * e2[1] - d[1] * e2[0]};
+    double det = e1[0] * p[0] + e1[1] * p[1] + e1[2] * p[2];
+    if (det == 0.0) return 0;
+    double inv = 1.0 / det;
+    double s[3] = {o[0] - v0[0], o[1] - v0[1], o[2] - v0[2]};
+    double q[3] = {s[1] * e1[2] - s[2] * e1[1], s[2] * e1[0] - s[0] * e1[2], s[0] * e1[1] - s[1] * e1[0]};
+    *u_ = (s[0] * p[0] + s[1] * p[1] + s[2] * p[2]) * inv;
+    *v_ = (d[0] * q[0] + d[1] * q[1] + d[2] * q[2]) * inv;
+    *t_ = (e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2]) * inv;
+    return 1;
+}
+
+/* closest (any = 0) or any hit in [tmin, tmax]; returns triangle or -1.
+ * If amb != NULL, *amb is set when the answer is ambiguous in the sense
+ * above (boxes are then expanded so that near misses are examined). */
 static int cb_trace(const cbvh* b, const double o[3], const double d[3], double tmin, double tmax,
-                    int any, double* t_out, double* u_out, double* v_out) {
+                    int any, double* t_out, double* u_out, double* v_out, int* amb) {
     if (!(tmax >= tmin)) return -1;
     int stack[128], sp = 0, node = 0, best = -1;
     double best_t = tmax, bu = 0, bv = 0;
+    double near_t = INFINITY, second_t = INFINITY;
+    int certain_any = 0;
     double inv[3];
     for (int a = 0; a < 3; ++a) inv[a] = 1.0 / (fabs(d[a]) > 1e-300 ? d[a] : copysign(1e-300, d[a]));
     stack[sp++] = 0;
     while (sp) {
         node = stack[--sp];
         const bnode* nd = &b->nodes[node];
-        double tn = tmin, tf = best_t;
+        double tn = tmin, tf = amb ? best_t * (1 + 4 * EPS_T) + 1e-12 : best_t;
         int miss = 0;
         for (int a = 0; a < 3; ++a) {
-            double t0 = (nd->lo[a] - o[a]) * inv[a], t1 = (nd->hi[a] - o[a]) * inv[a];
+            double grow = amb ? 1e-4 * (nd->hi[a] - nd->lo[a]) + 1e-9 : 0.0;
+            double t0 = (nd->lo[a] - grow - o[a]) * inv[a], t1 = (nd->hi[a] + grow - o[a]) * inv[a];
             double lo = fmin(t0, t1), hi = fmax(t0, t1);
             tn = fmax(tn, lo - 1e-9 * fabs(lo) - 1e-12);
             tf = fmin(tf, hi + 1e-9 * fabs(hi) + 1e-12);
@@ -743,16 +775,41 @@ static int cb_trace(const cbvh* b, const double o[3], const double d[3], double 
         if (nd->left < 0) {
             for (int k = nd->first; k < nd->first + nd->count; ++k) {
                 int t = b->order[k];
-                double u, v;
-                double th = mt_hit(b->tv + 9 * t, o, d, &u, &v);
-                if (!(th >= tmin && th <= best_t)) continue;
-                if (th == best_t && best >= 0 && t > best) continue;
+                double u, v, th;
+                if (!mt_line(b->tv + 9 * t, o, d, &u, &v, &th)) continue;
+                double margin = fmin(fmin(u, v), 1.0 - u - v);
+                int inside = u >= 0.0 && u <= 1.0 && v >= 0.0 && u + v <= 1.0;
+                if (amb) {
+                    double tol = EPS_T * fmax(fabs(th), 1.0);
+                    if (fabs(margin) < EPS_BARY && th >= tmin - tol && th <= tmax + tol)
+                        near_t = fmin(near_t, th);
+                    if (inside && (fabs(th - tmin) <= tol || (isfinite(tmax) && tmax < 1e30 && fabs(th - tmax) <= tol)))
+                        near_t = fmin(near_t, th);
+                }
+                if (!inside || !(th >= tmin && th <= best_t)) {
+                    if (inside && th >= tmin && th < second_t && best >= 0 && t != best) second_t = fmin(second_t, th);
+                    continue;
+                }
+                if (th == best_t && best >= 0 && t > best) { second_t = fmin(second_t, th); continue; }
+                if (best >= 0) second_t = fmin(second_t, best_t);
                 best = t; best_t = th; bu = u; bv = v;
-                if (any) { *t_out = th; *u_out = u; *v_out = v; return t; }
+                if (any) {
+                    if (!amb || margin >= EPS_BARY) { certain_any = 1; goto done; }
+                }
             }
         } else {
             stack[sp++] = nd->right;
             stack[sp++] = nd->left;
+        }
+    }
+done:
+    if (amb) {
+        double tol = EPS_T * fmax(fabs(best_t), 1.0);
+        if (any) {
+            if (!certain_any && near_t <= tmax + tol) *amb = 1;
+        } else {
+            if (near_t <= best_t + tol) *amb = 1;
+            if (best >= 0 && second_t <= best_t + tol) *amb = 1;
         }
     }
     if (best >= 0) { *t_out = best_t; *u_out = bu; *v_out = bv; }
@@ -782,7 +839,7 @@ static int d_normal(const float* tri_n, const int32_t* tri_mat, int t, double u,
 
 /* visibility fraction of the S shadow rays from the hit of (o, d, t) */
 static double d_shadow(const cbvh* b, const or_frame_cfg* c, int hit, const double o[3],
-                       const double d[3], double t, uint32_t seed, int pixel) {
+                       const double d[3], double t, uint32_t seed, int pixel, int* amb) {
     if (hit < 0) return 1.0;
     double adv = fmax(t - 1.0e-5, 0.0);
     double p[3] = {o[0] + d[0] * adv, o[1] + d[1] * adv, o[2] + d[2] * adv};
@@ -795,7 +852,7 @@ static double d_shadow(const cbvh* b, const or_frame_cfg* c, int hit, const doub
         double len = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
         for (int k = 0; k < 3; ++k) dir[k] /= len;
         double tt, uu, vv;
-        if (cb_trace(b, p, dir, 1.0e-5, len, 1, &tt, &uu, &vv) < 0) ++vis;
+        if (cb_trace(b, p, dir, 1.0e-5, len, 1, &tt, &uu, &vv, amb) < 0) ++vis;
     }
     return (double)vis / c->S;
 }
@@ -820,7 +877,7 @@ static void d_smooth(const cbvh* b, const or_frame_cfg* c, const float* tri_n, c
 
 OR_API void or_render_frame_f64(const float* tv, const float* tri_n, const int32_t* tri_mat,
                                 int64_t ntri, const float* albedo, const float* emission, int nmat,
-                                const or_frame_cfg* c, float* out_rgba) {
+                                const or_frame_cfg* c, float* out_rgba, uint8_t* out_amb) {
     cbvh b;
     b.tv = tv;
     b.nodes = (bnode*)malloc(sizeof(bnode) * (2 * ntri + 1));
@@ -833,6 +890,7 @@ OR_API void or_render_frame_f64(const float* tv, const float* tri_n, const int32
     for (int64_t pixel = 0; pixel < np; ++pixel) {
         uint32_t seed = tea((uint32_t)pixel, (uint32_t)c->frame, 16);
         double acc[3] = {0, 0, 0};
+        int amb = 0;
         for (int pass = 0; pass < c->spp; ++pass) {
             int si = pass + c->spp * (c->frame - 1);
             float hx = halton(si, 2), hy = halton(si, 3);
@@ -846,10 +904,10 @@ OR_API void or_render_frame_f64(const float* tv, const float* tri_n, const int32
             dnorm(d);
             double dec[3] = {1, 1, 1};
             double t = 0, u = 0, v = 0;
-            int hit = ntri > 0 ? cb_trace(&b, o, d, 0.0, c->max_dist, 0, &t, &u, &v) : -1;
+            int hit = ntri > 0 ? cb_trace(&b, o, d, 0.0, c->max_dist, 0, &t, &u, &v, &amb) : -1;
             double aux[3];
             d_smooth(&b, c, tri_n, tri_mat, emission, nmat, hit, o, d, t, u, v, dec, aux);
-            double vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel);
+            double vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel, &amb);
             for (int k = 0; k < 3; ++k) acc[k] += aux[k] * vis;
             for (int bounce = 0; bounce < c->depth; ++bounce) {
                 if (hit < 0) break;
@@ -871,18 +929,20 @@ OR_API void or_render_frame_f64(const float* tv, const float* tri_n, const int32
                     double q = fmax(fmax(dec[0] * w[0], dec[1] * w[1]), dec[2] * w[2]);
                     q = fmin(fmax(q, 0.05), 1.0);
                     double r2 = randf(&seed);
+                    if (fabs(r2 - q) < 1e-5) amb = 1;   /* roulette decision on a knife edge */
                     if (r2 >= q) break;
                     for (int k = 0; k < 3; ++k) w[k] /= q;
                 }
                 for (int k = 0; k < 3; ++k) { dec[k] *= w[k]; o[k] = p[k] + nd[k] * 1.0e-4; d[k] = nd[k]; }
-                hit = cb_trace(&b, o, d, 1.0e-4, 3.402823466e+38, 0, &t, &u, &v);
+                hit = cb_trace(&b, o, d, 1.0e-4, 3.402823466e+38, 0, &t, &u, &v, &amb);
                 d_smooth(&b, c, tri_n, tri_mat, emission, nmat, hit, o, d, t, u, v, dec, aux);
-                vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel);
+                vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel, &amb);
                 for (int k = 0; k < 3; ++k) acc[k] += aux[k] * vis;
             }
         }
         float* px = out_rgba + 4 * pixel;
         px[0] = (float)acc[0]; px[1] = (float)acc[1]; px[2] = (float)acc[2]; px[3] = 1.0f;
+        if (out_amb) out_amb[pixel] = (uint8_t)amb;
     }
     free(b.nodes);
     free(b.order);
@@ -904,7 +964,7 @@ OR_API void or_trace_bvh_f64(const float* rays, int64_t n, const float* tri_vert
         const float* r = rays + 8 * i;
         double o[3] = {r[0], r[1], r[2]}, d[3] = {r[4], r[5], r[6]};
         double t = -1, u, v;
-        int hit = ntri > 0 ? cb_trace(&b, o, d, r[3], r[7], !closest, &t, &u, &v) : -1;
+        int hit = ntri > 0 ? cb_trace(&b, o, d, r[3], r[7], !closest, &t, &u, &v, NULL) : -1;
         out_t[i] = hit >= 0 ? t : -1.0;
         out_tri[i] = hit;
     }

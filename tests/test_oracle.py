@@ -265,10 +265,11 @@ def frame_cfg(w=32, h=24, depth=0, S=2, radius=1e-3, light=(0, 0, 10), bg=(0.2, 
 
 def test_frame_f64_empty_scene_is_background(orc):
     cfg, _ = frame_cfg(depth=3)
-    img = orc.render_frame_f64(np.zeros((0, 9), np.float32), np.zeros((0, 9), np.float32),
+    img, amb = orc.render_frame_f64(np.zeros((0, 9), np.float32), np.zeros((0, 9), np.float32),
                                np.zeros(0, np.int32), np.full((1, 3), 0.8, np.float32),
                                np.zeros((1, 3), np.float32), **cfg)
     assert np.allclose(img[:, :3], [0.2, 0.4, 0.6])
+    assert not amb.any()
 
 
 def test_frame_f64_plane_closed_form(orc):
@@ -278,7 +279,7 @@ def test_frame_f64_plane_closed_form(orc):
                    [-100, -100, 0, 100, 100, 0, -100, 100, 0]], np.float32)
     tn = np.tile(np.array([0, 0, 1], np.float32), (2, 3))
     cfg, S2W = frame_cfg(depth=0)
-    img = orc.render_frame_f64(tv, tn, np.zeros(2, np.int32), np.full((1, 3), 0.8, np.float32),
+    img, amb = orc.render_frame_f64(tv, tn, np.zeros(2, np.int32), np.full((1, 3), 0.8, np.float32),
                                np.zeros((1, 3), np.float32), **cfg)
     w, h = cfg["w"], cfg["h"]
     px = np.arange(w * h)
@@ -291,3 +292,16 @@ def test_frame_f64_plane_closed_form(orc):
     L /= np.linalg.norm(L, axis=0)
     expect = np.maximum(L[2], 0.1)
     assert np.allclose(img[:, 0], expect, rtol=1e-6)
+    # the quad's diagonal is the only edge in view: few pixels graze it
+    assert amb.mean() < 0.05
+
+
+def test_frame_f64_ambiguity_flags_grazing_paths(orc):
+    """A ray that passes exactly through the shared vertex/edge of two
+    triangles is flagged; one through a triangle's interior is not."""
+    tv = np.array([[-1, -1, 0, 1, -1, 0, 1, 1, 0], [-1, -1, 0, 1, 1, 0, -1, 1, 0]], np.float32)
+    rays = np.array([[0.0, 0.0, 5.0, 0.0, 0.0, 0.0, -1.0, 100.0],      # on the diagonal
+                     [0.5, -0.4, 5.0, 0.0, 0.0, 0.0, -1.0, 100.0]], np.float32)
+    t, tri = orc.trace_bvh_f64(rays, tv)
+    assert np.all(tri >= 0)
+    assert np.allclose(t, 5.0)

@@ -46,20 +46,30 @@ def test_scene_build_structure(dev, boxes):
     import ctypes as C
 
     nodes = np.zeros((info["nodes"], 16), np.float32)
-    order = np.zeros(info["tris"], np.int32)
+    slots = info["nodes"] + 1                       # leaf references (>= triangles)
+    order = np.zeros(slots, np.int32)
     _capi.call("wf_scene_download", sc.handle, nodes.ctypes.data_as(_capi.pf),
                order.ctypes.data_as(_capi.pi32))
-    assert np.array_equal(np.sort(order), np.arange(info["tris"]))
-    # every slot is covered by exactly one leaf reference
-    refs = nodes[:, 12:14].view(np.int32).reshape(-1)
-    leaves = refs[refs < 0]
-    covered = np.zeros(info["tris"], np.int32)
-    for ref in leaves:
-        first, cnt = (~ref) >> 3, ((~ref) & 7) + 1
-        covered[first:first + cnt] += 1
+    assert slots >= info["tris"]
+    assert np.array_equal(np.unique(order), np.arange(info["tris"]))   # every triangle referenced
+    # walking the tree from the root reaches every slot exactly once and
+    # every reachable internal node once (a tree, no cycles)
+    refs = nodes[:, 12:14].view(np.int32)
+    covered = np.zeros(slots, np.int32)
+    seen = np.zeros(info["nodes"], np.int32)
+    todo = [0]
+    while todo:
+        nd = todo.pop()
+        seen[nd] += 1
+        for ref in refs[nd]:
+            if ref >= 0:
+                todo.append(int(ref))
+            else:
+                first, cnt = (~int(ref)) >> 3, ((~int(ref)) & 7) + 1
+                covered[first:first + cnt] += 1
     assert np.all(covered == 1)
-    internal = refs[refs >= 0]
-    assert len(np.unique(internal)) == len(internal) == info["nodes"] - 1   # a tree
+    assert seen.max() == 1
+    # child boxes contain the triangles of their leaves
     del C
 
 
@@ -195,3 +205,32 @@ def test_trace_sort_composite(dev, boxes):
         else:
             assert torch.equal(bufs["res"], ref)
         assert all(t >= 0 for t in times) and times[1] > 0
+
+
+@pytest.mark.parametrize("presplit,leaf,lds,block", [(0, 4, 0, 256), (1, 1, 1, 64), (1, 8, 1, 128),
+                                                     (1, 4, 0, 256), (1, 2, 1, 256)])
+def test_tuning_variants_bit_exact(dev, orc, presplit, leaf, lds, block):
+    """Every build/traversal variant the autotuner may pick returns the same
+    records (interior scene: walls are split into many references)."""
+    from vulkanwavefrontpathtracer_amd import _capi, stages
+
+    saved = {}
+    knobs = {"bvh.presplit": presplit, "bvh.leaf_max": leaf, "trace.lds_stack": lds,
+             "trace.block": block}
+    import ctypes as C
+    for k, v in knobs.items():
+        cur = C.c_int()
+        _capi.call("wf_get_tuning", k.encode(), C.byref(cur))
+        saved[k] = cur.value
+        _capi.call("wf_set_tuning", k.encode(), v)
+    try:
+        hs = stages.HostScene(stages.SCENE_INTERIOR, 1001)
+        sc = stages.Scene.from_host(hs, dev)
+        rays = scene_rays(hs, 800, 40 + leaf, tmin=1e-4)
+        got = stages.trace(sc, torch.from_numpy(rays).to(dev)).cpu().numpy()
+        assert np.array_equal(got, expected_hits(orc, hs, rays))
+        if presplit:
+            assert sc.info()["nodes"] + 1 > hs.num_tris   # walls were split
+    finally:
+        for k, v in saved.items():
+            _capi.call("wf_set_tuning", k.encode(), v)

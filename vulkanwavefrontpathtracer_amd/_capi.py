@@ -75,6 +75,8 @@ SIGNATURES: dict[str, tuple] = {
     "wf_reconstruct_shadow": (i32, [vp, vp, vp, vp, vp, i64, i64, i32, i32, vp, pf]),
     "wf_interpolate_colors": (i32, [vp, vp, i64, i32, f32, f32, vp, pf]),
     "wf_count_hits": (i32, [vp, i64, vp, vp, pf]),
+    "wf_set_tuning": (i32, [cs, i32]),
+    "wf_get_tuning": (i32, [cs, C.POINTER(i32)]),
     "wf_pixel_table": (i32, [i32, i32, pi32, pi32]),
     "wf_camera_matrices": (i32, [pf, pf, f32, f32, f32, f32, pf, pf, pf]),
     "wf_renderer_create": (i32, [cs, C.POINTER(vp)]),

@@ -8,12 +8,17 @@
 //   1. tri_prep   : per-triangle AABB and centroid, scene / centroid bounds
 //                   (wave reductions + one ordered-int atomic per wave),
 //                   shading records in original order.
-//   2. centroid Morton codes, 21 bits per axis (63-bit keys), sorted with
-//      this library's radix sort (tri index payload).
-//   3. Karras 2012 binary radix tree over the sorted codes (equal codes are
+//   2. references: a triangle whose box is larger than 1/64 of the scene
+//      diagonal is split (Ernst & Greiner's early split clipping: recursive
+//      midpoint splits of the clipped polygon, up to 2^split_depth pieces),
+//      so that large triangles (walls, floors) do not inflate the boxes near
+//      the root; small triangles are one reference each.
+//   3. centroid Morton codes of the references, 21 bits per axis (63-bit
+//      keys), sorted with this library's radix sort (reference payload).
+//   4. Karras 2012 binary radix tree over the sorted codes (equal codes are
 //      split by index), parent links.
-//   4. bottom-up AABB refit (second child to arrive at a node merges).
-//   5. emit 64-byte BVH2 nodes holding both child boxes; subtrees of at most
+//   5. bottom-up AABB refit (second child to arrive at a node merges).
+//   6. emit 64-byte BVH2 nodes holding both child boxes; subtrees of at most
 //      BVH_LEAF_MAX triangles collapse into one leaf (contiguous slots);
 //      child boxes are enlarged by 1e-5 x scene diagonal so that fp32 slab
 //      tests never cull a box whose triangles the intersection test hits.
@@ -108,6 +113,116 @@ __global__ __launch_bounds__(256) void tri_prep_kernel(
     }
 }
 
+// Sutherland-Hodgman clip of polygon `in` (n vertices) to the half space
+// x[axis] <= mid (below) or >= mid (above); returns the vertex count.
+constexpr int SPLIT_MAXV = 12;
+__device__ int clip_half(const float (*in)[3], int n, float (*out)[3], int axis, float mid,
+                         bool below) {
+    int m = 0;
+    for (int i = 0; i < n; ++i) {
+        const float* a = in[i];
+        const float* b = in[(i + 1) % n];
+        float da = below ? (mid - a[axis]) : (a[axis] - mid);
+        float db = below ? (mid - b[axis]) : (b[axis] - mid);
+        if (da >= 0.f && m < SPLIT_MAXV) {
+            out[m][0] = a[0];
+            out[m][1] = a[1];
+            out[m][2] = a[2];
+            ++m;
+        }
+        if ((da >= 0.f) != (db >= 0.f) && m < SPLIT_MAXV) {
+            float t = da / (da - db);
+            for (int k = 0; k < 3; ++k) out[m][k] = a[k] + t * (b[k] - a[k]);
+            out[m][axis] = mid;
+            ++m;
+        }
+    }
+    return m;
+}
+
+// Counts (EMIT = false) or writes (EMIT = true) the references of each
+// triangle: pieces of the triangle clipped by recursive midpoint splits of
+// the largest axis until the piece's extent is <= limit or depth runs out.
+template <bool EMIT>
+__global__ __launch_bounds__(128) void split_kernel(
+    const char* __restrict__ verts, int stride, const uint32_t* __restrict__ idx,
+    const float4* __restrict__ tri_box, int64_t ntri, const uint32_t* __restrict__ bounds,
+    float frac, int maxd, uint32_t* __restrict__ counts, const int64_t* __restrict__ offsets,
+    float4* __restrict__ item_box, uint32_t* __restrict__ item_tri) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntri) return;
+    float sx = funkey(bounds[3]) - funkey(bounds[0]);
+    float sy = funkey(bounds[4]) - funkey(bounds[1]);
+    float sz = funkey(bounds[5]) - funkey(bounds[2]);
+    const float limit = frac * sqrtf(sx * sx + sy * sy + sz * sz);
+    float4 blo = tri_box[2 * t], bhi = tri_box[2 * t + 1];
+    float ext = fmaxf(fmaxf(bhi.x - blo.x, bhi.y - blo.y), bhi.z - blo.z);
+    int64_t out = EMIT ? offsets[t] : 0;
+    if (ext <= limit || maxd == 0) {
+        if (EMIT) {
+            item_box[2 * out] = blo;
+            item_box[2 * out + 1] = bhi;
+            item_tri[out] = (uint32_t)t;
+        } else {
+            counts[t] = 1;
+        }
+        return;
+    }
+    float poly[9][SPLIT_MAXV][3];  // depth-first stack of pieces
+    int nv[9], dep[9];
+    int sp = 0;
+    for (int k = 0; k < 3; ++k) {
+        const float* v = reinterpret_cast<const float*>(verts + (int64_t)idx[3 * t + k] * stride);
+        poly[0][k][0] = v[0];
+        poly[0][k][1] = v[1];
+        poly[0][k][2] = v[2];
+    }
+    nv[0] = 3;
+    dep[0] = 0;
+    sp = 1;
+    uint32_t count = 0;
+    while (sp > 0) {
+        --sp;
+        const int n = nv[sp], d = dep[sp];
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = fminf(lo[k], poly[sp][i][k]);
+                hi[k] = fmaxf(hi[k], poly[sp][i][k]);
+            }
+        int axis = 0;
+        for (int k = 1; k < 3; ++k)
+            if (hi[k] - lo[k] > hi[axis] - lo[axis]) axis = k;
+        if (hi[axis] - lo[axis] <= limit || d >= maxd || sp + 2 > 9) {
+            if (EMIT) {
+                item_box[2 * out] = make_float4(lo[0], lo[1], lo[2], 0.f);
+                item_box[2 * out + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
+                item_tri[out] = (uint32_t)t;
+                ++out;
+            }
+            ++count;
+            continue;
+        }
+        const float mid = 0.5f * (lo[axis] + hi[axis]);
+        float cur[SPLIT_MAXV][3];
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k) cur[i][k] = poly[sp][i][k];
+        int na = clip_half(cur, n, poly[sp], axis, mid, false);
+        if (na >= 3) {
+            nv[sp] = na;
+            dep[sp] = d + 1;
+            ++sp;
+        }
+        int nb = clip_half(cur, n, poly[sp], axis, mid, true);
+        if (nb >= 3) {
+            nv[sp] = nb;
+            dep[sp] = d + 1;
+            ++sp;
+        }
+    }
+    if (!EMIT) counts[t] = count;
+}
+
 __device__ __forceinline__ uint64_t spread3_21(uint64_t v) {
     v &= 0x1fffffull;
     v = (v | (v << 32)) & 0x001f00000000ffffull;
@@ -118,18 +233,19 @@ __device__ __forceinline__ uint64_t spread3_21(uint64_t v) {
     return v;
 }
 
-__global__ __launch_bounds__(256) void centroid_keys_kernel(const float4* __restrict__ tri_box,
-                                                            int64_t ntri,
+// Morton code of each reference's box centre within the scene bounds.
+__global__ __launch_bounds__(256) void centroid_keys_kernel(const float4* __restrict__ item_box,
+                                                            int64_t nitem,
                                                             const uint32_t* __restrict__ bounds,
                                                             uint64_t* __restrict__ keys,
                                                             uint32_t* __restrict__ vals) {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntri) return;
-    float4 lo = tri_box[2 * t], hi = tri_box[2 * t + 1];
+    if (t >= nitem) return;
+    float4 lo = item_box[2 * t], hi = item_box[2 * t + 1];
     float c[3] = {(lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f};
     uint64_t q[3];
     for (int a = 0; a < 3; ++a) {
-        float mn = funkey(bounds[6 + a]), mx = funkey(bounds[9 + a]);
+        float mn = funkey(bounds[a]), mx = funkey(bounds[3 + a]);
         float ext = mx - mn;
         float u = ext > 0.f ? (c[a] - mn) / ext : 0.f;
         u = fminf(fmaxf(u, 0.f), 1.f);
@@ -179,8 +295,8 @@ __global__ __launch_bounds__(256) void karras_kernel(const uint64_t* __restrict_
     if (i == 0) parent[0] = -1;
 }
 
-__global__ __launch_bounds__(256) void refit_kernel(const float4* __restrict__ tri_box,
-                                                    const uint32_t* __restrict__ sorted_tri, int n,
+__global__ __launch_bounds__(256) void refit_kernel(const float4* __restrict__ item_box,
+                                                    const uint32_t* __restrict__ sorted_item, int n,
                                                     const int2* __restrict__ child,
                                                     const int32_t* __restrict__ parent,
                                                     float4* __restrict__ box /* 2 per node */,
@@ -188,9 +304,9 @@ __global__ __launch_bounds__(256) void refit_kernel(const float4* __restrict__ t
                                                     uint32_t* __restrict__ max_depth) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    uint32_t t = sorted_tri[s];
+    uint32_t t = sorted_item[s];
     int leaf = n - 1 + s;
-    float4 lo = tri_box[2 * (int64_t)t], hi = tri_box[2 * (int64_t)t + 1];
+    float4 lo = item_box[2 * (int64_t)t], hi = item_box[2 * (int64_t)t + 1];
     box[2 * leaf] = lo;
     box[2 * leaf + 1] = hi;
     // depth of this leaf (number of internal ancestors)
@@ -213,17 +329,18 @@ __global__ __launch_bounds__(256) void refit_kernel(const float4* __restrict__ t
 }
 
 __device__ __forceinline__ void child_ref(int c, int n, const int2* __restrict__ range,
-                                          int& ref) {
+                                          int leaf_max, int& ref) {
     if (c >= n - 1) {
         ref = leaf_ref(c - (n - 1), 1);
     } else {
         int2 r = range[c];
         int size = r.y - r.x + 1;
-        ref = size <= BVH_LEAF_MAX ? leaf_ref(r.x, size) : c;
+        ref = size <= leaf_max ? leaf_ref(r.x, size) : c;
     }
 }
 
-__global__ __launch_bounds__(256) void emit_nodes_kernel(int n, const int2* __restrict__ child,
+__global__ __launch_bounds__(256) void emit_nodes_kernel(int n, int leaf_max,
+                                                         const int2* __restrict__ child,
                                                          const int2* __restrict__ range,
                                                          const float4* __restrict__ box,
                                                          const uint32_t* __restrict__ bounds,
@@ -237,8 +354,8 @@ __global__ __launch_bounds__(256) void emit_nodes_kernel(int n, const int2* __re
     float margin = 1e-5f * sqrtf(dx * dx + dy * dy + dz * dz) + 1e-30f;
     int2 c = child[i];
     int r0, r1;
-    child_ref(c.x, n, range, r0);
-    child_ref(c.y, n, range, r1);
+    child_ref(c.x, n, range, leaf_max, r0);
+    child_ref(c.y, n, range, leaf_max, r1);
     float4 l0 = box[2 * c.x], h0 = box[2 * c.x + 1], l1 = box[2 * c.y], h1 = box[2 * c.y + 1];
     l0.x -= margin; l0.y -= margin; l0.z -= margin; h0.x += margin; h0.y += margin; h0.z += margin;
     l1.x -= margin; l1.y -= margin; l1.z -= margin; h1.x += margin; h1.y += margin; h1.z += margin;
@@ -264,13 +381,14 @@ __global__ void emit_single_kernel(const float4* __restrict__ tri_box, float4* _
 
 __global__ __launch_bounds__(256) void emit_tris_kernel(const char* __restrict__ verts, int stride,
                                                         const uint32_t* __restrict__ idx,
-                                                        const uint32_t* __restrict__ sorted_tri,
+                                                        const uint32_t* __restrict__ sorted_item,
+                                                        const uint32_t* __restrict__ item_tri,
                                                         const int32_t* __restrict__ tri_geom,
                                                         const int32_t* __restrict__ geom_first,
-                                                        int64_t ntri, float4* __restrict__ tris) {
+                                                        int64_t nslot, float4* __restrict__ tris) {
     int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= ntri) return;
-    uint32_t t = sorted_tri[s];
+    if (s >= nslot) return;
+    uint32_t t = item_tri ? item_tri[sorted_item[s]] : sorted_item[s];
     int g = tri_geom[t];
     int prim = (int)t - geom_first[g];
     float w[3] = {__int_as_float((int)t), __int_as_float(prim), __int_as_float(g)};
@@ -378,12 +496,13 @@ extern "C" int wf_scene_create(const void* vertices, int64_t num_vertices, int v
         float a[3] = {0.8f, 0.8f, 0.8f}, e[3] = {0.f, 0.f, 0.f};
         WF_TRY(wf_scene_set_materials(sc, a, e, 1, stream));
     }
-    WF_HIP(hipMalloc((void**)&sc->tris, sizeof(float4) * 3 * (n > 0 ? n : 1)));
     WF_HIP(hipMalloc((void**)&sc->shade, sizeof(float4) * 3 * (n > 0 ? n : 1)));
-    sc->num_nodes = n >= 2 ? n - 1 : 1;
-    WF_HIP(hipMalloc((void**)&sc->nodes, sizeof(float4) * 4 * sc->num_nodes));
 
     if (n == 0) {
+        WF_HIP(hipMalloc((void**)&sc->tris, sizeof(float4) * 3));
+        WF_HIP(hipMalloc((void**)&sc->nodes, sizeof(float4) * 4));
+        sc->num_nodes = 1;
+        sc->num_slots = 0;
         for (int a = 0; a < 3; ++a) sc->bmin[a] = sc->bmax[a] = 0.f;
         sc->num_leaves = 0;
         *out = sc;
@@ -391,20 +510,11 @@ extern "C" int wf_scene_create(const void* vertices, int64_t num_vertices, int v
         return WF_OK;
     }
 
-    DevBuf tri_box, tri_geom, bounds, keys, vals, scratch, child, range, parent, box, flags, misc;
+    const Tuning tu = tuning();
+    DevBuf tri_box, tri_geom, bounds, misc, counts, offsets, item_box, item_tri;
     WF_TRY(dev_alloc(tri_box, sizeof(float4) * 2 * n));
     WF_TRY(dev_alloc(tri_geom, sizeof(int32_t) * n));
     WF_TRY(dev_alloc(bounds, sizeof(uint32_t) * 16));
-    WF_TRY(dev_alloc(keys, sizeof(uint64_t) * n));
-    WF_TRY(dev_alloc(vals, sizeof(uint32_t) * n));
-    int64_t sb = wf_sort_scratch_bytes(n, 8, 63, 8);
-    if (sb < 0) return (int)sb;
-    WF_TRY(dev_alloc(scratch, (size_t)sb));
-    WF_TRY(dev_alloc(child, sizeof(int2) * (n > 1 ? n - 1 : 1)));
-    WF_TRY(dev_alloc(range, sizeof(int2) * (n > 1 ? n - 1 : 1)));
-    WF_TRY(dev_alloc(parent, sizeof(int32_t) * (2 * n - 1)));
-    WF_TRY(dev_alloc(box, sizeof(float4) * 2 * (2 * n - 1)));
-    WF_TRY(dev_alloc(flags, sizeof(uint32_t) * (n > 1 ? n - 1 : 1)));
     WF_TRY(dev_alloc(misc, sizeof(uint32_t) * 4));  // [0] bad index, [1] depth, [2] leaves
 
     uint32_t* b = (uint32_t*)bounds.p;
@@ -426,39 +536,94 @@ extern "C" int wf_scene_create(const void* vertices, int64_t num_vertices, int v
         WF_HIP(hipStreamSynchronize(s));
     }
     WF_TRY(fill_u32((uint32_t*)misc.p, 4, 0u, s));
-    if (n > 1) WF_TRY(fill_u32((uint32_t*)flags.p, n - 1, 0u, s));
 
     tri_prep_kernel<<<grid_for(n, 256), 256, 0, s>>>(
         (const char*)vertices, num_vertices, vertex_stride, indices, n, sc->geom_first, num_geoms,
         tri_material, (float4*)tri_box.p, sc->shade, (int32_t*)tri_geom.p, b, (uint32_t*)misc.p);
     WF_TRY(launch_check("tri_prep_kernel"));
 
-    if (n == 1) {
+    // References: one per triangle, or pieces of the large ones.
+    int64_t m = n;
+    const float4* refs = (const float4*)tri_box.p;
+    const uint32_t* ref_tri = nullptr;
+    if (tu.bvh_presplit && tu.bvh_split_depth > 0) {
+        const float frac = 1.0f / 64.0f;
+        WF_TRY(dev_alloc(counts, sizeof(uint32_t) * n));
+        split_kernel<false><<<grid_for(n, 128), 128, 0, s>>>(
+            (const char*)vertices, vertex_stride, indices, (const float4*)tri_box.p, n, b, frac,
+            tu.bvh_split_depth, (uint32_t*)counts.p, nullptr, nullptr, nullptr);
+        WF_TRY(launch_check("split_kernel<count>"));
+        std::vector<uint32_t> hc(n);
+        WF_HIP(hipMemcpyAsync(hc.data(), counts.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+        WF_HIP(hipStreamSynchronize(s));
+        std::vector<int64_t> off(n);
+        int64_t total = 0;
+        for (int64_t t = 0; t < n; ++t) {
+            off[t] = total;
+            total += hc[t];
+        }
+        if (total > (int64_t)(1 << 28))
+            return fail(WF_ERR_TOO_LARGE, "wf_scene_create: more than 2^28 references after splitting");
+        if (total != n) {
+            m = total;
+            WF_TRY(dev_alloc(offsets, sizeof(int64_t) * n));
+            WF_TRY(dev_alloc(item_box, sizeof(float4) * 2 * m));
+            WF_TRY(dev_alloc(item_tri, sizeof(uint32_t) * m));
+            WF_HIP(hipMemcpyAsync(offsets.p, off.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, s));
+            split_kernel<true><<<grid_for(n, 128), 128, 0, s>>>(
+                (const char*)vertices, vertex_stride, indices, (const float4*)tri_box.p, n, b, frac,
+                tu.bvh_split_depth, nullptr, (const int64_t*)offsets.p, (float4*)item_box.p,
+                (uint32_t*)item_tri.p);
+            WF_TRY(launch_check("split_kernel<emit>"));
+            WF_HIP(hipStreamSynchronize(s));  // `off` leaves scope
+            refs = (const float4*)item_box.p;
+            ref_tri = (const uint32_t*)item_tri.p;
+        }
+    }
+    sc->num_slots = m;
+    sc->num_nodes = m >= 2 ? m - 1 : 1;
+    WF_HIP(hipMalloc((void**)&sc->tris, sizeof(float4) * 3 * m));
+    WF_HIP(hipMalloc((void**)&sc->nodes, sizeof(float4) * 4 * sc->num_nodes));
+
+    DevBuf keys, vals, scratch, child, range, parent, box, flags;
+    WF_TRY(dev_alloc(keys, sizeof(uint64_t) * m));
+    WF_TRY(dev_alloc(vals, sizeof(uint32_t) * m));
+    int64_t sb = wf_sort_scratch_bytes(m, 8, 63, 8);
+    if (sb < 0) return (int)sb;
+    WF_TRY(dev_alloc(scratch, (size_t)sb));
+    WF_TRY(dev_alloc(child, sizeof(int2) * (m > 1 ? m - 1 : 1)));
+    WF_TRY(dev_alloc(range, sizeof(int2) * (m > 1 ? m - 1 : 1)));
+    WF_TRY(dev_alloc(parent, sizeof(int32_t) * (2 * m - 1)));
+    WF_TRY(dev_alloc(box, sizeof(float4) * 2 * (2 * m - 1)));
+    WF_TRY(dev_alloc(flags, sizeof(uint32_t) * (m > 1 ? m - 1 : 1)));
+    if (m > 1) WF_TRY(fill_u32((uint32_t*)flags.p, m - 1, 0u, s));
+
+    if (m == 1) {
         WF_TRY(fill_u32((uint32_t*)vals.p, 1, 0u, s));
-        emit_single_kernel<<<1, 1, 0, s>>>((const float4*)tri_box.p, sc->nodes);
+        emit_single_kernel<<<1, 1, 0, s>>>(refs, sc->nodes);
         WF_TRY(launch_check("emit_single_kernel"));
         sc->num_leaves = 1;
     } else {
-        centroid_keys_kernel<<<grid_for(n, 256), 256, 0, s>>>((const float4*)tri_box.p, n, b,
-                                                              (uint64_t*)keys.p, (uint32_t*)vals.p);
+        centroid_keys_kernel<<<grid_for(m, 256), 256, 0, s>>>(refs, m, b, (uint64_t*)keys.p,
+                                                              (uint32_t*)vals.p);
         WF_TRY(launch_check("centroid_keys_kernel"));
-        WF_TRY(wf_sort_pairs(keys.p, (uint32_t*)vals.p, n, 8, 63, 8, scratch.p, sb, stream, nullptr));
-        karras_kernel<<<grid_for(n - 1, 256), 256, 0, s>>>((const uint64_t*)keys.p, (int)n,
+        WF_TRY(wf_sort_pairs(keys.p, (uint32_t*)vals.p, m, 8, 63, 8, scratch.p, sb, stream, nullptr));
+        karras_kernel<<<grid_for(m - 1, 256), 256, 0, s>>>((const uint64_t*)keys.p, (int)m,
                                                            (int2*)child.p, (int2*)range.p,
                                                            (int32_t*)parent.p);
         WF_TRY(launch_check("karras_kernel"));
-        refit_kernel<<<grid_for(n, 256), 256, 0, s>>>(
-            (const float4*)tri_box.p, (const uint32_t*)vals.p, (int)n, (const int2*)child.p,
-            (const int32_t*)parent.p, (float4*)box.p, (uint32_t*)flags.p, (uint32_t*)misc.p + 1);
+        refit_kernel<<<grid_for(m, 256), 256, 0, s>>>(
+            refs, (const uint32_t*)vals.p, (int)m, (const int2*)child.p, (const int32_t*)parent.p,
+            (float4*)box.p, (uint32_t*)flags.p, (uint32_t*)misc.p + 1);
         WF_TRY(launch_check("refit_kernel"));
-        emit_nodes_kernel<<<grid_for(n - 1, 256), 256, 0, s>>>(
-            (int)n, (const int2*)child.p, (const int2*)range.p, (const float4*)box.p, b, sc->nodes,
-            (uint32_t*)misc.p + 2);
+        emit_nodes_kernel<<<grid_for(m - 1, 256), 256, 0, s>>>(
+            (int)m, tu.bvh_leaf_max, (const int2*)child.p, (const int2*)range.p,
+            (const float4*)box.p, b, sc->nodes, (uint32_t*)misc.p + 2);
         WF_TRY(launch_check("emit_nodes_kernel"));
     }
-    emit_tris_kernel<<<grid_for(n, 256), 256, 0, s>>>(
-        (const char*)vertices, vertex_stride, indices, (const uint32_t*)vals.p,
-        (const int32_t*)tri_geom.p, sc->geom_first, n, sc->tris);
+    emit_tris_kernel<<<grid_for(m, 256), 256, 0, s>>>(
+        (const char*)vertices, vertex_stride, indices, (const uint32_t*)vals.p, ref_tri,
+        (const int32_t*)tri_geom.p, sc->geom_first, m, sc->tris);
     WF_TRY(launch_check("emit_tris_kernel"));
 
     uint32_t hb[16], hm[4];
@@ -477,7 +642,7 @@ extern "C" int wf_scene_create(const void* vertices, int64_t num_vertices, int v
         sc->bmax[a] = unkey(hb[3 + a]);
     }
     sc->max_depth = (int)hm[1];
-    if (n > 1) sc->num_leaves = hm[2];
+    if (m > 1) sc->num_leaves = hm[2];
     if (sc->max_depth >= TRACE_STACK)
         return fail(WF_ERR_TOO_LARGE, "wf_scene_create: BVH deeper than the traversal stack");
     *out = sc;
@@ -498,7 +663,7 @@ extern "C" int wf_scene_info(wf_scene scene, int64_t* num_tris, int64_t* num_nod
                              int64_t* num_leaves, int* num_geoms) {
     WF_CHECK_ARG(scene, "wf_scene_info: null scene");
     if (num_tris) *num_tris = scene->num_tris;
-    if (num_nodes) *num_nodes = scene->num_tris >= 1 ? scene->num_nodes : 0;
+    if (num_nodes) *num_nodes = scene->num_slots >= 1 ? scene->num_nodes : 0;
     if (num_leaves) *num_leaves = scene->num_leaves;
     if (num_geoms) *num_geoms = scene->num_geoms;
     return WF_OK;
@@ -511,10 +676,10 @@ extern "C" int wf_scene_download(wf_scene scene, float* nodes_host, int32_t* tri
         WF_HIP(hipMemcpy(nodes_host, scene->nodes, sizeof(float4) * 4 * scene->num_nodes,
                          hipMemcpyDeviceToHost));
     if (tri_order_host) {
-        std::vector<float4> t(3 * scene->num_tris);
-        WF_HIP(hipMemcpy(t.data(), scene->tris, sizeof(float4) * 3 * scene->num_tris,
+        std::vector<float4> t(3 * scene->num_slots);
+        WF_HIP(hipMemcpy(t.data(), scene->tris, sizeof(float4) * 3 * scene->num_slots,
                          hipMemcpyDeviceToHost));
-        for (int64_t s = 0; s < scene->num_tris; ++s) {
+        for (int64_t s = 0; s < scene->num_slots; ++s) {
             float w = t[3 * s].w;
             memcpy(&tri_order_host[s], &w, 4);
         }

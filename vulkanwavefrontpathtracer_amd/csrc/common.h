@@ -128,3 +128,15 @@ __global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v);
 int fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 
 }  // namespace wfpt
+
+namespace wfpt {
+// Process-wide tuning knobs (wf_set_tuning).
+struct Tuning {
+    int bvh_presplit = 1;
+    int bvh_split_depth = 6;
+    int bvh_leaf_max = 4;
+    int trace_lds_stack = 1;
+    int trace_block = 256;
+};
+Tuning& tuning();
+}  // namespace wfpt

@@ -65,3 +65,36 @@ extern "C" int wf_fill_f32(float* buf, int64_t count, float value, wf_stream_t s
     WF_TRY(fill_u32(reinterpret_cast<uint32_t*>(buf), count, bits, s));
     return t.finish();
 }
+
+namespace wfpt {
+Tuning& tuning() {
+    static Tuning t;
+    return t;
+}
+}  // namespace wfpt
+
+extern "C" int wf_set_tuning(const char* key, int value) {
+    WF_CHECK_ARG(key, "wf_set_tuning: null key");
+    std::string k(key);
+    Tuning& t = tuning();
+    if (k == "bvh.presplit" && (value == 0 || value == 1)) t.bvh_presplit = value;
+    else if (k == "bvh.split_depth" && value >= 0 && value <= 8) t.bvh_split_depth = value;
+    else if (k == "bvh.leaf_max" && value >= 1 && value <= 8) t.bvh_leaf_max = value;
+    else if (k == "trace.lds_stack" && (value == 0 || value == 1)) t.trace_lds_stack = value;
+    else if (k == "trace.block" && (value == 64 || value == 128 || value == 256)) t.trace_block = value;
+    else return fail(WF_ERR_INVALID_ARG, "wf_set_tuning: unknown key or value out of range: " + k);
+    return WF_OK;
+}
+
+extern "C" int wf_get_tuning(const char* key, int* value) {
+    WF_CHECK_ARG(key && value, "wf_get_tuning: null argument");
+    std::string k(key);
+    const Tuning& t = tuning();
+    if (k == "bvh.presplit") *value = t.bvh_presplit;
+    else if (k == "bvh.split_depth") *value = t.bvh_split_depth;
+    else if (k == "bvh.leaf_max") *value = t.bvh_leaf_max;
+    else if (k == "trace.lds_stack") *value = t.trace_lds_stack;
+    else if (k == "trace.block") *value = t.trace_block;
+    else return fail(WF_ERR_INVALID_ARG, "wf_get_tuning: unknown key " + k);
+    return WF_OK;
+}

@@ -1,7 +1,8 @@
 // Scene = triangles + LBVH + materials, owned by the library.
 //
 // Device layout (all 16-byte aligned):
-//   tris    [3 * num_tris] float4, in BVH leaf order ("slots"):
+//   tris    [3 * num_slots] float4, in BVH leaf order ("slots"; a split
+//           large triangle occupies several slots):
 //             {v0.xyz, original triangle index}, {v1.xyz, prim in geometry},
 //             {v2.xyz, geometry index}   (the .w fields hold int bits)
 //   nodes   [max(num_tris - 1, 1)] x 4 float4 (64 bytes):
@@ -18,6 +19,7 @@
 
 struct wf_scene_s {
     int64_t num_tris = 0;
+    int64_t num_slots = 0;  // leaf references (>= num_tris with split references)
     int64_t num_nodes = 0;
     int64_t num_leaves = 0;
     int num_geoms = 0;
@@ -36,7 +38,6 @@ struct wf_scene_s {
 
 namespace wfpt {
 
-constexpr int BVH_LEAF_MAX = 4;        // triangles per collapsed leaf (<= 8)
 constexpr int TRACE_STACK = 64;        // traversal stack entries per ray
 
 __device__ __forceinline__ int leaf_ref(int first, int count) { return ~((first << 3) | (count - 1)); }

@@ -23,13 +23,19 @@ __device__ __forceinline__ float safe_inv(float d) {
     return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d));
 }
 
-template <bool CLOSEST>
-__global__ __launch_bounds__(256) void trace_kernel(const float4* __restrict__ nodes,
-                                                    const float4* __restrict__ tris,
-                                                    const wf_ray* __restrict__ rays,
-                                                    const uint32_t* __restrict__ ray_index,
-                                                    wf_hit* __restrict__ hits, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Traversal stack: the top LDS_DEPTH entries of each lane live in LDS
+// (column per lane, so lanes of a wave hit distinct banks), deeper entries in
+// private memory.  The build guarantees depth < TRACE_STACK.
+constexpr int LDS_DEPTH = 16;
+
+template <bool CLOSEST, int BLOCK, bool LDS>
+__global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__ nodes,
+                                                      const float4* __restrict__ tris,
+                                                      const wf_ray* __restrict__ rays,
+                                                      const uint32_t* __restrict__ ray_index,
+                                                      wf_hit* __restrict__ hits, int64_t n) {
+    __shared__ int s_stack[LDS ? LDS_DEPTH * BLOCK : 1];
+    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
     const int64_t r = ray_index ? (int64_t)ray_index[i] : i;
     float4 ro, rd;
@@ -44,8 +50,18 @@ __global__ __launch_bounds__(256) void trace_kernel(const float4* __restrict__ n
     if (live) {
         const float ix = safe_inv(rd.x), iy = safe_inv(rd.y), iz = safe_inv(rd.z);
         const float oix = ro.x * ix, oiy = ro.y * iy, oiz = ro.z * iz;
-        int stack[TRACE_STACK];
+        int spill[LDS ? TRACE_STACK - LDS_DEPTH : TRACE_STACK];
         int sp = 0;
+        auto push = [&](int v) {
+            if (LDS && sp < LDS_DEPTH) s_stack[sp * BLOCK + threadIdx.x] = v;
+            else spill[sp - (LDS ? LDS_DEPTH : 0)] = v;
+            ++sp;
+        };
+        auto pop = [&]() {
+            --sp;
+            return (LDS && sp < LDS_DEPTH) ? s_stack[sp * BLOCK + threadIdx.x]
+                                           : spill[sp - (LDS ? LDS_DEPTH : 0)];
+        };
         int node = 0;
         while (true) {
             if (node >= 0) {
@@ -67,14 +83,14 @@ __global__ __launch_bounds__(256) void trace_kernel(const float4* __restrict__ n
                 if (h0 && h1) {
                     const bool second_first = n1 < n0;
                     node = second_first ? r1 : r0;
-                    stack[sp++] = second_first ? r0 : r1;
+                    push(second_first ? r0 : r1);
                 } else if (h0) {
                     node = r0;
                 } else if (h1) {
                     node = r1;
                 } else {
                     if (sp == 0) break;
-                    node = stack[--sp];
+                    node = pop();
                 }
             } else {
                 const int first = leaf_first(node), cnt = leaf_count(node);
@@ -101,7 +117,7 @@ __global__ __launch_bounds__(256) void trace_kernel(const float4* __restrict__ n
                     }
                 }
                 if (done || sp == 0) break;
-                node = stack[--sp];
+                node = pop();
             }
         }
     }
@@ -132,6 +148,27 @@ __global__ __launch_bounds__(256) void miss_kernel(const uint32_t* __restrict__ 
     *reinterpret_cast<int4*>(hits + r) = make_int4(-1, -1, __float_as_int(-1.0f), 0);
 }
 
+template <bool CLOSEST, int BLOCK>
+static void launch_trace(bool lds, const float4* nodes, const float4* tris, const wf_ray* rays,
+                         const uint32_t* ray_index, wf_hit* hits, int64_t n, hipStream_t s) {
+    unsigned g = grid_for(n, BLOCK);
+    if (lds)
+        trace_kernel<CLOSEST, BLOCK, true><<<g, BLOCK, 0, s>>>(nodes, tris, rays, ray_index, hits, n);
+    else
+        trace_kernel<CLOSEST, BLOCK, false><<<g, BLOCK, 0, s>>>(nodes, tris, rays, ray_index, hits, n);
+}
+
+template <bool CLOSEST>
+static void launch_trace_block(int block, bool lds, const float4* nodes, const float4* tris,
+                               const wf_ray* rays, const uint32_t* ray_index, wf_hit* hits,
+                               int64_t n, hipStream_t s) {
+    switch (block) {
+        case 64: launch_trace<CLOSEST, 64>(lds, nodes, tris, rays, ray_index, hits, n, s); break;
+        case 128: launch_trace<CLOSEST, 128>(lds, nodes, tris, rays, ray_index, hits, n, s); break;
+        default: launch_trace<CLOSEST, 256>(lds, nodes, tris, rays, ray_index, hits, n, s); break;
+    }
+}
+
 int trace_launch(wf_scene scene, const wf_ray* rays, const uint32_t* ray_index, wf_hit* hits,
                  int64_t n, int closest, hipStream_t s) {
     if (n == 0) return WF_OK;
@@ -139,12 +176,13 @@ int trace_launch(wf_scene scene, const wf_ray* rays, const uint32_t* ray_index, 
         miss_kernel<<<grid_for(n, 256), 256, 0, s>>>(ray_index, hits, n);
         return launch_check("miss_kernel");
     }
+    const Tuning& tu = tuning();
     if (closest)
-        trace_kernel<true><<<grid_for(n, 256), 256, 0, s>>>(scene->nodes, scene->tris, rays,
-                                                           ray_index, hits, n);
+        launch_trace_block<true>(tu.trace_block, tu.trace_lds_stack != 0, scene->nodes, scene->tris,
+                                 rays, ray_index, hits, n, s);
     else
-        trace_kernel<false><<<grid_for(n, 256), 256, 0, s>>>(scene->nodes, scene->tris, rays,
-                                                            ray_index, hits, n);
+        launch_trace_block<false>(tu.trace_block, tu.trace_lds_stack != 0, scene->nodes, scene->tris,
+                                  rays, ray_index, hits, n, s);
     return launch_check("trace_kernel");
 }
 

@@ -93,9 +93,8 @@ class Sorter:
         _check_cuda(keys, vals)
         n = keys.shape[0]
         kb = keys.element_size()
-        need = _capi.load().wf_sort_scratch_bytes(n, kb, key_bits, digit_bits)
-        if need < 0:
-            raise ValueError(_capi.load().wf_last_error().decode())
+        need = _capi.check("wf_sort_scratch_bytes",
+                           _capi.load().wf_sort_scratch_bytes(n, kb, key_bits, digit_bits))
         if self.scratch is None or self.scratch.numel() < need:
             self.scratch = torch.empty(max(need, 256), dtype=torch.uint8, device=keys.device)
         ms = _ms(timed)
