@@ -56,8 +56,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU box has one "
-              "GPU); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the wavefront loop runs on a single device and has no "
+              "multi-GPU path; run with --gpus 1", file=sys.stderr)
         return 2
 
     import numpy as np
@@ -180,7 +180,7 @@ def main():
         step(record=True)
     torch.cuda.synchronize()
     stage_ms = {k: sum(a.elapsed_time(b_) for a, b_, _ in ev[k]) / 3 for k in names}
-    # Algorithmic bytes per element (SURVEY 8d): keys 32+K+4, sort P*2*(K+4)+K,
+    # Algorithmic bytes per element: keys 32+K+4, sort P*2*(K+4)+K,
     # reorder 80, trace 32 read + 16 write.
     passes = -(-kbits // args.digit_bits)
     bpe = {"keys": 32 + kbytes + 4, "sort": passes * 2 * (kbytes + 4) + kbytes, "reorder": 80,

@@ -158,6 +158,12 @@ int wf_scene_download(wf_scene scene, float* nodes_host, int32_t* tri_order_host
 int wf_trace(wf_scene scene, const wf_ray* rays, const uint32_t* ray_index,
              wf_hit* hits, int64_t n, int closest, wf_stream_t stream, float* ms);
 
+/* Diagnostic variant of wf_trace (same results) that also writes, per ray,
+ * the number of BVH nodes visited and triangles tested (u32 arrays indexed
+ * like hits).  For BVH quality analysis; not on the rendering path. */
+int wf_trace_stats(wf_scene scene, const wf_ray* rays, wf_hit* hits, int64_t n, int closest,
+                   uint32_t* node_visits, uint32_t* tri_tests, wf_stream_t stream);
+
 /* -------------------------------------------------------------- ray buffer */
 /* Mirror of RayBuffer (Ray/RayBuffer.h:22-70): double-buffered rays and LUTs.
  * All pointers are device pointers owned by the caller (the C++ host class
@@ -248,9 +254,9 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
 /* Process-wide knobs read by later calls (the autotuner sets them):
  *   "bvh.presplit"     0/1  split references of large triangles (default 1)
  *   "bvh.split_depth"  0-8  max binary splits per triangle (default 6)
- *   "bvh.leaf_max"     1-8  triangles per collapsed leaf (default 4)
+ *   "bvh.leaf_max"     1-8  triangles per collapsed leaf (default 8)
  *   "trace.lds_stack"  0/1  keep the top of the traversal stack in LDS (default 1)
- *   "trace.block"      64/128/256  threads per traversal workgroup (default 256)
+ *   "trace.block"      64/128/256  threads per traversal workgroup (default 64)
  * Returns WF_ERR_INVALID_ARG for an unknown key or value out of range. */
 int wf_set_tuning(const char* key, int value);
 int wf_get_tuning(const char* key, int* value);

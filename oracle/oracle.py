@@ -268,10 +268,12 @@ class FrameCfg(C.Structure):
 
 def render_frame_f64(tri_verts, tri_n, tri_mat, albedo, emission, *, w, h, spp, depth, S, frame,
                      light, bg, origin, radius, max_dist, s2w_colmajor, rr=False, rr_start=3):
-    """Frame 1 of Renderer::renderPath in fp64.  Returns ([w*h, 4] float32 linear
-    RGBA, [w*h] bool ambiguous): a pixel is ambiguous when some vertex of its
-    paths grazes a triangle edge, ties two hits or sits on an interval end
-    (within 1e-5), where an fp32 pipeline may follow another path."""
+    """Frame 1 of Renderer::renderPath, primary rays in fp32 (as raygenPrimary
+    makes them) and the paths followed in fp64.  Returns ([w*h, 4] float32
+    linear RGBA, [w*h] bool ambiguous): a pixel is ambiguous when some vertex
+    of its paths grazes a triangle edge, ties two hits or sits on an interval
+    end within the fp32 error bound propagated along the path, where an fp32
+    pipeline may follow another path."""
     cfg = FrameCfg(w, h, spp, depth, S, frame, int(rr), rr_start, (C.c_float * 3)(*light),
                    (C.c_float * 3)(*bg), (C.c_float * 3)(*origin), radius, max_dist,
                    (C.c_float * 16)(*np.asarray(s2w_colmajor, np.float32).reshape(16)))

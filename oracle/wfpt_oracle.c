@@ -375,23 +375,28 @@ static float halton(int i, int base) {
 OR_API float or_halton(int i, int base) { return halton(i, base); }
 
 /* raygenPrimary.comp:57-84; s2w column-major */
+static void primary_dir_f32(const float* m, const float* origin, int pixel, int w, int h, float hx,
+                            float hy, float d[3]) {
+    float x = (float)(pixel % w) + hx, y = (float)(pixel / w) + hy;
+    float nx = 2.0f * x / (float)w - 1.0f, ny = 2.0f * y / (float)h - 1.0f;
+    float wx = ((m[0] * nx + m[4] * ny) + m[8] * 0.0f) + m[12] * 1.0f;
+    float wy = ((m[1] * nx + m[5] * ny) + m[9] * 0.0f) + m[13] * 1.0f;
+    float wz = ((m[2] * nx + m[6] * ny) + m[10] * 0.0f) + m[14] * 1.0f;
+    float ww = ((m[3] * nx + m[7] * ny) + m[11] * 0.0f) + m[15] * 1.0f;
+    float px = wx / ww, py = wy / ww, pz = wz / ww;
+    float dx = px - origin[0], dy = py - origin[1], dz = pz - origin[2];
+    float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
+    d[0] = dx * inv; d[1] = dy * inv; d[2] = dz * inv;
+}
+
 OR_API void or_raygen_primary(const float* m, const float* origin, const int32_t* i2p, int w, int h,
                               int sample_index, float max_dist, float* rays) {
     float hx = halton(sample_index, 2), hy = halton(sample_index, 3);
     for (int64_t i = 0; i < (int64_t)w * h; ++i) {
-        int pixel = i2p[i];
-        float x = (float)(pixel % w) + hx, y = (float)(pixel / w) + hy;
-        float nx = 2.0f * x / (float)w - 1.0f, ny = 2.0f * y / (float)h - 1.0f;
-        float wx = ((m[0] * nx + m[4] * ny) + m[8] * 0.0f) + m[12] * 1.0f;
-        float wy = ((m[1] * nx + m[5] * ny) + m[9] * 0.0f) + m[13] * 1.0f;
-        float wz = ((m[2] * nx + m[6] * ny) + m[10] * 0.0f) + m[14] * 1.0f;
-        float ww = ((m[3] * nx + m[7] * ny) + m[11] * 0.0f) + m[15] * 1.0f;
-        float px = wx / ww, py = wy / ww, pz = wz / ww;
-        float dx = px - origin[0], dy = py - origin[1], dz = pz - origin[2];
-        float inv = 1.0f / sqrtf(dx * dx + dy * dy + dz * dz);
         float* r = rays + 8 * i;
+        primary_dir_f32(m, origin, i2p[i], w, h, hx, hy, r + 4);
         r[0] = origin[0]; r[1] = origin[1]; r[2] = origin[2]; r[3] = 0.0f;
-        r[4] = dx * inv; r[5] = dy * inv; r[6] = dz * inv; r[7] = max_dist;
+        r[7] = max_dist;
     }
 }
 
@@ -719,17 +724,30 @@ static double mt_hit(const float* tv, const double o[3], const double d[3], doub
 }
 
 /* Ambiguity of an fp64 query, used to recognise path vertices where an fp32
- * pipeline may legitimately decide differently: a triangle passed within
- * EPS_BARY (barycentric units) of one of its edges before the accepted hit,
- * two hits within EPS_T relative of each other (the tie rule), or a hit
- * within EPS_T of either end of the interval. */
+ * pipeline may legitimately decide differently.  The fp32 ray is the fp64 ray
+ * moved sideways by at most lat(t) = tol0 + tol_eps * t world units at
+ * distance t (origin error plus direction error), and the fp32 intersection
+ * itself is exact to about EPS_BARY barycentric units.  The query is
+ * ambiguous when
+ *   - the ray crosses a triangle's plane, before the accepted hit, within
+ *     EPS_BARY barycentric units or lat(t) / cos world units (the crossing
+ *     point slides by lat / cos on a plane seen at incidence cos) of one of
+ *     its edges;
+ *   - two hits lie within max(EPS_T * t, lat(t)) of each other (the tie rule);
+ *   - a hit lies that close to either end of the interval. */
 #define EPS_BARY 1e-5
 #define EPS_T 1e-5
 
-/* u, v, t of the ray's line against the triangle's plane (no inside test);
- * returns 0 for a parallel ray. */
-static int mt_line(const float* tv, const double o[3], const double d[3], double* u_, double* v_,
-                   double* t_) {
+typedef struct {
+    double u, v, t;
+    double cos;    /* |cos| of the angle between the ray and the plane's normal */
+    double edge;   /* signed distance (world units, in the plane) to the nearest edge
+                      line: >= 0 inside the triangle */
+} line_hit;
+
+/* The ray's line against the triangle's plane, without inside test; returns
+ * 0 for a parallel ray. */
+static int mt_line(const float* tv, const double o[3], const double d[3], line_hit* h) {
     double v0[3] = {tv[0], tv[1], tv[2]};
     double e1[3] = {tv[3] - v0[0], tv[4] - v0[1], tv[5] - v0[2]};
     double e2[3] = {tv[6] - v0[0], tv[7] - v0[1], tv[8] - v0[2]};
@@ -739,32 +757,56 @@ static int mt_line(const float* tv, const double o[3], const double d[3], double
     double inv = 1.0 / det;
     double s[3] = {o[0] - v0[0], o[1] - v0[1], o[2] - v0[2]};
     double q[3] = {s[1] * e1[2] - s[2] * e1[1], s[2] * e1[0] - s[0] * e1[2], s[0] * e1[1] - s[1] * e1[0]};
-    *u_ = (s[0] * p[0] + s[1] * p[1] + s[2] * p[2]) * inv;
-    *v_ = (d[0] * q[0] + d[1] * q[1] + d[2] * q[2]) * inv;
-    *t_ = (e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2]) * inv;
+    h->u = (s[0] * p[0] + s[1] * p[1] + s[2] * p[2]) * inv;
+    h->v = (d[0] * q[0] + d[1] * q[1] + d[2] * q[2]) * inv;
+    h->t = (e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2]) * inv;
+    double nx = e1[1] * e2[2] - e1[2] * e2[1], ny = e1[2] * e2[0] - e1[0] * e2[2],
+           nz = e1[0] * e2[1] - e1[1] * e2[0];
+    double area2 = sqrt(nx * nx + ny * ny + nz * nz);
+    double dl = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    h->cos = fabs(det) / (area2 * dl);
+    /* barycentric coordinate b_i times the height of vertex i over the
+     * opposite edge (= 2A / |edge|) is the distance to that edge's line */
+    double e3[3] = {e2[0] - e1[0], e2[1] - e1[1], e2[2] - e1[2]};
+    double l1 = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+    double l2 = sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+    double l3 = sqrt(e3[0] * e3[0] + e3[1] * e3[1] + e3[2] * e3[2]);
+    double w = 1.0 - h->u - h->v;
+    h->edge = fmin(fmin(w * area2 / l3, h->u * area2 / l2), h->v * area2 / l1);
     return 1;
 }
 
+typedef struct {
+    double tol0, tol_eps;   /* lateral uncertainty of the fp32 ray: tol0 + tol_eps * t */
+} ray_tol;
+
 /* closest (any = 0) or any hit in [tmin, tmax]; returns triangle or -1.
  * If amb != NULL, *amb is set when the answer is ambiguous in the sense
- * above (boxes are then expanded so that near misses are examined). */
+ * above (boxes are then expanded so that near misses are examined) and
+ * *cos_out receives the incidence of the accepted hit. */
 static int cb_trace(const cbvh* b, const double o[3], const double d[3], double tmin, double tmax,
-                    int any, double* t_out, double* u_out, double* v_out, int* amb) {
+                    int any, double* t_out, double* u_out, double* v_out, int* amb,
+                    const ray_tol* tl, double* cos_out) {
     if (!(tmax >= tmin)) return -1;
     int stack[128], sp = 0, node = 0, best = -1;
-    double best_t = tmax, bu = 0, bv = 0;
+    double best_t = tmax, bu = 0, bv = 0, bcos = 1;
     double near_t = INFINITY, second_t = INFINITY;
     int certain_any = 0;
     double inv[3];
+    const double dl = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double t0_ = tl ? tl->tol0 : 0.0, te_ = tl ? tl->tol_eps : 0.0;
+    /* lateral uncertainty (world units) at parameter t, and the same in t units */
+#define LAT(t) (t0_ + te_ * fabs(t) * dl)
+#define TTOL(t) fmax(EPS_T * fmax(fabs(t), 1.0), LAT(t) / dl)
     for (int a = 0; a < 3; ++a) inv[a] = 1.0 / (fabs(d[a]) > 1e-300 ? d[a] : copysign(1e-300, d[a]));
     stack[sp++] = 0;
     while (sp) {
         node = stack[--sp];
         const bnode* nd = &b->nodes[node];
-        double tn = tmin, tf = amb ? best_t * (1 + 4 * EPS_T) + 1e-12 : best_t;
+        double tn = tmin, tf = amb ? best_t + 4 * TTOL(best_t) + 1e-12 : best_t;
         int miss = 0;
         for (int a = 0; a < 3; ++a) {
-            double grow = amb ? 1e-4 * (nd->hi[a] - nd->lo[a]) + 1e-9 : 0.0;
+            double grow = amb ? 1e-4 * (nd->hi[a] - nd->lo[a]) + 1e-9 + 2 * LAT(tf < 1e30 ? tf : 0) : 0.0;
             double t0 = (nd->lo[a] - grow - o[a]) * inv[a], t1 = (nd->hi[a] + grow - o[a]) * inv[a];
             double lo = fmin(t0, t1), hi = fmax(t0, t1);
             tn = fmax(tn, lo - 1e-9 * fabs(lo) - 1e-12);
@@ -775,15 +817,23 @@ static int cb_trace(const cbvh* b, const double o[3], const double d[3], double 
         if (nd->left < 0) {
             for (int k = nd->first; k < nd->first + nd->count; ++k) {
                 int t = b->order[k];
-                double u, v, th;
-                if (!mt_line(b->tv + 9 * t, o, d, &u, &v, &th)) continue;
+                line_hit lh;
+                if (!mt_line(b->tv + 9 * t, o, d, &lh)) continue;
+                const double u = lh.u, v = lh.v, th = lh.t;
                 double margin = fmin(fmin(u, v), 1.0 - u - v);
                 int inside = u >= 0.0 && u <= 1.0 && v >= 0.0 && u + v <= 1.0;
                 if (amb) {
-                    double tol = EPS_T * fmax(fabs(th), 1.0);
-                    if (fabs(margin) < EPS_BARY && th >= tmin - tol && th <= tmax + tol)
+                    double tol = TTOL(th);
+                    double lat = LAT(th) / fmax(lh.cos, 1e-3);
+                    /* the fp32 origin also sits a few ulps of its coordinates
+                     * away, seen along the ray through this plane */
+                    double mag = fmax(fmax(fabs(o[0]), fabs(o[1])), fabs(o[2])) + 1.0;
+                    double tol_end = fmax(EPS_T * fabs(th), 1.2e-7 * mag / fmax(lh.cos, 1e-4) / dl);
+                    if ((fabs(margin) < EPS_BARY || fabs(lh.edge) < lat) && th >= tmin - tol &&
+                        th <= tmax + tol)
                         near_t = fmin(near_t, th);
-                    if (inside && (fabs(th - tmin) <= tol || (isfinite(tmax) && tmax < 1e30 && fabs(th - tmax) <= tol)))
+                    if (inside && (fabs(th - tmin) <= tol_end ||
+                                   (isfinite(tmax) && tmax < 1e30 && fabs(th - tmax) <= tol_end)))
                         near_t = fmin(near_t, th);
                 }
                 if (!inside || !(th >= tmin && th <= best_t)) {
@@ -792,9 +842,12 @@ static int cb_trace(const cbvh* b, const double o[3], const double d[3], double 
                 }
                 if (th == best_t && best >= 0 && t > best) { second_t = fmin(second_t, th); continue; }
                 if (best >= 0) second_t = fmin(second_t, best_t);
-                best = t; best_t = th; bu = u; bv = v;
+                best = t; best_t = th; bu = u; bv = v; bcos = lh.cos;
                 if (any) {
-                    if (!amb || margin >= EPS_BARY) { certain_any = 1; goto done; }
+                    if (!amb || (margin >= EPS_BARY && lh.edge >= LAT(th) / fmax(lh.cos, 1e-3))) {
+                        certain_any = 1;
+                        goto done;
+                    }
                 }
             }
         } else {
@@ -804,7 +857,7 @@ static int cb_trace(const cbvh* b, const double o[3], const double d[3], double 
     }
 done:
     if (amb) {
-        double tol = EPS_T * fmax(fabs(best_t), 1.0);
+        double tol = TTOL(best_t);
         if (any) {
             if (!certain_any && near_t <= tmax + tol) *amb = 1;
         } else {
@@ -812,7 +865,10 @@ done:
             if (best >= 0 && second_t <= best_t + tol) *amb = 1;
         }
     }
+#undef LAT
+#undef TTOL
     if (best >= 0) { *t_out = best_t; *u_out = bu; *v_out = bv; }
+    if (cos_out) *cos_out = bcos;
     return best;
 }
 
@@ -837,9 +893,39 @@ static int d_normal(const float* tri_n, const int32_t* tri_mat, int t, double u,
     return tri_mat[t];
 }
 
+/* Error model of the fp32 pipeline: a position rounded to fp32 after a few
+ * operations is within FP32_POS * |p| of the exact one, and a unit direction
+ * built from a few fp32 operations (sin/cos of 2*pi*r, sqrt, normalize) is
+ * within FP32_DIR radians of the exact one. */
+#define FP32_POS (4.0 * 5.960464477539063e-08)
+#define FP32_DIR 2e-6
+
+/* lateral uncertainty of an fp32 hit point: the ray's own uncertainty at
+ * distance t, slid along the surface at incidence cos, plus rounding of
+ * the point */
+static double hit_uncertainty(const ray_tol* tl, const double o[3], const double d[3], double t,
+                              double cs) {
+    double dl = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    double p[3] = {o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t};
+    double mag = fabs(p[0]) + fabs(p[1]) + fabs(p[2]) + fabs(t) * dl;
+    return (tl->tol0 + tl->tol_eps * fabs(t) * dl) / fmax(cs, 1e-3) + FP32_POS * mag;
+}
+
+/* direction uncertainty of a bounce: fp32 arithmetic plus the 16-bit
+ * barycentrics (packUnorm2x16) that interpolate the shading normal */
+static double bounce_dir_uncertainty(const float* tri_n, int t) {
+    const float* a = tri_n + 9 * (int64_t)t;
+    double var = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = i + 1; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) var = fmax(var, fabs((double)a[3 * i + k] - a[3 * j + k]));
+    return FP32_DIR + 2.0 * var / 65535.0;
+}
+
 /* visibility fraction of the S shadow rays from the hit of (o, d, t) */
 static double d_shadow(const cbvh* b, const or_frame_cfg* c, int hit, const double o[3],
-                       const double d[3], double t, uint32_t seed, int pixel, int* amb) {
+                       const double d[3], double t, uint32_t seed, int pixel, int* amb,
+                       double lat_hit) {
     if (hit < 0) return 1.0;
     double adv = fmax(t - 1.0e-5, 0.0);
     double p[3] = {o[0] + d[0] * adv, o[1] + d[1] * adv, o[2] + d[2] * adv};
@@ -852,7 +938,11 @@ static double d_shadow(const cbvh* b, const or_frame_cfg* c, int hit, const doub
         double len = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
         for (int k = 0; k < 3; ++k) dir[k] /= len;
         double tt, uu, vv;
-        if (cb_trace(b, p, dir, 1.0e-5, len, 1, &tt, &uu, &vv, amb) < 0) ++vis;
+        /* the fp32 shadow ray runs from a point lat_hit away to a light
+         * sample rounded to fp32 */
+        double lmag = fabs(c->light[0]) + fabs(c->light[1]) + fabs(c->light[2]) + c->radius;
+        ray_tol tl = {lat_hit + FP32_POS * lmag, 2 * FP32_DIR};
+        if (cb_trace(b, p, dir, 1.0e-5, len, 1, &tt, &uu, &vv, amb, &tl, NULL) < 0) ++vis;
     }
     return (double)vis / c->S;
 }
@@ -894,20 +984,21 @@ OR_API void or_render_frame_f64(const float* tv, const float* tri_n, const int32
         for (int pass = 0; pass < c->spp; ++pass) {
             int si = pass + c->spp * (c->frame - 1);
             float hx = halton(si, 2), hy = halton(si, 3);
-            double x = (double)(pixel % c->w) + hx, y = (double)(pixel / c->w) + hy;
-            double nx = 2.0 * x / c->w - 1.0, ny = 2.0 * y / c->h - 1.0;
-            const float* m = c->s2w;
-            double wx = m[0] * nx + m[4] * ny + m[12], wy = m[1] * nx + m[5] * ny + m[13];
-            double wz = m[2] * nx + m[6] * ny + m[14], ww = m[3] * nx + m[7] * ny + m[15];
+            /* primary rays are the fp32 ones of raygenPrimary (their near-plane
+             * construction loses ~1e-5 rad to cancellation; the stage is
+             * checked bit for bit on its own): the path is then followed in fp64 */
+            float df[3];
+            primary_dir_f32(c->s2w, c->origin, (int)pixel, c->w, c->h, hx, hy, df);
             double o[3] = {c->origin[0], c->origin[1], c->origin[2]};
-            double d[3] = {wx / ww - o[0], wy / ww - o[1], wz / ww - o[2]};
-            dnorm(d);
+            double d[3] = {df[0], df[1], df[2]};
             double dec[3] = {1, 1, 1};
-            double t = 0, u = 0, v = 0;
-            int hit = ntri > 0 ? cb_trace(&b, o, d, 0.0, c->max_dist, 0, &t, &u, &v, &amb) : -1;
+            double t = 0, u = 0, v = 0, cs = 1;
+            ray_tol tl = {0.0, FP32_POS};
+            int hit = ntri > 0 ? cb_trace(&b, o, d, 0.0, c->max_dist, 0, &t, &u, &v, &amb, &tl, &cs) : -1;
+            double lat = hit >= 0 ? hit_uncertainty(&tl, o, d, t, cs) : 0.0;
             double aux[3];
             d_smooth(&b, c, tri_n, tri_mat, emission, nmat, hit, o, d, t, u, v, dec, aux);
-            double vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel, &amb);
+            double vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel, &amb, lat);
             for (int k = 0; k < 3; ++k) acc[k] += aux[k] * vis;
             for (int bounce = 0; bounce < c->depth; ++bounce) {
                 if (hit < 0) break;
@@ -934,9 +1025,12 @@ OR_API void or_render_frame_f64(const float* tv, const float* tri_n, const int32
                     for (int k = 0; k < 3; ++k) w[k] /= q;
                 }
                 for (int k = 0; k < 3; ++k) { dec[k] *= w[k]; o[k] = p[k] + nd[k] * 1.0e-4; d[k] = nd[k]; }
-                hit = cb_trace(&b, o, d, 1.0e-4, 3.402823466e+38, 0, &t, &u, &v, &amb);
+                tl.tol0 = lat;
+                tl.tol_eps = bounce_dir_uncertainty(tri_n, hit);
+                hit = cb_trace(&b, o, d, 1.0e-4, 3.402823466e+38, 0, &t, &u, &v, &amb, &tl, &cs);
+                lat = hit >= 0 ? hit_uncertainty(&tl, o, d, t, cs) : 0.0;
                 d_smooth(&b, c, tri_n, tri_mat, emission, nmat, hit, o, d, t, u, v, dec, aux);
-                vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel, &amb);
+                vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel, &amb, lat);
                 for (int k = 0; k < 3; ++k) acc[k] += aux[k] * vis;
             }
         }
@@ -964,7 +1058,7 @@ OR_API void or_trace_bvh_f64(const float* rays, int64_t n, const float* tri_vert
         const float* r = rays + 8 * i;
         double o[3] = {r[0], r[1], r[2]}, d[3] = {r[4], r[5], r[6]};
         double t = -1, u, v;
-        int hit = ntri > 0 ? cb_trace(&b, o, d, r[3], r[7], !closest, &t, &u, &v, NULL) : -1;
+        int hit = ntri > 0 ? cb_trace(&b, o, d, r[3], r[7], !closest, &t, &u, &v, NULL, NULL, NULL) : -1;
         out_t[i] = hit >= 0 ? t : -1.0;
         out_tri[i] = hit;
     }

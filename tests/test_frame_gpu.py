@@ -1,6 +1,6 @@
 """GPU parity of ray generation, shade/extend and reconstruction against the
 oracle, and of whole frames against the fp64 CPU oracle under common random
-numbers (image RMSE <= 1e-3, the north-star tolerance)."""
+numbers (image RMSE <= 1e-3)."""
 import json
 
 import numpy as np
@@ -235,16 +235,18 @@ def frame_rmse(gpu, ref, amb):
 ])
 def test_frame_rmse_vs_fp64(dev, orc, env, overrides):
     """Image RMSE <= 1e-3 against the fp64 oracle under common random numbers,
-    over the pixels whose paths the fp64 oracle finds unambiguous (no vertex
-    within 1e-5 of an edge, tie or interval end, the tie rule applied along the
-    path); the ambiguous pixels must stay rare."""
+    over the pixels whose paths pass no decision (edge, tie, interval end,
+    roulette) within the fp32 error bound the oracle propagates along the
+    path; those flagged pixels, where the two may legitimately part (a
+    primary ray along the emitter's outline), must stay under 1%, and every
+    other pixel must agree to 1e-2 on its own."""
     flat = dict(overrides)
     gpu = render(flat, env).pixels()
     ref, amb = oracle_frame(orc, env, flat)
     rmse_clean, rmse_all, amb_frac, max_clean = frame_rmse(gpu, ref, amb)
-    print(f"RMSE unambiguous {rmse_clean:.3e} (max {max_clean:.3e}), all pixels {rmse_all:.3e}, "
-          f"ambiguous {100 * amb_frac:.3f}%")
-    assert rmse_clean <= RMSE_TOL, (rmse_clean, rmse_all, amb_frac)
+    print(f"RMSE all pixels {rmse_all:.3e}, unflagged {rmse_clean:.3e} (max {max_clean:.3e}), "
+          f"flagged {100 * amb_frac:.3f}%")
+    assert rmse_clean <= RMSE_TOL and max_clean <= 1e-2, (rmse_clean, max_clean, rmse_all)
     assert amb_frac < 0.01
     assert float(np.mean(ref[:, :3])) > 0.05      # a lit image, not a trivial match
 
@@ -258,3 +260,76 @@ def test_frame_stats_counts(dev):
     sm = st["stage_ms"]
     assert sm["keys"] > 0 and sm["sort"] > 0 and sm["reorder"] > 0 and sm["trace_sorted"] > 0
     assert sm["trace_unsorted"] > 0
+
+
+def compose_frame(dev, orc, hs, sc, cfg, W, H, use_gpu=True):
+    """Renderer::renderPath built from the individual stage calls (GPU) or
+    from the fp32 oracle stages (CPU); returns (frame, per-bounce counts)."""
+    from frame_compose import compose_frame_cpu
+    from vulkanwavefrontpathtracer_amd import stages
+
+    vec = lambda s: [float(x) for x in s.split()]  # noqa: E731
+    cam, R = cfg["Camera"], cfg["Renderer"]
+    _, _, S2W = stages.camera_matrices(vec(cam["position"]), vec(cam["rotation"]),
+                                       cam["fieldOfView"], W / H, cam["nearPlane"], cam["farPlane"])
+    light, bg = vec(cfg["Scene"]["light"]), vec(cfg["Scene"]["backgroundcolor"])
+    S, radius, depth = R["numberOfShadowSamples"], R["shadowRadius"], R["recursionDepth"]
+    i2p, _ = stages.pixel_table(W, H)
+    npx = W * H
+    tn = orc.tri_normals(hs.vertices, hs.indices)
+    counts = []
+    if use_gpu:
+        T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+        s2i = T(i2p)
+        rays = stages.raygen_primary(S2W, vec(cam["position"]), s2i, W, H, 0, cam["farPlane"])
+        hits = stages.trace(sc, rays)
+        seeds = stages.init_seeds(npx, 1, dev)
+        dec = torch.ones((npx, 4), dtype=torch.float32, device=dev)
+        aux = torch.zeros((npx, 4), dtype=torch.float32, device=dev)
+        fp = torch.zeros((npx, 4), dtype=torch.float32, device=dev)
+        stages.reconstruct_smooth(sc, rays, hits, s2i, aux, dec, light, bg, 1)
+
+        def shadow(rays, hits, s2i):
+            n = rays.shape[0]
+            sr, ss, si = stages.raygen_shadow(seeds, rays, hits, s2i, 0, n, S, light, radius)
+            sh = stages.trace(sc, sr, closest=False)
+            stages.reconstruct_shadow(sh, s2i, si, aux, fp, 0, n, S, 0)
+
+        shadow(rays, hits, s2i)
+        for _ in range(depth):
+            aux.zero_()
+            out, os2i, k = stages.shade_extend(sc, seeds, rays, hits, s2i, dec)
+            counts.append(k)
+            if k == 0:
+                break
+            oh = stages.trace(sc, out)
+            stages.reconstruct_smooth(sc, out, oh, os2i, aux, dec, light, bg, 1)
+            shadow(out, oh, os2i)
+            rays, hits, s2i = out, oh, os2i
+        torch.cuda.synchronize()
+        return fp.cpu().numpy(), counts
+
+    return compose_frame_cpu(orc, hs, S2W, cfg, W, H)
+
+
+def test_frame_loop_matches_stage_composition(dev, orc, boxes):
+    """The C++ frame loop equals the same stages called one by one (bitwise),
+    and the GPU stage composition equals the fp32 oracle composition."""
+    from vulkanwavefrontpathtracer_amd.renderer import env_path
+
+    hs, sc = boxes
+    W = H = 64
+    cfg = json.loads(env_path("config0.json").read_text())
+    cfg["Resolution"] = {"width": W, "height": H}
+    r = render({"Resolution.width": W, "Resolution.height": H, "Renderer.sortPathRays": False})
+    loop = r.pixels()
+    gpu, gcounts = compose_frame(dev, orc, hs, sc, cfg, W, H, use_gpu=True)
+    cpu, ccounts = compose_frame(dev, orc, hs, sc, cfg, W, H, use_gpu=False)
+    d_loop = np.abs(loop - gpu).max()
+    d_cpu = np.abs(gpu - cpu).max()
+    st = r.stats()
+    print(f"counts gpu {gcounts} cpu {ccounts}; renderer path rays {st['path_rays']}; "
+          f"|loop-gpu| {d_loop:.3e} |gpu-cpu| {d_cpu:.3e}")
+    assert gcounts == ccounts
+    assert d_cpu < 1e-3
+    assert np.array_equal(loop, gpu)

@@ -134,9 +134,9 @@ namespace wfpt {
 struct Tuning {
     int bvh_presplit = 1;
     int bvh_split_depth = 6;
-    int bvh_leaf_max = 4;
+    int bvh_leaf_max = 8;
     int trace_lds_stack = 1;
-    int trace_block = 256;
+    int trace_block = 64;
 };
 Tuning& tuning();
 }  // namespace wfpt

@@ -28,12 +28,15 @@ __device__ __forceinline__ float safe_inv(float d) {
 // private memory.  The build guarantees depth < TRACE_STACK.
 constexpr int LDS_DEPTH = 16;
 
-template <bool CLOSEST, int BLOCK, bool LDS>
+template <bool CLOSEST, int BLOCK, bool LDS, bool STATS = false>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__ nodes,
                                                       const float4* __restrict__ tris,
                                                       const wf_ray* __restrict__ rays,
                                                       const uint32_t* __restrict__ ray_index,
-                                                      wf_hit* __restrict__ hits, int64_t n) {
+                                                      wf_hit* __restrict__ hits, int64_t n,
+                                                      uint32_t* __restrict__ stat_nodes = nullptr,
+                                                      uint32_t* __restrict__ stat_tris = nullptr) {
+    uint32_t n_nodes = 0, n_tris = 0;
     __shared__ int s_stack[LDS ? LDS_DEPTH * BLOCK : 1];
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
@@ -65,6 +68,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__
         int node = 0;
         while (true) {
             if (node >= 0) {
+                if (STATS) ++n_nodes;
                 const float4* nd = nodes + 4 * (int64_t)node;
                 const float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3];
                 // child 0: lo (a.x a.y a.z) hi (a.w b.x b.y); child 1: lo (b.z b.w c.x) hi (c.y c.z c.w)
@@ -98,6 +102,7 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__
                 for (int k = 0; k < cnt; ++k) {
                     const float4* tp = tris + 3 * (int64_t)(first + k);
                     const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
+                    if (STATS) ++n_tris;
                     WtHit h;
                     if (!wt_intersect(wr, v0, v1, v2, h)) continue;
                     if (!(h.t >= tmin && h.t <= best_t)) continue;
@@ -138,6 +143,10 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__
         out.bary = 0u;
     }
     *reinterpret_cast<int4*>(hits + r) = make_int4(out.prim, out.geom, __float_as_int(out.t), (int)out.bary);
+    if (STATS) {
+        stat_nodes[r] = n_nodes;
+        stat_tris[r] = n_tris;
+    }
 }
 
 __global__ __launch_bounds__(256) void miss_kernel(const uint32_t* __restrict__ ray_index,
@@ -189,6 +198,23 @@ int trace_launch(wf_scene scene, const wf_ray* rays, const uint32_t* ray_index, 
 }  // namespace wfpt
 
 using namespace wfpt;
+
+extern "C" int wf_trace_stats(wf_scene scene, const wf_ray* rays, wf_hit* hits, int64_t n,
+                              int closest, uint32_t* node_visits, uint32_t* tri_tests,
+                              wf_stream_t stream) {
+    WF_CHECK_ARG(scene && n >= 0, "wf_trace_stats: bad scene or n");
+    if (n == 0) return WF_OK;
+    WF_CHECK_ARG(rays && hits && node_visits && tri_tests, "wf_trace_stats: null buffer");
+    WF_CHECK_ARG(scene->num_tris > 0, "wf_trace_stats: empty scene");
+    hipStream_t s = as_stream(stream);
+    if (closest)
+        trace_kernel<true, 64, true, true><<<grid_for(n, 64), 64, 0, s>>>(
+            scene->nodes, scene->tris, rays, nullptr, hits, n, node_visits, tri_tests);
+    else
+        trace_kernel<false, 64, true, true><<<grid_for(n, 64), 64, 0, s>>>(
+            scene->nodes, scene->tris, rays, nullptr, hits, n, node_visits, tri_tests);
+    return launch_check("trace_kernel<stats>");
+}
 
 extern "C" int wf_trace(wf_scene scene, const wf_ray* rays, const uint32_t* ray_index,
                         wf_hit* hits, int64_t n, int closest, wf_stream_t stream, float* ms) {
