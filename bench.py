@@ -54,6 +54,8 @@ def main():
     ap.add_argument("--frames", type=int, default=3, help="reference-style full frames timed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run one plain device copy of MIB MiB (scales HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the wavefront loop runs on a single device and has no "
@@ -68,6 +70,15 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        nb = args.copy_calibration << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {nb} bytes read + {nb} bytes written")
+        del src, dst
     env = env_path(args.env)
     cfg = json.loads(env.read_text())
     log(f"building scene for {args.env}")
@@ -201,6 +212,8 @@ def main():
     launch_bytes = bounces[big]["n"] * bpe[dominant]
     achieved = launch_bytes / (avg_launch / 1e3) / 1e9
 
+    traffic = measured_traffic(dominant, bounces[big]["n"])
+
     # Unsorted trace of the same wavefronts (sort ablation).
     ts = []
     for _ in range(3):
@@ -231,15 +244,17 @@ def main():
         r.render()
         frame_walls.append(time.perf_counter() - t1)
         fstats.append(r.stats())
-    fs = fstats[-1]
-    path_rays = fs["path_rays"]
-    ref_style = {
-        "path_rays": path_rays,
-        "path_mrays_s": round(path_rays / (fs["path_ms"] / 1e3) / 1e6, 1) if fs["path_ms"] > 0 else None,
-        "frame_ms_wall": round(1e3 * sum(frame_walls) / len(frame_walls), 2),
-        "rays_per_frame": fs["primary_rays"] + fs["shadow_rays"] + path_rays,
-        "stage_ms": {k: round(v, 3) for k, v in fs["stage_ms"].items()},
-    }
+    ref_style = None
+    if fstats:
+        fs = fstats[-1]
+        path_rays = fs["path_rays"]
+        ref_style = {
+            "path_rays": path_rays,
+            "path_mrays_s": round(path_rays / (fs["path_ms"] / 1e3) / 1e6, 1) if fs["path_ms"] > 0 else None,
+            "frame_ms_wall": round(1e3 * sum(frame_walls) / len(frame_walls), 2),
+            "rays_per_frame": fs["primary_rays"] + fs["shadow_rays"] + path_rays,
+            "stage_ms": {k: round(v, 3) for k, v in fs["stage_ms"].items()},
+        }
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -267,7 +282,7 @@ def main():
         },
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "launch_ms": round(avg_launch, 4),
+                     "traffic": traffic, "launch_ms": round(avg_launch, 4),
                      "launch_elems": bounces[big]["n"], "bytes_per_elem": bpe[dominant]},
         "stages": stages_roof,
         "sorted_pipeline_mrays_s": round(sorted_mrays, 1),
@@ -281,6 +296,24 @@ def main():
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_interior_1080p.txt"
+STAGE_KERNEL = {"trace": "trace_kernel", "sort": "sort_bin_kernel", "reorder": "reorder_kernel",
+                "keys": "keys_kernel"}
+
+
+def measured_traffic(stage, elems):
+    """HBM bytes per launch of the stage's kernel at this launch size, as
+    measured by the two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE, scaled
+    by a plain 1 GiB copy) summarised in profiles/; None when not measured."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        f = line.split()
+        if len(f) == 4 and f[0] == "traffic" and f[1] == STAGE_KERNEL.get(stage) and int(f[2]) == elems:
+            return int(f[3])
+    return None
 
 
 def cpu_baseline(b0, scene_h, smin, smax, mode, kbits, sample, renderer):
