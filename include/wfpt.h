@@ -131,12 +131,17 @@ int wf_scene_create(const void* vertices, int64_t num_vertices, int vertex_strid
                     const int32_t* geom_first_tri, int num_geoms,
                     const int32_t* tri_material, wf_stream_t stream, wf_scene* out);
 int wf_scene_destroy(wf_scene scene);
-/* Per-material albedo (rgb) and emission (rgb), device or host pointers not
- * accepted: host arrays of 3*num_materials floats, copied. Default one
- * material: albedo 0.8 (raygenPath.comp:138), emission 0. */
+/* Per-material albedo (rgb) and emission (rgb): host or device arrays of
+ * 3*num_materials floats, copied (the device copies are reused while the
+ * count stays the same). Default one material: albedo 0.8
+ * (raygenPath.comp:138), emission 0. */
 int wf_scene_set_materials(wf_scene scene, const float* albedo_rgb,
                            const float* emission_rgb, int num_materials,
                            wf_stream_t stream);
+/* Material count and, for non-NULL arrays (host or device), a copy of the
+ * albedo and emission tables. */
+int wf_scene_get_materials(wf_scene scene, int* num_materials, float* albedo_rgb,
+                           float* emission_rgb);
 /* Scene AABB (getSceneDimensions, base/VulkanglTFModel.cpp:1478-1488). */
 int wf_scene_bounds(wf_scene scene, float out_min[3], float out_max[3]);
 /* Sizes: triangles, BVH nodes, leaves. */
@@ -294,6 +299,21 @@ int wf_renderer_reset_frame_index(wf_renderer r);
 /* Copies width*height float4 of the accumulated (linear) or tone-mapped
  * framebuffer to host memory. */
 int wf_renderer_read_pixels(wf_renderer r, int tone_mapped, float* host_rgba);
+/* Device pointer of the same framebuffer (width*height float4), valid until
+ * the next render or destroy; waits for the renderer's stream. */
+int wf_renderer_pixels_device(wf_renderer r, int tone_mapped, const float** out);
+/* Path-replay adjoint of the last rendered frame (no counterpart in the
+ * reference, whose materials are a constant albedo, raygenPath.comp:138):
+ * given dLoss/dpixel (device, width*height float4, rgb used) of the linear
+ * frame estimate, writes dLoss/dalbedo and dLoss/demission (device,
+ * 3*num_materials floats each, overwritten).  Every path vertex is
+ * recomputed under the frame's random numbers; nothing is stored by the
+ * forward pass.  replay (device, optional) receives the replayed frame
+ * estimate, equal to the rendered one.  Needs Renderer.rayType = path,
+ * Russian roulette off, recursionDepth <= 8, and the settings of that frame;
+ * frame = the frame index to replay (0 = the last rendered). */
+int wf_renderer_adjoint(wf_renderer r, int frame, const float* d_pixels, float* grad_albedo,
+                        float* grad_emission, float* replay, float* ms);
 int wf_renderer_size(wf_renderer r, int* width, int* height);
 /* Stats of the last frame (Renderer.cpp:660-706):
  * counts[3] = primary, shadow, path rays; times[3] = trace ms per type;
@@ -313,16 +333,68 @@ int wf_renderer_set_capture(wf_renderer r, int enable);
 int wf_renderer_captured(wf_renderer r, int bounce, const wf_ray** rays,
                          const int32_t** slot_to_index, int64_t* n);
 
+/* ------------------------------------------------------ batched rendering */
+/* Many scenes and views of uneven size rendered in one persistent wavefront
+ * queue (no counterpart in the reference, which renders one view per
+ * Renderer::render, Renderer.cpp:552-654).  Rays of all views share one
+ * queue; path rays are sorted by 64-bit keys with the scene index in the
+ * top 6 bits (scene | 30-bit origin | 28-bit octahedral direction); counts
+ * stay on the device and one pass of the bounce loop is captured in a HIP
+ * graph and replayed for every sample pass.  Each view's frame equals the
+ * single-view frame loop's (wf_renderer_render, frame 1, Russian roulette
+ * off) for the same camera, scene and settings. */
+typedef struct {
+    int scene;                 /* index into the scenes of wf_batch_create */
+    int width, height, spp;
+    float position[3], rotation[3];   /* camera (camera.hpp conventions) */
+    float fov, znear, zfar;
+    float light[3], background[3];
+    float shadow_radius;
+} wf_view;
+typedef struct wf_batch_s* wf_batch;
+/* Scenes stay owned by the caller and must outlive the batch. */
+int wf_batch_create(const wf_scene* scenes, int num_scenes, const wf_view* views, int num_views,
+                    int depth, int shadow_samples, wf_batch* out);
+int wf_batch_destroy(wf_batch b);
+/* Sort digit width {4, 8, 11}, trace workgroup size {64, 128, 256}, reorder
+ * period (sort path rays every k-th bounce; 0 = never). */
+int wf_batch_set_tuning(wf_batch b, int digit_bits, int block, int reorder_period);
+int wf_batch_get_tuning(wf_batch b, int* digit_bits, int* block, int* reorder_period);
+/* Times one full pass (the largest) for every combination of the three
+ * knobs and keeps the fastest; *best_ms = its pass time. */
+int wf_batch_autotune(wf_batch b, float* best_ms);
+/* 1 (default): replay the captured pass graph; 0: launch the same sequence
+ * eagerly. */
+int wf_batch_set_graph(wf_batch b, int enable);
+/* Frame 1 of every view; *ms = GPU time of the frame. */
+int wf_batch_render(wf_batch b, float* ms);
+/* Linear frame estimate of one view (caller's view index), width*height
+ * float4 to host memory. */
+int wf_batch_read_pixels(wf_batch b, int view, float* host_rgba);
+/* Rays of the last frame {primary, shadow, path} and the queue capacity. */
+int wf_batch_stats(wf_batch b, uint64_t counts[3], int64_t* total_pixels);
+
 /* ------------------------------------------------------- procedural scenes */
 /* Host-side generators for the five workloads (seeded SplitMix64).  They
  * write host arrays; call first with NULL arrays to get the sizes.
- * Vertex records are {pos.xyz, normal.xyz} (stride 24 bytes). */
-int wf_procedural_scene(int kind, uint64_t seed, int64_t* num_vertices, int64_t* num_tris,
+ * Vertex records are {pos.xyz, normal.xyz} (stride 24 bytes).
+ * material_groups > 0 regroups the non-emissive triangles into contiguous
+ * runs with one material each (material_groups materials in all, emissive
+ * materials kept first), as the differentiable workload needs; 0 keeps the
+ * generator's materials. */
+int wf_procedural_scene(int kind, uint64_t seed, int material_groups, int64_t* num_vertices,
+                        int64_t* num_tris,
                         int* num_geoms, int* num_materials, float* vertices,
                         uint32_t* indices, int32_t* geom_first_tri, int32_t* tri_material,
                         float* albedo, float* emission, float scene_min[3],
                         float scene_max[3]);
 enum { WF_SCENE_BOXES = 0, WF_SCENE_INTERIOR = 1, WF_SCENE_CITY = 2 };
+/* The boxes generator with num_boxes boxes (192 triangles each) instead of
+ * 64, plus the ground and light quads: the scenes of the batched workload. */
+int wf_procedural_boxes(uint64_t seed, int num_boxes, int64_t* num_vertices, int64_t* num_tris,
+                        int* num_geoms, int* num_materials, float* vertices, uint32_t* indices,
+                        int32_t* geom_first_tri, int32_t* tri_material, float* albedo,
+                        float* emission, float scene_min[3], float scene_max[3]);
 
 #ifdef __cplusplus
 }

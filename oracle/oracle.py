@@ -267,13 +267,18 @@ class FrameCfg(C.Structure):
 
 
 def render_frame_f64(tri_verts, tri_n, tri_mat, albedo, emission, *, w, h, spp, depth, S, frame,
-                     light, bg, origin, radius, max_dist, s2w_colmajor, rr=False, rr_start=3):
+                     light, bg, origin, radius, max_dist, s2w_colmajor, rr=False, rr_start=3,
+                     d_pixels=None):
     """Frame 1 of Renderer::renderPath, primary rays in fp32 (as raygenPrimary
     makes them) and the paths followed in fp64.  Returns ([w*h, 4] float32
     linear RGBA, [w*h] bool ambiguous): a pixel is ambiguous when some vertex
     of its paths grazes a triangle edge, ties two hits or sits on an interval
     end within the fp32 error bound propagated along the path, where an fp32
-    pipeline may follow another path."""
+    pipeline may follow another path.
+
+    With d_pixels ([w*h, 4] float32, dLoss/dpixel), also returns the fp64
+    adjoint: (image, ambiguous, grad_albedo [nmat, 3], grad_emission [nmat, 3])
+    with grad = sum over pixels of d_pixels . dpixel/dparameter."""
     cfg = FrameCfg(w, h, spp, depth, S, frame, int(rr), rr_start, (C.c_float * 3)(*light),
                    (C.c_float * 3)(*bg), (C.c_float * 3)(*origin), radius, max_dist,
                    (C.c_float * 16)(*np.asarray(s2w_colmajor, np.float32).reshape(16)))
@@ -284,6 +289,13 @@ def render_frame_f64(tri_verts, tri_n, tri_mat, albedo, emission, *, w, h, spp, 
     em = np.ascontiguousarray(emission, np.float32)
     out = np.zeros((w * h, 4), np.float32)
     amb = np.zeros(w * h, np.uint8)
+    if d_pixels is None:
+        load().or_render_frame_f64(_p(tv), _p(tn), _p(tm), C.c_int64(len(tv)), _p(al), _p(em),
+                                   C.c_int(len(al)), C.byref(cfg), _p(out), _p(amb), None, None, None)
+        return out, amb.astype(bool)
+    dp = np.ascontiguousarray(d_pixels, np.float32).reshape(w * h, 4)
+    ga = np.zeros((len(al), 3), np.float64)
+    ge = np.zeros((len(al), 3), np.float64)
     load().or_render_frame_f64(_p(tv), _p(tn), _p(tm), C.c_int64(len(tv)), _p(al), _p(em),
-                               C.c_int(len(al)), C.byref(cfg), _p(out), _p(amb))
-    return out, amb.astype(bool)
+                               C.c_int(len(al)), C.byref(cfg), _p(out), _p(amb), _p(dp), _p(ga), _p(ge))
+    return out, amb.astype(bool), ga, ge

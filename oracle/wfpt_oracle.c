@@ -947,13 +947,19 @@ static double d_shadow(const cbvh* b, const or_frame_cfg* c, int hit, const doub
     return (double)vis / c->S;
 }
 
-static void d_smooth(const cbvh* b, const or_frame_cfg* c, const float* tri_n, const int32_t* tri_mat,
-                     const float* emission, int nmat, int hit, const double o[3], const double d[3],
-                     double t, double u, double v, const double dec[3], double out[3]) {
+/* reconstructSmooth of one vertex: out = dec * y with y = (max(N.L, 0.1) +
+ * emission[m]) / spp for a hit and background / spp for a miss; returns the
+ * material (-1 for a miss) and y. */
+static int d_smooth(const cbvh* b, const or_frame_cfg* c, const float* tri_n, const int32_t* tri_mat,
+                    const float* emission, int nmat, int hit, const double o[3], const double d[3],
+                    double t, double u, double v, const double dec[3], double out[3], double y[3]) {
     (void)b;
     if (hit < 0) {
-        for (int k = 0; k < 3; ++k) out[k] = dec[k] * c->bg[k] / c->spp;
-        return;
+        for (int k = 0; k < 3; ++k) {
+            y[k] = c->bg[k] / c->spp;
+            out[k] = dec[k] * y[k];
+        }
+        return -1;
     }
     double n[3];
     int m = d_normal(tri_n, tri_mat, hit, u, v, d, n);
@@ -962,12 +968,43 @@ static void d_smooth(const cbvh* b, const or_frame_cfg* c, const float* tri_n, c
     double L[3] = {c->light[0] - p[0], c->light[1] - p[1], c->light[2] - p[2]};
     dnorm(L);
     double alpha = fmax(n[0] * L[0] + n[1] * L[1] + n[2] * L[2], 0.1);
-    for (int k = 0; k < 3; ++k) out[k] = dec[k] * (alpha + emission[3 * m + k]) / c->spp;
+    for (int k = 0; k < 3; ++k) {
+        y[k] = (alpha + emission[3 * m + k]) / c->spp;
+        out[k] = dec[k] * y[k];
+    }
+    return m;
 }
 
+#define OR_MAX_VERTS 16
+
+/* Gradient bookkeeping of one path (the fp64 twin of the adjoint kernel):
+ * E[k] = throughput without the albedo factor of extension k, mat[k] its
+ * material. */
+typedef struct {
+    double E[OR_MAX_VERTS][3];
+    int mat[OR_MAX_VERTS];
+    int nv;
+} path_grad;
+
+/* vertex contribution dec * y * vis with loss gradient g: emission gradient
+ * of the vertex's material and albedo gradients of the earlier extensions */
+static void grad_vertex(const path_grad* pg, int m, const double dec[3], const double y[3],
+                        double vis, const double g[3], double spp, double* ga, double* ge) {
+    if (!ga) return;
+    for (int c = 0; c < 3; ++c) {
+        if (m >= 0) ge[3 * m + c] += g[c] * dec[c] * vis / spp;
+        for (int k = 0; k < pg->nv; ++k) ga[3 * pg->mat[k] + c] += g[c] * pg->E[k][c] * y[c] * vis;
+    }
+}
+
+/* Frame 1 of Renderer::renderPath in fp64 (see the ambiguity model above).
+ * With d_pixels (w*h float4, rgb used) non-NULL, also the fp64 adjoint:
+ * grad_albedo / grad_emission (3*nmat doubles, overwritten) receive
+ * sum_pixels d_pixels . dpixel/dparameter. */
 OR_API void or_render_frame_f64(const float* tv, const float* tri_n, const int32_t* tri_mat,
                                 int64_t ntri, const float* albedo, const float* emission, int nmat,
-                                const or_frame_cfg* c, float* out_rgba, uint8_t* out_amb) {
+                                const or_frame_cfg* c, float* out_rgba, uint8_t* out_amb,
+                                const float* d_pixels, double* grad_albedo, double* grad_emission) {
     cbvh b;
     b.tv = tv;
     b.nodes = (bnode*)malloc(sizeof(bnode) * (2 * ntri + 1));
@@ -976,67 +1013,96 @@ OR_API void or_render_frame_f64(const float* tv, const float* tri_n, const int32
     for (int64_t i = 0; i < ntri; ++i) b.order[i] = (int)i;
     if (ntri > 0) cb_build(&b, 0, (int)ntri);
     const int64_t np = (int64_t)c->w * c->h;
-#pragma omp parallel for schedule(dynamic, 256)
-    for (int64_t pixel = 0; pixel < np; ++pixel) {
-        uint32_t seed = tea((uint32_t)pixel, (uint32_t)c->frame, 16);
-        double acc[3] = {0, 0, 0};
-        int amb = 0;
-        for (int pass = 0; pass < c->spp; ++pass) {
-            int si = pass + c->spp * (c->frame - 1);
-            float hx = halton(si, 2), hy = halton(si, 3);
-            /* primary rays are the fp32 ones of raygenPrimary (their near-plane
-             * construction loses ~1e-5 rad to cancellation; the stage is
-             * checked bit for bit on its own): the path is then followed in fp64 */
-            float df[3];
-            primary_dir_f32(c->s2w, c->origin, (int)pixel, c->w, c->h, hx, hy, df);
-            double o[3] = {c->origin[0], c->origin[1], c->origin[2]};
-            double d[3] = {df[0], df[1], df[2]};
-            double dec[3] = {1, 1, 1};
-            double t = 0, u = 0, v = 0, cs = 1;
-            ray_tol tl = {0.0, FP32_POS};
-            int hit = ntri > 0 ? cb_trace(&b, o, d, 0.0, c->max_dist, 0, &t, &u, &v, &amb, &tl, &cs) : -1;
-            double lat = hit >= 0 ? hit_uncertainty(&tl, o, d, t, cs) : 0.0;
-            double aux[3];
-            d_smooth(&b, c, tri_n, tri_mat, emission, nmat, hit, o, d, t, u, v, dec, aux);
-            double vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel, &amb, lat);
-            for (int k = 0; k < 3; ++k) acc[k] += aux[k] * vis;
-            for (int bounce = 0; bounce < c->depth; ++bounce) {
-                if (hit < 0) break;
-                double n[3];
-                int mt = d_normal(tri_n, tri_mat, hit, u, v, d, n);
-                if (mt < 0 || mt >= nmat) mt = 0;
-                double r0 = randf(&seed), r1 = randf(&seed);
-                double p[3] = {o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t};
-                double theta = 2.0 * 3.14159265358979323846 * r1, rad = sqrt(r0);
-                double xx = rad * sin(theta), zz = rad * cos(theta), yy = sqrt(1.0 - r0);
-                double s = n[2] >= 0.0 ? 1.0 : -1.0;
-                double a = -1.0 / (s + n[2]), bb = n[0] * n[1] * a;
-                double b1[3] = {1.0 + s * n[0] * n[0] * a, s * bb, -s * n[0]};
-                double b2[3] = {bb, s + n[1] * n[1] * a, -n[1]};
-                double nd[3];
-                for (int k = 0; k < 3; ++k) nd[k] = xx * b1[k] + zz * b2[k] + yy * n[k];
-                double w[3] = {albedo[3 * mt], albedo[3 * mt + 1], albedo[3 * mt + 2]};
-                if (c->rr && bounce >= c->rr_start) {
-                    double q = fmax(fmax(dec[0] * w[0], dec[1] * w[1]), dec[2] * w[2]);
-                    q = fmin(fmax(q, 0.05), 1.0);
-                    double r2 = randf(&seed);
-                    if (fabs(r2 - q) < 1e-5) amb = 1;   /* roulette decision on a knife edge */
-                    if (r2 >= q) break;
-                    for (int k = 0; k < 3; ++k) w[k] /= q;
-                }
-                for (int k = 0; k < 3; ++k) { dec[k] *= w[k]; o[k] = p[k] + nd[k] * 1.0e-4; d[k] = nd[k]; }
-                tl.tol0 = lat;
-                tl.tol_eps = bounce_dir_uncertainty(tri_n, hit);
-                hit = cb_trace(&b, o, d, 1.0e-4, 3.402823466e+38, 0, &t, &u, &v, &amb, &tl, &cs);
-                lat = hit >= 0 ? hit_uncertainty(&tl, o, d, t, cs) : 0.0;
-                d_smooth(&b, c, tri_n, tri_mat, emission, nmat, hit, o, d, t, u, v, dec, aux);
-                vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel, &amb, lat);
+    const int want_grad = d_pixels && grad_albedo && grad_emission;
+    if (want_grad)
+        for (int k = 0; k < 3 * nmat; ++k) grad_albedo[k] = grad_emission[k] = 0.0;
+#pragma omp parallel
+    {
+        double* ga = want_grad ? (double*)calloc(6 * (size_t)nmat, sizeof(double)) : NULL;
+        double* ge = ga ? ga + 3 * nmat : NULL;
+#pragma omp for schedule(dynamic, 256)
+        for (int64_t pixel = 0; pixel < np; ++pixel) {
+            uint32_t seed = tea((uint32_t)pixel, (uint32_t)c->frame, 16);
+            double acc[3] = {0, 0, 0};
+            double g[3] = {0, 0, 0};
+            if (ga)
+                for (int k = 0; k < 3; ++k) g[k] = d_pixels[4 * pixel + k];
+            int amb = 0;
+            for (int pass = 0; pass < c->spp; ++pass) {
+                int si = pass + c->spp * (c->frame - 1);
+                float hx = halton(si, 2), hy = halton(si, 3);
+                /* primary rays are the fp32 ones of raygenPrimary (their near-plane
+                 * construction loses ~1e-5 rad to cancellation; the stage is
+                 * checked bit for bit on its own): the path is then followed in fp64 */
+                float df[3];
+                primary_dir_f32(c->s2w, c->origin, (int)pixel, c->w, c->h, hx, hy, df);
+                double o[3] = {c->origin[0], c->origin[1], c->origin[2]};
+                double d[3] = {df[0], df[1], df[2]};
+                double dec[3] = {1, 1, 1};
+                double t = 0, u = 0, v = 0, cs = 1;
+                ray_tol tl = {0.0, FP32_POS};
+                path_grad pg;
+                pg.nv = 0;
+                int hit = ntri > 0 ? cb_trace(&b, o, d, 0.0, c->max_dist, 0, &t, &u, &v, &amb, &tl, &cs) : -1;
+                double lat = hit >= 0 ? hit_uncertainty(&tl, o, d, t, cs) : 0.0;
+                double aux[3], y[3];
+                int m = d_smooth(&b, c, tri_n, tri_mat, emission, nmat, hit, o, d, t, u, v, dec, aux, y);
+                double vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel, &amb, lat);
                 for (int k = 0; k < 3; ++k) acc[k] += aux[k] * vis;
+                grad_vertex(&pg, m, dec, y, vis, g, c->spp, ga, ge);
+                for (int bounce = 0; bounce < c->depth; ++bounce) {
+                    if (hit < 0) break;
+                    double n[3];
+                    int mt = d_normal(tri_n, tri_mat, hit, u, v, d, n);
+                    if (mt < 0 || mt >= nmat) mt = 0;
+                    double r0 = randf(&seed), r1 = randf(&seed);
+                    double p[3] = {o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t};
+                    double theta = 2.0 * 3.14159265358979323846 * r1, rad = sqrt(r0);
+                    double xx = rad * sin(theta), zz = rad * cos(theta), yy = sqrt(1.0 - r0);
+                    double s = n[2] >= 0.0 ? 1.0 : -1.0;
+                    double a = -1.0 / (s + n[2]), bb = n[0] * n[1] * a;
+                    double b1[3] = {1.0 + s * n[0] * n[0] * a, s * bb, -s * n[0]};
+                    double b2[3] = {bb, s + n[1] * n[1] * a, -n[1]};
+                    double nd[3];
+                    for (int k = 0; k < 3; ++k) nd[k] = xx * b1[k] + zz * b2[k] + yy * n[k];
+                    double w[3] = {albedo[3 * mt], albedo[3 * mt + 1], albedo[3 * mt + 2]};
+                    if (c->rr && bounce >= c->rr_start) {
+                        double q = fmax(fmax(dec[0] * w[0], dec[1] * w[1]), dec[2] * w[2]);
+                        q = fmin(fmax(q, 0.05), 1.0);
+                        double r2 = randf(&seed);
+                        if (fabs(r2 - q) < 1e-5) amb = 1;   /* roulette decision on a knife edge */
+                        if (r2 >= q) break;
+                        for (int k = 0; k < 3; ++k) w[k] /= q;
+                    }
+                    if (pg.nv < OR_MAX_VERTS) {
+                        for (int k = 0; k < pg.nv; ++k)
+                            for (int q = 0; q < 3; ++q) pg.E[k][q] *= w[q];
+                        for (int q = 0; q < 3; ++q) pg.E[pg.nv][q] = dec[q];
+                        pg.mat[pg.nv++] = mt;
+                    }
+                    for (int k = 0; k < 3; ++k) { dec[k] *= w[k]; o[k] = p[k] + nd[k] * 1.0e-4; d[k] = nd[k]; }
+                    tl.tol0 = lat;
+                    tl.tol_eps = bounce_dir_uncertainty(tri_n, hit);
+                    hit = cb_trace(&b, o, d, 1.0e-4, 3.402823466e+38, 0, &t, &u, &v, &amb, &tl, &cs);
+                    lat = hit >= 0 ? hit_uncertainty(&tl, o, d, t, cs) : 0.0;
+                    m = d_smooth(&b, c, tri_n, tri_mat, emission, nmat, hit, o, d, t, u, v, dec, aux, y);
+                    vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel, &amb, lat);
+                    for (int k = 0; k < 3; ++k) acc[k] += aux[k] * vis;
+                    grad_vertex(&pg, m, dec, y, vis, g, c->spp, ga, ge);
+                }
             }
+            float* px = out_rgba + 4 * pixel;
+            px[0] = (float)acc[0]; px[1] = (float)acc[1]; px[2] = (float)acc[2]; px[3] = 1.0f;
+            if (out_amb) out_amb[pixel] = (uint8_t)amb;
         }
-        float* px = out_rgba + 4 * pixel;
-        px[0] = (float)acc[0]; px[1] = (float)acc[1]; px[2] = (float)acc[2]; px[3] = 1.0f;
-        if (out_amb) out_amb[pixel] = (uint8_t)amb;
+        if (ga) {
+#pragma omp critical
+            for (int k = 0; k < 3 * nmat; ++k) {
+                grad_albedo[k] += ga[k];
+                grad_emission[k] += ge[k];
+            }
+            free(ga);
+        }
     }
     free(b.nodes);
     free(b.order);

@@ -58,7 +58,8 @@ SIGNATURES: dict[str, tuple] = {
     "wf_reorder": (i32, [vp, vp, vp, vp, vp, vp, i64, vp, pf]),
     "wf_scene_create": (i32, [vp, i64, i32, vp, i64, pi32, i32, vp, vp, C.POINTER(vp)]),
     "wf_scene_destroy": (i32, [vp]),
-    "wf_scene_set_materials": (i32, [vp, pf, pf, i32, vp]),
+    "wf_scene_set_materials": (i32, [vp, vp, vp, i32, vp]),
+    "wf_scene_get_materials": (i32, [vp, C.POINTER(i32), vp, vp]),
     "wf_scene_bounds": (i32, [vp, pf, pf]),
     "wf_scene_info": (i32, [vp, pi64, pi64, pi64, C.POINTER(i32)]),
     "wf_scene_download": (i32, [vp, pf, pi32]),
@@ -89,6 +90,8 @@ SIGNATURES: dict[str, tuple] = {
     "wf_renderer_render": (i32, [vp, pf]),
     "wf_renderer_reset_frame_index": (i32, [vp]),
     "wf_renderer_read_pixels": (i32, [vp, i32, pf]),
+    "wf_renderer_pixels_device": (i32, [vp, i32, C.POINTER(vp)]),
+    "wf_renderer_adjoint": (i32, [vp, i32, vp, vp, vp, vp, pf]),
     "wf_renderer_size": (i32, [vp, C.POINTER(i32), C.POINTER(i32)]),
     "wf_renderer_stats": (i32, [vp, pu64, pf, pf, pu64]),
     "wf_env_get_int": (i32, [vp, cs, C.POINTER(i32)]),
@@ -97,7 +100,18 @@ SIGNATURES: dict[str, tuple] = {
     "wf_env_get_vector": (i32, [vp, cs, pf]),
     "wf_renderer_set_capture": (i32, [vp, i32]),
     "wf_renderer_captured": (i32, [vp, i32, C.POINTER(vp), C.POINTER(vp), pi64]),
-    "wf_procedural_scene": (i32, [i32, u64, pi64, pi64, C.POINTER(i32), C.POINTER(i32), pf, pu32,
+    "wf_procedural_boxes": (i32, [u64, i32, pi64, pi64, C.POINTER(i32), C.POINTER(i32), pf, pu32,
+                                  pi32, pi32, pf, pf, pf, pf]),
+    "wf_batch_create": (i32, [C.POINTER(vp), i32, vp, i32, i32, i32, C.POINTER(vp)]),
+    "wf_batch_destroy": (i32, [vp]),
+    "wf_batch_set_tuning": (i32, [vp, i32, i32, i32]),
+    "wf_batch_get_tuning": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
+    "wf_batch_autotune": (i32, [vp, pf]),
+    "wf_batch_set_graph": (i32, [vp, i32]),
+    "wf_batch_render": (i32, [vp, pf]),
+    "wf_batch_read_pixels": (i32, [vp, i32, pf]),
+    "wf_batch_stats": (i32, [vp, pu64, pi64]),
+    "wf_procedural_scene": (i32, [i32, u64, i32, pi64, pi64, C.POINTER(i32), C.POINTER(i32), pf, pu32,
                                   pi32, pi32, pf, pf, pf, pf]),
 }
 
@@ -145,6 +159,21 @@ def check(fn_name: str, status: int) -> int:
 
 def call(name: str, *args):
     return check(name, getattr(load(), name)(*args))
+
+
+_hip = None
+
+
+def memcpy_d2d(dst: int, src: int, nbytes: int) -> None:
+    """Synchronous device copy between raw device addresses (hipMemcpy)."""
+    global _hip
+    if _hip is None:
+        _hip = C.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip.hipMemcpy.restype = C.c_int
+    rc = _hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), 3)
+    if rc != 0:
+        raise WfError(f"hipMemcpy failed ({rc})")
 
 
 def f3(v) -> C.Array:

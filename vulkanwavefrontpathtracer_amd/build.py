@@ -51,6 +51,37 @@ def _compile(src: Path) -> Path:
     return obj
 
 
+TORCH_SRC = PKG / "torch_ext" / "render_autograd.cpp"
+TORCH_LIB = PKG / "_wfpt_torch.so"
+
+
+def build_torch_ext(verbose: bool = True) -> Path:
+    """The libtorch autograd Function (torch_ext/render_autograd.cpp) as the
+    Python module _wfpt_torch, linked against libwfpt.so next to it."""
+    import sysconfig
+
+    import torch
+    from torch.utils import cpp_extension as ce
+
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    inc = [f"-I{p}" for p in ce.include_paths()] + [f"-I{sysconfig.get_paths()['include']}",
+                                                    f"-I{ROOT / 'include'}"]
+    libdirs = ce.library_paths()
+    tmp = TORCH_LIB.with_suffix(".so.tmp")
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
+           "-DTORCH_EXTENSION_NAME=_wfpt_torch", "-DTORCH_API_INCLUDE_EXTENSION_H", *inc,
+           str(TORCH_SRC), "-o", str(tmp), *[f"-L{d}" for d in libdirs], f"-L{PKG}",
+           "-lwfpt", "-lc10", "-ltorch", "-ltorch_cpu", "-ltorch_python",
+           "-Wl,-rpath,$ORIGIN", *[f"-Wl,-rpath,{d}" for d in libdirs]]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"torch extension build failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, TORCH_LIB)
+    if verbose:
+        print(f"[build] {TORCH_LIB}", flush=True)
+    return TORCH_LIB
+
+
 def build(verbose: bool = True, jobs: int | None = None) -> Path:
     OBJ.mkdir(exist_ok=True)
     srcs = sources()
@@ -70,4 +101,5 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
 
 if __name__ == "__main__":
     build()
+    build_torch_ext()
     sys.exit(0)

@@ -438,19 +438,37 @@ extern "C" int wf_scene_set_materials(wf_scene scene, const float* albedo_rgb,
                                       wf_stream_t stream) {
     WF_CHECK_ARG(scene, "wf_scene_set_materials: null scene");
     WF_CHECK_ARG(num_materials >= 1 && albedo_rgb && emission_rgb,
-                 "wf_scene_set_materials: need >= 1 material and host albedo/emission arrays");
+                 "wf_scene_set_materials: need >= 1 material and albedo/emission arrays");
     hipStream_t s = as_stream(stream);
-    float *a = nullptr, *e = nullptr;
-    WF_HIP(hipMalloc((void**)&a, sizeof(float) * 3 * num_materials));
-    WF_HIP(hipMalloc((void**)&e, sizeof(float) * 3 * num_materials));
-    WF_HIP(hipMemcpyAsync(a, albedo_rgb, sizeof(float) * 3 * num_materials, hipMemcpyHostToDevice, s));
-    WF_HIP(hipMemcpyAsync(e, emission_rgb, sizeof(float) * 3 * num_materials, hipMemcpyHostToDevice, s));
+    const size_t bytes = sizeof(float) * 3 * (size_t)num_materials;
+    if (num_materials != scene->num_materials || !scene->albedo || !scene->emission) {
+        float *a = nullptr, *e = nullptr;
+        WF_HIP(hipMalloc((void**)&a, bytes));
+        if (hipMalloc((void**)&e, bytes) != hipSuccess) {
+            (void)hipFree(a);
+            return fail(WF_ERR_OUT_OF_MEMORY, "wf_scene_set_materials: out of device memory");
+        }
+        WF_HIP(hipStreamSynchronize(s));
+        if (scene->albedo) (void)hipFree(scene->albedo);
+        if (scene->emission) (void)hipFree(scene->emission);
+        scene->albedo = a;
+        scene->emission = e;
+        scene->num_materials = num_materials;
+    }
+    // host or device source arrays (unified addressing)
+    WF_HIP(hipMemcpyAsync(scene->albedo, albedo_rgb, bytes, hipMemcpyDefault, s));
+    WF_HIP(hipMemcpyAsync(scene->emission, emission_rgb, bytes, hipMemcpyDefault, s));
     WF_HIP(hipStreamSynchronize(s));
-    if (scene->albedo) (void)hipFree(scene->albedo);
-    if (scene->emission) (void)hipFree(scene->emission);
-    scene->albedo = a;
-    scene->emission = e;
-    scene->num_materials = num_materials;
+    return WF_OK;
+}
+
+extern "C" int wf_scene_get_materials(wf_scene scene, int* num_materials, float* albedo_rgb,
+                                      float* emission_rgb) {
+    WF_CHECK_ARG(scene, "wf_scene_get_materials: null scene");
+    if (num_materials) *num_materials = scene->num_materials;
+    const size_t bytes = sizeof(float) * 3 * (size_t)scene->num_materials;
+    if (albedo_rgb) WF_HIP(hipMemcpy(albedo_rgb, scene->albedo, bytes, hipMemcpyDefault));
+    if (emission_rgb) WF_HIP(hipMemcpy(emission_rgb, scene->emission, bytes, hipMemcpyDefault));
     return WF_OK;
 }
 

@@ -63,7 +63,9 @@ struct HostScene {
     std::vector<float> albedo, emission;  // 3 per material
     float bmin[3], bmax[3];
 };
-int procedural_scene(int kind, uint64_t seed, HostScene& out);
+int procedural_scene(int kind, uint64_t seed, HostScene& out, int num_boxes = 64);
+// Regroups the non-emissive triangles into contiguous runs, one material each.
+int regroup_materials(HostScene& s, int groups);
 
 // ----------------------------------------------------------- device memory
 struct DeviceBuffer {

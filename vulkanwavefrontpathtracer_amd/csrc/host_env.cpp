@@ -262,6 +262,7 @@ Environment::Environment() {
     // This library's keys (procedural workloads and sort settings).
     reg("Scene.procedural", OPT_STRING);          // boxes | interior | city
     reg("Scene.seed", "1000", OPT_INT);
+    reg("Scene.materialGroups", "0", OPT_INT);     // > 0: contiguous material groups
     reg("Renderer.keyMode", "aila32", OPT_STRING);  // aila32 | origin30 | origin30_oct30
     reg("Renderer.digitBits", "8", OPT_INT);        // 4 | 8 | 11
     reg("Renderer.reorderPeriod", "1", OPT_INT);    // sort every k-th bounce

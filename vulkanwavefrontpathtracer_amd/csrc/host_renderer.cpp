@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "adjoint.h"
 #include "common.h"
 #include "host.h"
 #include "scene.h"
@@ -520,6 +521,9 @@ extern "C" int wf_renderer_build_scene(wf_renderer r) {
     r->env.get_int("Scene.seed", seed);
     HostScene hs;
     WF_TRY(procedural_scene(k, (uint64_t)seed, hs));
+    int groups = 0;
+    r->env.get_int("Scene.materialGroups", groups);
+    if (groups > 0) WF_TRY(regroup_materials(hs, groups));
     const int64_t nv = (int64_t)hs.vertices.size() / 6, nt = (int64_t)hs.indices.size() / 3;
     DeviceBuffer v, idx, mat;
     WF_TRY(v.ensure(sizeof(float) * hs.vertices.size()));
@@ -582,6 +586,72 @@ extern "C" int wf_renderer_render(wf_renderer r, float* ms_out) {
 extern "C" int wf_renderer_reset_frame_index(wf_renderer r) {
     WF_CHECK_ARG(r, "wf_renderer_reset_frame_index: null renderer");
     r->frame_index = 1;
+    return WF_OK;
+}
+
+extern "C" int wf_renderer_pixels_device(wf_renderer r, int tone_mapped, const float** out) {
+    WF_CHECK_ARG(r && out, "wf_renderer_pixels_device: null argument");
+    const DeviceBuffer& b = tone_mapped ? r->frame_pixels : r->pixels;
+    WF_CHECK_ARG(b.p && r->stream, "wf_renderer_pixels_device: nothing rendered yet");
+    WF_HIP(hipStreamSynchronize(r->stream));
+    *out = b.as<float>();
+    return WF_OK;
+}
+
+// Path-replay adjoint of the last rendered frame (csrc/adjoint.hip).
+extern "C" int wf_renderer_adjoint(wf_renderer r, int frame, const float* d_pixels,
+                                   float* grad_albedo, float* grad_emission, float* replay,
+                                   float* ms) {
+    WF_CHECK_ARG(r && d_pixels && grad_albedo && grad_emission,
+                 "wf_renderer_adjoint: null argument");
+    WF_CHECK_ARG(r->scene, "wf_renderer_adjoint: no scene");
+    WF_CHECK_ARG(r->frame_index > 1 && r->lut_w == r->width && r->lut_h == r->height,
+                 "wf_renderer_adjoint: render a frame first");
+    WF_CHECK_ARG(frame >= 0, "wf_renderer_adjoint: frame < 0");
+    if (frame == 0) frame = r->frame_index - 1;
+    WF_CHECK_ARG(r->ray_type == "path", "wf_renderer_adjoint: needs Renderer.rayType = path");
+    WF_CHECK_ARG(!r->russian_roulette, "wf_renderer_adjoint: Russian roulette is not differentiable here");
+    WF_CHECK_ARG(r->depth >= 0 && r->depth <= ADJ_MAX_VERTS,
+                 "wf_renderer_adjoint: recursionDepth must be <= 8");
+    WF_TRY(ensure_stream(r));
+    hipStream_t s = r->stream;
+    wf_scene sc = r->scene;
+    const int nmat = sc->num_materials;
+    WF_TRY(fill_u32(reinterpret_cast<uint32_t*>(grad_albedo), 3 * (int64_t)nmat, 0u, s));
+    WF_TRY(fill_u32(reinterpret_cast<uint32_t*>(grad_emission), 3 * (int64_t)nmat, 0u, s));
+    AdjointParams p{};
+    p.nodes = sc->nodes;
+    p.tris = sc->tris;
+    p.shade = sc->shade;
+    p.albedo = sc->albedo;
+    p.emission = sc->emission;
+    p.nmat = nmat;
+    CameraMatrices cm;
+    camera_matrices(r->cam_pos, r->cam_rot, r->fov, (float)r->width / (float)r->height, r->znear,
+                    r->zfar, cm);
+    for (int k = 0; k < 16; ++k) p.s2w.m[k] = cm.screen_to_world[k];
+    for (int a = 0; a < 3; ++a) {
+        p.origin[a] = r->cam_pos[a];
+        p.light[a] = r->light[a];
+        p.background[a] = r->background[a];
+    }
+    p.index_to_pixel = r->i2p.as<int32_t>();
+    p.width = r->width;
+    p.height = r->height;
+    p.spp = r->spp;
+    p.depth = r->depth;
+    p.frame = frame;
+    p.S = r->shadow_samples;
+    p.radius = r->shadow_radius;
+    p.max_dist = r->zfar;
+    p.d_pixels = reinterpret_cast<const float4*>(d_pixels);
+    p.grad_albedo = grad_albedo;
+    p.grad_emission = grad_emission;
+    p.replay = reinterpret_cast<float4*>(replay);
+    StageTimer t(s, ms);
+    WF_TRY(adjoint_launch(p, s));
+    WF_TRY(t.finish());
+    WF_HIP(hipStreamSynchronize(s));
     return WF_OK;
 }
 

@@ -1,6 +1,8 @@
 // Camera matrices, primary-ray pixel order and the seeded procedural scenes.
 #include <cmath>
 #include <cstring>
+#include <algorithm>
+#include <vector>
 
 #include "common.h"
 #include "host.h"
@@ -281,10 +283,10 @@ void finish(HostScene& s) {
 // 64 boxes (centres U[-10,10]^3, edges U(0.2,2), 6 faces x 4x4 x 2 = 192
 // triangles, albedo U(0.1,0.9)), a 20x20 ground quad and an emissive quad
 // light of radiance 10: 12,292 triangles.
-void scene_boxes(uint64_t seed, HostScene& s) {
+void scene_boxes(uint64_t seed, HostScene& s, int num_boxes = 64) {
     SplitMix64 g{seed};
     Builder b(s);
-    for (int k = 0; k < 64; ++k) {
+    for (int k = 0; k < num_boxes; ++k) {
         F3 c{g.uni(-10, 10), g.uni(-10, 10), g.uni(-10, 10)};
         F3 e{g.uni(0.2f, 2.f), g.uni(0.2f, 2.f), g.uni(0.2f, 2.f)};
         F3 alb{g.uni(0.1f, 0.9f), g.uni(0.1f, 0.9f), g.uni(0.1f, 0.9f)};
@@ -294,10 +296,10 @@ void scene_boxes(uint64_t seed, HostScene& s) {
     }
     b.material({0.5f, 0.5f, 0.5f}, {0, 0, 0});
     b.begin_geom();
-    b.grid({-10, -11, -10}, {0, 0, 20}, {20, 0, 0}, 1, 1, {0, 1, 0}, 64);
+    b.grid({-10, -11, -10}, {0, 0, 20}, {20, 0, 0}, 1, 1, {0, 1, 0}, num_boxes);
     b.material({0, 0, 0}, {10, 10, 10});
     b.begin_geom();
-    b.grid({-2, 12, -2}, {4, 0, 0}, {0, 0, 4}, 1, 1, {0, -1, 0}, 65);
+    b.grid({-2, 12, -2}, {4, 0, 0}, {0, 0, 4}, 1, 1, {0, -1, 0}, num_boxes + 1);
     finish(s);
 }
 
@@ -383,14 +385,61 @@ void scene_city(uint64_t seed, HostScene& s) {
 
 }  // namespace
 
-int procedural_scene(int kind, uint64_t seed, HostScene& out) {
+int procedural_scene(int kind, uint64_t seed, HostScene& out, int num_boxes) {
     out = HostScene();
     switch (kind) {
-        case WF_SCENE_BOXES: scene_boxes(seed, out); break;
+        case WF_SCENE_BOXES:
+            if (num_boxes < 1 || num_boxes > (1 << 20))
+                return fail(WF_ERR_INVALID_ARG, "box count must be in [1, 2^20]");
+            scene_boxes(seed, out, num_boxes);
+            break;
         case WF_SCENE_INTERIOR: scene_interior(seed, out); break;
         case WF_SCENE_CITY: scene_city(seed, out); break;
         default: return fail(WF_ERR_INVALID_ARG, "procedural scene kind must be 0, 1 or 2");
     }
+    return WF_OK;
+}
+
+// Emissive materials keep their own slots (first); the non-emissive
+// triangles, in order, are cut into groups - n_emissive contiguous runs, each
+// with the albedo of its first triangle's original material.
+int regroup_materials(HostScene& s, int groups) {
+    const int nm = (int)(s.albedo.size() / 3);
+    std::vector<int> emissive_slot(nm, -1);
+    std::vector<float> alb, em;
+    int ne = 0;
+    for (int m = 0; m < nm; ++m)
+        if (s.emission[3 * m] > 0 || s.emission[3 * m + 1] > 0 || s.emission[3 * m + 2] > 0) {
+            emissive_slot[m] = ne++;
+            alb.insert(alb.end(), {s.albedo[3 * m], s.albedo[3 * m + 1], s.albedo[3 * m + 2]});
+            em.insert(em.end(), {s.emission[3 * m], s.emission[3 * m + 1], s.emission[3 * m + 2]});
+        }
+    const int g = groups - ne;
+    if (g < 1) return fail(WF_ERR_INVALID_ARG, "material_groups must exceed the emissive materials");
+    int64_t plain = 0;
+    for (int32_t m : s.tri_material)
+        if (m < 0 || m >= nm || emissive_slot[m] < 0) ++plain;
+    alb.resize(3 * (size_t)groups, 0.8f);
+    em.resize(3 * (size_t)groups, 0.0f);
+    std::vector<char> seen(g, 0);
+    int64_t rank = 0;
+    for (auto& m : s.tri_material) {
+        if (m >= 0 && m < nm && emissive_slot[m] >= 0) {
+            m = emissive_slot[m];
+            continue;
+        }
+        const int grp = (int)(rank * g / std::max<int64_t>(plain, 1));
+        ++rank;
+        const int slot = ne + grp;
+        if (!seen[grp]) {
+            seen[grp] = 1;
+            const int src = (m >= 0 && m < nm) ? m : 0;
+            for (int c = 0; c < 3; ++c) alb[3 * slot + c] = s.albedo[3 * src + c];
+        }
+        m = slot;
+    }
+    s.albedo = alb;
+    s.emission = em;
     return WF_OK;
 }
 
@@ -423,13 +472,40 @@ extern "C" int wf_camera_matrices(const float position[3], const float rotation_
     return WF_OK;
 }
 
-extern "C" int wf_procedural_scene(int kind, uint64_t seed, int64_t* num_vertices,
+static int export_scene(const HostScene& s, int64_t* num_vertices, int64_t* num_tris,
+                        int* num_geoms, int* num_materials, float* vertices, uint32_t* indices,
+                        int32_t* geom_first_tri, int32_t* tri_material, float* albedo,
+                        float* emission, float scene_min[3], float scene_max[3]);
+
+extern "C" int wf_procedural_boxes(uint64_t seed, int num_boxes, int64_t* num_vertices,
                                    int64_t* num_tris, int* num_geoms, int* num_materials,
                                    float* vertices, uint32_t* indices, int32_t* geom_first_tri,
                                    int32_t* tri_material, float* albedo, float* emission,
                                    float scene_min[3], float scene_max[3]) {
     HostScene s;
+    WF_TRY(procedural_scene(WF_SCENE_BOXES, seed, s, num_boxes));
+    return export_scene(s, num_vertices, num_tris, num_geoms, num_materials, vertices, indices,
+                        geom_first_tri, tri_material, albedo, emission, scene_min, scene_max);
+}
+
+extern "C" int wf_procedural_scene(int kind, uint64_t seed, int material_groups,
+                                   int64_t* num_vertices, int64_t* num_tris, int* num_geoms,
+                                   int* num_materials,
+                                   float* vertices, uint32_t* indices, int32_t* geom_first_tri,
+                                   int32_t* tri_material, float* albedo, float* emission,
+                                   float scene_min[3], float scene_max[3]) {
+    HostScene s;
     WF_TRY(procedural_scene(kind, seed, s));
+    WF_CHECK_ARG(material_groups >= 0, "wf_procedural_scene: material_groups < 0");
+    if (material_groups > 0) WF_TRY(regroup_materials(s, material_groups));
+    return export_scene(s, num_vertices, num_tris, num_geoms, num_materials, vertices, indices,
+                        geom_first_tri, tri_material, albedo, emission, scene_min, scene_max);
+}
+
+static int export_scene(const HostScene& s, int64_t* num_vertices, int64_t* num_tris,
+                        int* num_geoms, int* num_materials, float* vertices, uint32_t* indices,
+                        int32_t* geom_first_tri, int32_t* tri_material, float* albedo,
+                        float* emission, float scene_min[3], float scene_max[3]) {
     if (num_vertices) *num_vertices = (int64_t)(s.vertices.size() / 6);
     if (num_tris) *num_tris = (int64_t)(s.indices.size() / 3);
     if (num_geoms) *num_geoms = (int)s.geom_first.size() - 1;

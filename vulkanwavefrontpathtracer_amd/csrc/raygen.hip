@@ -15,10 +15,6 @@ __global__ __launch_bounds__(256) void init_seeds_kernel(uint32_t* __restrict__ 
 }
 
 // ---------------------------------------------------------------- primary
-struct Mat4 {
-    float m[16];  // column-major (glm layout)
-};
-
 // raygenPrimary.comp:44-87.
 __global__ __launch_bounds__(256) void raygen_primary_kernel(const Mat4 s2w, float ox, float oy,
                                                              float oz,
@@ -29,19 +25,7 @@ __global__ __launch_bounds__(256) void raygen_primary_kernel(const Mat4 s2w, flo
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t np = (int64_t)width * height;
     if (i >= np) return;
-    int pixel = i2p[i];
-    float x = (float)(pixel % width) + hx;
-    float y = (float)(pixel / width) + hy;
-    float nx = 2.0f * x / (float)width - 1.0f;
-    float ny = 2.0f * y / (float)height - 1.0f;
-    const float* m = s2w.m;
-    // mat4 * vec4(nx, ny, 0, 1), GLSL column-major
-    float wx = ((m[0] * nx + m[4] * ny) + m[8] * 0.0f) + m[12] * 1.0f;
-    float wy = ((m[1] * nx + m[5] * ny) + m[9] * 0.0f) + m[13] * 1.0f;
-    float wz = ((m[2] * nx + m[6] * ny) + m[10] * 0.0f) + m[14] * 1.0f;
-    float ww = ((m[3] * nx + m[7] * ny) + m[11] * 0.0f) + m[15] * 1.0f;
-    V3 wp = v3(wx / ww, wy / ww, wz / ww);
-    V3 d = normalize3(v3(wp.x - ox, wp.y - oy, wp.z - oz));
+    V3 d = primary_direction(s2w, i2p[i], width, height, hx, hy, ox, oy, oz);
     store_ray(rays, i, make_float4(ox, oy, oz, 0.0f), make_float4(d.x, d.y, d.z, max_dist));
 }
 
@@ -57,63 +41,23 @@ __global__ __launch_bounds__(256) void raygen_shadow_kernel(
     if (j >= count) return;
     int64_t src = begin + j;
     int pixel = in_s2i[src];
-    uint32_t seed = seeds[pixel];
     float4 ro, rd;
     load_ray(in_rays, src, ro, rd);
     const int4 hit = *reinterpret_cast<const int4*>(in_hits + src);
-    uint32_t ha = seed + (uint32_t)pixel, hb = 0x9e3779b9u, hc = 0x9e3779b9u;
-    jenkins_mix(ha, hb, hc);
-    jenkins_mix(ha, hb, hc);
-    const float two_m32 = 2.3283064365386963e-10f;  // exp2(-32)
-    float offx = (float)ha * two_m32, offy = (float)hb * two_m32, offz = (float)hc * two_m32;
-    int tri = hit.x;
-    float ht = __int_as_float(hit.z);
-    float adv = fmaxf(ht - 1.0e-5f, 0.0f);
-    V3 o = v3(ro.x + rd.x * adv, ro.y + rd.y * adv, ro.z + rd.z * adv);
-    if (tri == -1) o = v3(0.f, 0.f, 0.f);
+    const V3 off = shadow_offset(seeds[pixel], pixel);
+    const bool miss = hit.x == -1;
+    const V3 o = shadow_origin(ro, rd, __int_as_float(hit.z), miss);
     int64_t out = (int64_t)S * j;
     for (int i = 0; i < S; ++i, ++out) {
-        float px, py;
-        sobol2((uint32_t)i, px, py);
-        float pz = ((float)i + 0.5f) / (float)S;
-        px += offx;
-        py += offy;
-        pz += offz;
-        if (px >= 1.0f) px -= 1.0f;
-        if (py >= 1.0f) py -= 1.0f;
-        if (pz >= 1.0f) pz -= 1.0f;
-        px = 2.0f * px - 1.0f;
-        py = 2.0f * py - 1.0f;
-        pz = 2.0f * pz - 1.0f;
-        V3 target = v3(lx + radius * px, ly + radius * py, lz + radius * pz);
-        V3 dir = v3(target.x - o.x, target.y - o.y, target.z - o.z);
-        float len = sqrtf(dot3(dir, dir));
-        V3 nd = tri == -1 ? v3(1.f, 0.f, 0.f) : normalize3(dir);
-        store_ray(out_rays, out, make_float4(o.x, o.y, o.z, 1.0e-5f),
-                  make_float4(nd.x, nd.y, nd.z, tri == -1 ? -1.0f : len));
+        float4 ra, rb;
+        shadow_ray(i, S, off, o, miss, lx, ly, lz, radius, ra, rb);
+        store_ray(out_rays, out, ra, rb);
         out_s2i[out] = (int32_t)out;
         out_i2s[out] = (int32_t)out;
     }
 }
 
 // ----------------------------------------------------------- shade/extend
-// raygenPath.comp:28-49: Duff et al. 2017 orthonormal basis and cosine lobe.
-__device__ __forceinline__ V3 cosine_direction(float r0, float r1, V3 n) {
-    const float two_pi = 2.0f * 3.14159265358979323846f;
-    float theta = two_pi * r1;
-    float radius = sqrtf(r0);
-    float x = radius * sinf(theta);
-    float z = radius * cosf(theta);
-    float y = sqrtf(1.0f - r0);
-    float s = n.z >= 0.0f ? 1.0f : -1.0f;
-    float a = -1.0f / (s + n.z);
-    float b = n.x * n.y * a;
-    V3 b1 = v3(1.0f + s * n.x * n.x * a, s * b, -s * n.x);
-    V3 b2 = v3(b, s + n.y * n.y * a, -n.y);
-    return v3(x * b1.x + z * b2.x + y * n.x, x * b1.y + z * b2.y + y * n.y,
-              x * b1.z + z * b2.z + y * n.z);
-}
-
 constexpr int SHADE_BLOCK = 256;
 
 // One thread per input slot.  Survivors are written in input order: each

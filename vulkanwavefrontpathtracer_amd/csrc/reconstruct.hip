@@ -33,9 +33,8 @@ __global__ __launch_bounds__(256) void reconstruct_smooth_kernel(
         int m = clamp_material(shading_normal(shade, tri, (uint32_t)hit.w, nrm), nmat);
         float t = __int_as_float(hit.z);
         V3 p = v3(ro.x + rd.x * t, ro.y + rd.y * t, ro.z + rd.z * t);
-        V3 L = normalize3(v3(lx - p.x, ly - p.y, lz - p.z));
         V3 ff = faceforward3(nrm, v3(rd.x, rd.y, rd.z));
-        float alpha = fmaxf(dot3(ff, L), 0.1f);
+        float alpha = smooth_alpha(ff, p, lx, ly, lz);
         float er = emission[3 * m], eg = emission[3 * m + 1], eb = emission[3 * m + 2];
         cr += dec.x * (alpha + er) / spp;
         cg += dec.y * (alpha + eg) / spp;

@@ -80,8 +80,10 @@ template <typename K>
 __global__ __launch_bounds__(HIST_BLOCK) void sort_hist_kernel(const K* __restrict__ keys,
                                                                int64_t n, int passes,
                                                                int digit_bits, int key_bits,
-                                                               uint32_t* __restrict__ hist) {
+                                                               uint32_t* __restrict__ hist,
+                                                               const uint32_t* __restrict__ d_n) {
     extern __shared__ uint32_t s_hist[];  // passes * radix
+    if (d_n) n = min(n, (int64_t)*d_n);  // element count on the device (graph replays)
     const int radix = 1 << digit_bits;
     const int total = passes * radix;
     for (int i = threadIdx.x; i < total; i += HIST_BLOCK) s_hist[i] = 0;
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
     const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     K* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n, int shift, int width,
     uint32_t* __restrict__ tile_counter, const uint32_t* __restrict__ digit_offsets,
-    uint32_t* __restrict__ status) {
+    uint32_t* __restrict__ status, const uint32_t* __restrict__ d_n) {
     constexpr int RADIX = 1 << DBITS;
     constexpr int DPT = RADIX >= SORT_BLOCK ? RADIX / SORT_BLOCK : 1;  // digits per thread
     __shared__ uint32_t s_wave[SORT_WAVES][RADIX];  // per-wave counts -> wave prefixes
@@ -191,6 +193,11 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
     __syncthreads();
     const int64_t tile = s_tile;
     const int64_t base = tile * SORT_TILE;
+    // With the count on the device the grid covers the capacity: tiles are
+    // taken in arrival order, so the surplus workgroups are the last ones
+    // and nothing waits on them.
+    if (d_n) n = min(n, (int64_t)*d_n);
+    if (base >= n) return;
 
     // Wave-striped load: element index = base + wave*64*ITEMS + it*64 + lane.
     K key[SORT_ITEMS];
@@ -306,7 +313,9 @@ template <typename K>
 __global__ __launch_bounds__(256) void copy_pairs_kernel(const K* __restrict__ ks,
                                                          const uint32_t* __restrict__ vs,
                                                          K* __restrict__ kd,
-                                                         uint32_t* __restrict__ vd, int64_t n) {
+                                                         uint32_t* __restrict__ vd, int64_t n,
+                                                         const uint32_t* __restrict__ d_n) {
+    if (d_n) n = min(n, (int64_t)*d_n);
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         kd[i] = ks[i];
@@ -317,15 +326,15 @@ __global__ __launch_bounds__(256) void copy_pairs_kernel(const K* __restrict__ k
 template <typename K, int DBITS>
 static void launch_bin(const void* ki, const uint32_t* vi, void* ko, uint32_t* vo, int64_t n,
                        int shift, int width, uint32_t* ctr, const uint32_t* offs,
-                       uint32_t* status, unsigned tiles, hipStream_t s) {
+                       uint32_t* status, unsigned tiles, const uint32_t* d_n, hipStream_t s) {
     sort_bin_kernel<K, DBITS><<<tiles, SORT_BLOCK, 0, s>>>(
         static_cast<const K*>(ki), vi, static_cast<K*>(ko), vo, n, shift, width, ctr, offs,
-        status);
+        status, d_n);
 }
 
 template <typename K>
 static int sort_impl(void* keys, uint32_t* vals, int64_t n, int key_bits, int digit_bits,
-                     char* scratch, const SortPlan& p, hipStream_t s) {
+                     char* scratch, const SortPlan& p, const uint32_t* d_n, hipStream_t s) {
     uint32_t* counters = reinterpret_cast<uint32_t*>(scratch + p.off_counters);
     uint32_t* hist = reinterpret_cast<uint32_t*>(scratch + p.off_hist);
     uint32_t* offsets = reinterpret_cast<uint32_t*>(scratch + p.off_offsets);
@@ -345,7 +354,7 @@ static int sort_impl(void* keys, uint32_t* vals, int64_t n, int key_bits, int di
         if (blocks > 2048) blocks = 2048;
         size_t lds = sizeof(uint32_t) * (size_t)p.passes * p.radix;
         sort_hist_kernel<K><<<(unsigned)blocks, HIST_BLOCK, lds, s>>>(
-            static_cast<const K*>(keys), n, p.passes, digit_bits, key_bits, hist);
+            static_cast<const K*>(keys), n, p.passes, digit_bits, key_bits, hist, d_n);
         WF_TRY(launch_check("sort_hist_kernel"));
         sort_scan_kernel<<<p.passes, 256, 0, s>>>(hist, offsets, p.radix);
         WF_TRY(launch_check("sort_scan_kernel"));
@@ -382,18 +391,35 @@ static int sort_impl(void* keys, uint32_t* vals, int64_t n, int key_bits, int di
         const Buf& out = seq[pass + 1];
         unsigned tiles = (unsigned)p.tiles;
         switch (digit_bits) {
-            case 4: launch_bin<K, 4>(in.k, in.v, out.k, out.v, n, shift, width, ctr, offs, st, tiles, s); break;
-            case 8: launch_bin<K, 8>(in.k, in.v, out.k, out.v, n, shift, width, ctr, offs, st, tiles, s); break;
-            default: launch_bin<K, 11>(in.k, in.v, out.k, out.v, n, shift, width, ctr, offs, st, tiles, s); break;
+            case 4: launch_bin<K, 4>(in.k, in.v, out.k, out.v, n, shift, width, ctr, offs, st, tiles, d_n, s); break;
+            case 8: launch_bin<K, 8>(in.k, in.v, out.k, out.v, n, shift, width, ctr, offs, st, tiles, d_n, s); break;
+            default: launch_bin<K, 11>(in.k, in.v, out.k, out.v, n, shift, width, ctr, offs, st, tiles, d_n, s); break;
         }
         WF_TRY(launch_check("sort_bin_kernel"));
     }
     if (p.passes == 1) {
         copy_pairs_kernel<K><<<grid_for(n, 256), 256, 0, s>>>(
-            static_cast<const K*>(ka), va, static_cast<K*>(keys), vals, n);
+            static_cast<const K*>(ka), va, static_cast<K*>(keys), vals, n, d_n);
         WF_TRY(launch_check("copy_pairs_kernel"));
     }
     return WF_OK;
+}
+
+int sort_pairs_launch(void* keys, uint32_t* vals, int64_t n_cap, const uint32_t* d_n,
+                      int key_bytes, int key_bits, int digit_bits, void* scratch,
+                      int64_t scratch_bytes, hipStream_t s) {
+    SortPlan p;
+    WF_CHECK_ARG(make_plan(n_cap, key_bytes, key_bits, digit_bits, p),
+                 "sort: key_bytes must be 4 or 8, 1 <= key_bits <= 8*key_bytes, "
+                 "digit_bits in {4, 8, 11}, n >= 0");
+    if (n_cap > WF_MAX_ELEMENTS) return fail(WF_ERR_TOO_LARGE, "sort: n exceeds 2^30-1");
+    if (n_cap <= 1) return WF_OK;
+    WF_CHECK_ARG(keys && vals && scratch, "sort: null buffer");
+    if (scratch_bytes < p.total)
+        return fail(WF_ERR_SCRATCH_TOO_SMALL, "sort: scratch smaller than wf_sort_scratch_bytes");
+    return key_bytes == 4
+               ? sort_impl<uint32_t>(keys, vals, n_cap, key_bits, digit_bits, (char*)scratch, p, d_n, s)
+               : sort_impl<uint64_t>(keys, vals, n_cap, key_bits, digit_bits, (char*)scratch, p, d_n, s);
 }
 
 }  // namespace wfpt
@@ -412,23 +438,18 @@ extern "C" int64_t wf_sort_scratch_bytes(int64_t n, int key_bytes, int key_bits,
 extern "C" int wf_sort_pairs(void* keys, uint32_t* vals, int64_t n, int key_bytes, int key_bits,
                              int digit_bits, void* scratch, int64_t scratch_bytes,
                              wf_stream_t stream, float* ms) {
-    SortPlan p;
-    WF_CHECK_ARG(make_plan(n, key_bytes, key_bits, digit_bits, p),
-                 "wf_sort_pairs: key_bytes must be 4 or 8, 1 <= key_bits <= 8*key_bytes, "
-                 "digit_bits in {4, 8, 11}, n >= 0");
-    if (n > WF_MAX_ELEMENTS) return fail(WF_ERR_TOO_LARGE, "wf_sort_pairs: n exceeds 2^30-1");
-    if (n <= 1) {
-        if (ms) *ms = 0.f;
-        return WF_OK;
-    }
-    WF_CHECK_ARG(keys && vals && scratch, "wf_sort_pairs: null buffer");
-    if (scratch_bytes < p.total)
-        return fail(WF_ERR_SCRATCH_TOO_SMALL, "wf_sort_pairs: scratch smaller than wf_sort_scratch_bytes");
     hipStream_t s = as_stream(stream);
     StageTimer t(s, ms);
-    int st = key_bytes == 4
-                 ? sort_impl<uint32_t>(keys, vals, n, key_bits, digit_bits, (char*)scratch, p, s)
-                 : sort_impl<uint64_t>(keys, vals, n, key_bits, digit_bits, (char*)scratch, p, s);
-    if (st != WF_OK) return st;
+    WF_TRY(sort_pairs_launch(keys, vals, n, nullptr, key_bytes, key_bits, digit_bits, scratch,
+                             scratch_bytes, s));
     return t.finish();
+}
+
+extern "C" int wf_sort_pairs_count(void* keys, uint32_t* vals, int64_t capacity,
+                                   const uint32_t* d_count, int key_bytes, int key_bits,
+                                   int digit_bits, void* scratch, int64_t scratch_bytes,
+                                   wf_stream_t stream) {
+    WF_CHECK_ARG(d_count, "wf_sort_pairs_count: null count");
+    return sort_pairs_launch(keys, vals, capacity, d_count, key_bytes, key_bits, digit_bits,
+                             scratch, scratch_bytes, as_stream(stream));
 }

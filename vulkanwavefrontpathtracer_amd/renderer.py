@@ -71,6 +71,66 @@ class Renderer:
              out.ctypes.data_as(_capi.pf))
         return out
 
+    def pixels_tensor(self, tone_mapped=False):
+        """The framebuffer as a [h, w, 4] float32 torch tensor on the GPU (a
+        copy; the renderer's buffer is reused by the next frame)."""
+        import torch
+
+        w, h = self.size
+        p = C.c_void_p()
+        call("wf_renderer_pixels_device", self.handle, 1 if tone_mapped else 0, C.byref(p))
+        out = torch.empty((h, w, 4), dtype=torch.float32, device="cuda")
+        _capi.memcpy_d2d(out.data_ptr(), p.value, out.numel() * 4)
+        return out
+
+    def set_materials(self, albedo, emission):
+        """Per-material albedo and emission ([nmat, 3] float32, host arrays or
+        GPU tensors) of the renderer's scene."""
+        import torch
+
+        a = albedo.detach().float().contiguous() if torch.is_tensor(albedo) else \
+            np.ascontiguousarray(albedo, np.float32)
+        e = emission.detach().float().contiguous() if torch.is_tensor(emission) else \
+            np.ascontiguousarray(emission, np.float32)
+        pa = a.data_ptr() if torch.is_tensor(a) else a.ctypes.data
+        pe = e.data_ptr() if torch.is_tensor(e) else e.ctypes.data
+        if torch.is_tensor(a) and a.is_cuda:
+            torch.cuda.synchronize()
+        call("wf_scene_set_materials", self.scene_handle(), pa, pe, a.shape[0], None)
+
+    def adjoint(self, d_pixels, replay=False, frame=0):
+        """Path-replay adjoint of the last frame: d_pixels = dLoss/dpixel of
+        the linear frame estimate ([h, w, 4] or [h*w, 4] float32 on the GPU).
+        Returns (grad_albedo, grad_emission[, replayed frame]) as tensors, and
+        the kernel's GPU milliseconds in self.last_adjoint_ms."""
+        import torch
+
+        nm = self.num_materials()
+        dp = d_pixels.detach().float().contiguous()
+        ga = torch.empty((nm, 3), dtype=torch.float32, device=dp.device)
+        ge = torch.empty_like(ga)
+        rp = torch.empty_like(dp) if replay else None
+        ms = C.c_float(0)
+        torch.cuda.synchronize()
+        call("wf_renderer_adjoint", self.handle, frame, dp.data_ptr(), ga.data_ptr(), ge.data_ptr(),
+             rp.data_ptr() if replay else None, C.byref(ms))
+        self.last_adjoint_ms = ms.value
+        return (ga, ge, rp) if replay else (ga, ge)
+
+    def num_materials(self):
+        n = C.c_int()
+        call("wf_scene_get_materials", self.scene_handle(), C.byref(n), None, None)
+        return n.value
+
+    def materials(self):
+        """(albedo, emission) of the renderer's scene as [nmat, 3] numpy arrays."""
+        nm = self.num_materials()
+        a = np.zeros((nm, 3), np.float32)
+        e = np.zeros((nm, 3), np.float32)
+        n = C.c_int()
+        call("wf_scene_get_materials", self.scene_handle(), C.byref(n), a.ctypes.data, e.ctypes.data)
+        return a, e
+
     def stats(self) -> dict:
         counts = (C.c_uint64 * 3)()
         times = (C.c_float * 3)()

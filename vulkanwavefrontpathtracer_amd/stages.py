@@ -138,14 +138,20 @@ SCENE_BOXES, SCENE_INTERIOR, SCENE_CITY = 0, 1, 2
 class HostScene:
     """Host arrays of a procedural scene (vertices: [nv, 6] pos+normal)."""
 
-    def __init__(self, kind: int, seed: int):
+    def __init__(self, kind: int, seed: int, material_groups: int = 0, num_boxes: int = 0):
+        """kind = SCENE_BOXES / SCENE_INTERIOR / SCENE_CITY; num_boxes > 0 builds
+        the boxes generator with that many boxes (wf_procedural_boxes)."""
         import numpy as np
 
         lib = _capi.load()
         nv, nt = C.c_int64(), C.c_int64()
         ng, nm = C.c_int(), C.c_int()
-        call("wf_procedural_scene", kind, seed, C.byref(nv), C.byref(nt), C.byref(ng), C.byref(nm),
-             None, None, None, None, None, None, None, None)
+        if num_boxes > 0:
+            gen = lambda *a: call("wf_procedural_boxes", seed, num_boxes, *a)  # noqa: E731
+        else:
+            gen = lambda *a: call("wf_procedural_scene", kind, seed, material_groups, *a)  # noqa: E731
+        gen(C.byref(nv), C.byref(nt), C.byref(ng), C.byref(nm),
+            None, None, None, None, None, None, None, None)
         self.vertices = np.zeros((nv.value, 6), np.float32)
         self.indices = np.zeros((nt.value, 3), np.uint32)
         self.geom_first = np.zeros(ng.value + 1, np.int32)
@@ -158,7 +164,7 @@ class HostScene:
         def fp(a):
             return a.ctypes.data_as(_capi.pf)
 
-        call("wf_procedural_scene", kind, seed, None, None, None, None, fp(self.vertices),
+        gen(None, None, None, None, fp(self.vertices),
              self.indices.ctypes.data_as(_capi.pu32), self.geom_first.ctypes.data_as(_capi.pi32),
              self.tri_material.ctypes.data_as(_capi.pi32), fp(self.albedo), fp(self.emission),
              fp(self.bmin), fp(self.bmax))
@@ -194,8 +200,7 @@ class Scene:
         if albedo is not None:
             a = np.ascontiguousarray(albedo, np.float32)
             e = np.ascontiguousarray(emission, np.float32)
-            call("wf_scene_set_materials", h, a.ctypes.data_as(_capi.pf),
-                 e.ctypes.data_as(_capi.pf), len(a), _stream())
+            call("wf_scene_set_materials", h, a.ctypes.data, e.ctypes.data, len(a), _stream())
 
     @classmethod
     def from_host(cls, hs: "HostScene", device="cuda"):
