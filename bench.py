@@ -55,6 +55,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable workload")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
+    ap.add_argument("--no-large", action="store_true", help="skip the large-scene reorder ablation")
+    ap.add_argument("--batch-views", type=int, default=64)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (scales HBM counters)")
     args = ap.parse_args()
@@ -257,7 +260,9 @@ def main():
             "stage_ms": {k: round(v, 3) for k, v in fs["stage_ms"].items()},
         }
 
+    large = None if args.no_large else large_section(dev)
     adjoint = None if args.no_adjoint else adjoint_section(dev)
+    batched = None if args.no_batch else batch_section(dev, args.batch_views)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -295,7 +300,9 @@ def main():
         "sorted_vs_unsorted_hits_equal": bool(same),
         "reference_style_frame": ref_style,
         "untimed_capture_frame": {"path_rays": frame_stats["path_rays"]},
+        "large_ablation": large,
         "adjoint": adjoint,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -354,6 +361,216 @@ def adjoint_section(dev, reps=3):
             "rays_per_frame": rays, "forward_mrays_s": round(rays / (f_ms / 1e3) / 1e6, 1),
             "adjoint_mrays_s": round(rays / (a_ms / 1e3) / 1e6, 1), "loss": loss,
             "grad_norm": float(torch.sqrt((ga.double() ** 2).sum() + (ge.double() ** 2).sum()))}
+
+
+def large_section(dev, reps=3):
+    """Large scene reorder ablation: the city (~2.8M triangles), 1920x1080,
+    4 spp, 8 bounces.  For each key width (30-bit origin keys in u32, 60-bit
+    origin + octahedral keys in u64) and sort period (every bounce, every 2nd,
+    never) a frame of the C++ frame loop with those settings captures every
+    pass's path wavefronts; the 4 passes of each bounce are concatenated into
+    one wavefront (8.3M rays, all samples in flight) and the timed step runs
+    keys -> sort -> reorder -> closest-hit trace on the bounces the period
+    sorts and the trace alone on the others."""
+    import torch
+
+    from vulkanwavefrontpathtracer_amd import stages
+    from vulkanwavefrontpathtracer_amd.renderer import Renderer, env_path
+
+    env = env_path("config2.json")
+    cfg = json.loads(env.read_text())
+    depth, spp = cfg["Renderer"]["recursionDepth"], cfg["Renderer"]["numberOfPrimarySamples"]
+    log("large ablation: building the city scene")
+    base = Renderer(env)
+    base.build_scene()
+
+    class SceneRef:
+        handle = base.scene_handle()
+
+    smin, smax = (C.c_float * 3)(), (C.c_float * 3)()
+    stages.call("wf_scene_bounds", SceneRef.handle, smin, smax)
+    lib = stages._capi.load()
+    stream = torch.cuda.current_stream()
+    s_ptr = C.c_void_p(stream.cuda_stream)
+    modes = [("32-bit", "origin30", 1, 4, 30), ("60-bit", "origin30_oct30", 2, 8, 60)]
+    rows, sort_rates = [], {}
+    for name, key_mode, mode, kbytes, kbits in modes:
+        for period in (1, 2, 0):
+            r = Renderer(env, {"Renderer.keyMode": key_mode, "Renderer.reorderPeriod": period,
+                               "Renderer.sortPathRays": period != 0,
+                               "Renderer.reorderPathRays": period != 0})
+            r.set_scene(SceneRef, list(smin), list(smax))
+            r.set_capture(True)
+            r.render()
+            waves = []
+            for b in range(depth):
+                parts = [r.captured(p * depth + b) for p in range(spp)]
+                n = sum(x[2] for x in parts)
+                if n == 0:
+                    break
+                rays = torch.empty((n, 8), dtype=torch.float32, device=dev)
+                s2i = torch.empty(n, dtype=torch.int32, device=dev)
+                o = 0
+                torch.cuda.synchronize()
+                for rp, sp, k in parts:
+                    if k:
+                        d2d(rays[o:].data_ptr(), rp, 32 * k)
+                        d2d(s2i[o:].data_ptr(), sp, 4 * k)
+                        o += k
+                waves.append(dict(n=n, rays=rays, s2i=s2i))
+            del r
+            nmax = max(w["n"] for w in waves)
+            keys = torch.empty(nmax, dtype=torch.int64 if kbytes == 8 else torch.int32, device=dev)
+            idx = torch.empty(nmax, dtype=torch.int32, device=dev)
+            out_rays = torch.empty((nmax, 8), dtype=torch.float32, device=dev)
+            out_s2i = torch.empty(nmax, dtype=torch.int32, device=dev)
+            out_i2s = torch.empty(1920 * 1080, dtype=torch.int32, device=dev)
+            hits = torch.empty((nmax, 4), dtype=torch.int32, device=dev)
+            best = None
+            for digit_bits in ((11, 8) if period == 1 else (sort_rates[name]["digit_bits"],)):
+                need = lib.wf_sort_scratch_bytes(nmax, kbytes, kbits, digit_bits)
+                scratch = torch.empty(need, dtype=torch.uint8, device=dev)
+                ms = {"keys": 0.0, "sort": 0.0, "reorder": 0.0, "trace": 0.0}
+                sort_big = []
+                for rep in range(reps + 1):
+                    evs = []
+                    for b, w in enumerate(waves):
+                        n = w["n"]
+                        srt = period > 0 and b % period == 0
+                        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+                        ev[0].record(stream)
+                        if srt:
+                            stages._capi.check("wf_morton_keys", lib.wf_morton_keys(
+                                C.c_void_p(w["rays"].data_ptr()), C.c_void_p(keys.data_ptr()),
+                                C.c_void_p(idx.data_ptr()), n, mode, smin, smax, s_ptr, None))
+                        ev[1].record(stream)
+                        if srt:
+                            stages._capi.check("wf_sort_pairs", lib.wf_sort_pairs(
+                                C.c_void_p(keys.data_ptr()), C.c_void_p(idx.data_ptr()), n, kbytes,
+                                kbits, digit_bits, C.c_void_p(scratch.data_ptr()), need, s_ptr, None))
+                        ev[2].record(stream)
+                        if srt:
+                            stages._capi.check("wf_reorder", lib.wf_reorder(
+                                C.c_void_p(idx.data_ptr()), C.c_void_p(w["rays"].data_ptr()),
+                                C.c_void_p(out_rays.data_ptr()), C.c_void_p(w["s2i"].data_ptr()),
+                                C.c_void_p(out_s2i.data_ptr()), C.c_void_p(out_i2s.data_ptr()), n,
+                                s_ptr, None))
+                        ev[3].record(stream)
+                        src = out_rays if srt else w["rays"]
+                        stages._capi.check("wf_trace", lib.wf_trace(
+                            SceneRef.handle, C.c_void_p(src.data_ptr()), None,
+                            C.c_void_p(hits.data_ptr()), n, 1, s_ptr, None))
+                        ev[4].record(stream)
+                        evs.append((ev, n, srt))
+                    torch.cuda.synchronize()
+                    if rep == 0:
+                        continue                     # warm-up
+                    for ev, n, srt in evs:
+                        for k, nm in enumerate(["keys", "sort", "reorder", "trace"]):
+                            ms[nm] += ev[k].elapsed_time(ev[k + 1]) / reps
+                        if srt and n == nmax:
+                            sort_big.append(ev[1].elapsed_time(ev[2]))
+                total_rays = sum(w["n"] for w in waves)
+                t = sum(ms.values())
+                res = {"keys": name, "period": period, "digit_bits": digit_bits,
+                       "mrays_s": round(total_rays / (t / 1e3) / 1e6, 1),
+                       "ms_per_frame": {k: round(v, 3) for k, v in ms.items()},
+                       "rays": total_rays, "rays_per_bounce": [w["n"] for w in waves]}
+                if sort_big:
+                    passes = -(-kbits // digit_bits)
+                    bpe = passes * 2 * (kbytes + 4) + kbytes
+                    gbs = nmax * bpe / (min(sort_big) / 1e3) / 1e9
+                    res["sort_GBs"] = round(gbs, 1)
+                    res["sort_frac_hbm"] = round(gbs / HBM_PEAK_GBS, 4)
+                if best is None or res["mrays_s"] > best["mrays_s"]:
+                    best = res
+                del scratch
+            if period == 1:
+                sort_rates[name] = best
+            rows.append(best)
+            log(f"large ablation {name} period {period}: {best['mrays_s']} Mrays/s")
+    out = {"workload": "city ~2.8M tris, 1920x1080, 4 spp in flight (8.3M rays/bounce), 8 bounces",
+           "rows": rows}
+    for name in ("32-bit", "60-bit"):
+        every = next(x for x in rows if x["keys"] == name and x["period"] == 1)
+        never = next(x for x in rows if x["keys"] == name and x["period"] == 0)
+        out[f"speedup_every_vs_never_{name}"] = round(every["mrays_s"] / never["mrays_s"], 3)
+    return out
+
+
+def batch_section(dev, num_views=64, reps=2):
+    """Batched workload: num_views scenes from the boxes generator (triangles
+    log-uniform in [1k, 500k]), one view each with resolution in
+    {128, 256, 512, 1024}^2 and spp in 1..16, 6 bounces, 2 shadow samples.
+    The batch (one queue, scene-prefixed keys, graph-replayed passes,
+    autotuned) against the same views rendered one after another by the
+    single-view frame loop (one renderer per view, buffers allocated by an
+    untimed warm-up frame)."""
+    import torch
+
+    from vulkanwavefrontpathtracer_amd import stages
+    from vulkanwavefrontpathtracer_amd.batch import Batch, boxes_env_view, workload
+    from vulkanwavefrontpathtracer_amd.renderer import Renderer, env_path
+
+    boxes, seeds, vspec = workload(num_views=num_views)
+    log(f"batched workload: building {num_views} scenes ({sum(boxes) * 192:,} triangles)")
+    hosts = [stages.HostScene(0, sd, num_boxes=nb) for nb, sd in zip(boxes, seeds)]
+    scenes = [stages.Scene.from_host(h, dev) for h in hosts]
+    env = env_path("config0.json")
+    views = [boxes_env_view(env, v["scene"], v["width"], v["height"], v["spp"]) for v in vspec]
+    depth, S = 6, 2
+    b = Batch(scenes, views, depth=depth, shadow_samples=S)
+    log("batched workload: autotuning")
+    t0 = time.perf_counter()
+    best_pass_ms = b.autotune()
+    tune_s = time.perf_counter() - t0
+    tuning = b.tuning()
+    b.render()                                   # graph capture + warm-up
+    walls = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gpu_ms = b.render()
+        walls.append(time.perf_counter() - t0)
+    st = b.stats()
+    rays = st["primary_rays"] + st["shadow_rays"] + st["path_rays"]
+    batch_s = min(walls)
+
+    log("batched workload: sequential single-view frames")
+    cfg = json.loads(env.read_text())
+    rs = []
+    for v in views:
+        c = json.loads(json.dumps(cfg))
+        c["Resolution"] = {"width": v.width, "height": v.height}
+        c["Renderer"].update({"numberOfPrimarySamples": v.spp, "numberOfShadowSamples": S,
+                              "recursionDepth": depth, "rayType": "path", "russianRoulette": False,
+                              "sortPathRays": True, "reorderPathRays": True,
+                              "keyMode": "origin30_oct30"})
+        r = Renderer(c)
+        r.set_scene(scenes[v.scene], hosts[v.scene].bmin, hosts[v.scene].bmax)
+        r.render()                               # allocation + warm-up
+        rs.append(r)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for r in rs:
+        r.reset_frame_index()
+        r.render()
+    torch.cuda.synchronize()
+    seq_s = time.perf_counter() - t0
+    # spot parity: the largest and the smallest view against the batch
+    import numpy as np
+    sizes = [v.width * v.height * v.spp for v in views]
+    check = sorted({int(np.argmax(sizes)), int(np.argmin(sizes))})
+    equal = all(np.array_equal(b.pixels(k)[:, :3], rs[k].pixels()[:, :3]) for k in check)
+    return {"workload": f"{num_views} boxes scenes (tris log-uniform 1k-500k), res in "
+                        "{128,256,512,1024}^2, spp U{1..16}, 6 bounces, 2 shadow samples",
+            "triangles": sum(boxes) * 192 + 4 * num_views, "pixels": st["pixels"],
+            "rays_per_frame": rays, "batched_s": round(batch_s, 4),
+            "batched_gpu_ms": round(gpu_ms, 2), "batched_mrays_s": round(rays / batch_s / 1e6, 1),
+            "sequential_s": round(seq_s, 4), "sequential_mrays_s": round(rays / seq_s / 1e6, 1),
+            "speedup_vs_sequential": round(seq_s / batch_s, 3), "tuning": tuning,
+            "autotune_s": round(tune_s, 2), "autotune_best_pass_ms": round(best_pass_ms, 3),
+            "views_checked_equal": equal}
 
 
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_interior_1080p.txt"

@@ -328,7 +328,8 @@ int wf_env_get_string(wf_renderer r, const char* name, char* buf, int buf_len);
 int wf_env_get_vector(wf_renderer r, const char* name, float value[3]);
 /* Capture: when enabled, each path bounce's pre-sort wavefront (rays and
  * slot-to-pixel LUT) is copied into renderer-owned device buffers;
- * wf_renderer_captured returns bounce b's buffers (n = 0 past the last). */
+ * wf_renderer_captured returns the buffers of slot
+ * pass * recursionDepth + bounce (n = 0 where nothing was captured). */
 int wf_renderer_set_capture(wf_renderer r, int enable);
 int wf_renderer_captured(wf_renderer r, int bounce, const wf_ray** rays,
                          const int32_t** slot_to_index, int64_t* n);

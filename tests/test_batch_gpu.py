@@ -1,0 +1,91 @@
+"""GPU parity of the batched wavefront queue (wf_batch_*): every view of a
+batch of uneven scenes, resolutions and sample counts equals the
+single-view frame loop's frame for the same scene and settings bit for bit,
+for every tuning the autotuner may pick, with and without the captured HIP
+graph; the ray counts are those of the views."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DEPTH, S = 4, 2
+
+
+@pytest.fixture(scope="module")
+def small_batch(dev):
+    from vulkanwavefrontpathtracer_amd import stages
+    from vulkanwavefrontpathtracer_amd.batch import boxes_env_view
+    from vulkanwavefrontpathtracer_amd.renderer import env_path
+
+    counts = [3, 20, 64, 150]
+    hosts = [stages.HostScene(0, 3000 + k, num_boxes=n) for k, n in enumerate(counts)]
+    scenes = [stages.Scene.from_host(h, dev) for h in hosts]
+    env = env_path("config0.json")
+    shapes = [(32, 32, 1), (48, 40, 3), (64, 64, 2), (40, 56, 4), (24, 24, 3)]
+    views = [boxes_env_view(env, k % len(scenes), w, h, spp) for k, (w, h, spp) in enumerate(shapes)]
+    return hosts, scenes, views
+
+
+def single_view_frame(scene, host, view):
+    """The single-view frame loop (wf_renderer_render, frame 1) for a view."""
+    import json
+
+    from vulkanwavefrontpathtracer_amd.renderer import Renderer, env_path
+
+    cfg = json.loads(env_path("config0.json").read_text())
+    f = lambda v: " ".join(str(float(x)) for x in v)  # noqa: E731
+    cfg["Resolution"] = {"width": view.width, "height": view.height}
+    cfg["Renderer"].update({"numberOfPrimarySamples": view.spp, "numberOfShadowSamples": S,
+                            "recursionDepth": DEPTH, "rayType": "path", "russianRoulette": False,
+                            "shadowRadius": view.shadow_radius})
+    cfg["Scene"].update({"light": f(view.light), "backgroundcolor": f(view.background)})
+    cfg["Camera"].update({"position": f(view.position), "rotation": f(view.rotation),
+                          "fieldOfView": view.fov, "nearPlane": view.znear, "farPlane": view.zfar})
+    r = Renderer(cfg)
+    r.set_scene(scene, host.bmin, host.bmax)
+    r.render()
+    return r.pixels(), r.stats()
+
+
+@pytest.fixture(scope="module")
+def expected(small_batch):
+    hosts, scenes, views = small_batch
+    return [single_view_frame(scenes[v.scene], hosts[v.scene], v) for v in views]
+
+
+@pytest.mark.parametrize("digit_bits,block,period,graph", [
+    (8, 64, 1, True), (11, 128, 2, True), (4, 256, 4, True), (8, 64, 0, True), (8, 128, 1, False)])
+def test_batch_views_equal_single_view_frames(dev, small_batch, expected, digit_bits, block, period,
+                                              graph):
+    from vulkanwavefrontpathtracer_amd.batch import Batch
+
+    hosts, scenes, views = small_batch
+    b = Batch(scenes, views, depth=DEPTH, shadow_samples=S)
+    b.set_tuning(digit_bits, block, period)
+    b.set_graph(graph)
+    for rep in range(2):              # graph replays give the same frame every time
+        ms = b.render()
+        assert ms > 0
+        for k, v in enumerate(views):
+            got = b.pixels(k)
+            want, _ = expected[k]
+            assert np.array_equal(got[:, :3], want[:, :3]), (rep, k, np.abs(got - want).max())
+    st = b.stats()
+    assert st["primary_rays"] == sum(v.width * v.height * v.spp for v in views)
+    assert st["pixels"] == sum(v.width * v.height for v in views)
+    assert st["path_rays"] > 0 and st["shadow_rays"] == S * (st["primary_rays"] + st["path_rays"])
+
+
+def test_batch_autotune_keeps_results(dev, small_batch, expected):
+    from vulkanwavefrontpathtracer_amd.batch import Batch
+
+    hosts, scenes, views = small_batch
+    b = Batch(scenes, views, depth=DEPTH, shadow_samples=S)
+    best = b.autotune()
+    t = b.tuning()
+    assert best > 0
+    assert t["digit_bits"] in (4, 8, 11) and t["block"] in (64, 128, 256)
+    assert t["reorder_period"] in (1, 2, 4)
+    b.render()
+    for k in range(len(views)):
+        assert np.array_equal(b.pixels(k)[:, :3], expected[k][0][:, :3])

@@ -251,3 +251,22 @@ def test_adjoint_argument_checks_without_gpu():
         r_call("wf_renderer_adjoint", r.handle, 1, None, None, None, None, None)
     with pytest.raises(_capi.WfError, match="no scene"):
         _capi.call("wf_renderer_adjoint", r.handle, 1, 64, 64, 64, None, None)
+
+
+def test_sized_boxes_and_batched_workload_spec():
+    from vulkanwavefrontpathtracer_amd import stages
+    from vulkanwavefrontpathtracer_amd.batch import workload
+
+    hs = stages.HostScene(0, 3000, num_boxes=10)
+    assert hs.num_tris == 10 * 192 + 4 and hs.albedo.shape == (12, 3)
+    assert hs.emission[11].max() == 10.0                      # the light quad's material
+    base = stages.HostScene(0, 1000)
+    same = stages.HostScene(0, 1000, num_boxes=64)
+    assert np.array_equal(base.vertices, same.vertices)        # 64 boxes = the boxes scene
+    boxes, seeds, views = workload()
+    tris = np.array(boxes) * 192
+    assert len(views) == 64 and seeds[0] == 3000 and seeds[-1] == 3063
+    assert tris.min() >= 900 and tris.max() <= 500_200
+    assert {v["width"] for v in views} <= {128, 256, 512, 1024}
+    assert all(1 <= v["spp"] <= 16 for v in views)
+    assert np.log(tris).std() > 1.0                            # spread over decades

@@ -392,16 +392,18 @@ struct Frame {
             WF_HIP(hipStreamSynchronize(s));
             out.size = count;
             if (r->capture && count > 0) {
-                if ((int)r->cap.size() <= bounce) r->cap.emplace_back(new DeviceBuffer());
-                if ((int)r->cap_n.size() <= bounce) r->cap_n.push_back(0);
+                // slot = pass * recursionDepth + bounce
+                const size_t slot = (size_t)pass * (size_t)r->depth + (size_t)bounce;
+                while (r->cap.size() <= slot) r->cap.emplace_back(new DeviceBuffer());
+                if (r->cap_n.size() <= slot) r->cap_n.resize(slot + 1, 0);
                 // rays followed by the slot-to-pixel LUT
-                WF_TRY(r->cap[bounce]->ensure((sizeof(wf_ray) + sizeof(int32_t)) * count));
-                WF_HIP(hipMemcpyAsync(r->cap[bounce]->p, out.ray_buf(), sizeof(wf_ray) * count,
+                WF_TRY(r->cap[slot]->ensure((sizeof(wf_ray) + sizeof(int32_t)) * count));
+                WF_HIP(hipMemcpyAsync(r->cap[slot]->p, out.ray_buf(), sizeof(wf_ray) * count,
                                       hipMemcpyDeviceToDevice, s));
-                WF_HIP(hipMemcpyAsync(r->cap[bounce]->as<char>() + sizeof(wf_ray) * count,
+                WF_HIP(hipMemcpyAsync(r->cap[slot]->as<char>() + sizeof(wf_ray) * count,
                                       out.s2i_buf(), sizeof(int32_t) * count,
                                       hipMemcpyDeviceToDevice, s));
-                r->cap_n[bounce] = count;
+                r->cap_n[slot] = count;
             }
             WF_TRY(trace_rays(out, RK_PATH, r->sort_path, r->reorder_path));
             WF_TRY(reconstruct_smooth(out, aux));
