@@ -58,6 +58,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--no-large", action="store_true", help="skip the large-scene reorder ablation")
     ap.add_argument("--batch-views", type=int, default=64)
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="wf_set_tuning knob applied before the scene is built (repeatable)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (scales HBM counters)")
     args = ap.parse_args()
@@ -74,6 +76,9 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    for kv in args.tune:
+        k, v = kv.split("=")
+        stages.call("wf_set_tuning", k.encode(), int(v))
     if args.copy_calibration:
         nb = args.copy_calibration << 20
         src = torch.ones(nb // 4, dtype=torch.float32, device=dev)
@@ -288,6 +293,7 @@ def main():
             "rays_per_bounce": [x["n"] for x in bounces], "digit_bits": args.digit_bits,
             "parallelism": "1 GPU",
         },
+        "tuning": tuning_state(),
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "launch_ms": round(avg_launch, 4),
@@ -571,6 +577,18 @@ def batch_section(dev, num_views=64, reps=2):
             "speedup_vs_sequential": round(seq_s / batch_s, 3), "tuning": tuning,
             "autotune_s": round(tune_s, 2), "autotune_best_pass_ms": round(best_pass_ms, 3),
             "views_checked_equal": equal}
+
+
+def tuning_state():
+    from vulkanwavefrontpathtracer_amd import stages
+
+    out = {}
+    for k in ("bvh.presplit", "bvh.split_depth", "bvh.leaf_max", "trace.lds_stack", "trace.block",
+              "trace.persistent", "trace.refill"):
+        v = C.c_int()
+        stages.call("wf_get_tuning", k.encode(), C.byref(v))
+        out[k] = v.value
+    return out
 
 
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_interior_1080p.txt"

@@ -142,6 +142,10 @@ int wf_scene_set_materials(wf_scene scene, const float* albedo_rgb,
  * albedo and emission tables. */
 int wf_scene_get_materials(wf_scene scene, int* num_materials, float* albedo_rgb,
                            float* emission_rgb);
+/* The 4-wide quantized hierarchy: node count, traversal stack bound and,
+ * for a non-NULL host array (num_nodes4 * 16 floats), a copy of the nodes
+ * (layout in csrc/scene.h). */
+int wf_scene_wide_info(wf_scene scene, int64_t* num_nodes4, int* max_stack4, float* nodes4_host);
 /* Scene AABB (getSceneDimensions, base/VulkanglTFModel.cpp:1478-1488). */
 int wf_scene_bounds(wf_scene scene, float out_min[3], float out_max[3]);
 /* Sizes: triangles, BVH nodes, leaves. */
@@ -262,6 +266,11 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
  *   "bvh.leaf_max"     1-8  triangles per collapsed leaf (default 8)
  *   "trace.lds_stack"  0/1  keep the top of the traversal stack in LDS (default 1)
  *   "trace.block"      64/128/256  threads per traversal workgroup (default 64)
+ *   "trace.persistent" 0/1  persistent waves refilling idle lanes from a ray
+ *                           counter (default 1)
+ *   "trace.refill"     1-64 idle lanes of a wave that trigger a refill (default 16)
+ *   "trace.width"      2/4  traverse the 2-wide or the 4-wide quantized hierarchy
+ *                           (both are built; default 4)
  * Returns WF_ERR_INVALID_ARG for an unknown key or value out of range. */
 int wf_set_tuning(const char* key, int value);
 int wf_get_tuning(const char* key, int* value);

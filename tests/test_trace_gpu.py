@@ -73,6 +73,76 @@ def test_scene_build_structure(dev, boxes):
     del C
 
 
+def decode_wide(nodes4):
+    """Child boxes (float64), refs and counts of the 4-wide quantized nodes."""
+    raw = nodes4.view(np.uint32).reshape(-1, 16)
+    origin = nodes4.reshape(-1, 16)[:, :3].astype(np.float64)
+    meta = raw[:, 3]
+    nc = (meta >> 24).astype(int)
+    scale = np.stack([np.ldexp(1.0, ((meta >> (8 * a)) & 255).astype(int) - 127) for a in range(3)], 1)
+    q = raw[:, 4:10]                      # qlo x y z, qhi x y z
+    qb = np.stack([(q >> (8 * k)) & 255 for k in range(4)], 1).astype(np.float64)   # [n, child, 6]
+    lo = origin[:, None, :] + qb[:, :, :3] * scale[:, None, :]
+    hi = origin[:, None, :] + qb[:, :, 3:] * scale[:, None, :]
+    refs = raw[:, 10:14].view(np.int32)
+    return lo, hi, refs, nc
+
+
+def test_wide_hierarchy_structure_and_conservative_boxes(dev, boxes):
+    """The 4-wide hierarchy reaches every leaf slot once, each node once, and
+    every decoded (quantized) child box contains all (unsplit) triangles below
+    it."""
+    import ctypes as C
+
+    from vulkanwavefrontpathtracer_amd import _capi
+
+    hs, sc = boxes
+    info = sc.info()
+    n4, st4 = C.c_int64(), C.c_int()
+    _capi.call("wf_scene_wide_info", sc.handle, C.byref(n4), C.byref(st4), None)
+    nodes4 = np.zeros((n4.value, 16), np.float32)
+    _capi.call("wf_scene_wide_info", sc.handle, None, None, nodes4.ctypes.data_as(_capi.pf))
+    assert 0 < n4.value < info["nodes"] and 0 < st4.value <= 96
+    slots = info["nodes"] + 1
+    order = np.zeros(slots, np.int32)
+    n2 = np.zeros((info["nodes"], 16), np.float32)
+    _capi.call("wf_scene_download", sc.handle, n2.ctypes.data_as(_capi.pf),
+               order.ctypes.data_as(_capi.pi32))
+    lo, hi, refs, nc = decode_wide(nodes4)
+    # split triangles (several references) are covered by their references'
+    # clipped boxes, not whole; containment is checked for the others
+    whole = np.bincount(order, minlength=hs.num_tris)[order] == 1
+    assert np.all((nc >= 2) | (n4.value == 1)) and np.all(nc <= 4)
+    tv = hs.tri_verts().reshape(-1, 3, 3).astype(np.float64)
+    seen = np.zeros(n4.value, np.int32)
+    covered = np.zeros(slots, np.int32)
+
+    def subtree_slots(ref):
+        if ref < 0:
+            first, cnt = (~int(ref)) >> 3, ((~int(ref)) & 7) + 1
+            return list(range(first, first + cnt))
+        out = []
+        for k in range(nc[ref]):
+            out += subtree_slots(refs[ref, k])
+        return out
+
+    todo = [0]
+    while todo:
+        nd = todo.pop()
+        seen[nd] += 1
+        for k in range(nc[nd]):
+            r = int(refs[nd, k])
+            sl = subtree_slots(r)
+            pts = tv[order[[x for x in sl if whole[x]]]].reshape(-1, 3)
+            assert np.all(pts >= lo[nd, k] - 0.0) and np.all(pts <= hi[nd, k] + 0.0), (nd, k)
+            if r >= 0:
+                todo.append(r)
+            else:
+                covered[sl] += 1
+    assert seen.max() == 1 and seen.sum() == n4.value
+    assert np.all(covered == 1)
+
+
 @pytest.mark.parametrize("seed", [1, 2])
 def test_closest_hit_bit_exact_random_rays(dev, orc, boxes, seed):
     from vulkanwavefrontpathtracer_amd import stages
@@ -207,16 +277,20 @@ def test_trace_sort_composite(dev, boxes):
         assert all(t >= 0 for t in times) and times[1] > 0
 
 
-@pytest.mark.parametrize("presplit,leaf,lds,block", [(0, 4, 0, 256), (1, 1, 1, 64), (1, 8, 1, 128),
-                                                     (1, 4, 0, 256), (1, 2, 1, 256)])
-def test_tuning_variants_bit_exact(dev, orc, presplit, leaf, lds, block):
-    """Every build/traversal variant the autotuner may pick returns the same
-    records (interior scene: walls are split into many references)."""
+@pytest.mark.parametrize("presplit,leaf,lds,block,persistent,refill,width", [
+    (0, 4, 0, 256, 0, 16, 2), (1, 1, 1, 64, 0, 16, 2), (1, 8, 1, 128, 0, 16, 2), (1, 4, 0, 256, 0, 16, 4),
+    (1, 2, 1, 256, 0, 16, 4), (1, 8, 1, 64, 1, 16, 2), (1, 8, 1, 64, 1, 1, 4), (1, 8, 0, 128, 1, 64, 4),
+    (1, 4, 1, 256, 1, 32, 2), (1, 8, 1, 64, 1, 16, 4), (0, 1, 1, 64, 0, 16, 4)])
+def test_tuning_variants_bit_exact(dev, orc, presplit, leaf, lds, block, persistent, refill, width):
+    """Every build/traversal variant the autotuner may pick (including the
+    persistent kernel with its refill threshold) returns the same records
+    (interior scene: walls are split into many references)."""
     from vulkanwavefrontpathtracer_amd import _capi, stages
 
     saved = {}
     knobs = {"bvh.presplit": presplit, "bvh.leaf_max": leaf, "trace.lds_stack": lds,
-             "trace.block": block}
+             "trace.block": block, "trace.persistent": persistent, "trace.refill": refill,
+             "trace.width": width}
     import ctypes as C
     for k, v in knobs.items():
         cur = C.c_int()
@@ -227,8 +301,15 @@ def test_tuning_variants_bit_exact(dev, orc, presplit, leaf, lds, block):
         hs = stages.HostScene(stages.SCENE_INTERIOR, 1001)
         sc = stages.Scene.from_host(hs, dev)
         rays = scene_rays(hs, 800, 40 + leaf, tmin=1e-4)
+        rays[::97, 7] = -1.0                        # dead rays in the mix
         got = stages.trace(sc, torch.from_numpy(rays).to(dev)).cpu().numpy()
         assert np.array_equal(got, expected_hits(orc, hs, rays))
+        # any-hit visibility and indirect order through the same variant
+        tr = torch.from_numpy(rays).to(dev)
+        anyh = stages.trace(sc, tr, closest=False).cpu().numpy()
+        assert np.array_equal(anyh[:, 0] >= 0, got[:, 0] >= 0)
+        perm = torch.randperm(len(rays), device=dev).to(torch.int32)
+        assert np.array_equal(stages.trace(sc, tr, ray_index=perm).cpu().numpy(), got)
         if presplit:
             assert sc.info()["nodes"] + 1 > hs.num_tris   # walls were split
     finally:

@@ -60,6 +60,7 @@ SIGNATURES: dict[str, tuple] = {
     "wf_scene_destroy": (i32, [vp]),
     "wf_scene_set_materials": (i32, [vp, vp, vp, i32, vp]),
     "wf_scene_get_materials": (i32, [vp, C.POINTER(i32), vp, vp]),
+    "wf_scene_wide_info": (i32, [vp, pi64, C.POINTER(i32), pf]),
     "wf_scene_bounds": (i32, [vp, pf, pf]),
     "wf_scene_info": (i32, [vp, pi64, pi64, pi64, C.POINTER(i32)]),
     "wf_scene_download": (i32, [vp, pf, pi32]),

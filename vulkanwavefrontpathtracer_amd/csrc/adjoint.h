@@ -12,6 +12,7 @@ constexpr size_t ADJ_LDS_GRAD_BYTES = 32 * 1024;  // gradients kept in LDS up to
 
 struct AdjointParams {
     const float4* nodes;
+    const float4* nodes4;
     const float4* tris;
     const float4* shade;
     const float* albedo;

@@ -33,8 +33,9 @@ namespace wfpt {
 
 constexpr int ADJ_BLOCK = 64;
 
-template <bool LDS_GRAD>
+template <bool LDS_GRAD, bool WIDE>
 __global__ __launch_bounds__(ADJ_BLOCK) void adjoint_kernel(const AdjointParams p) {
+    const float4* nodes = WIDE ? p.nodes4 : p.nodes;
     __shared__ int s_stack[LDS_DEPTH * ADJ_BLOCK];
     extern __shared__ float s_grad[];  // [3 * nmat] albedo, then [3 * nmat] emission
     const int nm3 = 3 * p.nmat;
@@ -42,8 +43,16 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adjoint_kernel(const AdjointParams 
         for (int k = threadIdx.x; k < 2 * nm3; k += ADJ_BLOCK) s_grad[k] = 0.0f;
         __syncthreads();
     }
-    float* ga = LDS_GRAD ? s_grad : p.grad_albedo;
-    float* ge = LDS_GRAD ? s_grad + nm3 : p.grad_emission;
+    // gradient accumulation into LDS or straight into the outputs; the
+    // address space is fixed at compile time (no generic-pointer atomics)
+    auto add_ga = [&](int k, float v) {
+        if (LDS_GRAD) atomicAdd(&s_grad[k], v);
+        else atomicAdd(&p.grad_albedo[k], v);
+    };
+    auto add_ge = [&](int k, float v) {
+        if (LDS_GRAD) atomicAdd(&s_grad[nm3 + k], v);
+        else atomicAdd(&p.grad_emission[k], v);
+    };
 
     const int64_t np = (int64_t)p.width * p.height;
     const int64_t i = (int64_t)blockIdx.x * ADJ_BLOCK + threadIdx.x;
@@ -74,7 +83,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adjoint_kernel(const AdjointParams 
                 WtRay wr;
                 const bool live = (rd.w >= ro.w) && wt_setup(ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, wr);
                 const TraceOut tr =
-                    traverse<true, ADJ_BLOCK, true, false>(p.nodes, p.tris, ro, rd, wr, live, s_stack);
+                    traverse<true, ADJ_BLOCK, true, false, WIDE>(nodes, p.tris, ro, rd, wr, live, s_stack);
                 int tri;
                 const int4 hit = hit_record(p.tris, wr, tr, &tri);
                 float Y[3];
@@ -114,7 +123,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adjoint_kernel(const AdjointParams 
                     WtRay swr;
                     const bool slive = (sb.w >= sa.w) && wt_setup(sa.x, sa.y, sa.z, sb.x, sb.y, sb.z, swr);
                     const TraceOut st =
-                        traverse<false, ADJ_BLOCK, true, false>(p.nodes, p.tris, sa, sb, swr, slive, s_stack);
+                        traverse<false, ADJ_BLOCK, true, false, WIDE>(nodes, p.tris, sa, sb, swr, slive, s_stack);
                     if (st.slot < 0) vis += 1.0f;
                 }
                 vis /= (float)p.S;
@@ -124,7 +133,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adjoint_kernel(const AdjointParams 
                 for (int c = 0; c < 3; ++c) {
                     Y[c] = (alpha + e[c]) * vis / spp;
                     const float de = g[c] * T[c] * vis / spp;
-                    if (de != 0.0f) atomicAdd(&ge[3 * m + c], de);
+                    if (de != 0.0f) add_ge(3 * m + c, de);
                 }
 #pragma unroll
                 for (int k = 0; k < ADJ_MAX_VERTS; ++k)
@@ -155,7 +164,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adjoint_kernel(const AdjointParams 
             for (int k = 0; k < ADJ_MAX_VERTS; ++k)
                 if (k < nv)
                     for (int c = 0; c < 3; ++c)
-                        if (acc[k][c] != 0.0f) atomicAdd(&ga[3 * mat[k] + c], acc[k][c]);
+                        if (acc[k][c] != 0.0f) add_ga(3 * mat[k] + c, acc[k][c]);
         }
         if (p.replay) p.replay[pixel] = make_float4(Lr, Lg, Lb, 1.0f);
     }
@@ -172,10 +181,14 @@ int adjoint_launch(const AdjointParams& p, hipStream_t s) {
     const int64_t np = (int64_t)p.width * p.height;
     if (np == 0) return WF_OK;
     const size_t lds = sizeof(float) * 6 * (size_t)p.nmat;
+    // The replay always walks the 4-wide hierarchy: closest hits are exact
+    // and any-hit visibility is the same whichever hierarchy the forward
+    // pass traversed.
+    const unsigned g = grid_for(np, ADJ_BLOCK);
     if (lds <= ADJ_LDS_GRAD_BYTES)
-        adjoint_kernel<true><<<grid_for(np, ADJ_BLOCK), ADJ_BLOCK, lds, s>>>(p);
+        adjoint_kernel<true, true><<<g, ADJ_BLOCK, lds, s>>>(p);
     else
-        adjoint_kernel<false><<<grid_for(np, ADJ_BLOCK), ADJ_BLOCK, 0, s>>>(p);
+        adjoint_kernel<false, true><<<g, ADJ_BLOCK, 0, s>>>(p);
     return launch_check("adjoint_kernel");
 }
 

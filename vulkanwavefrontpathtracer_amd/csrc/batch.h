@@ -13,6 +13,7 @@ namespace wfpt {
 
 struct BatchScene {
     const float4* nodes;
+    const float4* nodes4;
     const float4* tris;
     const float4* shade;
     const int32_t* geom_first;
@@ -71,6 +72,7 @@ struct BatchBuffers {
 
 struct BatchTuning {
     int digit_bits = 8, block = 64, reorder_period = 1;
+    int width = 4;  // hierarchy traversed (the process-wide trace.width at render time)
 };
 
 // Enqueue one pass (primary rays of every view with spp > pass, then

@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void batch_primary_kernel(const BatchTables t,
 
 // ----------------------------------------------------------------- trace
 // Ray i belongs to the path owner_s2i[i / mult] (mult = S for shadow rays).
-template <bool CLOSEST, int BLOCK>
+template <bool CLOSEST, int BLOCK, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void batch_trace_kernel(const BatchTables t,
                                                             const uint32_t* __restrict__ q, int b,
                                                             int mult,
@@ -106,7 +106,8 @@ __global__ __launch_bounds__(BLOCK) void batch_trace_kernel(const BatchTables t,
     load_ray(rays, i, ro, rd);
     WtRay wr;
     const bool live = (rd.w >= ro.w) && wt_setup(ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, wr);
-    const TraceOut tr = traverse<CLOSEST, BLOCK, true, false>(sc.nodes, sc.tris, ro, rd, wr, live, s_stack);
+    const TraceOut tr =
+        traverse<CLOSEST, BLOCK, true, false, WIDE>(WIDE ? sc.nodes4 : sc.nodes, sc.tris, ro, rd, wr, live, s_stack);
     *reinterpret_cast<int4*>(hits + i) = hit_record(sc.tris, wr, tr);
 }
 
@@ -323,21 +324,29 @@ __global__ __launch_bounds__(256) void batch_reorder_kernel(const uint32_t* __re
 }
 
 // ------------------------------------------------------------------ host
-template <bool CLOSEST>
-static void launch_batch_trace(int block, const BatchTables& t, const uint32_t* q, int b, int mult,
-                               const wf_ray* rays, const int32_t* owner, wf_hit* hits,
-                               int64_t cap, hipStream_t s) {
+template <bool CLOSEST, bool WIDE>
+static void launch_batch_trace_w(int block, const BatchTables& t, const uint32_t* q, int b, int mult,
+                                 const wf_ray* rays, const int32_t* owner, wf_hit* hits,
+                                 int64_t cap, hipStream_t s) {
     switch (block) {
         case 128:
-            batch_trace_kernel<CLOSEST, 128><<<grid_for(cap, 128), 128, 0, s>>>(t, q, b, mult, rays, owner, hits);
+            batch_trace_kernel<CLOSEST, 128, WIDE><<<grid_for(cap, 128), 128, 0, s>>>(t, q, b, mult, rays, owner, hits);
             break;
         case 256:
-            batch_trace_kernel<CLOSEST, 256><<<grid_for(cap, 256), 256, 0, s>>>(t, q, b, mult, rays, owner, hits);
+            batch_trace_kernel<CLOSEST, 256, WIDE><<<grid_for(cap, 256), 256, 0, s>>>(t, q, b, mult, rays, owner, hits);
             break;
         default:
-            batch_trace_kernel<CLOSEST, 64><<<grid_for(cap, 64), 64, 0, s>>>(t, q, b, mult, rays, owner, hits);
+            batch_trace_kernel<CLOSEST, 64, WIDE><<<grid_for(cap, 64), 64, 0, s>>>(t, q, b, mult, rays, owner, hits);
             break;
     }
+}
+
+template <bool CLOSEST>
+static void launch_batch_trace(const BatchTuning& tu, const BatchTables& t, const uint32_t* q, int b,
+                               int mult, const wf_ray* rays, const int32_t* owner, wf_hit* hits,
+                               int64_t cap, hipStream_t s) {
+    if (tu.width == 4) launch_batch_trace_w<CLOSEST, true>(tu.block, t, q, b, mult, rays, owner, hits, cap, s);
+    else launch_batch_trace_w<CLOSEST, false>(tu.block, t, q, b, mult, rays, owner, hits, cap, s);
 }
 
 int batch_init_frame(const BatchTables& t, const BatchBuffers& bb, int64_t total_pixels, int frame,
@@ -362,7 +371,7 @@ static int batch_vertex(const BatchTables& t, const BatchBuffers& bb, const Batc
                         int cur, hipStream_t s) {
     const int64_t cap = bb.capacity;
     const int S = t.shadow_samples;
-    launch_batch_trace<true>(tu.block, t, bb.q, b, 1, bb.rays[cur], bb.s2i[cur], bb.hits, cap, s);
+    launch_batch_trace<true>(tu, t, bb.q, b, 1, bb.rays[cur], bb.s2i[cur], bb.hits, cap, s);
     WF_TRY(launch_check("batch_trace_kernel<closest>"));
     batch_smooth_kernel<<<grid_for(cap, 256), 256, 0, s>>>(t, bb.q, b, bb.rays[cur], bb.hits,
                                                           bb.s2i[cur], bb.aux, bb.dec);
@@ -370,7 +379,7 @@ static int batch_vertex(const BatchTables& t, const BatchBuffers& bb, const Batc
     batch_shadow_gen_kernel<<<grid_for(cap, 256), 256, 0, s>>>(t, bb.q, b, bb.seeds, bb.rays[cur],
                                                               bb.hits, bb.s2i[cur], bb.shadow_rays);
     WF_TRY(launch_check("batch_shadow_gen_kernel"));
-    launch_batch_trace<false>(tu.block, t, bb.q, b, S, bb.shadow_rays, bb.s2i[cur], bb.shadow_hits,
+    launch_batch_trace<false>(tu, t, bb.q, b, S, bb.shadow_rays, bb.s2i[cur], bb.shadow_hits,
                               cap * S, s);
     WF_TRY(launch_check("batch_trace_kernel<any>"));
     batch_shadow_recon_kernel<<<grid_for(cap, 256), 256, 0, s>>>(bb.q, b, S, bb.shadow_hits,

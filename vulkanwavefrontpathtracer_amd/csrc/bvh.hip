@@ -398,6 +398,200 @@ __global__ __launch_bounds__(256) void emit_tris_kernel(const char* __restrict__
     }
 }
 
+// ------------------------------------------------------- 4-wide hierarchy
+// Every interior node of the 2-wide hierarchy at even depth becomes a 4-wide
+// node whose children are its grandchildren (a leaf child is kept as it is).
+// A 2-wide node is real (emitted) when it is the root or holds more than
+// leaf_max references; its depth counts its real ancestors, all of which are
+// real since ranges nest.
+__global__ __launch_bounds__(256) void wide_mark_kernel(int n, int leaf_max,
+                                                        const int2* __restrict__ range,
+                                                        const int32_t* __restrict__ parent,
+                                                        uint32_t* __restrict__ flag,
+                                                        uint32_t* __restrict__ depth_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int2 r = range[i];
+    const bool real = i == 0 || (r.y - r.x + 1) > leaf_max;
+    uint32_t depth = 0;
+    for (int p = parent[i]; p >= 0; p = parent[p]) ++depth;
+    flag[i] = (real && (depth & 1u) == 0u) ? 1u : 0u;
+    if (real) atomicMax(depth_out, depth);
+}
+
+// Exclusive prefix of 0/1 flags in index order (decoupled look-back, one
+// element per thread); the last tile writes the total.
+__global__ __launch_bounds__(256) void flag_scan_kernel(const uint32_t* __restrict__ flag, int64_t n,
+                                                        uint32_t* __restrict__ out,
+                                                        uint32_t* __restrict__ tile_counter,
+                                                        uint32_t* __restrict__ status,
+                                                        uint32_t* __restrict__ total) {
+    __shared__ uint32_t s_tile, s_excl, s_wave[4];
+    const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+    __syncthreads();
+    const int64_t tile = s_tile;
+    const int64_t ntiles = (n + 255) / 256;
+    const int64_t i = tile * 256 + tid;
+    const bool f = i < n && flag[i] != 0u;
+    const uint64_t bal = ballot64(f);
+    if (lane == 0) s_wave[wave] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t wbase = 0, sum = 0;
+    for (int w = 0; w < 4; ++w) {
+        if (w < (int)wave) wbase += s_wave[w];
+        sum += s_wave[w];
+    }
+    if (tid == 0) {
+        uint32_t excl = 0;
+        if (tile == 0) {
+            lb_store(&status[0], LB_FLAG_INC | sum);
+        } else {
+            lb_store(&status[tile], LB_FLAG_AGG | sum);
+            for (int64_t k = tile - 1;; --k) {
+                uint32_t v;
+                do {
+                    v = lb_load(&status[k]);
+                } while ((v & ~LB_MASK) == 0u);
+                excl += v & LB_MASK;
+                if (v & LB_FLAG_INC) break;
+            }
+            lb_store(&status[tile], LB_FLAG_INC | (excl + sum));
+        }
+        s_excl = excl;
+        if (tile == ntiles - 1) *total = excl + sum;
+    }
+    __syncthreads();
+    if (i < n) out[i] = s_excl + wbase + popc_below(bal);
+}
+
+__device__ __forceinline__ int wide_exponent(float extent, int e_min) {
+    int e = e_min;
+    const float x = extent / 252.0f;
+    if (x > 0.0f) {
+        int ex = 0;
+        (void)frexpf(x, &ex);  // x = m * 2^ex, m in [0.5, 1): 2^ex >= x
+        e = max(e, ex);
+    }
+    return min(max(e, -126), 127);
+}
+
+// Quantized 4-wide node from a flagged 2-wide node.  e_min keeps every
+// quantum at least 2^-19 of the larger of the scene's diagonal and its
+// largest coordinate magnitude, far above the rounding of the slab test
+// (a few ulps, 2^-22 of the same), so the outward rounding by one quantum
+// keeps the boxes conservative.
+__device__ __forceinline__ int wide_min_exponent(const uint32_t* __restrict__ bounds) {
+    float m = 0.0f, d2 = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        const float lo = funkey(bounds[a]), hi = funkey(bounds[3 + a]);
+        m = fmaxf(m, fmaxf(fabsf(lo), fabsf(hi)));
+        d2 += (hi - lo) * (hi - lo);
+    }
+    const float size = fmaxf(fmaxf(m, sqrtf(d2)), 1e-30f);
+    int ex = 0;
+    (void)frexpf(size, &ex);
+    return ex - 18;  // quantum >= 2^-18 * 2^ex > 2^-18 * size... >= 2^-19 * size
+}
+
+// Single-reference scene: one 4-wide node with the leaf as its only child.
+__global__ void wide_single_kernel(const float4* __restrict__ nodes, const uint32_t* __restrict__ bounds,
+                                   float4* __restrict__ nodes4) {
+    const float4 a = nodes[0], b = nodes[1];
+    const float lo[3] = {a.x, a.y, a.z}, hi[3] = {a.w, b.x, b.y};
+    const int e_min = wide_min_exponent(bounds);
+    float origin[3];
+    uint32_t meta = 1u << 24, qlo[3], qhi[3];
+    for (int k = 0; k < 3; ++k) {
+        const int e = wide_exponent(hi[k] - lo[k], e_min);
+        const float scale = ldexpf(1.0f, e);
+        origin[k] = lo[k] - scale;
+        meta |= (uint32_t)(e + 127) << (8 * k);
+        qlo[k] = (uint32_t)fminf(fmaxf(floorf((lo[k] - origin[k]) / scale) - 1.0f, 0.0f), 255.0f);
+        qhi[k] = (uint32_t)fminf(fmaxf(ceilf((hi[k] - origin[k]) / scale) + 1.0f, 0.0f), 255.0f);
+    }
+    nodes4[0] = make_float4(origin[0], origin[1], origin[2], __uint_as_float(meta));
+    nodes4[1] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qlo[1]), __uint_as_float(qlo[2]),
+                            __uint_as_float(qhi[0]));
+    nodes4[2] = make_float4(__uint_as_float(qhi[1]), __uint_as_float(qhi[2]),
+                            __int_as_float(leaf_ref(0, 1)), 0.f);
+    nodes4[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(256) void wide_emit_kernel(int n, const uint32_t* __restrict__ flag,
+                                                        const uint32_t* __restrict__ idx4,
+                                                        const float4* __restrict__ nodes,
+                                                        float4* __restrict__ nodes4,
+                                                        const uint32_t* __restrict__ bounds) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1 || !flag[i]) return;
+    const int e_min = wide_min_exponent(bounds);
+    float lo[4][3], hi[4][3];
+    int ref[4];
+    int nc = 0;
+    auto node_child = [&](int node, int c, float* l, float* h, int& r) {
+        const float4* nd = nodes + 4 * (int64_t)node;
+        const float4 a = nd[0], b = nd[1], cc = nd[2], d = nd[3];
+        if (c == 0) {
+            l[0] = a.x; l[1] = a.y; l[2] = a.z; h[0] = a.w; h[1] = b.x; h[2] = b.y;
+            r = __float_as_int(d.x);
+        } else {
+            l[0] = b.z; l[1] = b.w; l[2] = cc.x; h[0] = cc.y; h[1] = cc.z; h[2] = cc.w;
+            r = __float_as_int(d.y);
+        }
+    };
+    for (int c = 0; c < 2; ++c) {
+        float l[3], h[3];
+        int r;
+        node_child(i, c, l, h, r);
+        if (r >= 0) {
+            for (int g = 0; g < 2; ++g) {
+                int rg;
+                node_child(r, g, lo[nc], hi[nc], rg);
+                ref[nc++] = rg >= 0 ? (int)idx4[rg] : rg;
+            }
+        } else {
+            for (int a = 0; a < 3; ++a) {
+                lo[nc][a] = l[a];
+                hi[nc][a] = h[a];
+            }
+            ref[nc++] = r;
+        }
+    }
+    float ulo[3] = {INFINITY, INFINITY, INFINITY}, uhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < nc; ++k)
+        for (int a = 0; a < 3; ++a) {
+            ulo[a] = fminf(ulo[a], lo[k][a]);
+            uhi[a] = fmaxf(uhi[a], hi[k][a]);
+        }
+    float origin[3], scale[3];
+    uint32_t meta = (uint32_t)nc << 24;
+    for (int a = 0; a < 3; ++a) {
+        const int e = wide_exponent(uhi[a] - ulo[a], e_min);
+        scale[a] = ldexpf(1.0f, e);
+        origin[a] = ulo[a] - scale[a];
+        meta |= (uint32_t)(e + 127) << (8 * a);
+    }
+    uint32_t qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0};
+    for (int k = 0; k < nc; ++k)
+        for (int a = 0; a < 3; ++a) {
+            const float ql = floorf((lo[k][a] - origin[a]) / scale[a]) - 1.0f;
+            const float qh = ceilf((hi[k][a] - origin[a]) / scale[a]) + 1.0f;
+            const uint32_t l = (uint32_t)fminf(fmaxf(ql, 0.0f), 255.0f);
+            const uint32_t h = (uint32_t)fminf(fmaxf(qh, 0.0f), 255.0f);
+            qlo[a] |= l << (8 * k);
+            qhi[a] |= h << (8 * k);
+        }
+    for (int k = nc; k < 4; ++k) ref[k] = 0;
+    float4* o = nodes4 + 4 * (int64_t)idx4[i];
+    o[0] = make_float4(origin[0], origin[1], origin[2], __uint_as_float(meta));
+    o[1] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qlo[1]), __uint_as_float(qlo[2]),
+                       __uint_as_float(qhi[0]));
+    o[2] = make_float4(__uint_as_float(qhi[1]), __uint_as_float(qhi[2]), __int_as_float(ref[0]),
+                       __int_as_float(ref[1]));
+    o[3] = make_float4(__int_as_float(ref[2]), __int_as_float(ref[3]), 0.f, 0.f);
+}
+
 // ------------------------------------------------------------------ host
 struct DevBuf {
     void* p = nullptr;
@@ -417,6 +611,7 @@ static void free_scene(wf_scene_s* s) {
     if (!s) return;
     if (s->tris) (void)hipFree(s->tris);
     if (s->nodes) (void)hipFree(s->nodes);
+    if (s->nodes4) (void)hipFree(s->nodes4);
     if (s->shade) (void)hipFree(s->shade);
     if (s->geom_first) (void)hipFree(s->geom_first);
     if (s->albedo) (void)hipFree(s->albedo);
@@ -519,7 +714,16 @@ extern "C" int wf_scene_create(const void* vertices, int64_t num_vertices, int v
     if (n == 0) {
         WF_HIP(hipMalloc((void**)&sc->tris, sizeof(float4) * 3));
         WF_HIP(hipMalloc((void**)&sc->nodes, sizeof(float4) * 4));
+        WF_HIP(hipMalloc((void**)&sc->nodes4, sizeof(float4) * 4));
+        // empty nodes: inverted boxes / no children, so any traversal ends at the root
+        const float inf = INFINITY;
+        const float empty2[16] = {inf, inf, inf, -inf, -inf, -inf, inf, inf,
+                                  inf, -inf, -inf, -inf, 0.f, 0.f, 0.f, 0.f};
+        const float empty4[16] = {0.f};
+        WF_HIP(hipMemcpy(sc->nodes, empty2, sizeof(empty2), hipMemcpyHostToDevice));
+        WF_HIP(hipMemcpy(sc->nodes4, empty4, sizeof(empty4), hipMemcpyHostToDevice));
         sc->num_nodes = 1;
+        sc->num_nodes4 = 1;
         sc->num_slots = 0;
         for (int a = 0; a < 3; ++a) sc->bmin[a] = sc->bmax[a] = 0.f;
         sc->num_leaves = 0;
@@ -639,6 +843,46 @@ extern "C" int wf_scene_create(const void* vertices, int64_t num_vertices, int v
             (const float4*)box.p, b, sc->nodes, (uint32_t*)misc.p + 2);
         WF_TRY(launch_check("emit_nodes_kernel"));
     }
+    // 4-wide hierarchy
+    {
+        DevBuf flag4, idx4, scan_state, wide_misc;
+        const int64_t nin = m > 1 ? m - 1 : 1;
+        WF_TRY(dev_alloc(wide_misc, sizeof(uint32_t) * 4));  // [0] depth, [1] nodes
+        WF_TRY(fill_u32((uint32_t*)wide_misc.p, 4, 0u, s));
+        uint32_t hw[4] = {0, 1, 0, 0};
+        if (m > 1) {
+            WF_TRY(dev_alloc(flag4, sizeof(uint32_t) * nin));
+            WF_TRY(dev_alloc(idx4, sizeof(uint32_t) * nin));
+            const int64_t tiles = (nin + 255) / 256;
+            WF_TRY(dev_alloc(scan_state, sizeof(uint32_t) * (1 + tiles)));
+            WF_TRY(fill_u32((uint32_t*)scan_state.p, 1 + tiles, 0u, s));
+            wide_mark_kernel<<<grid_for(nin, 256), 256, 0, s>>>(
+                (int)m, tu.bvh_leaf_max, (const int2*)range.p, (const int32_t*)parent.p,
+                (uint32_t*)flag4.p, (uint32_t*)wide_misc.p);
+            WF_TRY(launch_check("wide_mark_kernel"));
+            flag_scan_kernel<<<(unsigned)tiles, 256, 0, s>>>(
+                (const uint32_t*)flag4.p, nin, (uint32_t*)idx4.p, (uint32_t*)scan_state.p,
+                (uint32_t*)scan_state.p + 1, (uint32_t*)wide_misc.p + 1);
+            WF_TRY(launch_check("flag_scan_kernel"));
+            WF_HIP(hipMemcpyAsync(hw, wide_misc.p, sizeof(hw), hipMemcpyDeviceToHost, s));
+            WF_HIP(hipStreamSynchronize(s));
+        }
+        sc->num_nodes4 = hw[1];
+        // at most 3 pushes per 4-wide level; levels = real 2-wide depth / 2 + 1
+        sc->max_stack4 = 3 * ((int)hw[0] / 2 + 1);
+        if (sc->max_stack4 > TRACE_STACK4)
+            return fail(WF_ERR_TOO_LARGE, "wf_scene_create: 4-wide BVH deeper than the traversal stack");
+        WF_HIP(hipMalloc((void**)&sc->nodes4, sizeof(float4) * 4 * sc->num_nodes4));
+        if (m > 1) {
+            wide_emit_kernel<<<grid_for(nin, 256), 256, 0, s>>>(
+                (int)m, (const uint32_t*)flag4.p, (const uint32_t*)idx4.p, sc->nodes, sc->nodes4, b);
+            WF_TRY(launch_check("wide_emit_kernel"));
+        } else {
+            wide_single_kernel<<<1, 1, 0, s>>>(sc->nodes, b, sc->nodes4);
+            WF_TRY(launch_check("wide_single_kernel"));
+        }
+        WF_HIP(hipStreamSynchronize(s));  // the temporaries leave scope
+    }
     emit_tris_kernel<<<grid_for(m, 256), 256, 0, s>>>(
         (const char*)vertices, vertex_stride, indices, (const uint32_t*)vals.p, ref_tri,
         (const int32_t*)tri_geom.p, sc->geom_first, m, sc->tris);
@@ -684,6 +928,17 @@ extern "C" int wf_scene_info(wf_scene scene, int64_t* num_tris, int64_t* num_nod
     if (num_nodes) *num_nodes = scene->num_slots >= 1 ? scene->num_nodes : 0;
     if (num_leaves) *num_leaves = scene->num_leaves;
     if (num_geoms) *num_geoms = scene->num_geoms;
+    return WF_OK;
+}
+
+extern "C" int wf_scene_wide_info(wf_scene scene, int64_t* num_nodes4, int* max_stack4,
+                                  float* nodes4_host) {
+    WF_CHECK_ARG(scene, "wf_scene_wide_info: null scene");
+    if (num_nodes4) *num_nodes4 = scene->num_nodes4;
+    if (max_stack4) *max_stack4 = scene->max_stack4;
+    if (nodes4_host)
+        WF_HIP(hipMemcpy(nodes4_host, scene->nodes4, sizeof(float4) * 4 * scene->num_nodes4,
+                         hipMemcpyDeviceToHost));
     return WF_OK;
 }
 

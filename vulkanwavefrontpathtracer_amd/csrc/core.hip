@@ -82,6 +82,9 @@ extern "C" int wf_set_tuning(const char* key, int value) {
     else if (k == "bvh.leaf_max" && value >= 1 && value <= 8) t.bvh_leaf_max = value;
     else if (k == "trace.lds_stack" && (value == 0 || value == 1)) t.trace_lds_stack = value;
     else if (k == "trace.block" && (value == 64 || value == 128 || value == 256)) t.trace_block = value;
+    else if (k == "trace.persistent" && (value == 0 || value == 1)) t.trace_persistent = value;
+    else if (k == "trace.refill" && value >= 1 && value <= 64) t.trace_refill = value;
+    else if (k == "trace.width" && (value == 2 || value == 4)) t.trace_width = value;
     else return fail(WF_ERR_INVALID_ARG, "wf_set_tuning: unknown key or value out of range: " + k);
     return WF_OK;
 }
@@ -95,6 +98,9 @@ extern "C" int wf_get_tuning(const char* key, int* value) {
     else if (k == "bvh.leaf_max") *value = t.bvh_leaf_max;
     else if (k == "trace.lds_stack") *value = t.trace_lds_stack;
     else if (k == "trace.block") *value = t.trace_block;
+    else if (k == "trace.persistent") *value = t.trace_persistent;
+    else if (k == "trace.refill") *value = t.trace_refill;
+    else if (k == "trace.width") *value = t.trace_width;
     else return fail(WF_ERR_INVALID_ARG, "wf_get_tuning: unknown key " + k);
     return WF_OK;
 }

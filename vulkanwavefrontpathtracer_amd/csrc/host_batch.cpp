@@ -117,11 +117,13 @@ static int capture_pass(wf_batch_s* b) {
 }
 
 static bool same_tuning(const BatchTuning& a, const BatchTuning& c) {
-    return a.digit_bits == c.digit_bits && a.block == c.block && a.reorder_period == c.reorder_period;
+    return a.digit_bits == c.digit_bits && a.block == c.block &&
+           a.reorder_period == c.reorder_period && a.width == c.width;
 }
 
 static int run_pass(wf_batch_s* b, int frame, int pass) {
     const BatchBuffers bb = buffers(b);
+    b->tuning.width = tuning().trace_width;
     WF_TRY(batch_set_pass(bb, frame, pass, b->pass_count[pass], b->depth, b->stream));
     if (b->use_graph) {
         if (!b->graph || !same_tuning(b->graph_tuning, b->tuning)) WF_TRY(capture_pass(b));
@@ -216,6 +218,7 @@ extern "C" int wf_batch_create(const wf_scene* scenes, int num_scenes, const wf_
         const wf_scene sc = scenes[s];
         BatchScene bs{};
         bs.nodes = sc->nodes;
+        bs.nodes4 = sc->nodes4;
         bs.tris = sc->tris;
         bs.shade = sc->shade;
         bs.geom_first = sc->geom_first;

@@ -623,6 +623,7 @@ extern "C" int wf_renderer_adjoint(wf_renderer r, int frame, const float* d_pixe
     WF_TRY(fill_u32(reinterpret_cast<uint32_t*>(grad_emission), 3 * (int64_t)nmat, 0u, s));
     AdjointParams p{};
     p.nodes = sc->nodes;
+    p.nodes4 = sc->nodes4;
     p.tris = sc->tris;
     p.shade = sc->shade;
     p.albedo = sc->albedo;

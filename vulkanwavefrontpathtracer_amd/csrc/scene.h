@@ -9,6 +9,15 @@
 //             {c0.lo.xyz, c0.hi.x}, {c0.hi.yz, c1.lo.xy}, {c1.lo.z, c1.hi.xyz},
 //             {ref0, ref1, 0, 0} (int bits).  ref >= 0: internal node index;
 //             ref < 0: leaf ~(first_slot << 3 | (count - 1)), count <= 8.
+//   nodes4  [num_nodes4] x 4 float4 (64 bytes), the same hierarchy with
+//           every second level collapsed (2-4 children per node) and the child
+//           boxes quantized to 8 bits per plane against the node's own frame:
+//             {origin.xyz, exponents ex | ey << 8 | ez << 16 | nchild << 24},
+//             {qlo.x, qlo.y, qlo.z, qhi.x}, {qhi.y, qhi.z, ref0, ref1},
+//             {ref2, ref3, 0, 0}; byte k of each q word belongs to child k;
+//           child box plane = origin + q * 2^(e - 127), rounded outwards by
+//           at least one quantum (conservative); refs as in nodes, interior
+//           refs index nodes4.
 //   shade   [3 * num_tris] float4 in ORIGINAL triangle order:
 //             {n0.xyz, material}, {n1.xyz, 0}, {n2.xyz, 0}
 //   geom_first [num_geoms + 1] int: first original triangle of each geometry.
@@ -29,6 +38,9 @@ struct wf_scene_s {
     float bmax[3] = {0, 0, 0};
     float4* tris = nullptr;
     float4* nodes = nullptr;
+    float4* nodes4 = nullptr;
+    int64_t num_nodes4 = 0;
+    int max_stack4 = 0;  // traversal stack bound of nodes4
     float4* shade = nullptr;
     int32_t* geom_first = nullptr;
     float* albedo = nullptr;
@@ -38,7 +50,8 @@ struct wf_scene_s {
 
 namespace wfpt {
 
-constexpr int TRACE_STACK = 64;        // traversal stack entries per ray
+constexpr int TRACE_STACK = 64;        // traversal stack entries per ray (2-wide)
+constexpr int TRACE_STACK4 = 96;       // traversal stack entries per ray (4-wide)
 
 __device__ __forceinline__ int leaf_ref(int first, int count) { return ~((first << 3) | (count - 1)); }
 __device__ __forceinline__ int leaf_first(int ref) { return (~ref) >> 3; }

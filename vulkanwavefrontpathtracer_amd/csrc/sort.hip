@@ -2,20 +2,22 @@
 //
 // Replaces the vulkan_radix_sort dependency called by Tracer::radixSort
 // (projects/WaveFrontPathTracer/Tracer/Tracer.cpp:39-78 ->
-// external/vulkan_radix_sort/vk_radix_sort.h:2579-2585).  The reference runs
-// 4 x (upsweep, spine, downsweep) with 8-bit digits on u32 keys; this is a
-// onesweep-style design for CDNA4:
+// external/vulkan_radix_sort/vk_radix_sort.h:2579-2585), which runs 4 fixed
+// 8-bit passes on u32 keys.  Here:
 //   1. one histogram kernel reads every key once and counts the digits of
-//      all passes (LDS histograms, one global atomic per non-empty bin);
-//   2. one tiny scan kernel turns them into per-pass digit start offsets;
-//   3. one binning kernel per pass: each 256-thread workgroup (4 wave64)
-//      takes a 4096-element tile in arrival order (atomic tile counter),
-//      ranks its keys stably with wave64 ballot matching, publishes its
-//      per-digit counts and resolves the exclusive prefix of earlier tiles by
-//      decoupled look-back, stages the tile in LDS in digit order and writes
-//      each digit run contiguously.
+//      all passes (LDS histograms, one global atomic per non-empty bin), and
+//      a tiny scan turns them into per-pass digit start offsets;
+//   2. per pass, reduce-then-scan with no dependency between workgroups of
+//      a kernel: an upsweep counts each 4096-element tile's digits, a spine
+//      scans every digit's row of tile counts (one workgroup per digit) and
+//      adds the digit's start, and a downsweep ranks each tile stably with
+//      wave64 ballot matching, stages it in LDS in digit order and writes
+//      each digit run contiguously to its scanned position.
+// Decoupled look-back (single-pass) was measured slower here: its chained
+// status reads cross the XCDs' separate L2 caches and, with ~1000 tiles
+// resident at once, walk hundreds of predecessors.
 // HBM traffic per element: key read once for the histograms plus, per pass,
-// one read and one write of key and value.
+// a key read (upsweep) and one read and one write of key and value.
 #include "common.h"
 
 namespace wfpt {
@@ -56,8 +58,8 @@ static bool make_plan(int64_t n, int key_bytes, int key_bits, int digit_bits, So
     o += align256(sizeof(uint32_t) * (int64_t)p.passes * p.radix);
     p.off_offsets = o;
     o += align256(sizeof(uint32_t) * (int64_t)p.passes * p.radix);
-    p.off_status = o;
-    o += align256(sizeof(uint32_t) * (int64_t)p.passes * p.tiles * p.radix);
+    p.off_status = o;  // tile counts of one pass, digit-major
+    o += align256(sizeof(uint32_t) * p.tiles * p.radix);
     p.off_keys_a = o;
     o += align256((int64_t)key_bytes * n);
     p.off_vals_a = o;
@@ -169,35 +171,89 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp,
     return wbase + x - v;
 }
 
+// Upsweep: digit counts of one tile, written digit-major (counts[d * tiles
+// + tile]) so that the spine scans contiguous rows.  Tiles past the element
+// count write zeros.
+template <typename K, int DBITS>
+__global__ __launch_bounds__(SORT_BLOCK) void sort_upsweep_kernel(const K* __restrict__ keys,
+                                                                  int64_t n, int shift, int width,
+                                                                  uint32_t* __restrict__ counts,
+                                                                  int64_t tiles,
+                                                                  const uint32_t* __restrict__ d_n) {
+    constexpr int RADIX = 1 << DBITS;
+    __shared__ uint32_t s_cnt[SORT_WAVES][RADIX];
+    const unsigned tid = threadIdx.x, wave = tid >> 6;
+    for (int i = tid; i < SORT_WAVES * RADIX; i += SORT_BLOCK) (&s_cnt[0][0])[i] = 0;
+    __syncthreads();
+    if (d_n) n = min(n, (int64_t)*d_n);
+    const uint32_t mask = (1u << width) - 1u;
+    const int64_t tile = blockIdx.x;
+    const int64_t base = tile * SORT_TILE;
+    if (base < n) {
+        K kk[SORT_ITEMS];  // all loads in flight before the counting
+#pragma unroll
+        for (int it = 0; it < SORT_ITEMS; ++it) {
+            const int64_t idx = base + (int64_t)it * SORT_BLOCK + tid;
+            kk[it] = idx < n ? keys[idx] : K(0);
+        }
+#pragma unroll
+        for (int it = 0; it < SORT_ITEMS; ++it) {
+            const int64_t idx = base + (int64_t)it * SORT_BLOCK + tid;
+            if (idx < n) atomicAdd(&s_cnt[wave][digit_of(kk[it], shift, mask)], 1u);
+        }
+    }
+    __syncthreads();
+    for (int d = tid; d < RADIX; d += SORT_BLOCK) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < SORT_WAVES; ++w) c += s_cnt[w][d];
+        counts[(int64_t)d * tiles + tile] = c;
+    }
+}
+
+// Spine: one workgroup per digit scans its row of tile counts in place
+// (exclusive) and adds the digit's start in the output.
+__global__ __launch_bounds__(256) void sort_spine_kernel(uint32_t* __restrict__ counts,
+                                                         int64_t tiles,
+                                                         const uint32_t* __restrict__ digit_start) {
+    __shared__ uint32_t s_tmp[SORT_WAVES];
+    uint32_t* row = counts + (int64_t)blockIdx.x * tiles;
+    uint32_t carry = digit_start[blockIdx.x];
+    for (int64_t base = 0; base < tiles; base += 256) {
+        const int64_t i = base + threadIdx.x;
+        const uint32_t v = i < tiles ? row[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_excl_scan(v, s_tmp, &total);
+        if (i < tiles) row[i] = carry + ex;
+        carry += total;
+    }
+}
+
+// Downsweep: stable ranking of a tile, staged in LDS in digit order, each
+// digit run written contiguously from its scanned position.
 template <typename K, int DBITS>
 __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
     const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     K* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n, int shift, int width,
-    uint32_t* __restrict__ tile_counter, const uint32_t* __restrict__ digit_offsets,
-    uint32_t* __restrict__ status, const uint32_t* __restrict__ d_n) {
+    const uint32_t* __restrict__ tile_offsets, int64_t tiles, const uint32_t* __restrict__ d_n) {
     constexpr int RADIX = 1 << DBITS;
     constexpr int DPT = RADIX >= SORT_BLOCK ? RADIX / SORT_BLOCK : 1;  // digits per thread
     __shared__ uint32_t s_wave[SORT_WAVES][RADIX];  // per-wave counts -> wave prefixes
     __shared__ uint32_t s_lstart[RADIX];            // tile-local start of each digit
     __shared__ uint32_t s_goff[RADIX];              // global base - local start
-    __shared__ K s_keys[SORT_TILE];
-    __shared__ uint32_t s_vals[SORT_TILE];
+    // Staging area, keys first, then values (one buffer keeps LDS per
+    // workgroup small enough for several resident workgroups per CU).
+    __shared__ K s_stage[SORT_TILE];
     __shared__ uint32_t s_tmp[SORT_WAVES];
-    __shared__ uint32_t s_tile;
 
     const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const uint32_t mask = (1u << width) - 1u;
-
-    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+    if (d_n) n = min(n, (int64_t)*d_n);
+    const int64_t tile = blockIdx.x;
+    const int64_t base = tile * SORT_TILE;
+    if (base >= n) return;  // uniform: surplus tiles of a capacity-sized grid
     for (int i = tid; i < SORT_WAVES * RADIX; i += SORT_BLOCK) (&s_wave[0][0])[i] = 0;
     __syncthreads();
-    const int64_t tile = s_tile;
-    const int64_t base = tile * SORT_TILE;
-    // With the count on the device the grid covers the capacity: tiles are
-    // taken in arrival order, so the surplus workgroups are the last ones
-    // and nothing waits on them.
-    if (d_n) n = min(n, (int64_t)*d_n);
-    if (base >= n) return;
 
     // Wave-striped load: element index = base + wave*64*ITEMS + it*64 + lane.
     K key[SORT_ITEMS];
@@ -248,8 +304,6 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
                 s_wave[w][d] = sum;
                 sum += c;
             }
-            // Publish the tile's aggregate (tile 0 publishes its inclusive value).
-            lb_store(&status[tile * RADIX + d], (tile == 0 ? LB_FLAG_INC : LB_FLAG_AGG) | sum);
         }
         cnt[k] = sum;
         my_total += sum;
@@ -258,54 +312,51 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
     uint32_t tile_total;
     uint32_t lstart = block_excl_scan(my_total, s_tmp, &tile_total);
     (void)tile_total;
-
-    // Decoupled look-back: exclusive prefix of each digit over earlier tiles.
 #pragma unroll
     for (int k = 0; k < DPT; ++k) {
         int d = tid * DPT + k;
         if (d < RADIX) {
-            uint32_t excl = 0;
-            if (tile > 0) {
-                int64_t j = tile - 1;
-                while (true) {
-                    uint32_t v;
-                    do {
-                        v = lb_load(&status[j * RADIX + d]);
-                    } while ((v & ~LB_MASK) == 0u);
-                    excl += v & LB_MASK;
-                    if (v & LB_FLAG_INC) break;
-                    --j;
-                }
-                lb_store(&status[tile * RADIX + d], LB_FLAG_INC | (excl + cnt[k]));
-            }
             s_lstart[d] = lstart;
-            s_goff[d] = digit_offsets[d] + excl - lstart;
+            s_goff[d] = tile_offsets[(int64_t)d * tiles + tile] - lstart;
             lstart += cnt[k];
         }
     }
     __syncthreads();
 
-    // Stage the tile in LDS in digit order.
+    // Position of every item in the tile's digit order.
+    uint32_t pos[SORT_ITEMS];
 #pragma unroll
     for (int it = 0; it < SORT_ITEMS; ++it) {
-        int64_t idx = wbase + it * 64 + lane;
-        if (idx < n) {
-            uint32_t d = digit_of(key[it], shift, mask);
-            uint32_t pos = s_lstart[d] + s_wave[wave][d] + off[it];
-            s_keys[pos] = key[it];
-            s_vals[pos] = val[it];
+        uint32_t d = digit_of(key[it], shift, mask);
+        pos[it] = s_lstart[d] + s_wave[wave][d] + off[it];
+    }
+    // Keys: stage in digit order, then each digit run goes out contiguously.
+    const int count = (int)min((int64_t)SORT_TILE, n - base);
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it)
+        if (wbase + it * 64 + lane < n) s_stage[pos[it]] = key[it];
+    __syncthreads();
+    uint32_t gpos[SORT_ITEMS];
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        const int j = it * SORT_BLOCK + (int)tid;
+        if (j < count) {
+            const K k = s_stage[j];
+            gpos[it] = s_goff[digit_of(k, shift, mask)] + (uint32_t)j;
+            keys_out[gpos[it]] = k;
         }
     }
     __syncthreads();
-
-    // Contiguous runs per digit go out to their global positions.
-    const int count = (int)min((int64_t)SORT_TILE, n - base);
-    for (int j = tid; j < count; j += SORT_BLOCK) {
-        K k = s_keys[j];
-        uint32_t d = digit_of(k, shift, mask);
-        uint32_t g = s_goff[d] + (uint32_t)j;
-        keys_out[g] = k;
-        vals_out[g] = s_vals[j];
+    // Values through the same staging buffer and the same global positions.
+    uint32_t* s_vstage = reinterpret_cast<uint32_t*>(s_stage);
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it)
+        if (wbase + it * 64 + lane < n) s_vstage[pos[it]] = val[it];
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        const int j = it * SORT_BLOCK + (int)tid;
+        if (j < count) vals_out[gpos[it]] = s_vstage[j];
     }
 }
 
@@ -324,12 +375,18 @@ __global__ __launch_bounds__(256) void copy_pairs_kernel(const K* __restrict__ k
 }
 
 template <typename K, int DBITS>
-static void launch_bin(const void* ki, const uint32_t* vi, void* ko, uint32_t* vo, int64_t n,
-                       int shift, int width, uint32_t* ctr, const uint32_t* offs,
-                       uint32_t* status, unsigned tiles, const uint32_t* d_n, hipStream_t s) {
-    sort_bin_kernel<K, DBITS><<<tiles, SORT_BLOCK, 0, s>>>(
-        static_cast<const K*>(ki), vi, static_cast<K*>(ko), vo, n, shift, width, ctr, offs,
-        status, d_n);
+static int launch_pass(const void* ki, const uint32_t* vi, void* ko, uint32_t* vo, int64_t n,
+                       int shift, int width, uint32_t* counts, const uint32_t* digit_start,
+                       int64_t tiles, const uint32_t* d_n, hipStream_t s) {
+    constexpr int RADIX = 1 << DBITS;
+    sort_upsweep_kernel<K, DBITS><<<(unsigned)tiles, SORT_BLOCK, 0, s>>>(
+        static_cast<const K*>(ki), n, shift, width, counts, tiles, d_n);
+    WF_TRY(launch_check("sort_upsweep_kernel"));
+    sort_spine_kernel<<<RADIX, 256, 0, s>>>(counts, tiles, digit_start);
+    WF_TRY(launch_check("sort_spine_kernel"));
+    sort_bin_kernel<K, DBITS><<<(unsigned)tiles, SORT_BLOCK, 0, s>>>(
+        static_cast<const K*>(ki), vi, static_cast<K*>(ko), vo, n, shift, width, counts, tiles, d_n);
+    return launch_check("sort_bin_kernel");
 }
 
 template <typename K>
@@ -344,8 +401,8 @@ static int sort_impl(void* keys, uint32_t* vals, int64_t n, int key_bits, int di
     void* kb = scratch + p.off_keys_b;
     uint32_t* vb = reinterpret_cast<uint32_t*>(scratch + p.off_vals_b);
 
-    // Counters, histograms and look-back words start at zero.
-    const int64_t zero_words = (p.off_keys_a - p.off_counters) / 4;
+    // Counters and histograms start at zero (tile counts are written whole).
+    const int64_t zero_words = (p.off_status - p.off_counters) / 4;
     WF_TRY(fill_u32(counters, zero_words, 0u, s));
 
     {
@@ -384,18 +441,14 @@ static int sort_impl(void* keys, uint32_t* vals, int64_t n, int key_bits, int di
     for (int pass = 0; pass < p.passes; ++pass) {
         const int shift = pass * digit_bits;
         const int width = std::min(digit_bits, key_bits - shift);
-        uint32_t* ctr = counters + pass;
         const uint32_t* offs = offsets + (int64_t)pass * p.radix;
-        uint32_t* st = status + (int64_t)pass * p.tiles * p.radix;
         const Buf& in = seq[pass];
         const Buf& out = seq[pass + 1];
-        unsigned tiles = (unsigned)p.tiles;
         switch (digit_bits) {
-            case 4: launch_bin<K, 4>(in.k, in.v, out.k, out.v, n, shift, width, ctr, offs, st, tiles, d_n, s); break;
-            case 8: launch_bin<K, 8>(in.k, in.v, out.k, out.v, n, shift, width, ctr, offs, st, tiles, d_n, s); break;
-            default: launch_bin<K, 11>(in.k, in.v, out.k, out.v, n, shift, width, ctr, offs, st, tiles, d_n, s); break;
+            case 4: WF_TRY((launch_pass<K, 4>(in.k, in.v, out.k, out.v, n, shift, width, status, offs, p.tiles, d_n, s))); break;
+            case 8: WF_TRY((launch_pass<K, 8>(in.k, in.v, out.k, out.v, n, shift, width, status, offs, p.tiles, d_n, s))); break;
+            default: WF_TRY((launch_pass<K, 11>(in.k, in.v, out.k, out.v, n, shift, width, status, offs, p.tiles, d_n, s))); break;
         }
-        WF_TRY(launch_check("sort_bin_kernel"));
     }
     if (p.passes == 1) {
         copy_pairs_kernel<K><<<grid_for(n, 256), 256, 0, s>>>(

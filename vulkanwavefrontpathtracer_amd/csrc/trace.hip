@@ -16,9 +16,12 @@
 // the result equals a brute-force scan whatever the traversal order.
 #include "traverse.h"
 
+#include <algorithm>
+#include <mutex>
+
 namespace wfpt {
 
-template <bool CLOSEST, int BLOCK, bool LDS, bool STATS = false>
+template <bool CLOSEST, int BLOCK, bool LDS, bool STATS = false, bool WIDE = false>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__ nodes,
                                                       const float4* __restrict__ tris,
                                                       const wf_ray* __restrict__ rays,
@@ -34,11 +37,68 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__
     load_ray(rays, r, ro, rd);
     WtRay wr;
     const bool live = (rd.w >= ro.w) && wt_setup(ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, wr);
-    const TraceOut tr = traverse<CLOSEST, BLOCK, LDS, STATS>(nodes, tris, ro, rd, wr, live, s_stack);
+    const TraceOut tr = traverse<CLOSEST, BLOCK, LDS, STATS, WIDE>(nodes, tris, ro, rd, wr, live, s_stack);
     *reinterpret_cast<int4*>(hits + r) = hit_record(tris, wr, tr);
     if (STATS) {
         stat_nodes[r] = tr.nodes_visited;
         stat_tris[r] = tr.tris_tested;
+    }
+}
+
+// Persistent variant: a grid sized to the resident waves; each wave keeps
+// its lanes busy by handing rays out of a global counter to idle lanes (one
+// atomic per refill of `refill` or more idle lanes), so a wave is no longer
+// held by its slowest of 64 rays (Aila & Laine 2009).  Rays are taken in
+// index order, so sorted wavefronts keep their coherence; each ray's
+// traversal is the same step sequence as in trace_kernel.
+template <bool CLOSEST, int BLOCK, bool LDS, bool WIDE>
+__global__ __launch_bounds__(BLOCK) void trace_persistent_kernel(
+    const float4* __restrict__ nodes, const float4* __restrict__ tris,
+    const wf_ray* __restrict__ rays, const uint32_t* __restrict__ ray_index,
+    wf_hit* __restrict__ hits, int64_t n, uint32_t* __restrict__ counter, int refill) {
+    __shared__ int s_stack[LDS ? LDS_DEPTH * BLOCK : 1];
+    const unsigned lane = lane_id();
+    constexpr int STACK = WIDE ? TRACE_STACK4 : TRACE_STACK;
+    TravState<LDS, STACK> st;
+    int spill[TravState<LDS, STACK>::SPILL];
+    WtRay wr;
+    int64_t r = 0;
+    bool active = false, exhausted = false;
+    while (true) {
+        const uint64_t idle = ballot64(!active);
+        const int nidle = __popcll(idle);
+        if (!exhausted && nidle >= refill) {
+            const int leader = __ffsll((unsigned long long)idle) - 1;
+            uint32_t base = 0;
+            if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)nidle);
+            base = __shfl(base, leader, 64);
+            if ((int64_t)base + nidle >= n) exhausted = true;
+            if (!active) {
+                const int64_t k = (int64_t)base + popc_below(idle);
+                if (k < n) {
+                    r = ray_index ? (int64_t)ray_index[k] : k;
+                    float4 ro, rd;
+                    load_ray(rays, r, ro, rd);
+                    const bool live = (rd.w >= ro.w) && wt_setup(ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, wr);
+                    trav_begin(st, ro, rd);
+                    if (live) active = true;
+                    else *reinterpret_cast<int4*>(hits + r) = hit_record(tris, wr, TraceOut{-1, rd.w, 0u, 0u});
+                }
+            }
+        }
+        if (ballot64(active) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        bool fin = false;
+        if (active) {
+            if (WIDE) fin = trav_step4<CLOSEST, BLOCK, LDS, false>(st, spill, nodes, tris, wr, s_stack);
+            else fin = trav_step<CLOSEST, BLOCK, LDS, false>(st, spill, nodes, tris, wr, s_stack);
+        }
+        if (fin) {
+            *reinterpret_cast<int4*>(hits + r) = hit_record(tris, wr, TraceOut{st.best_slot, st.best_t, 0u, 0u});
+            active = false;
+        }
     }
 }
 
@@ -50,24 +110,73 @@ __global__ __launch_bounds__(256) void miss_kernel(const uint32_t* __restrict__ 
     *reinterpret_cast<int4*>(hits + r) = make_int4(-1, -1, __float_as_int(-1.0f), 0);
 }
 
-template <bool CLOSEST, int BLOCK>
-static void launch_trace(bool lds, const float4* nodes, const float4* tris, const wf_ray* rays,
-                         const uint32_t* ray_index, wf_hit* hits, int64_t n, hipStream_t s) {
-    unsigned g = grid_for(n, BLOCK);
-    if (lds)
-        trace_kernel<CLOSEST, BLOCK, true><<<g, BLOCK, 0, s>>>(nodes, tris, rays, ray_index, hits, n);
-    else
-        trace_kernel<CLOSEST, BLOCK, false><<<g, BLOCK, 0, s>>>(nodes, tris, rays, ray_index, hits, n);
+// Ray counters of persistent launches: a small ring per device, one slot
+// per launch (a slot is reused after TRACE_COUNTER_SLOTS launches).
+constexpr int TRACE_COUNTER_SLOTS = 1024;
+constexpr int TRACE_MAX_DEVICES = 64;
+
+static uint32_t* counter_slot(int device) {
+    static std::mutex mu;
+    static uint32_t* ring[TRACE_MAX_DEVICES] = {};
+    static unsigned next[TRACE_MAX_DEVICES] = {};
+    std::lock_guard<std::mutex> lock(mu);
+    if (device < 0 || device >= TRACE_MAX_DEVICES) return nullptr;
+    if (!ring[device] &&
+        hipMalloc((void**)&ring[device], sizeof(uint32_t) * TRACE_COUNTER_SLOTS) != hipSuccess) {
+        ring[device] = nullptr;
+        return nullptr;
+    }
+    return ring[device] + (next[device]++ % TRACE_COUNTER_SLOTS);
 }
 
-template <bool CLOSEST>
-static void launch_trace_block(int block, bool lds, const float4* nodes, const float4* tris,
-                               const wf_ray* rays, const uint32_t* ray_index, wf_hit* hits,
-                               int64_t n, hipStream_t s) {
-    switch (block) {
-        case 64: launch_trace<CLOSEST, 64>(lds, nodes, tris, rays, ray_index, hits, n, s); break;
-        case 128: launch_trace<CLOSEST, 128>(lds, nodes, tris, rays, ray_index, hits, n, s); break;
-        default: launch_trace<CLOSEST, 256>(lds, nodes, tris, rays, ray_index, hits, n, s); break;
+template <bool CLOSEST, int BLOCK, bool LDS, bool WIDE>
+static int launch_persistent(const float4* nodes, const float4* tris, const wf_ray* rays,
+                             const uint32_t* ray_index, wf_hit* hits, int64_t n, int refill,
+                             hipStream_t s) {
+    static int resident[TRACE_MAX_DEVICES] = {};
+    int dev = 0;
+    WF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= TRACE_MAX_DEVICES) return fail(WF_ERR_HIP, "trace: device index out of range");
+    if (resident[dev] == 0) {
+        int per_cu = 0, cus = 0;
+        WF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, trace_persistent_kernel<CLOSEST, BLOCK, LDS, WIDE>, BLOCK, 0));
+        WF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        resident[dev] = std::max(1, per_cu) * std::max(1, cus);
+    }
+    uint32_t* ctr = counter_slot(dev);
+    if (!ctr) return fail(WF_ERR_OUT_OF_MEMORY, "trace: counter allocation failed");
+    WF_TRY(fill_u32(ctr, 1, 0u, s));
+    const int64_t blocks = std::min<int64_t>(resident[dev], (n + BLOCK - 1) / BLOCK);
+    trace_persistent_kernel<CLOSEST, BLOCK, LDS, WIDE><<<(unsigned)blocks, BLOCK, 0, s>>>(
+        nodes, tris, rays, ray_index, hits, n, ctr, refill);
+    return WF_OK;
+}
+
+template <bool CLOSEST, int BLOCK, bool WIDE>
+static int launch_trace(bool lds, bool persistent, int refill, const float4* nodes,
+                        const float4* tris, const wf_ray* rays, const uint32_t* ray_index,
+                        wf_hit* hits, int64_t n, hipStream_t s) {
+    if (persistent)
+        return lds ? launch_persistent<CLOSEST, BLOCK, true, WIDE>(nodes, tris, rays, ray_index, hits, n, refill, s)
+                   : launch_persistent<CLOSEST, BLOCK, false, WIDE>(nodes, tris, rays, ray_index, hits, n, refill, s);
+    unsigned g = grid_for(n, BLOCK);
+    if (lds)
+        trace_kernel<CLOSEST, BLOCK, true, false, WIDE><<<g, BLOCK, 0, s>>>(nodes, tris, rays, ray_index, hits, n);
+    else
+        trace_kernel<CLOSEST, BLOCK, false, false, WIDE><<<g, BLOCK, 0, s>>>(nodes, tris, rays, ray_index, hits, n);
+    return WF_OK;
+}
+
+template <bool CLOSEST, bool WIDE>
+static int launch_trace_block(const Tuning& tu, const float4* nodes, const float4* tris,
+                              const wf_ray* rays, const uint32_t* ray_index, wf_hit* hits,
+                              int64_t n, hipStream_t s) {
+    const bool lds = tu.trace_lds_stack != 0, pers = tu.trace_persistent != 0;
+    switch (tu.trace_block) {
+        case 64: return launch_trace<CLOSEST, 64, WIDE>(lds, pers, tu.trace_refill, nodes, tris, rays, ray_index, hits, n, s);
+        case 128: return launch_trace<CLOSEST, 128, WIDE>(lds, pers, tu.trace_refill, nodes, tris, rays, ray_index, hits, n, s);
+        default: return launch_trace<CLOSEST, 256, WIDE>(lds, pers, tu.trace_refill, nodes, tris, rays, ray_index, hits, n, s);
     }
 }
 
@@ -79,12 +188,17 @@ int trace_launch(wf_scene scene, const wf_ray* rays, const uint32_t* ray_index, 
         return launch_check("miss_kernel");
     }
     const Tuning& tu = tuning();
-    if (closest)
-        launch_trace_block<true>(tu.trace_block, tu.trace_lds_stack != 0, scene->nodes, scene->tris,
-                                 rays, ray_index, hits, n, s);
-    else
-        launch_trace_block<false>(tu.trace_block, tu.trace_lds_stack != 0, scene->nodes, scene->tris,
-                                  rays, ray_index, hits, n, s);
+    if (tu.trace_width == 4) {
+        if (closest)
+            WF_TRY((launch_trace_block<true, true>(tu, scene->nodes4, scene->tris, rays, ray_index, hits, n, s)));
+        else
+            WF_TRY((launch_trace_block<false, true>(tu, scene->nodes4, scene->tris, rays, ray_index, hits, n, s)));
+    } else {
+        if (closest)
+            WF_TRY((launch_trace_block<true, false>(tu, scene->nodes, scene->tris, rays, ray_index, hits, n, s)));
+        else
+            WF_TRY((launch_trace_block<false, false>(tu, scene->nodes, scene->tris, rays, ray_index, hits, n, s)));
+    }
     return launch_check("trace_kernel");
 }
 
@@ -100,12 +214,20 @@ extern "C" int wf_trace_stats(wf_scene scene, const wf_ray* rays, wf_hit* hits, 
     WF_CHECK_ARG(rays && hits && node_visits && tri_tests, "wf_trace_stats: null buffer");
     WF_CHECK_ARG(scene->num_tris > 0, "wf_trace_stats: empty scene");
     hipStream_t s = as_stream(stream);
-    if (closest)
-        trace_kernel<true, 64, true, true><<<grid_for(n, 64), 64, 0, s>>>(
-            scene->nodes, scene->tris, rays, nullptr, hits, n, node_visits, tri_tests);
+    const bool wide = tuning().trace_width == 4;
+    const float4* nodes = wide ? scene->nodes4 : scene->nodes;
+    if (closest && wide)
+        trace_kernel<true, 64, true, true, true><<<grid_for(n, 64), 64, 0, s>>>(
+            nodes, scene->tris, rays, nullptr, hits, n, node_visits, tri_tests);
+    else if (closest)
+        trace_kernel<true, 64, true, true, false><<<grid_for(n, 64), 64, 0, s>>>(
+            nodes, scene->tris, rays, nullptr, hits, n, node_visits, tri_tests);
+    else if (wide)
+        trace_kernel<false, 64, true, true, true><<<grid_for(n, 64), 64, 0, s>>>(
+            nodes, scene->tris, rays, nullptr, hits, n, node_visits, tri_tests);
     else
-        trace_kernel<false, 64, true, true><<<grid_for(n, 64), 64, 0, s>>>(
-            scene->nodes, scene->tris, rays, nullptr, hits, n, node_visits, tri_tests);
+        trace_kernel<false, 64, true, true, false><<<grid_for(n, 64), 64, 0, s>>>(
+            nodes, scene->tris, rays, nullptr, hits, n, node_visits, tri_tests);
     return launch_check("trace_kernel<stats>");
 }
 
