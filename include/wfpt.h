@@ -263,12 +263,12 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
 /* Process-wide knobs read by later calls (the autotuner sets them):
  *   "bvh.presplit"     0/1  split references of large triangles (default 1)
  *   "bvh.split_depth"  0-8  max binary splits per triangle (default 6)
- *   "bvh.leaf_max"     1-8  triangles per collapsed leaf (default 8)
+ *   "bvh.leaf_max"     1-8  triangles per collapsed leaf (default 1)
  *   "trace.lds_stack"  0/1  keep the top of the traversal stack in LDS (default 1)
  *   "trace.block"      64/128/256  threads per traversal workgroup (default 64)
  *   "trace.persistent" 0/1  persistent waves refilling idle lanes from a ray
- *                           counter (default 1)
- *   "trace.refill"     1-64 idle lanes of a wave that trigger a refill (default 16)
+ *                           counter (default 0)
+ *   "trace.refill"     1-64 idle lanes of a wave that trigger a refill (default 24)
  *   "trace.width"      2/4  traverse the 2-wide or the 4-wide quantized hierarchy
  *                           (both are built; default 4)
  * Returns WF_ERR_INVALID_ARG for an unknown key or value out of range. */
@@ -308,6 +308,14 @@ int wf_renderer_reset_frame_index(wf_renderer r);
 /* Copies width*height float4 of the accumulated (linear) or tone-mapped
  * framebuffer to host memory. */
 int wf_renderer_read_pixels(wf_renderer r, int tone_mapped, float* host_rgba);
+/* Image output in place of present.comp (present.comp:19-31) and the
+ * swapchain: linear = 0 writes the tone-mapped frame as an 8-bit RGB PNG
+ * (UNORM conversion of the rgba8 swapchain image), linear = 1 the
+ * accumulated linear estimate as a PFM (float RGB). */
+int wf_renderer_save_image(wf_renderer r, const char* path, int linear);
+/* The same writers for host float4 images (row 0 = top). */
+int wf_write_png(const char* path, const float* host_rgba, int width, int height);
+int wf_write_pfm(const char* path, const float* host_rgba, int width, int height);
 /* Device pointer of the same framebuffer (width*height float4), valid until
  * the next render or destroy; waits for the renderer's stream. */
 int wf_renderer_pixels_device(wf_renderer r, int tone_mapped, const float** out);

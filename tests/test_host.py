@@ -270,3 +270,42 @@ def test_sized_boxes_and_batched_workload_spec():
     assert {v["width"] for v in views} <= {128, 256, 512, 1024}
     assert all(1 <= v["spp"] <= 16 for v in views)
     assert np.log(tris).std() > 1.0                            # spread over decades
+
+
+def test_png_and_pfm_writers(tmp_path):
+    """present.comp's rgba8 conversion (clamp, x * 255 to nearest) in a PNG
+    that zlib and the PNG chunk CRCs accept; PFM rows bottom to top."""
+    import struct
+    import zlib
+
+    from vulkanwavefrontpathtracer_amd import _capi
+
+    w, h = 70, 3                                    # > 65535-byte stored blocks need not apply
+    rng = np.random.default_rng(2)
+    img = rng.uniform(-0.2, 1.2, (h * w, 4)).astype(np.float32)
+    img[0, :3] = [np.nan, 0.5 / 255, 1.0]
+    p = tmp_path / "a.png"
+    _capi.call("wf_write_png", str(p).encode(), img.ctypes.data_as(_capi.pf), w, h)
+    data = p.read_bytes()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, chunks = 8, {}
+    while pos < len(data):
+        n, = struct.unpack(">I", data[pos:pos + 4])
+        typ, body = data[pos + 4:pos + 8], data[pos + 8:pos + 8 + n]
+        crc, = struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])
+        assert crc == zlib.crc32(typ + body)
+        chunks[typ] = chunks.get(typ, b"") + body
+        pos += 12 + n
+    assert struct.unpack(">IIBBBBB", chunks[b"IHDR"]) == (w, h, 8, 2, 0, 0, 0)
+    raw = np.frombuffer(zlib.decompress(chunks[b"IDAT"]), np.uint8).reshape(h, 3 * w + 1)
+    assert np.all(raw[:, 0] == 0)
+    got = raw[:, 1:].reshape(h * w, 3)
+    x = np.nan_to_num(img[:, :3], nan=0.0)
+    want = np.where(x <= 0, 0, np.where(x >= 1, 255, np.rint(x * 255))).astype(np.uint8)
+    assert np.array_equal(got, want)
+    q = tmp_path / "a.pfm"
+    _capi.call("wf_write_pfm", str(q).encode(), img.ctypes.data_as(_capi.pf), w, h)
+    head, rest = q.read_bytes().split(b"\n-1.0\n", 1)
+    assert head == f"PF\n{w} {h}".encode()
+    pf = np.frombuffer(rest, "<f4").reshape(h, w, 3)[::-1].reshape(h * w, 3)
+    assert np.array_equal(pf, img[:, :3], equal_nan=True)

@@ -92,6 +92,9 @@ SIGNATURES: dict[str, tuple] = {
     "wf_renderer_reset_frame_index": (i32, [vp]),
     "wf_renderer_read_pixels": (i32, [vp, i32, pf]),
     "wf_renderer_pixels_device": (i32, [vp, i32, C.POINTER(vp)]),
+    "wf_renderer_save_image": (i32, [vp, cs, i32]),
+    "wf_write_png": (i32, [cs, pf, i32, i32]),
+    "wf_write_pfm": (i32, [cs, pf, i32, i32]),
     "wf_renderer_adjoint": (i32, [vp, i32, vp, vp, vp, vp, pf]),
     "wf_renderer_size": (i32, [vp, C.POINTER(i32), C.POINTER(i32)]),
     "wf_renderer_stats": (i32, [vp, pu64, pf, pf, pu64]),

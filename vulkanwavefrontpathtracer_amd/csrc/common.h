@@ -134,11 +134,11 @@ namespace wfpt {
 struct Tuning {
     int bvh_presplit = 1;
     int bvh_split_depth = 6;
-    int bvh_leaf_max = 8;
+    int bvh_leaf_max = 1;
     int trace_lds_stack = 1;
     int trace_block = 64;
-    int trace_persistent = 1;   // persistent waves that refill idle lanes with new rays
-    int trace_refill = 16;      // idle lanes of a wave that trigger a refill
+    int trace_persistent = 0;   // persistent waves that refill idle lanes with new rays
+    int trace_refill = 24;      // idle lanes of a wave that trigger a refill
     int trace_width = 4;        // traverse the 2-wide or the 4-wide hierarchy
 };
 Tuning& tuning();

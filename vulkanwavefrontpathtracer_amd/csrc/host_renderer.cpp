@@ -658,6 +658,14 @@ extern "C" int wf_renderer_adjoint(wf_renderer r, int frame, const float* d_pixe
     return WF_OK;
 }
 
+extern "C" int wf_renderer_save_image(wf_renderer r, const char* path, int linear) {
+    WF_CHECK_ARG(r && path, "wf_renderer_save_image: null argument");
+    std::vector<float> px(4 * (size_t)r->width * r->height);
+    WF_TRY(wf_renderer_read_pixels(r, linear ? 0 : 1, px.data()));
+    return linear ? wf_write_pfm(path, px.data(), r->width, r->height)
+                  : wf_write_png(path, px.data(), r->width, r->height);
+}
+
 extern "C" int wf_renderer_read_pixels(wf_renderer r, int tone_mapped, float* host_rgba) {
     WF_CHECK_ARG(r && host_rgba, "wf_renderer_read_pixels: null argument");
     const DeviceBuffer& b = tone_mapped ? r->frame_pixels : r->pixels;

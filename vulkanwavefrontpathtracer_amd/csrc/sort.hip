@@ -4,20 +4,17 @@
 // (projects/WaveFrontPathTracer/Tracer/Tracer.cpp:39-78 ->
 // external/vulkan_radix_sort/vk_radix_sort.h:2579-2585), which runs 4 fixed
 // 8-bit passes on u32 keys.  Here:
-//   1. one histogram kernel reads every key once and counts the digits of
-//      all passes (LDS histograms, one global atomic per non-empty bin), and
-//      a tiny scan turns them into per-pass digit start offsets;
-//   2. per pass, reduce-then-scan with no dependency between workgroups of
-//      a kernel: an upsweep counts each 4096-element tile's digits, a spine
-//      scans every digit's row of tile counts (one workgroup per digit) and
-//      adds the digit's start, and a downsweep ranks each tile stably with
-//      wave64 ballot matching, stages it in LDS in digit order and writes
-//      each digit run contiguously to its scanned position.
+// per pass, reduce-then-scan with no dependency between the workgroups of a
+// kernel: an upsweep counts each 4096-element tile's digits, a spine scans
+// every digit's row of tile counts (one workgroup per digit) and writes the
+// digit's total, and a downsweep scans the digit totals into digit starts,
+// ranks its tile stably with wave64 ballot matching, stages it in LDS in
+// digit order and writes each digit run contiguously to its position.
 // Decoupled look-back (single-pass) was measured slower here: its chained
 // status reads cross the XCDs' separate L2 caches and, with ~1000 tiles
 // resident at once, walk hundreds of predecessors.
-// HBM traffic per element: key read once for the histograms plus, per pass,
-// a key read (upsweep) and one read and one write of key and value.
+// HBM traffic per element and pass: a key read (upsweep) and one read and
+// one write of key and value (downsweep).
 #include "common.h"
 
 namespace wfpt {
@@ -27,8 +24,6 @@ constexpr int SORT_WAVES = SORT_BLOCK / 64;
 constexpr int SORT_ITEMS = 16;
 constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
 constexpr int SORT_MAX_PASSES = 16;
-constexpr int HIST_BLOCK = 256;
-constexpr int HIST_ITEMS = 32;
 
 inline int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
@@ -37,7 +32,7 @@ struct SortPlan {
     int radix = 0;
     int64_t tiles = 0;
     // byte offsets inside scratch
-    int64_t off_counters = 0, off_hist = 0, off_offsets = 0, off_status = 0;
+    int64_t off_counts = 0, off_totals = 0;
     int64_t off_keys_a = 0, off_vals_a = 0, off_keys_b = 0, off_vals_b = 0;
     int64_t total = 0;
 };
@@ -52,14 +47,10 @@ static bool make_plan(int64_t n, int key_bytes, int key_bits, int digit_bits, So
     p.radix = 1 << digit_bits;
     p.tiles = (n + SORT_TILE - 1) / SORT_TILE;
     int64_t o = 0;
-    p.off_counters = o;
-    o += align256(sizeof(uint32_t) * SORT_MAX_PASSES);
-    p.off_hist = o;
-    o += align256(sizeof(uint32_t) * (int64_t)p.passes * p.radix);
-    p.off_offsets = o;
-    o += align256(sizeof(uint32_t) * (int64_t)p.passes * p.radix);
-    p.off_status = o;  // tile counts of one pass, digit-major
+    p.off_counts = o;  // tile counts of one pass, digit-major, scanned in place
     o += align256(sizeof(uint32_t) * p.tiles * p.radix);
+    p.off_totals = o;  // digit totals of one pass
+    o += align256(sizeof(uint32_t) * p.radix);
     p.off_keys_a = o;
     o += align256((int64_t)key_bytes * n);
     p.off_vals_a = o;
@@ -75,73 +66,6 @@ static bool make_plan(int64_t n, int key_bytes, int key_bits, int digit_bits, So
 template <typename K>
 __device__ __forceinline__ uint32_t digit_of(K key, int shift, uint32_t mask) {
     return (uint32_t)(key >> shift) & mask;
-}
-
-// ------------------------------------------------------------- histograms
-template <typename K>
-__global__ __launch_bounds__(HIST_BLOCK) void sort_hist_kernel(const K* __restrict__ keys,
-                                                               int64_t n, int passes,
-                                                               int digit_bits, int key_bits,
-                                                               uint32_t* __restrict__ hist,
-                                                               const uint32_t* __restrict__ d_n) {
-    extern __shared__ uint32_t s_hist[];  // passes * radix
-    if (d_n) n = min(n, (int64_t)*d_n);  // element count on the device (graph replays)
-    const int radix = 1 << digit_bits;
-    const int total = passes * radix;
-    for (int i = threadIdx.x; i < total; i += HIST_BLOCK) s_hist[i] = 0;
-    __syncthreads();
-    const int64_t chunk = (int64_t)HIST_BLOCK * HIST_ITEMS;
-    for (int64_t base = (int64_t)blockIdx.x * chunk; base < n; base += (int64_t)gridDim.x * chunk) {
-        K kk[HIST_ITEMS];
-        bool ok[HIST_ITEMS];
-#pragma unroll
-        for (int it = 0; it < HIST_ITEMS; ++it) {
-            int64_t idx = base + (int64_t)it * HIST_BLOCK + threadIdx.x;
-            ok[it] = idx < n;
-            kk[it] = ok[it] ? keys[idx] : K(0);
-        }
-        for (int p = 0; p < passes; ++p) {
-            const int shift = p * digit_bits;
-            const int width = min(digit_bits, key_bits - shift);
-            const uint32_t mask = (1u << width) - 1u;
-#pragma unroll
-            for (int it = 0; it < HIST_ITEMS; ++it)
-                if (ok[it]) atomicAdd(&s_hist[p * radix + digit_of(kk[it], shift, mask)], 1u);
-        }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < total; i += HIST_BLOCK) {
-        uint32_t c = s_hist[i];
-        if (c) atomicAdd(&hist[i], c);
-    }
-}
-
-// Exclusive scan of each pass's histogram (one workgroup per pass).
-__global__ __launch_bounds__(256) void sort_scan_kernel(const uint32_t* __restrict__ hist,
-                                                        uint32_t* __restrict__ offsets,
-                                                        int radix) {
-    __shared__ uint32_t s_part[256];
-    const int p = blockIdx.x;
-    const int per = (radix + 255) / 256;  // digits per thread, contiguous
-    const int d0 = threadIdx.x * per;
-    uint32_t local = 0;
-    for (int k = 0; k < per; ++k)
-        if (d0 + k < radix) local += hist[p * radix + d0 + k];
-    s_part[threadIdx.x] = local;
-    __syncthreads();
-    // Hillis-Steele over 256 partial sums.
-    for (int off = 1; off < 256; off <<= 1) {
-        uint32_t v = threadIdx.x >= off ? s_part[threadIdx.x - off] : 0u;
-        __syncthreads();
-        s_part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = s_part[threadIdx.x] - local;  // exclusive
-    for (int k = 0; k < per; ++k)
-        if (d0 + k < radix) {
-            offsets[p * radix + d0 + k] = run;
-            run += hist[p * radix + d0 + k];
-        }
 }
 
 // ---------------------------------------------------------------- binning
@@ -212,13 +136,13 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_upsweep_kernel(const K* __res
 }
 
 // Spine: one workgroup per digit scans its row of tile counts in place
-// (exclusive) and adds the digit's start in the output.
+// (exclusive) and writes the row total, the digit's count in the pass.
 __global__ __launch_bounds__(256) void sort_spine_kernel(uint32_t* __restrict__ counts,
                                                          int64_t tiles,
-                                                         const uint32_t* __restrict__ digit_start) {
+                                                         uint32_t* __restrict__ totals) {
     __shared__ uint32_t s_tmp[SORT_WAVES];
     uint32_t* row = counts + (int64_t)blockIdx.x * tiles;
-    uint32_t carry = digit_start[blockIdx.x];
+    uint32_t carry = 0;
     for (int64_t base = 0; base < tiles; base += 256) {
         const int64_t i = base + threadIdx.x;
         const uint32_t v = i < tiles ? row[i] : 0u;
@@ -227,6 +151,7 @@ __global__ __launch_bounds__(256) void sort_spine_kernel(uint32_t* __restrict__ 
         if (i < tiles) row[i] = carry + ex;
         carry += total;
     }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
 // Downsweep: stable ranking of a tile, staged in LDS in digit order, each
@@ -235,7 +160,8 @@ template <typename K, int DBITS>
 __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
     const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     K* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n, int shift, int width,
-    const uint32_t* __restrict__ tile_offsets, int64_t tiles, const uint32_t* __restrict__ d_n) {
+    const uint32_t* __restrict__ tile_offsets, int64_t tiles, const uint32_t* __restrict__ totals,
+    const uint32_t* __restrict__ d_n) {
     constexpr int RADIX = 1 << DBITS;
     constexpr int DPT = RADIX >= SORT_BLOCK ? RADIX / SORT_BLOCK : 1;  // digits per thread
     __shared__ uint32_t s_wave[SORT_WAVES][RADIX];  // per-wave counts -> wave prefixes
@@ -312,13 +238,26 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
     uint32_t tile_total;
     uint32_t lstart = block_excl_scan(my_total, s_tmp, &tile_total);
     (void)tile_total;
+    // Digit starts in the output: exclusive scan of the pass's digit totals.
+    uint32_t tot[DPT];
+    uint32_t my_tot = 0;
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        const int d = tid * DPT + k;
+        tot[k] = d < RADIX ? totals[d] : 0u;
+        my_tot += tot[k];
+    }
+    uint32_t all_total;
+    uint32_t dstart = block_excl_scan(my_tot, s_tmp, &all_total);
+    (void)all_total;
 #pragma unroll
     for (int k = 0; k < DPT; ++k) {
         int d = tid * DPT + k;
         if (d < RADIX) {
             s_lstart[d] = lstart;
-            s_goff[d] = tile_offsets[(int64_t)d * tiles + tile] - lstart;
+            s_goff[d] = dstart + tile_offsets[(int64_t)d * tiles + tile] - lstart;
             lstart += cnt[k];
+            dstart += tot[k];
         }
     }
     __syncthreads();
@@ -376,46 +315,30 @@ __global__ __launch_bounds__(256) void copy_pairs_kernel(const K* __restrict__ k
 
 template <typename K, int DBITS>
 static int launch_pass(const void* ki, const uint32_t* vi, void* ko, uint32_t* vo, int64_t n,
-                       int shift, int width, uint32_t* counts, const uint32_t* digit_start,
-                       int64_t tiles, const uint32_t* d_n, hipStream_t s) {
+                       int shift, int width, uint32_t* counts, uint32_t* totals, int64_t tiles,
+                       const uint32_t* d_n, hipStream_t s) {
     constexpr int RADIX = 1 << DBITS;
     sort_upsweep_kernel<K, DBITS><<<(unsigned)tiles, SORT_BLOCK, 0, s>>>(
         static_cast<const K*>(ki), n, shift, width, counts, tiles, d_n);
     WF_TRY(launch_check("sort_upsweep_kernel"));
-    sort_spine_kernel<<<RADIX, 256, 0, s>>>(counts, tiles, digit_start);
+    sort_spine_kernel<<<RADIX, 256, 0, s>>>(counts, tiles, totals);
     WF_TRY(launch_check("sort_spine_kernel"));
     sort_bin_kernel<K, DBITS><<<(unsigned)tiles, SORT_BLOCK, 0, s>>>(
-        static_cast<const K*>(ki), vi, static_cast<K*>(ko), vo, n, shift, width, counts, tiles, d_n);
+        static_cast<const K*>(ki), vi, static_cast<K*>(ko), vo, n, shift, width, counts, tiles,
+        totals, d_n);
     return launch_check("sort_bin_kernel");
 }
 
 template <typename K>
 static int sort_impl(void* keys, uint32_t* vals, int64_t n, int key_bits, int digit_bits,
                      char* scratch, const SortPlan& p, const uint32_t* d_n, hipStream_t s) {
-    uint32_t* counters = reinterpret_cast<uint32_t*>(scratch + p.off_counters);
-    uint32_t* hist = reinterpret_cast<uint32_t*>(scratch + p.off_hist);
-    uint32_t* offsets = reinterpret_cast<uint32_t*>(scratch + p.off_offsets);
-    uint32_t* status = reinterpret_cast<uint32_t*>(scratch + p.off_status);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(scratch + p.off_counts);
+    uint32_t* totals = reinterpret_cast<uint32_t*>(scratch + p.off_totals);
     void* ka = scratch + p.off_keys_a;
     uint32_t* va = reinterpret_cast<uint32_t*>(scratch + p.off_vals_a);
     void* kb = scratch + p.off_keys_b;
     uint32_t* vb = reinterpret_cast<uint32_t*>(scratch + p.off_vals_b);
-
-    // Counters and histograms start at zero (tile counts are written whole).
-    const int64_t zero_words = (p.off_status - p.off_counters) / 4;
-    WF_TRY(fill_u32(counters, zero_words, 0u, s));
-
-    {
-        const int64_t chunk = (int64_t)HIST_BLOCK * HIST_ITEMS;
-        int64_t blocks = (n + chunk - 1) / chunk;
-        if (blocks > 2048) blocks = 2048;
-        size_t lds = sizeof(uint32_t) * (size_t)p.passes * p.radix;
-        sort_hist_kernel<K><<<(unsigned)blocks, HIST_BLOCK, lds, s>>>(
-            static_cast<const K*>(keys), n, p.passes, digit_bits, key_bits, hist, d_n);
-        WF_TRY(launch_check("sort_hist_kernel"));
-        sort_scan_kernel<<<p.passes, 256, 0, s>>>(hist, offsets, p.radix);
-        WF_TRY(launch_check("sort_scan_kernel"));
-    }
+    (void)key_bits;
 
     // Buffer sequence: starts and ends in the caller's buffers, never reads
     // and writes the same buffer in one pass.
@@ -441,13 +364,12 @@ static int sort_impl(void* keys, uint32_t* vals, int64_t n, int key_bits, int di
     for (int pass = 0; pass < p.passes; ++pass) {
         const int shift = pass * digit_bits;
         const int width = std::min(digit_bits, key_bits - shift);
-        const uint32_t* offs = offsets + (int64_t)pass * p.radix;
         const Buf& in = seq[pass];
         const Buf& out = seq[pass + 1];
         switch (digit_bits) {
-            case 4: WF_TRY((launch_pass<K, 4>(in.k, in.v, out.k, out.v, n, shift, width, status, offs, p.tiles, d_n, s))); break;
-            case 8: WF_TRY((launch_pass<K, 8>(in.k, in.v, out.k, out.v, n, shift, width, status, offs, p.tiles, d_n, s))); break;
-            default: WF_TRY((launch_pass<K, 11>(in.k, in.v, out.k, out.v, n, shift, width, status, offs, p.tiles, d_n, s))); break;
+            case 4: WF_TRY((launch_pass<K, 4>(in.k, in.v, out.k, out.v, n, shift, width, counts, totals, p.tiles, d_n, s))); break;
+            case 8: WF_TRY((launch_pass<K, 8>(in.k, in.v, out.k, out.v, n, shift, width, counts, totals, p.tiles, d_n, s))); break;
+            default: WF_TRY((launch_pass<K, 11>(in.k, in.v, out.k, out.v, n, shift, width, counts, totals, p.tiles, d_n, s))); break;
         }
     }
     if (p.passes == 1) {

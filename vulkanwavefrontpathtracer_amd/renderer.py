@@ -131,6 +131,10 @@ class Renderer:
         call("wf_scene_get_materials", self.scene_handle(), C.byref(n), a.ctypes.data, e.ctypes.data)
         return a, e
 
+    def save_image(self, path, linear=False):
+        """Tone-mapped PNG (present.comp's rgba8 conversion) or linear PFM."""
+        call("wf_renderer_save_image", self.handle, str(path).encode(), 1 if linear else 0)
+
     def stats(self) -> dict:
         counts = (C.c_uint64 * 3)()
         times = (C.c_float * 3)()
