@@ -7,7 +7,9 @@ bits + 30 octahedral direction bits) sorted as u64 keys with a u32 payload,
 bounce's path-ray wavefront (rays + slot-to-pixel LUT) in HBM.  A timed step
 is one pass of the hot path over all captured bounces of that frame:
     keys -> sort -> reorder -> closest-hit trace          (Tracer::traceSort)
-value = rays through the step / wall time of the timed steps.
+value = rays through the step / wall time of the timed steps, the step
+captured once in a HIP graph and replayed (the eager-launch rate is
+reported beside it).
 
 Also reported: per-stage GPU times (hipEvents on the stream the kernels run
 on), the roofline of every stage and of the dominant one, the unsorted-trace
@@ -144,7 +146,8 @@ def main():
     names = ["keys", "sort", "reorder", "trace"]
     ev = {k: [] for k in names}
 
-    def step(record=False):
+    def step(record=False, sp=None):
+        s_ptr = sp if sp is not None else C.c_void_p(stream.cuda_stream)
         for x in bounces:
             n = x["n"]
             marks = []
@@ -185,15 +188,38 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    log(f"timing {args.steps} steps")
+    log(f"timing {args.steps} eager steps")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
+    wall_eager = time.perf_counter() - t0
+    value_eager = total_rays * args.steps / wall_eager / 1e6
+    log(f"eager: {value_eager:.1f} Mrays/s ({wall_eager * 1e3 / args.steps:.3f} ms/step)")
+
+    # The same step captured once in a HIP graph (every stage call is
+    # stream-ordered with no host synchronisation) and replayed.
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        step(sp=C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    for _ in range(args.warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} graph-replayed steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.replay()
+    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
     value = total_rays * args.steps / wall / 1e6
-    log(f"{value:.1f} Mrays/s ({ms_per_step:.3f} ms/step)")
+    log(f"graph: {value:.1f} Mrays/s ({ms_per_step:.3f} ms/step)")
+    # the graph's last replay must equal an eager step's hits
+    graph_hits = [x["hits"].clone() for x in bounces]
+    step()
+    torch.cuda.synchronize()
+    graph_equal = all(torch.equal(h, x["hits"]) for h, x in zip(graph_hits, bounces))
+    del graph
 
     # Per-stage times on the kernels' stream (separate, event-bracketed steps).
     for _ in range(3):
@@ -299,6 +325,8 @@ def main():
                      "traffic": traffic, "launch_ms": round(avg_launch, 4),
                      "launch_elems": bounces[big]["n"], "bytes_per_elem": bpe[dominant]},
         "stages": stages_roof,
+        "value_eager_launches": round(value_eager, 2),
+        "graph_replay_equals_eager": bool(graph_equal),
         "sorted_pipeline_mrays_s": round(sorted_mrays, 1),
         "trace_only_sorted_mrays_s": round(trace_only_sorted, 1),
         "unsorted_trace_mrays_s": round(unsorted_mrays, 1),
