@@ -289,15 +289,24 @@ int wf_camera_matrices(const float position[3], const float rotation_deg[3], flo
 
 /* -------------------------------------------------------------- renderer */
 /* Mirror of Renderer (Renderer/Renderer.h:212-276) driven by an env.json
- * (Environment/AppEnvironment.cpp:10-43 keys and defaults, plus the keys
- * listed in the DESIGN notes of this library). */
+ * (Environment/AppEnvironment.cpp:10-43 keys and defaults, plus this
+ * library's Scene.procedural, Scene.seed, Scene.materialGroups,
+ * Renderer.keyMode, Renderer.digitBits, Renderer.reorderPeriod and
+ * Renderer.russianRouletteStart). */
 typedef struct wf_renderer_s* wf_renderer;
 int wf_renderer_create(const char* env_json_path, wf_renderer* out);
 int wf_renderer_create_from_string(const char* env_json_text, wf_renderer* out);
 int wf_renderer_destroy(wf_renderer r);
-/* Builds the procedural scene the env names (Scene.procedural) or the one
- * given by wf_renderer_set_scene. */
+/* Builds and binds the scene the env names: Scene.procedural (a generator
+ * below) if set, else Scene.filename, a glTF file loaded as by wf_gltf_load
+ * (VulkanExample::prepare, WaveFrontPathTracer.cpp:505-535), with its
+ * bounds from the file as the reference's model.dimensions.  A relative
+ * filename is prefixed with the asset path: "<env file dir>/../assets/" for
+ * wf_renderer_create, "./../assets/" (getAssetPath, base/VulkanTools.cpp:13-29)
+ * for wf_renderer_create_from_string, or what wf_renderer_set_asset_path set
+ * (the reference's resource path).  Scene.materialGroups applies to both. */
 int wf_renderer_build_scene(wf_renderer r);
+int wf_renderer_set_asset_path(wf_renderer r, const char* dir);
 int wf_renderer_set_scene(wf_renderer r, wf_scene scene, const float scene_min[3],
                           const float scene_max[3]);
 int wf_renderer_get_scene(wf_renderer r, wf_scene* out);
@@ -407,6 +416,27 @@ int wf_procedural_scene(int kind, uint64_t seed, int material_groups, int64_t* n
                         float* albedo, float* emission, float scene_min[3],
                         float scene_max[3]);
 enum { WF_SCENE_BOXES = 0, WF_SCENE_INTERIOR = 1, WF_SCENE_CITY = 2 };
+
+/* ------------------------------------------------------------ glTF scenes */
+/* A glTF 2.0 scene (.gltf with file or base64 buffers, or .glb) as
+ * vkglTF::Model::loadFromFile (base/VulkanglTFModel.cpp:767-968,
+ * :1178-1338) and the path tracer's acceleration-structure build
+ * (WaveFrontPathTracer.cpp:202-275) see it, in the procedural generators'
+ * host layout: vertices {pos.xyz, normal.xyz} with positions transformed by
+ * their node's world matrix and normals normalised but untransformed (as
+ * the shading reads them); u32 indices rebased per primitive (vertexStart);
+ * one geometry per indexed primitive in node post-order; tri_material the
+ * primitive's glTF material (primitives without one on the extra last
+ * material); albedo 0.8 and emission 0 for every material (the reference's
+ * constant shading); scene_min/max as Model::getSceneDimensions.  Call
+ * first with NULL arrays for the sizes (this reads only the JSON).  Errors:
+ * WF_ERR_IO (missing file or buffer), WF_ERR_PARSE (malformed JSON, glb or
+ * buffer), WF_ERR_INVALID_ARG (unsupported or inconsistent content). */
+int wf_gltf_load(const char* path, int64_t* num_vertices, int64_t* num_tris, int* num_geoms,
+                 int* num_materials, float* vertices, uint32_t* indices,
+                 int32_t* geom_first_tri, int32_t* tri_material, float* albedo,
+                 float* emission, float scene_min[3], float scene_max[3]);
+
 /* The boxes generator with num_boxes boxes (192 triangles each) instead of
  * 64, plus the ground and light quads: the scenes of the batched workload. */
 int wf_procedural_boxes(uint64_t seed, int num_boxes, int64_t* num_vertices, int64_t* num_tris,

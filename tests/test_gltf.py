@@ -1,0 +1,252 @@
+"""The glTF scene loader (wf_gltf_load) against the reference's loading
+semantics (base/VulkanglTFModel.cpp:767-968, :1459-1488 and
+WaveFrontPathTracer.cpp:202-275), with expectations computed independently
+here: node world matrices from scipy's quaternion rotation, positions
+transformed and normals normalised in float64, and the post-order of
+geometries written out by hand.  Runs on the CPU (the loader is host code)."""
+import json
+
+import numpy as np
+import pytest
+from scipy.spatial.transform import Rotation
+
+from gltf_writer import U8, U16, U32, GltfBuilder  # noqa: F401
+
+rng = np.random.default_rng(7)
+
+
+def trs(t=(0, 0, 0), q=(0, 0, 0, 1), s=(1, 1, 1)):
+    m = np.eye(4)
+    m[:3, :3] = Rotation.from_quat(q).as_matrix() @ np.diag(s)
+    m[:3, 3] = t
+    return m
+
+
+def hierarchy_scene():
+    """Two roots; the first has a child, the second instances the child's
+    mesh again.  Primitives cover u16/u8/u32 indices, an interleaved view,
+    missing normals, a leftover index, no indices and an empty index list."""
+    b = GltfBuilder()
+    pa = rng.uniform(-1, 1, (4, 3))
+    na = rng.uniform(-2, 2, (4, 3))
+    pb = rng.uniform(-1, 1, (5, 3))
+    nb = rng.uniform(-2, 2, (5, 3))
+    pc = rng.uniform(-1, 1, (3, 3))
+    pd = rng.uniform(-1, 1, (3, 3))
+    pe = rng.uniform(-3, 3, (3, 3))
+    ia = np.array([0, 1, 2, 0, 2, 3])
+    ib = np.array([4, 3, 0, 1, 2, 4, 1])   # 7 indices: two triangles, one left over
+    ic = np.array([2, 1, 0])
+    pos_b, nrm_b = b.interleaved(pb, nb)
+    mesh0 = b.mesh([
+        {"attributes": {"POSITION": b.vec3(pa), "NORMAL": b.vec3(na, False)},
+         "indices": b.indices(ia, np.uint16), "material": 1},
+        {"attributes": {"POSITION": pos_b, "NORMAL": nrm_b}, "indices": b.indices(ib, np.uint8)},
+    ])
+    mesh1 = b.mesh([
+        {"attributes": {"POSITION": b.vec3(pc)}, "indices": b.indices(ic, np.uint32), "material": 0},
+        {"attributes": {"POSITION": b.vec3(pd)}},                       # no indices: skipped
+        {"attributes": {"POSITION": b.vec3(pe)}, "indices": b.indices([], np.uint16)},
+    ])
+    m1 = trs((-4, 0, 2), Rotation.from_euler("z", 30, degrees=True).as_quat(), (1.5, 1, 0.5))
+    q0 = Rotation.from_rotvec(0.7 * np.array([1, 1, 0]) / np.sqrt(2)).as_quat()
+    n1 = b.node(matrix=[float(x) for x in m1.T.reshape(-1)], mesh=mesh0)
+    n0 = b.node(translation=[1, 2, 3], rotation=[float(x) for x in q0], scale=[2, 0.5, 1.5],
+                children=[n1], mesh=mesh1)
+    n2 = b.node(translation=[10, 20, 30], mesh=mesh0)
+    n3 = b.node(mesh=mesh1)
+    b.doc["scenes"] = [{"nodes": [n3]}, {"nodes": [n0, n2]}]
+    b.doc["scene"] = 1
+    b.doc["materials"] = [{}, {}]
+    w0 = trs((1, 2, 3), q0, (2, 0.5, 1.5))
+    w1 = w0 @ m1
+    w2 = trs((10, 20, 30))
+    # post-order: n1's mesh0, then n0's mesh1, then n2's mesh0
+    prims = [  # (positions, normals, indices, world, material or None for no geometry)
+        (pa, na, ia, w1, 1), (pb, nb, ib, w1, 2),
+        (pc, None, ic, w0, 0), (pe, None, np.array([], int), w0, None),
+        (pa, na, ia, w2, 1), (pb, nb, ib, w2, 2),
+    ]
+    return b, prims
+
+
+def expected(prims):
+    verts, idx, gf, mats = [], [], [0], []
+    lo = np.full(3, np.inf)
+    hi = np.full(3, -np.inf)
+    start = 0
+    for p, n, ind, w, mat in prims:
+        p32 = p.astype(np.float32).astype(np.float64)
+        world = (w @ np.c_[p32, np.ones(len(p32))].T).T[:, :3]
+        if n is None:
+            nn = np.zeros_like(p32)
+        else:
+            n32 = n.astype(np.float32).astype(np.float64)
+            nn = n32 / np.linalg.norm(n32, axis=1, keepdims=True)
+        verts.append(np.c_[world, nn])
+        # Model::getNodeDimensions: the row vector (min, 1) times the matrix
+        mn = np.r_[p32.min(axis=0), 1.0] @ w
+        mx = np.r_[p32.max(axis=0), 1.0] @ w
+        lo = np.minimum(lo, mn[:3])
+        hi = np.maximum(hi, mx[:3])
+        ntri = len(ind) // 3
+        if ntri:
+            idx.append(ind[:3 * ntri] + start)
+            gf.append(gf[-1] + ntri)
+            mats += [mat] * ntri
+        start += len(p)
+    return (np.concatenate(verts), np.concatenate(idx).reshape(-1, 3), np.array(gf), np.array(mats),
+            lo, hi)
+
+
+@pytest.mark.parametrize("mode", ["data", "file", "glb"])
+def test_hierarchy_matches_reference_semantics(tmp_path, mode):
+    from vulkanwavefrontpathtracer_amd.stages import HostScene
+
+    b, prims = hierarchy_scene()
+    path = b.write(tmp_path / ("scene.glb" if mode == "glb" else "scene.gltf"), mode)
+    hs = HostScene.from_gltf(path)
+    v, idx, gf, mats, lo, hi = expected(prims)
+    assert hs.vertices.shape == (24, 6)
+    np.testing.assert_allclose(hs.vertices[:, :3], v[:, :3], rtol=1e-6, atol=1e-5)
+    np.testing.assert_allclose(hs.vertices[:, 3:], v[:, 3:], rtol=1e-6, atol=1e-6)
+    assert np.array_equal(hs.indices, idx)
+    assert np.array_equal(hs.geom_first, gf)
+    assert np.array_equal(hs.tri_material, mats)
+    assert hs.albedo.shape == (3, 3) and np.all(hs.albedo == np.float32(0.8))
+    assert np.all(hs.emission == 0)
+    np.testing.assert_allclose(hs.bmin, lo, rtol=1e-6, atol=1e-5)
+    np.testing.assert_allclose(hs.bmax, hi, rtol=1e-6, atol=1e-5)
+
+
+def test_scene_dimensions_drop_translation(tmp_path):
+    """Known answer of getSceneDimensions' row-vector product: a node that
+    only translates leaves the bounds at the accessor's own min/max, while the
+    vertices move."""
+    from vulkanwavefrontpathtracer_amd.stages import HostScene
+
+    b = GltfBuilder()
+    p = np.array([[-1, -2, -3], [1, 2, 3], [1, -2, 0]], np.float32)
+    m = b.mesh([{"attributes": {"POSITION": b.vec3(p)}, "indices": b.indices([0, 1, 2], np.uint16)}])
+    b.doc["scenes"] = [{"nodes": [b.node(translation=[10, 20, 30], mesh=m)]}]
+    hs = HostScene.from_gltf(b.write(tmp_path / "t.gltf"))
+    assert np.array_equal(hs.bmin, [-1, -2, -3]) and np.array_equal(hs.bmax, [1, 2, 3])
+    assert np.array_equal(hs.vertices[:, :3], p + [10, 20, 30])
+    # a quarter turn about z: the product applies the transposed rotation
+    b.doc["nodes"][0] = {"rotation": [0, 0, float(np.sin(np.pi / 4)), float(np.cos(np.pi / 4))],
+                         "mesh": m}
+    hs = HostScene.from_gltf(b.write(tmp_path / "r.gltf"))
+    # R^T (x, y, z) = (y, -x, z) for min (-1,-2,-3) and max (1,2,3)
+    np.testing.assert_allclose(hs.bmin, [-2, 1, -3], atol=1e-6)
+    np.testing.assert_allclose(hs.bmax, [2, -1, 3], atol=1e-6)
+    np.testing.assert_allclose(hs.vertices[:, :3], p @ np.array([[0, 1, 0], [-1, 0, 0], [0, 0, 1]]),
+                               atol=1e-6)
+
+
+def test_sizes_call_reads_only_json(tmp_path):
+    """The sizing call (NULL arrays) succeeds without the buffer file."""
+    import ctypes as C
+
+    from vulkanwavefrontpathtracer_amd._capi import call
+
+    b, _ = hierarchy_scene()
+    path = b.write(tmp_path / "s.gltf", "file")
+    (tmp_path / "s buffer.bin").unlink()
+    nv, nt = C.c_int64(), C.c_int64()
+    ng, nm = C.c_int(), C.c_int()
+    call("wf_gltf_load", str(path).encode(), C.byref(nv), C.byref(nt), C.byref(ng), C.byref(nm),
+         None, None, None, None, None, None, None, None)
+    assert (nv.value, nt.value, ng.value, nm.value) == (24, 9, 5, 3)
+
+
+def _status(fn):
+    from vulkanwavefrontpathtracer_amd._capi import WfError
+
+    with pytest.raises(WfError) as e:
+        fn()
+    return e.value.status
+
+
+def test_errors(tmp_path):
+    from vulkanwavefrontpathtracer_amd.stages import HostScene
+
+    load = lambda p: HostScene.from_gltf(p)  # noqa: E731
+    assert _status(lambda: load(tmp_path / "missing.gltf")) == -7
+    (tmp_path / "bad.gltf").write_text("{\"asset\": ")
+    assert _status(lambda: load(tmp_path / "bad.gltf")) == -6
+
+    def variant(name, edit):
+        b, _ = hierarchy_scene()
+        edit(b.doc)
+        return b.write(tmp_path / name)
+
+    def mode_lines(d):
+        d["meshes"][0]["primitives"][0]["mode"] = 1
+    assert _status(lambda: load(variant("lines.gltf", mode_lines))) == -1
+
+    def required_ext(d):
+        d["extensionsRequired"] = ["KHR_draco_mesh_compression"]
+    assert _status(lambda: load(variant("ext.gltf", required_ext))) == -1
+
+    def cycle(d):
+        d["nodes"][0]["children"] = [1]   # n1 -> n0 -> n1
+    assert _status(lambda: load(variant("cycle.gltf", cycle))) == -1
+
+    def no_scene(d):
+        d["scenes"] = []
+    assert _status(lambda: load(variant("noscene.gltf", no_scene))) == -1
+
+    b = GltfBuilder()
+    m = b.mesh([{"attributes": {"POSITION": b.vec3(np.eye(3))}, "indices": b.indices([0, 1, 3], np.uint16)}])
+    b.doc["scenes"] = [{"nodes": [b.node(mesh=m)]}]
+    assert _status(lambda: load(b.write(tmp_path / "oob.gltf"))) == -1
+
+    b = GltfBuilder()
+    m = b.mesh([{"attributes": {"POSITION": b.vec3(np.eye(3))}, "indices": b.indices([0, 1, 2], np.uint16)}])
+    b.doc["scenes"] = [{"nodes": [b.node(mesh=m)]}]
+    p = b.write(tmp_path / "short.gltf")
+    d = json.loads(p.read_text())
+    d["buffers"][0]["byteLength"] += 64
+    p.write_text(json.dumps(d))
+    assert _status(lambda: load(p)) == -6
+
+
+def test_procedural_scene_round_trip(tmp_path):
+    """A procedural scene exported one primitive per geometry loads back with
+    the same triangles, geometry split, materials and bounds."""
+    from gltf_writer import export_host_scene
+
+    from vulkanwavefrontpathtracer_amd import stages
+
+    src = stages.HostScene(stages.SCENE_BOXES, 1000)
+    hs = stages.HostScene.from_gltf(export_host_scene(src, tmp_path / "boxes.glb", "glb"))
+    assert np.array_equal(hs.tri_verts(), src.tri_verts())
+    assert np.array_equal(hs.geom_first, src.geom_first)
+    assert np.array_equal(hs.tri_material, src.tri_material)
+    assert np.array_equal(hs.bmin, src.bmin) and np.array_equal(hs.bmax, src.bmax)
+    n_src = src.vertices[:, 3:][src.indices.reshape(-1)]
+    n_got = hs.vertices[:, 3:][hs.indices.reshape(-1)]
+    assert np.array_equal(n_got, n_src)
+
+
+def test_renderer_resolves_scene_filename(tmp_path):
+    """Scene.filename relative to the asset path: <env dir>/../assets/ for an
+    env file, "./../assets/" for env text, or the one set explicitly.  A
+    missing file is an I/O error that names the resolved path (checked
+    before any device work, so it runs without a GPU)."""
+    from vulkanwavefrontpathtracer_amd._capi import WfError
+    from vulkanwavefrontpathtracer_amd.renderer import Renderer, env_path
+
+    cfg = json.loads(env_path("config0.json").read_text())
+    cfg["Scene"].pop("procedural", None)
+    cfg["Scene"]["filename"] = "models/nothere.glb"
+    (tmp_path / "env").mkdir()
+    envf = tmp_path / "env" / "scene.json"
+    envf.write_text(json.dumps(cfg))
+    for r, where in [(Renderer(envf), str(tmp_path / "assets")),
+                     (Renderer(cfg), "./../assets"),
+                     (Renderer(cfg, asset_path=tmp_path / "elsewhere"), str(tmp_path / "elsewhere"))]:
+        with pytest.raises(WfError) as e:
+            r.build_scene()
+        assert e.value.status == -7
+        assert where + "/models/nothere.glb" in str(e.value), str(e.value)

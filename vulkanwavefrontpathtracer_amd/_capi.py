@@ -86,6 +86,7 @@ SIGNATURES: dict[str, tuple] = {
     "wf_renderer_create_from_string": (i32, [cs, C.POINTER(vp)]),
     "wf_renderer_destroy": (i32, [vp]),
     "wf_renderer_build_scene": (i32, [vp]),
+    "wf_renderer_set_asset_path": (i32, [vp, cs]),
     "wf_renderer_set_scene": (i32, [vp, vp, pf, pf]),
     "wf_renderer_get_scene": (i32, [vp, C.POINTER(vp)]),
     "wf_renderer_render": (i32, [vp, pf]),
@@ -117,6 +118,8 @@ SIGNATURES: dict[str, tuple] = {
     "wf_batch_stats": (i32, [vp, pu64, pi64]),
     "wf_procedural_scene": (i32, [i32, u64, i32, pi64, pi64, C.POINTER(i32), C.POINTER(i32), pf, pu32,
                                   pi32, pi32, pf, pf, pf, pf]),
+    "wf_gltf_load": (i32, [cs, pi64, pi64, C.POINTER(i32), C.POINTER(i32), pf, pu32, pi32, pi32, pf,
+                           pf, pf, pf]),
 }
 
 _lib = None

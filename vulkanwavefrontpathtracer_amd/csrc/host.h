@@ -64,6 +64,12 @@ struct HostScene {
     float bmin[3], bmax[3];
 };
 int procedural_scene(int kind, uint64_t seed, HostScene& out, int num_boxes = 64);
+// glTF 2.0 scene (host_gltf.cpp).  out == nullptr: sizes only.
+struct GltfSizes {
+    int64_t num_vertices = 0, num_tris = 0;
+    int num_geoms = 0, num_materials = 0;
+};
+int load_gltf(const std::string& path, HostScene* out, GltfSizes& sizes);
 // Regroups the non-emissive triangles into contiguous runs, one material each.
 int regroup_materials(HostScene& s, int groups);
 

@@ -147,6 +147,8 @@ struct wf_renderer_s {
     float light[3] = {5, -2, 1}, background[3] = {0, 0, 1};
     float cam_pos[3] = {0, 0, 0}, cam_rot[3] = {0, 0, 0}, znear = 0.001f, zfar = 3.f, fov = 45.f;
     int frame_index = 1;
+    // prefix of a relative Scene.filename (getAssetPath, base/VulkanTools.cpp:13-29)
+    std::string asset_path = "./../assets/";
     // buffers
     wfpt::DeviceBuffer pixels, frame_pixels, aux_pixels, decreases, seeds, i2p, counters, compact;
     wfpt::RayBuffer primary, shadow, queue[2];
@@ -234,6 +236,12 @@ static int create(const std::string& text, bool is_path, wf_renderer* out) {
     auto* r = new wf_renderer_s();
     int st = is_path ? r->env.read_file(text) : r->env.read_string(text);
     if (st == WF_OK) st = load_settings(r);
+    if (is_path) {
+        // env/<name>.json beside assets/, as in the reference's tree
+        const size_t slash = text.find_last_of('/');
+        r->asset_path = (slash == std::string::npos ? std::string(".") : text.substr(0, slash)) +
+                        "/../assets/";
+    }
     if (st != WF_OK) {
         delete r;
         return st;
@@ -508,21 +516,28 @@ extern "C" int wf_renderer_destroy(wf_renderer r) {
     return WF_OK;
 }
 
+// The scene of Scene.procedural (this library's generators) or of
+// Scene.filename (a glTF file, VulkanExample::prepare,
+// WaveFrontPathTracer.cpp:505-535), uploaded and bound with its bounds.
 extern "C" int wf_renderer_build_scene(wf_renderer r) {
     WF_CHECK_ARG(r, "wf_renderer_build_scene: null renderer");
-    std::string kind;
-    if (!r->env.get_string("Scene.procedural", kind))
-        return fail(WF_ERR_INVALID_ARG,
-                    "Scene.procedural is not set (glTF scenes go through wf_renderer_set_scene)");
-    int k = kind == "boxes" ? WF_SCENE_BOXES
-            : kind == "interior" ? WF_SCENE_INTERIOR
-            : kind == "city" ? WF_SCENE_CITY : -1;
-    WF_CHECK_ARG(k >= 0, "Scene.procedural must be boxes, interior or city");
-    WF_TRY(ensure_stream(r));
-    int seed = 1000;
-    r->env.get_int("Scene.seed", seed);
+    std::string kind, file;
     HostScene hs;
-    WF_TRY(procedural_scene(k, (uint64_t)seed, hs));
+    if (r->env.get_string("Scene.procedural", kind) && !kind.empty()) {
+        int k = kind == "boxes" ? WF_SCENE_BOXES
+                : kind == "interior" ? WF_SCENE_INTERIOR
+                : kind == "city" ? WF_SCENE_CITY : -1;
+        WF_CHECK_ARG(k >= 0, "Scene.procedural must be boxes, interior or city");
+        int seed = 1000;
+        r->env.get_int("Scene.seed", seed);
+        WF_TRY(procedural_scene(k, (uint64_t)seed, hs));
+    } else if (r->env.get_string("Scene.filename", file) && !file.empty()) {
+        GltfSizes sz;
+        WF_TRY(load_gltf(file[0] == '/' ? file : r->asset_path + file, &hs, sz));
+    } else {
+        return fail(WF_ERR_INVALID_ARG, "neither Scene.procedural nor Scene.filename is set");
+    }
+    WF_TRY(ensure_stream(r));
     int groups = 0;
     r->env.get_int("Scene.materialGroups", groups);
     if (groups > 0) WF_TRY(regroup_materials(hs, groups));
@@ -553,6 +568,13 @@ extern "C" int wf_renderer_build_scene(wf_renderer r) {
         r->smax[a] = hs.bmax[a];
     }
     r->frame_index = 1;
+    return WF_OK;
+}
+
+extern "C" int wf_renderer_set_asset_path(wf_renderer r, const char* dir) {
+    WF_CHECK_ARG(r && dir, "wf_renderer_set_asset_path: null argument");
+    r->asset_path = dir;
+    if (!r->asset_path.empty() && r->asset_path.back() != '/') r->asset_path += '/';
     return WF_OK;
 }
 

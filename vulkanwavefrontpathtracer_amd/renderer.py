@@ -24,7 +24,11 @@ def env_path(name: str) -> Path:
 
 
 class Renderer:
-    def __init__(self, env: str | Path | dict, overrides: dict | None = None):
+    def __init__(self, env: str | Path | dict, overrides: dict | None = None,
+                 asset_path: str | Path | None = None):
+        """env: an env.json path or its parsed dict; overrides: dotted keys
+        ("Scene.filename": ...) set on top; asset_path: prefix of a relative
+        Scene.filename (default: the env file's ../assets/)."""
         h = C.c_void_p()
         if isinstance(env, dict) or overrides:
             data = env if isinstance(env, dict) else json.loads(Path(env).read_text())
@@ -39,6 +43,10 @@ class Renderer:
         else:
             call("wf_renderer_create", str(env).encode(), C.byref(h))
         self.handle = h
+        if asset_path is None and not isinstance(env, dict):
+            asset_path = Path(env).resolve().parent.parent / "assets"
+        if asset_path is not None:
+            call("wf_renderer_set_asset_path", self.handle, str(asset_path).encode())
 
     def build_scene(self):
         call("wf_renderer_build_scene", self.handle)

@@ -136,17 +136,21 @@ SCENE_BOXES, SCENE_INTERIOR, SCENE_CITY = 0, 1, 2
 
 
 class HostScene:
-    """Host arrays of a procedural scene (vertices: [nv, 6] pos+normal)."""
+    """Host arrays of a procedural or glTF scene (vertices: [nv, 6] pos+normal)."""
 
-    def __init__(self, kind: int, seed: int, material_groups: int = 0, num_boxes: int = 0):
+    def __init__(self, kind: int, seed: int, material_groups: int = 0, num_boxes: int = 0,
+                 gltf: str | None = None):
         """kind = SCENE_BOXES / SCENE_INTERIOR / SCENE_CITY; num_boxes > 0 builds
-        the boxes generator with that many boxes (wf_procedural_boxes)."""
+        the boxes generator with that many boxes (wf_procedural_boxes);
+        gltf = a glTF file to load instead (wf_gltf_load, kind and seed unused)."""
         import numpy as np
 
         lib = _capi.load()
         nv, nt = C.c_int64(), C.c_int64()
         ng, nm = C.c_int(), C.c_int()
-        if num_boxes > 0:
+        if gltf is not None:
+            gen = lambda *a: call("wf_gltf_load", str(gltf).encode(), *a)  # noqa: E731
+        elif num_boxes > 0:
             gen = lambda *a: call("wf_procedural_boxes", seed, num_boxes, *a)  # noqa: E731
         else:
             gen = lambda *a: call("wf_procedural_scene", kind, seed, material_groups, *a)  # noqa: E731
@@ -169,6 +173,10 @@ class HostScene:
              self.tri_material.ctypes.data_as(_capi.pi32), fp(self.albedo), fp(self.emission),
              fp(self.bmin), fp(self.bmax))
         del lib
+
+    @classmethod
+    def from_gltf(cls, path) -> "HostScene":
+        return cls(0, 0, gltf=str(path))
 
     @property
     def num_tris(self) -> int:
