@@ -611,8 +611,9 @@ def tuning_state():
     from vulkanwavefrontpathtracer_amd import stages
 
     out = {}
-    for k in ("bvh.presplit", "bvh.split_depth", "bvh.leaf_max", "trace.lds_stack", "trace.block",
-              "trace.persistent", "trace.refill"):
+    for k in ("bvh.presplit", "bvh.split_depth", "bvh.leaf_max", "bvh.builder", "bvh.collapse",
+              "trace.lds_stack", "trace.block", "trace.persistent", "trace.refill", "trace.width",
+              "trace.loop"):
         v = C.c_int()
         stages.call("wf_get_tuning", k.encode(), C.byref(v))
         out[k] = v.value

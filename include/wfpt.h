@@ -104,6 +104,14 @@ int64_t wf_sort_scratch_bytes(int64_t n, int key_bytes, int key_bits, int digit_
 int wf_sort_pairs(void* keys, uint32_t* vals, int64_t n, int key_bytes, int key_bits,
                   int digit_bits, void* scratch, int64_t scratch_bytes,
                   wf_stream_t stream, float* ms);
+/* As wf_sort_pairs, with the element count read on the device from
+ * *d_count (clamped to capacity; scratch sized for capacity), so a captured
+ * bounce loop can sort the compacted wavefront without the host readback of
+ * the count that RayGen::path makes (RayGen/RayGen.cpp:178-183). */
+int wf_sort_pairs_count(void* keys, uint32_t* vals, int64_t capacity,
+                        const uint32_t* d_count, int key_bytes, int key_bits,
+                        int digit_bits, void* scratch, int64_t scratch_bytes,
+                        wf_stream_t stream);
 
 /* ----------------------------------------------------------------- reorder */
 /* Replaces Tracer::reorderRays (Tracer/Tracer.cpp:80-98), reorderRays.comp:
@@ -264,6 +272,15 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
  *   "bvh.presplit"     0/1  split references of large triangles (default 1)
  *   "bvh.split_depth"  0-8  max binary splits per triangle (default 6)
  *   "bvh.leaf_max"     1-8  triangles per collapsed leaf (default 1)
+ *   "bvh.builder"      0/1  binary hierarchy: 0 Karras LBVH over centroid Morton
+ *                           codes (default), 1 PLOC clustering of the
+ *                           Morton-sorted references (Meister & Bittner 2018);
+ *                           a PLOC tree too deep for the traversal stack is
+ *                           rebuilt as the LBVH
+ *   "bvh.ploc_radius"  1-32 PLOC nearest-neighbour search radius (default 16)
+ *   "bvh.collapse"     0/1  4-wide nodes: 0 collapse every second level,
+ *                           1 open the largest-area child until 4 children
+ *                           (default 1)
  *   "trace.lds_stack"  0/1  keep the top of the traversal stack in LDS (default 1)
  *   "trace.block"      64/128/256  threads per traversal workgroup (default 64)
  *   "trace.persistent" 0/1  persistent waves refilling idle lanes from a ray
@@ -271,6 +288,10 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
  *   "trace.refill"     1-64 idle lanes of a wave that trigger a refill (default 24)
  *   "trace.width"      2/4  traverse the 2-wide or the 4-wide quantized hierarchy
  *                           (both are built; default 4)
+ *   "trace.loop"       0-2  4-wide traversal loop: 0 branch on node/leaf, then
+ *                           load; 1 one load per step for node or leaf
+ *                           (default); 2 leaves postponed until every lane of
+ *                           the wave holds one (Aila & Laine 2009)
  * Returns WF_ERR_INVALID_ARG for an unknown key or value out of range. */
 int wf_set_tuning(const char* key, int value);
 int wf_get_tuning(const char* key, int* value);

@@ -1,0 +1,256 @@
+"""GPU checks of the hierarchy builders.  Both binary builders (the Karras
+LBVH over centroid Morton codes and PLOC clustering) and both 4-wide
+collapses (every second level, greedy largest-area) must give a valid tree:
+every leaf slot reached once and every node once, child boxes that contain
+the triangles below them, a traversal stack bound that holds on every
+root-to-leaf path, and closest-hit records equal to the oracle's bit for
+bit.  Degenerate inputs (coincident triangles, two- and three-triangle
+scenes) must build with a bounded depth."""
+import contextlib
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [(0, 0), (0, 1), (1, 0), (1, 1)]   # (bvh.builder, bvh.collapse)
+
+
+@contextlib.contextmanager
+def knobs(**kv):
+    from vulkanwavefrontpathtracer_amd import _capi
+
+    saved = {}
+    for k, v in kv.items():
+        key = k.replace("__", ".").encode()
+        cur = C.c_int()
+        _capi.call("wf_get_tuning", key, C.byref(cur))
+        saved[key] = cur.value
+        _capi.call("wf_set_tuning", key, v)
+    try:
+        yield
+    finally:
+        for key, v in saved.items():
+            _capi.call("wf_set_tuning", key, v)
+
+
+def rays_in_box(bmin, bmax, n, seed, tmin=1e-4):
+    rng = np.random.default_rng(seed)
+    r = np.zeros((n, 8), np.float32)
+    r[:, :3] = rng.uniform(bmin, bmax, (n, 3))
+    d = rng.standard_normal((n, 3))
+    r[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    r[:, 3] = tmin
+    r[:, 7] = 3.402823466e38
+    return r
+
+
+def expected_hits(orc, tri_verts, geom_first, rays):
+    ref = orc.trace_f32(rays, tri_verts, closest=True)
+    prim, geom = orc.hits_to_prim_geom(ref[:, 0], geom_first)
+    return np.stack([prim, geom, ref[:, 1], ref[:, 2]], axis=1).astype(np.int32)
+
+
+def binary_tree(sc):
+    from vulkanwavefrontpathtracer_amd import _capi
+
+    info = sc.info()
+    nodes = np.zeros((info["nodes"], 16), np.float32)
+    order = np.zeros(info["nodes"] + 1, np.int32)
+    _capi.call("wf_scene_download", sc.handle, nodes.ctypes.data_as(_capi.pf),
+               order.ctypes.data_as(_capi.pi32))
+    return nodes, order
+
+
+def check_binary(num_tris, sc):
+    """Walk the 2-wide tree: returns its depth (interior levels)."""
+    nodes, order = binary_tree(sc)
+    slots = len(order)
+    assert np.array_equal(np.unique(order), np.arange(num_tris))
+    refs = nodes[:, 12:14].view(np.int32)
+    covered = np.zeros(slots, np.int32)
+    seen = np.zeros(len(nodes), np.int32)
+    todo, depth = [(0, 1)], 0
+    while todo:
+        nd, d = todo.pop()
+        seen[nd] += 1
+        depth = max(depth, d)
+        for ref in refs[nd]:
+            if ref >= 0:
+                todo.append((int(ref), d + 1))
+            else:
+                first, cnt = (~int(ref)) >> 3, ((~int(ref)) & 7) + 1
+                covered[first:first + cnt] += 1
+    assert np.all(covered == 1) and seen.max() == 1
+    return depth
+
+
+def decode_wide(nodes4):
+    raw = nodes4.view(np.uint32).reshape(-1, 16)
+    origin = nodes4.reshape(-1, 16)[:, :3].astype(np.float64)
+    meta = raw[:, 3]
+    nc = (meta >> 24).astype(int)
+    scale = np.stack([np.ldexp(1.0, ((meta >> (8 * a)) & 255).astype(int) - 127) for a in range(3)], 1)
+    q = raw[:, 4:10]
+    qb = np.stack([(q >> (8 * k)) & 255 for k in range(4)], 1).astype(np.float64)
+    lo = origin[:, None, :] + qb[:, :, :3] * scale[:, None, :]
+    hi = origin[:, None, :] + qb[:, :, 3:] * scale[:, None, :]
+    refs = raw[:, 10:14].view(np.int32)
+    return lo, hi, refs, nc
+
+
+def check_wide(tri_verts, sc):
+    """4-wide tree: every slot once, every node once, decoded child boxes
+    contain the (unsplit) triangles below them, and the sum of (children - 1)
+    along any root-to-node path stays within the reported stack bound."""
+    from vulkanwavefrontpathtracer_amd import _capi
+
+    n4, st4 = C.c_int64(), C.c_int()
+    _capi.call("wf_scene_wide_info", sc.handle, C.byref(n4), C.byref(st4), None)
+    nodes4 = np.zeros((n4.value, 16), np.float32)
+    _capi.call("wf_scene_wide_info", sc.handle, None, None, nodes4.ctypes.data_as(_capi.pf))
+    _, order = binary_tree(sc)
+    slots = len(order)
+    lo, hi, refs, nc = decode_wide(nodes4)
+    assert np.all(nc <= 4) and np.all((nc >= 2) | (n4.value == 1))
+    whole = np.bincount(order, minlength=len(tri_verts))[order] == 1
+    tv = tri_verts.reshape(-1, 3, 3).astype(np.float64)
+    slot_lo = np.where(whole[:, None], tv[order].min(axis=1), np.inf)
+    slot_hi = np.where(whole[:, None], tv[order].max(axis=1), -np.inf)
+    seen = np.zeros(n4.value, np.int32)
+    covered = np.zeros(slots, np.int32)
+    # post-order: bounds of what lies under each node, checked against the
+    # parent's decoded child box
+    sub_lo = np.full((n4.value, 3), np.inf)
+    sub_hi = np.full((n4.value, 3), -np.inf)
+    order_nodes, todo, max_need = [], [(0, 0)], 0
+    while todo:
+        nd, need = todo.pop()
+        seen[nd] += 1
+        order_nodes.append(nd)
+        need += nc[nd] - 1
+        max_need = max(max_need, need)
+        for k in range(nc[nd]):
+            r = int(refs[nd, k])
+            if r >= 0:
+                todo.append((r, need))
+            else:
+                first, cnt = (~r) >> 3, ((~r) & 7) + 1
+                covered[first:first + cnt] += 1
+    assert seen.max() == 1 and seen.sum() == n4.value
+    assert np.all(covered == 1)
+    for nd in reversed(order_nodes):
+        for k in range(nc[nd]):
+            r = int(refs[nd, k])
+            if r >= 0:
+                clo, chi = sub_lo[r], sub_hi[r]
+            else:
+                first, cnt = (~r) >> 3, ((~r) & 7) + 1
+                clo = slot_lo[first:first + cnt].min(axis=0)
+                chi = slot_hi[first:first + cnt].max(axis=0)
+            assert np.all(clo >= lo[nd, k]) and np.all(chi <= hi[nd, k]), (nd, k)
+            sub_lo[nd] = np.minimum(sub_lo[nd], clo)
+            sub_hi[nd] = np.maximum(sub_hi[nd], chi)
+    assert max_need <= st4.value <= 96
+    return max_need, st4.value
+
+
+@pytest.mark.parametrize("builder,collapse", VARIANTS)
+def test_builders_structure_boxes(dev, builder, collapse):
+    from vulkanwavefrontpathtracer_amd import stages
+
+    with knobs(bvh__builder=builder, bvh__collapse=collapse):
+        hs = stages.HostScene(stages.SCENE_BOXES, 1000)
+        sc = stages.Scene.from_host(hs, dev)
+        depth = check_binary(hs.num_tris, sc)
+        need, bound = check_wide(hs.tri_verts(), sc)
+    assert depth < 64
+    if collapse == 1:
+        assert need == bound            # the greedy marking measures the exact bound
+
+
+_INTERIOR = {}
+
+
+@pytest.mark.parametrize("leaf", [1, 4])
+@pytest.mark.parametrize("builder,collapse", VARIANTS)
+def test_builders_bit_exact_interior(dev, orc, builder, collapse, leaf):
+    """Interior scene (walls split into many references): every builder and
+    collapse returns the oracle's closest-hit records, any-hit agrees on
+    visibility, in both the 4-wide and 2-wide traversal."""
+    from vulkanwavefrontpathtracer_amd import stages
+
+    if leaf not in _INTERIOR:
+        hs = stages.HostScene(stages.SCENE_INTERIOR, 1001)
+        rays = rays_in_box(hs.bmin, hs.bmax, 1500, 7 + leaf)
+        rays[::101, 7] = -1.0
+        _INTERIOR[leaf] = (hs, rays, expected_hits(orc, hs.tri_verts(), hs.geom_first, rays))
+    hs, rays, exp = _INTERIOR[leaf]
+    with knobs(bvh__builder=builder, bvh__collapse=collapse, bvh__leaf_max=leaf):
+        sc = stages.Scene.from_host(hs, dev)
+        tr = torch.from_numpy(rays).to(dev)
+        for width in (4, 2):
+            with knobs(trace__width=width):
+                got = stages.trace(sc, tr).cpu().numpy()
+                bad = np.nonzero(np.any(got != exp, axis=1))[0]
+                assert len(bad) == 0, f"width {width}: {len(bad)} mismatches, e.g. {bad[:3]}"
+                anyh = stages.trace(sc, tr, closest=False).cpu().numpy()
+                assert np.array_equal(anyh[:, 0] >= 0, got[:, 0] >= 0)
+    assert np.mean(exp[:, 0] >= 0) > 0.5
+
+
+@pytest.mark.parametrize("presplit", [0, 1])
+@pytest.mark.parametrize("builder,collapse", VARIANTS)
+def test_builders_degenerate_inputs(dev, orc, builder, collapse, presplit):
+    """Coincident triangles (identical boxes, tied hits: the smaller original
+    index wins) and two- and three-triangle scenes.  With split references
+    the large overlapping pieces can make a clustering tree too deep for the
+    traversal stack; the build then falls back to the Karras tree."""
+    from vulkanwavefrontpathtracer_amd import stages
+
+    rng = np.random.default_rng(5)
+    k = 600
+    base = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    rand = rng.uniform(-3, 3, (200, 3, 3)).astype(np.float32)
+    tris = np.concatenate([np.repeat(base[None], k, axis=0), rand])
+    verts = np.zeros((len(tris) * 3, 6), np.float32)
+    verts[:, :3] = tris.reshape(-1, 3)
+    verts[:, 5] = 1.0
+    idx = np.arange(len(tris) * 3, dtype=np.uint32).reshape(-1, 3)
+    with knobs(bvh__builder=builder, bvh__collapse=collapse, bvh__presplit=presplit):
+        for nt in (2, 3, len(tris)):
+            gf = np.array([0, nt], np.int32)
+            sc = stages.Scene(verts[:3 * nt], idx[:nt], gf, device=dev)
+            depth = check_binary(nt, sc)
+            check_wide(tris[:nt].reshape(-1, 9), sc)
+            assert depth < 64
+            rays = rays_in_box((-3, -3, -3), (3, 3, 3), 800, nt)
+            rays[:100, :3] = rng.uniform(0.05, 0.3, (100, 3)) * [1, 1, 0] + [0, 0, 2]
+            rays[:100, 4:7] = [0, 0, -1]
+            got = stages.trace(sc, torch.from_numpy(rays).to(dev)).cpu().numpy()
+            assert np.array_equal(got, expected_hits(orc, tris[:nt].reshape(-1, 9), gf, rays))
+
+
+def test_builder_node_visits(dev):
+    """Traversal work per ray on the interior scene for each builder and
+    collapse (printed); PLOC must not visit more nodes than the LBVH."""
+    from vulkanwavefrontpathtracer_amd import _capi, stages
+
+    hs = stages.HostScene(stages.SCENE_INTERIOR, 1001)
+    rays = torch.from_numpy(rays_in_box(hs.bmin, hs.bmax, 20000, 3)).to(dev)
+    hits = torch.empty((len(rays), 4), dtype=torch.int32, device=dev)
+    visits = {}
+    for builder, collapse in VARIANTS:
+        with knobs(bvh__builder=builder, bvh__collapse=collapse):
+            sc = stages.Scene.from_host(hs, dev)
+            nv = torch.zeros(len(rays), dtype=torch.int32, device=dev)
+            nt = torch.zeros_like(nv)
+            _capi.call("wf_trace_stats", sc.handle, C.c_void_p(rays.data_ptr()),
+                       C.c_void_p(hits.data_ptr()), len(rays), 1, C.c_void_p(nv.data_ptr()),
+                       C.c_void_p(nt.data_ptr()), None)
+            torch.cuda.synchronize()
+            visits[(builder, collapse)] = (nv.float().mean().item(), nt.float().mean().item())
+    print("node visits / triangle tests per ray (builder, collapse):", visits)
+    assert visits[(1, 0)][0] <= visits[(0, 0)][0] * 1.05

@@ -79,7 +79,7 @@ SORT_CASES = [
 
 
 @pytest.mark.parametrize("dtype,bits,digit", SORT_CASES)
-@pytest.mark.parametrize("n", [0, 1, 2, 4095, 4096, 4097, 70_001])
+@pytest.mark.parametrize("n", [0, 1, 2, 4095, 4096, 4097, 8191, 8192, 8193, 70_001])
 def test_sort_bit_exact(dev, orc, dtype, bits, digit, n):
     from vulkanwavefrontpathtracer_amd import stages
 
@@ -122,6 +122,37 @@ def test_sort_full_size_properties(dev, dtype, bits, digit):
     assert torch.equal(orig[vals.long()], keys)                  # payload follows its key
     same = k[1:] == k[:-1]
     assert bool((vals[1:][same] > vals[:-1][same]).all())       # stable
+
+
+@pytest.mark.parametrize("dtype,bits,digit", [(np.uint32, 30, 8), (np.uint64, 60, 8),
+                                              (np.uint64, 64, 4), (np.uint32, 32, 11)])
+@pytest.mark.parametrize("capacity,count", [(8192, 5000), (8192, 8192), (8192, 0),
+                                            (20_000, 8000), (70_001, 9000), (70_001, 70_001)])
+def test_sort_count_on_device_bit_exact(dev, orc, dtype, bits, digit, capacity, count):
+    """wf_sort_pairs_count sorts the first *d_count pairs of a capacity-sized
+    buffer (the count stays on the device) and leaves the rest untouched."""
+    import ctypes as C
+
+    from vulkanwavefrontpathtracer_amd import _capi
+
+    rng = np.random.default_rng(capacity + count + bits)
+    keys = rng.integers(0, np.iinfo(dtype).max, size=capacity, dtype=dtype, endpoint=True)
+    keys[: count // 3] = keys[0]
+    vals = rng.permutation(capacity).astype(np.uint32)
+    kb = np.dtype(dtype).itemsize
+    tk = torch.from_numpy(keys.view(np.int32 if kb == 4 else np.int64)).to(dev)
+    tv = torch.from_numpy(vals.view(np.int32)).to(dev)
+    d_count = torch.tensor([count], dtype=torch.int32, device=dev)
+    need = _capi.load().wf_sort_scratch_bytes(capacity, kb, bits, digit)
+    scratch = torch.empty(need, dtype=torch.uint8, device=dev)
+    _capi.call("wf_sort_pairs_count", C.c_void_p(tk.data_ptr()), C.c_void_p(tv.data_ptr()), capacity,
+               C.c_void_p(d_count.data_ptr()), kb, bits, digit, C.c_void_p(scratch.data_ptr()),
+               need, None)
+    torch.cuda.synchronize()
+    rk, rv = orc.sort_pairs(keys[:count], vals[:count], bits)
+    gk, gv = tk.cpu().numpy().view(dtype), tv.cpu().numpy().view(np.uint32)
+    assert np.array_equal(gk[:count], rk) and np.array_equal(gv[:count], rv)
+    assert np.array_equal(gk[count:], keys[count:]) and np.array_equal(gv[count:], vals[count:])
 
 
 def test_sort_rejects_bad_args(dev):

@@ -55,6 +55,7 @@ SIGNATURES: dict[str, tuple] = {
     "wf_morton_keys_batched": (i32, [vp, vp, vp, vp, i64, pf, pf, i32, vp, pf]),
     "wf_sort_scratch_bytes": (i64, [i64, i32, i32, i32]),
     "wf_sort_pairs": (i32, [vp, vp, i64, i32, i32, i32, vp, i64, vp, pf]),
+    "wf_sort_pairs_count": (i32, [vp, vp, i64, vp, i32, i32, i32, vp, i64, vp]),
     "wf_reorder": (i32, [vp, vp, vp, vp, vp, vp, i64, vp, pf]),
     "wf_scene_create": (i32, [vp, i64, i32, vp, i64, pi32, i32, vp, vp, C.POINTER(vp)]),
     "wf_scene_destroy": (i32, [vp]),

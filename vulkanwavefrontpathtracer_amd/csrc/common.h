@@ -135,11 +135,15 @@ struct Tuning {
     int bvh_presplit = 1;
     int bvh_split_depth = 6;
     int bvh_leaf_max = 1;
+    int bvh_builder = 0;        // 0: Karras LBVH over Morton codes; 1: PLOC clustering
+    int bvh_ploc_radius = 16;   // PLOC nearest-neighbour search radius (positions)
+    int bvh_collapse = 1;       // 4-wide nodes: 0 every second level; 1 greedy largest-area
     int trace_lds_stack = 1;
     int trace_block = 64;
     int trace_persistent = 0;   // persistent waves that refill idle lanes with new rays
     int trace_refill = 24;      // idle lanes of a wave that trigger a refill
     int trace_width = 4;        // traverse the 2-wide or the 4-wide hierarchy
+    int trace_loop = 1;         // 4-wide loop: 0 branch then load, 1 one load per step, 2 postponed leaves
 };
 Tuning& tuning();
 }  // namespace wfpt

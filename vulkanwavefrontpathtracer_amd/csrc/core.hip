@@ -80,11 +80,15 @@ extern "C" int wf_set_tuning(const char* key, int value) {
     if (k == "bvh.presplit" && (value == 0 || value == 1)) t.bvh_presplit = value;
     else if (k == "bvh.split_depth" && value >= 0 && value <= 8) t.bvh_split_depth = value;
     else if (k == "bvh.leaf_max" && value >= 1 && value <= 8) t.bvh_leaf_max = value;
+    else if (k == "bvh.builder" && (value == 0 || value == 1)) t.bvh_builder = value;
+    else if (k == "bvh.ploc_radius" && value >= 1 && value <= 32) t.bvh_ploc_radius = value;
+    else if (k == "bvh.collapse" && (value == 0 || value == 1)) t.bvh_collapse = value;
     else if (k == "trace.lds_stack" && (value == 0 || value == 1)) t.trace_lds_stack = value;
     else if (k == "trace.block" && (value == 64 || value == 128 || value == 256)) t.trace_block = value;
     else if (k == "trace.persistent" && (value == 0 || value == 1)) t.trace_persistent = value;
     else if (k == "trace.refill" && value >= 1 && value <= 64) t.trace_refill = value;
     else if (k == "trace.width" && (value == 2 || value == 4)) t.trace_width = value;
+    else if (k == "trace.loop" && value >= 0 && value <= 2) t.trace_loop = value;
     else return fail(WF_ERR_INVALID_ARG, "wf_set_tuning: unknown key or value out of range: " + k);
     return WF_OK;
 }
@@ -96,11 +100,15 @@ extern "C" int wf_get_tuning(const char* key, int* value) {
     if (k == "bvh.presplit") *value = t.bvh_presplit;
     else if (k == "bvh.split_depth") *value = t.bvh_split_depth;
     else if (k == "bvh.leaf_max") *value = t.bvh_leaf_max;
+    else if (k == "bvh.builder") *value = t.bvh_builder;
+    else if (k == "bvh.ploc_radius") *value = t.bvh_ploc_radius;
+    else if (k == "bvh.collapse") *value = t.bvh_collapse;
     else if (k == "trace.lds_stack") *value = t.trace_lds_stack;
     else if (k == "trace.block") *value = t.trace_block;
     else if (k == "trace.persistent") *value = t.trace_persistent;
     else if (k == "trace.refill") *value = t.trace_refill;
     else if (k == "trace.width") *value = t.trace_width;
+    else if (k == "trace.loop") *value = t.trace_loop;
     else return fail(WF_ERR_INVALID_ARG, "wf_get_tuning: unknown key " + k);
     return WF_OK;
 }

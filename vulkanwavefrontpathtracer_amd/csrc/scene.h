@@ -50,7 +50,7 @@ struct wf_scene_s {
 
 namespace wfpt {
 
-constexpr int TRACE_STACK = 64;        // traversal stack entries per ray (2-wide)
+constexpr int TRACE_STACK = 96;        // traversal stack entries per ray (2-wide)
 constexpr int TRACE_STACK4 = 96;       // traversal stack entries per ray (4-wide)
 
 __device__ __forceinline__ int leaf_ref(int first, int count) { return ~((first << 3) | (count - 1)); }

@@ -299,6 +299,134 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
     }
 }
 
+// ------------------------------------------------------------ small sorts
+// Up to SMALL_N pairs: one workgroup of 1024 threads holds 8 pairs per thread
+// in registers and runs every pass in LDS (rank by wave64 ballot matching,
+// per-digit prefixes over the 16 waves and over the digits, scatter to the
+// LDS copy in digit order, read back in index order), then writes the
+// result once.  One launch instead of three per pass: the tail bounces of a
+// frame have a few thousand rays, where the multi-kernel passes are bound by
+// launch and memory latency, not bandwidth.
+constexpr int SMALL_BLOCK = 1024;
+constexpr int SMALL_WAVES = SMALL_BLOCK / 64;
+constexpr int SMALL_ITEMS = 8;
+constexpr int SMALL_N = SMALL_BLOCK * SMALL_ITEMS;
+
+// Block-wide (1024 threads) exclusive scan of one value per thread.
+__device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t* s_tmp) {
+    const unsigned lane = lane_id();
+    const unsigned wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= (unsigned)off) x += y;
+    }
+    if (lane == 63) s_tmp[wave] = x;
+    __syncthreads();
+    uint32_t wbase = 0;
+#pragma unroll
+    for (int w = 0; w < SMALL_WAVES; ++w)
+        if (w < (int)wave) wbase += s_tmp[w];
+    __syncthreads();
+    return wbase + x - v;
+}
+
+template <typename K, int DBITS>
+__global__ __launch_bounds__(SMALL_BLOCK) void sort_small_kernel(K* __restrict__ keys,
+                                                                 uint32_t* __restrict__ vals,
+                                                                 int64_t n, int key_bits,
+                                                                 const uint32_t* __restrict__ d_n) {
+    constexpr int RADIX = 1 << DBITS;
+    static_assert(RADIX <= SMALL_BLOCK, "one thread per digit");
+    __shared__ uint32_t s_wave[SMALL_WAVES][RADIX];  // per-wave counts -> wave prefixes
+    __shared__ uint32_t s_start[RADIX];              // digit starts in the block
+    __shared__ K s_key[SMALL_N];
+    __shared__ uint32_t s_val[SMALL_N];
+    __shared__ uint32_t s_tmp[SMALL_WAVES];
+    if (d_n) n = min(n, (int64_t)*d_n);
+    const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    // Wave-striped layout: element e = wave * 64 * ITEMS + it * 64 + lane,
+    // so (wave, item, lane) order is index order and the ranking is stable.
+    const int wbase = (int)wave * 64 * SMALL_ITEMS;
+    K key[SMALL_ITEMS];
+    uint32_t val[SMALL_ITEMS];
+#pragma unroll
+    for (int it = 0; it < SMALL_ITEMS; ++it) {
+        const int e = wbase + it * 64 + (int)lane;
+        const bool ok = e < n;
+        key[it] = ok ? keys[e] : K(0);
+        val[it] = ok ? vals[e] : 0u;
+    }
+    const int passes = (key_bits + DBITS - 1) / DBITS;
+    for (int pass = 0; pass < passes; ++pass) {
+        const int shift = pass * DBITS;
+        const int width = min(DBITS, key_bits - shift);
+        const uint32_t mask = (1u << width) - 1u;
+        for (int i = tid; i < SMALL_WAVES * RADIX; i += SMALL_BLOCK) (&s_wave[0][0])[i] = 0;
+        __syncthreads();
+        uint32_t off[SMALL_ITEMS];
+#pragma unroll
+        for (int it = 0; it < SMALL_ITEMS; ++it) {
+            const bool ok = wbase + it * 64 + (int)lane < n;
+            const uint32_t d = digit_of(key[it], shift, mask);
+            uint64_t peers = ballot64(ok);
+#pragma unroll
+            for (int b = 0; b < DBITS; ++b) {
+                const bool bit = (d >> b) & 1u;
+                const uint64_t bal = ballot64(bit);
+                peers &= bit ? bal : ~bal;
+            }
+            const uint32_t rank = popc_below(peers);
+            const uint32_t cnt = (uint32_t)__popcll(peers);
+            uint32_t pre = 0;
+            if (ok) pre = s_wave[wave][d];
+            off[it] = pre + rank;
+            if (ok && rank == 0) s_wave[wave][d] = pre + cnt;
+        }
+        __syncthreads();
+        uint32_t total = 0;
+        if (tid < RADIX) {
+#pragma unroll
+            for (int w = 0; w < SMALL_WAVES; ++w) {
+                const uint32_t c = s_wave[w][tid];
+                s_wave[w][tid] = total;
+                total += c;
+            }
+        }
+        const uint32_t start = block_excl_scan_1024(total, s_tmp);
+        if (tid < RADIX) s_start[tid] = start;
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < SMALL_ITEMS; ++it) {
+            if (wbase + it * 64 + (int)lane < n) {
+                const uint32_t d = digit_of(key[it], shift, mask);
+                const uint32_t pos = s_start[d] + s_wave[wave][d] + off[it];
+                s_key[pos] = key[it];
+                s_val[pos] = val[it];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < SMALL_ITEMS; ++it) {
+            const int e = wbase + it * 64 + (int)lane;
+            if (e < n) {
+                key[it] = s_key[e];
+                val[it] = s_val[e];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int it = 0; it < SMALL_ITEMS; ++it) {
+        const int e = wbase + it * 64 + (int)lane;
+        if (e < n) {
+            keys[e] = key[it];
+            vals[e] = val[it];
+        }
+    }
+}
+
 template <typename K>
 __global__ __launch_bounds__(256) void copy_pairs_kernel(const K* __restrict__ ks,
                                                          const uint32_t* __restrict__ vs,
@@ -392,6 +520,17 @@ int sort_pairs_launch(void* keys, uint32_t* vals, int64_t n_cap, const uint32_t*
     WF_CHECK_ARG(keys && vals && scratch, "sort: null buffer");
     if (scratch_bytes < p.total)
         return fail(WF_ERR_SCRATCH_TOO_SMALL, "sort: scratch smaller than wf_sort_scratch_bytes");
+    if (n_cap <= SMALL_N && digit_bits <= 8) {
+        if (key_bytes == 4 && digit_bits == 8)
+            sort_small_kernel<uint32_t, 8><<<1, SMALL_BLOCK, 0, s>>>((uint32_t*)keys, vals, n_cap, key_bits, d_n);
+        else if (key_bytes == 4)
+            sort_small_kernel<uint32_t, 4><<<1, SMALL_BLOCK, 0, s>>>((uint32_t*)keys, vals, n_cap, key_bits, d_n);
+        else if (digit_bits == 8)
+            sort_small_kernel<uint64_t, 8><<<1, SMALL_BLOCK, 0, s>>>((uint64_t*)keys, vals, n_cap, key_bits, d_n);
+        else
+            sort_small_kernel<uint64_t, 4><<<1, SMALL_BLOCK, 0, s>>>((uint64_t*)keys, vals, n_cap, key_bits, d_n);
+        return launch_check("sort_small_kernel");
+    }
     return key_bytes == 4
                ? sort_impl<uint32_t>(keys, vals, n_cap, key_bits, digit_bits, (char*)scratch, p, d_n, s)
                : sort_impl<uint64_t>(keys, vals, n_cap, key_bits, digit_bits, (char*)scratch, p, d_n, s);

@@ -17,11 +17,11 @@
 #include "traverse.h"
 
 #include <algorithm>
-#include <mutex>
 
 namespace wfpt {
 
-template <bool CLOSEST, int BLOCK, bool LDS, bool STATS = false, bool WIDE = false>
+template <bool CLOSEST, int BLOCK, bool LDS, bool STATS = false, bool WIDE = false,
+          int LOOP = TRACE_LOOP_DEFAULT>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__ nodes,
                                                       const float4* __restrict__ tris,
                                                       const wf_ray* __restrict__ rays,
@@ -37,7 +37,8 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__
     load_ray(rays, r, ro, rd);
     WtRay wr;
     const bool live = (rd.w >= ro.w) && wt_setup(ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, wr);
-    const TraceOut tr = traverse<CLOSEST, BLOCK, LDS, STATS, WIDE>(nodes, tris, ro, rd, wr, live, s_stack);
+    const TraceOut tr =
+        traverse<CLOSEST, BLOCK, LDS, STATS, WIDE, LOOP>(nodes, tris, ro, rd, wr, live, s_stack);
     *reinterpret_cast<int4*>(hits + r) = hit_record(tris, wr, tr);
     if (STATS) {
         stat_nodes[r] = tr.nodes_visited;
@@ -46,35 +47,38 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__
 }
 
 // Persistent variant: a grid sized to the resident waves; each wave keeps
-// its lanes busy by handing rays out of a global counter to idle lanes (one
-// atomic per refill of `refill` or more idle lanes), so a wave is no longer
-// held by its slowest of 64 rays (Aila & Laine 2009).  Rays are taken in
-// index order, so sorted wavefronts keep their coherence; each ray's
-// traversal is the same step sequence as in trace_kernel.
+// its lanes busy by handing new rays to idle lanes whenever `refill` or more
+// are idle, so a wave is no longer held by its slowest of 64 rays (Aila &
+// Laine 2009).  The rays are dealt statically: wave w of W owns the 64-ray
+// chunks w, w + W, w + 2W, ... and takes them in order, so no counter (and
+// no device-scope atomic) is involved, a chunk keeps the coherence of a
+// sorted wavefront, and every wave samples the whole wavefront, which
+// balances the work.  Each ray's traversal is the same step sequence as in
+// trace_kernel, and the hit record fields are kept when a hit is accepted,
+// so a finished lane stores its record without reloading the triangle.
 template <bool CLOSEST, int BLOCK, bool LDS, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void trace_persistent_kernel(
     const float4* __restrict__ nodes, const float4* __restrict__ tris,
     const wf_ray* __restrict__ rays, const uint32_t* __restrict__ ray_index,
-    wf_hit* __restrict__ hits, int64_t n, uint32_t* __restrict__ counter, int refill) {
+    wf_hit* __restrict__ hits, int64_t n, int refill) {
     __shared__ int s_stack[LDS ? LDS_DEPTH * BLOCK : 1];
     const unsigned lane = lane_id();
     constexpr int STACK = WIDE ? TRACE_STACK4 : TRACE_STACK;
+    const int64_t num_waves = (int64_t)gridDim.x * (BLOCK / 64);
+    const int64_t wave = (int64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6);
+    // j-th ray of this wave's sequence (monotone in j)
+    auto ray_of = [&](int64_t j) { return (((j >> 6) * num_waves + wave) << 6) + (j & 63); };
     TravState<LDS, STACK> st;
     int spill[TravState<LDS, STACK>::SPILL];
     WtRay wr;
-    int64_t r = 0;
-    bool active = false, exhausted = false;
+    int64_t r = 0, next = 0;
+    bool active = false, exhausted = ray_of(0) >= n;
     while (true) {
         const uint64_t idle = ballot64(!active);
         const int nidle = __popcll(idle);
         if (!exhausted && nidle >= refill) {
-            const int leader = __ffsll((unsigned long long)idle) - 1;
-            uint32_t base = 0;
-            if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)nidle);
-            base = __shfl(base, leader, 64);
-            if ((int64_t)base + nidle >= n) exhausted = true;
             if (!active) {
-                const int64_t k = (int64_t)base + popc_below(idle);
+                const int64_t k = ray_of(next + popc_below(idle));
                 if (k < n) {
                     r = ray_index ? (int64_t)ray_index[k] : k;
                     float4 ro, rd;
@@ -82,9 +86,11 @@ __global__ __launch_bounds__(BLOCK) void trace_persistent_kernel(
                     const bool live = (rd.w >= ro.w) && wt_setup(ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, wr);
                     trav_begin(st, ro, rd);
                     if (live) active = true;
-                    else *reinterpret_cast<int4*>(hits + r) = hit_record(tris, wr, TraceOut{-1, rd.w, 0u, 0u});
+                    else *reinterpret_cast<int4*>(hits + r) = make_int4(-1, -1, __float_as_int(-1.0f), 0);
                 }
             }
+            next += nidle;
+            exhausted = ray_of(next) >= n;
         }
         if (ballot64(active) == 0) {
             if (exhausted) break;
@@ -92,11 +98,14 @@ __global__ __launch_bounds__(BLOCK) void trace_persistent_kernel(
         }
         bool fin = false;
         if (active) {
-            if (WIDE) fin = trav_step4<CLOSEST, BLOCK, LDS, false>(st, spill, nodes, tris, wr, s_stack);
-            else fin = trav_step<CLOSEST, BLOCK, LDS, false>(st, spill, nodes, tris, wr, s_stack);
+            if (WIDE) fin = trav_step4<CLOSEST, BLOCK, LDS, false, STACK, true>(st, spill, nodes, tris, wr, s_stack);
+            else fin = trav_step<CLOSEST, BLOCK, LDS, false, STACK, true>(st, spill, nodes, tris, wr, s_stack);
         }
         if (fin) {
-            *reinterpret_cast<int4*>(hits + r) = hit_record(tris, wr, TraceOut{st.best_slot, st.best_t, 0u, 0u});
+            *reinterpret_cast<int4*>(hits + r) =
+                st.best_slot < 0 ? make_int4(-1, -1, __float_as_int(-1.0f), 0)
+                                 : make_int4(st.best_prim, st.best_geom, __float_as_int(st.best_t),
+                                             (int)st.best_bary);
             active = false;
         }
     }
@@ -110,24 +119,7 @@ __global__ __launch_bounds__(256) void miss_kernel(const uint32_t* __restrict__ 
     *reinterpret_cast<int4*>(hits + r) = make_int4(-1, -1, __float_as_int(-1.0f), 0);
 }
 
-// Ray counters of persistent launches: a small ring per device, one slot
-// per launch (a slot is reused after TRACE_COUNTER_SLOTS launches).
-constexpr int TRACE_COUNTER_SLOTS = 1024;
 constexpr int TRACE_MAX_DEVICES = 64;
-
-static uint32_t* counter_slot(int device) {
-    static std::mutex mu;
-    static uint32_t* ring[TRACE_MAX_DEVICES] = {};
-    static unsigned next[TRACE_MAX_DEVICES] = {};
-    std::lock_guard<std::mutex> lock(mu);
-    if (device < 0 || device >= TRACE_MAX_DEVICES) return nullptr;
-    if (!ring[device] &&
-        hipMalloc((void**)&ring[device], sizeof(uint32_t) * TRACE_COUNTER_SLOTS) != hipSuccess) {
-        ring[device] = nullptr;
-        return nullptr;
-    }
-    return ring[device] + (next[device]++ % TRACE_COUNTER_SLOTS);
-}
 
 template <bool CLOSEST, int BLOCK, bool LDS, bool WIDE>
 static int launch_persistent(const float4* nodes, const float4* tris, const wf_ray* rays,
@@ -144,27 +136,34 @@ static int launch_persistent(const float4* nodes, const float4* tris, const wf_r
         WF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         resident[dev] = std::max(1, per_cu) * std::max(1, cus);
     }
-    uint32_t* ctr = counter_slot(dev);
-    if (!ctr) return fail(WF_ERR_OUT_OF_MEMORY, "trace: counter allocation failed");
-    WF_TRY(fill_u32(ctr, 1, 0u, s));
     const int64_t blocks = std::min<int64_t>(resident[dev], (n + BLOCK - 1) / BLOCK);
     trace_persistent_kernel<CLOSEST, BLOCK, LDS, WIDE><<<(unsigned)blocks, BLOCK, 0, s>>>(
-        nodes, tris, rays, ray_index, hits, n, ctr, refill);
+        nodes, tris, rays, ray_index, hits, n, refill);
     return WF_OK;
 }
 
+template <bool CLOSEST, int BLOCK, bool LDS, bool WIDE>
+static void launch_trace_loop(int loop, unsigned g, const float4* nodes, const float4* tris,
+                              const wf_ray* rays, const uint32_t* ray_index, wf_hit* hits,
+                              int64_t n, hipStream_t s) {
+    if (WIDE && loop == LOOP_FUSED)
+        trace_kernel<CLOSEST, BLOCK, LDS, false, WIDE, LOOP_FUSED><<<g, BLOCK, 0, s>>>(nodes, tris, rays, ray_index, hits, n);
+    else if (WIDE && loop == LOOP_POSTPONED)
+        trace_kernel<CLOSEST, BLOCK, LDS, false, WIDE, LOOP_POSTPONED><<<g, BLOCK, 0, s>>>(nodes, tris, rays, ray_index, hits, n);
+    else
+        trace_kernel<CLOSEST, BLOCK, LDS, false, WIDE, LOOP_BRANCH><<<g, BLOCK, 0, s>>>(nodes, tris, rays, ray_index, hits, n);
+}
+
 template <bool CLOSEST, int BLOCK, bool WIDE>
-static int launch_trace(bool lds, bool persistent, int refill, const float4* nodes,
+static int launch_trace(bool lds, bool persistent, int loop, int refill, const float4* nodes,
                         const float4* tris, const wf_ray* rays, const uint32_t* ray_index,
                         wf_hit* hits, int64_t n, hipStream_t s) {
     if (persistent)
         return lds ? launch_persistent<CLOSEST, BLOCK, true, WIDE>(nodes, tris, rays, ray_index, hits, n, refill, s)
                    : launch_persistent<CLOSEST, BLOCK, false, WIDE>(nodes, tris, rays, ray_index, hits, n, refill, s);
-    unsigned g = grid_for(n, BLOCK);
-    if (lds)
-        trace_kernel<CLOSEST, BLOCK, true, false, WIDE><<<g, BLOCK, 0, s>>>(nodes, tris, rays, ray_index, hits, n);
-    else
-        trace_kernel<CLOSEST, BLOCK, false, false, WIDE><<<g, BLOCK, 0, s>>>(nodes, tris, rays, ray_index, hits, n);
+    const unsigned g = grid_for(n, BLOCK);
+    if (lds) launch_trace_loop<CLOSEST, BLOCK, true, WIDE>(loop, g, nodes, tris, rays, ray_index, hits, n, s);
+    else launch_trace_loop<CLOSEST, BLOCK, false, WIDE>(loop, g, nodes, tris, rays, ray_index, hits, n, s);
     return WF_OK;
 }
 
@@ -173,10 +172,11 @@ static int launch_trace_block(const Tuning& tu, const float4* nodes, const float
                               const wf_ray* rays, const uint32_t* ray_index, wf_hit* hits,
                               int64_t n, hipStream_t s) {
     const bool lds = tu.trace_lds_stack != 0, pers = tu.trace_persistent != 0;
+    const int loop = tu.trace_loop;
     switch (tu.trace_block) {
-        case 64: return launch_trace<CLOSEST, 64, WIDE>(lds, pers, tu.trace_refill, nodes, tris, rays, ray_index, hits, n, s);
-        case 128: return launch_trace<CLOSEST, 128, WIDE>(lds, pers, tu.trace_refill, nodes, tris, rays, ray_index, hits, n, s);
-        default: return launch_trace<CLOSEST, 256, WIDE>(lds, pers, tu.trace_refill, nodes, tris, rays, ray_index, hits, n, s);
+        case 64: return launch_trace<CLOSEST, 64, WIDE>(lds, pers, loop, tu.trace_refill, nodes, tris, rays, ray_index, hits, n, s);
+        case 128: return launch_trace<CLOSEST, 128, WIDE>(lds, pers, loop, tu.trace_refill, nodes, tris, rays, ray_index, hits, n, s);
+        default: return launch_trace<CLOSEST, 256, WIDE>(lds, pers, loop, tu.trace_refill, nodes, tris, rays, ray_index, hits, n, s);
     }
 }
 
@@ -216,7 +216,14 @@ extern "C" int wf_trace_stats(wf_scene scene, const wf_ray* rays, wf_hit* hits, 
     hipStream_t s = as_stream(stream);
     const bool wide = tuning().trace_width == 4;
     const float4* nodes = wide ? scene->nodes4 : scene->nodes;
-    if (closest && wide)
+    const int loop = tuning().trace_loop;
+    if (closest && wide && loop == LOOP_BRANCH)
+        trace_kernel<true, 64, true, true, true, LOOP_BRANCH><<<grid_for(n, 64), 64, 0, s>>>(
+            nodes, scene->tris, rays, nullptr, hits, n, node_visits, tri_tests);
+    else if (closest && wide && loop == LOOP_POSTPONED)
+        trace_kernel<true, 64, true, true, true, LOOP_POSTPONED><<<grid_for(n, 64), 64, 0, s>>>(
+            nodes, scene->tris, rays, nullptr, hits, n, node_visits, tri_tests);
+    else if (closest && wide)
         trace_kernel<true, 64, true, true, true><<<grid_for(n, 64), 64, 0, s>>>(
             nodes, scene->tris, rays, nullptr, hits, n, node_visits, tri_tests);
     else if (closest)

@@ -21,6 +21,7 @@ __device__ __forceinline__ float safe_inv(float d) {
 // (column per lane, so lanes of a wave hit distinct banks), deeper entries in
 // private memory.  The build guarantees depth < TRACE_STACK.
 constexpr int LDS_DEPTH = 16;
+typedef __attribute__((address_space(3))) int lds_int;
 
 struct TraceOut {
     int slot;      // leaf slot of the accepted triangle, -1 for a miss
@@ -37,6 +38,9 @@ struct TravState {
     float ix, iy, iz, oix, oiy, oiz, tmin, best_t;
     int node, sp, best_slot, best_orig;
     uint32_t nodes_visited, tris_tested;
+    // hit record fields kept at acceptance (RECORD traversals only)
+    int best_prim, best_geom;
+    uint32_t best_bary;
 };
 
 template <bool LDS, int STACK>
@@ -55,6 +59,9 @@ __device__ __forceinline__ void trav_begin(TravState<LDS, STACK>& st, float4 ro,
     st.best_orig = 0x7fffffff;
     st.nodes_visited = 0;
     st.tris_tested = 0;
+    st.best_prim = -1;
+    st.best_geom = -1;
+    st.best_bary = 0u;
 }
 
 // Stack of a lane: top LDS_DEPTH entries in LDS (s_stack column of the
@@ -63,46 +70,60 @@ template <bool LDS, int BLOCK, int STACK>
 __device__ __forceinline__ void stack_push(TravState<LDS, STACK>& st,
                                            int (&spill)[TravState<LDS, STACK>::SPILL], int* s_stack,
                                            int v) {
-    if (LDS && st.sp < LDS_DEPTH) s_stack[st.sp * BLOCK + threadIdx.x] = v;
+    if (LDS && st.sp < LDS_DEPTH) ((lds_int*)s_stack)[st.sp * BLOCK + threadIdx.x] = v;
     else spill[st.sp - (LDS ? LDS_DEPTH : 0)] = v;
     ++st.sp;
 }
 
+// LDS entries are read through an address-space-3 pointer: through the
+// generic pointer the compiler merges the LDS and the private load of the
+// two branches into one flat load of a selected address, which goes through
+// the vector memory pipeline (and its wait counter) even for LDS entries.
 template <bool LDS, int BLOCK, int STACK>
 __device__ __forceinline__ int stack_pop(TravState<LDS, STACK>& st,
                                          int (&spill)[TravState<LDS, STACK>::SPILL], int* s_stack) {
     --st.sp;
-    int v;
-    if (LDS && st.sp < LDS_DEPTH) v = s_stack[st.sp * BLOCK + threadIdx.x];
-    else v = spill[st.sp - (LDS ? LDS_DEPTH : 0)];
+    if (!LDS) return spill[st.sp];
+    int v = ((lds_int*)s_stack)[min(st.sp, LDS_DEPTH - 1) * BLOCK + threadIdx.x];
+    if (st.sp >= LDS_DEPTH) v = spill[st.sp - LDS_DEPTH];
     return v;
 }
 
-// Leaf of up to 8 triangle slots; returns true when an any-hit query is done.
-template <bool CLOSEST, bool LDS, int STACK, bool STATS>
-__device__ __forceinline__ bool trav_leaf(TravState<LDS, STACK>& st, const float4* __restrict__ tris,
-                                          const WtRay& wr) {
-    const int first = leaf_first(st.node), cnt = leaf_count(st.node);
+// Triangle in leaf slot `slot` (vertices v0..v2, original index in v0.w);
+// returns true when an any-hit query is done.  RECORD also keeps the hit
+// record fields of the accepted triangle (prim in v1.w, geometry in v2.w,
+// packed barycentrics as hit_record computes them), so a finished lane can
+// store its record without reloading the triangle.
+template <bool CLOSEST, bool LDS, int STACK, bool STATS, bool RECORD = false>
+__device__ __forceinline__ bool trav_tri(TravState<LDS, STACK>& st, int slot, float4 v0, float4 v1,
+                                         float4 v2, const WtRay& wr) {
+    if (STATS) ++st.tris_tested;
+    WtHit h;
+    if (!wt_intersect(wr, v0, v1, v2, h)) return false;
+    if (!(h.t >= st.tmin && h.t <= st.best_t)) return false;
+    const int orig = __float_as_int(v0.w);
+    if (CLOSEST && !(h.t < st.best_t || orig < st.best_orig)) return false;
+    st.best_t = h.t;
+    st.best_orig = orig;
+    st.best_slot = slot;
+    if (RECORD) {
+        st.best_prim = __float_as_int(v1.w);
+        st.best_geom = __float_as_int(v2.w);
+        st.best_bary = pack_unorm2x16(h.v / h.det, h.w / h.det);
+    }
+    return !CLOSEST;
+}
+
+// Leaf `leaf` (reference of up to 8 triangle slots); returns true when an
+// any-hit query is done.
+template <bool CLOSEST, bool LDS, int STACK, bool STATS, bool RECORD = false>
+__device__ __forceinline__ bool trav_leaf(TravState<LDS, STACK>& st, int leaf,
+                                          const float4* __restrict__ tris, const WtRay& wr) {
+    const int first = leaf_first(leaf), cnt = leaf_count(leaf);
     for (int k = 0; k < cnt; ++k) {
         const float4* tp = tris + 3 * (int64_t)(first + k);
-        const float4 v0 = tp[0], v1 = tp[1], v2 = tp[2];
-        if (STATS) ++st.tris_tested;
-        WtHit h;
-        if (!wt_intersect(wr, v0, v1, v2, h)) continue;
-        if (!(h.t >= st.tmin && h.t <= st.best_t)) continue;
-        const int orig = __float_as_int(v0.w);
-        if (CLOSEST) {
-            if (h.t < st.best_t || orig < st.best_orig) {
-                st.best_t = h.t;
-                st.best_orig = orig;
-                st.best_slot = first + k;
-            }
-        } else {
-            st.best_t = h.t;
-            st.best_orig = orig;
-            st.best_slot = first + k;
+        if (trav_tri<CLOSEST, LDS, STACK, STATS, RECORD>(st, first + k, tp[0], tp[1], tp[2], wr))
             return true;
-        }
     }
     return false;
 }
@@ -110,7 +131,7 @@ __device__ __forceinline__ bool trav_leaf(TravState<LDS, STACK>& st, const float
 // One traversal step of the 2-wide hierarchy: an interior node (both child
 // boxes) or a whole leaf.  Returns true when the ray is finished (stack
 // empty, or any-hit found).
-template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, int STACK>
+template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, int STACK, bool RECORD = false>
 __device__ __forceinline__ bool trav_step(TravState<LDS, STACK>& st,
                                           int (&spill)[TravState<LDS, STACK>::SPILL],
                                           const float4* __restrict__ nodes,
@@ -150,7 +171,7 @@ __device__ __forceinline__ bool trav_step(TravState<LDS, STACK>& st,
         }
         return false;
     }
-    if (trav_leaf<CLOSEST, LDS, STACK, STATS>(st, tris, wr)) return true;
+    if (trav_leaf<CLOSEST, LDS, STACK, STATS, RECORD>(st, st.node, tris, wr)) return true;
     if (st.sp == 0) return true;
     st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
     return false;
@@ -167,69 +188,161 @@ __device__ __forceinline__ void cswap(float& ta, int& ra, float& tb, int& rb) {
     }
 }
 
-// One traversal step of the 4-wide quantized hierarchy: up to four child
+// Interior node st.node of the 4-wide quantized hierarchy: up to four child
 // boxes decoded from 8-bit planes (t = q * 2^e / d + (origin - o) / d, one
 // fused multiply-add per plane), the hit children visited nearest first.
-template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, int STACK>
-__device__ __forceinline__ bool trav_step4(TravState<LDS, STACK>& st,
-                                           int (&spill)[TravState<LDS, STACK>::SPILL],
-                                           const float4* __restrict__ nodes4,
-                                           const float4* __restrict__ tris, const WtRay& wr,
-                                           int* s_stack) {
-    if (st.node >= 0) {
-        if (STATS) ++st.nodes_visited;
-        const float4* nd = nodes4 + 4 * (int64_t)st.node;
-        const float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3];
-        const uint32_t meta = __float_as_uint(a.w);
-        const int nc = (int)(meta >> 24);
-        const float sax = __uint_as_float((meta & 0xffu) << 23) * st.ix;
-        const float say = __uint_as_float(((meta >> 8) & 0xffu) << 23) * st.iy;
-        const float saz = __uint_as_float(((meta >> 16) & 0xffu) << 23) * st.iz;
-        const float bx = fmaf(a.x, st.ix, -st.oix), by = fmaf(a.y, st.iy, -st.oiy),
-                    bz = fmaf(a.z, st.iz, -st.oiz);
-        const uint32_t qlx = __float_as_uint(b.x), qly = __float_as_uint(b.y),
-                       qlz = __float_as_uint(b.z), qhx = __float_as_uint(b.w),
-                       qhy = __float_as_uint(c.x), qhz = __float_as_uint(c.y);
-        float tn[4];
-        int ref[4] = {__float_as_int(c.z), __float_as_int(c.w), __float_as_int(d.x), __float_as_int(d.y)};
+// Returns false when the lane has no node left (no child hit, stack empty).
+template <int BLOCK, bool LDS, bool STATS, int STACK>
+__device__ __forceinline__ bool trav_node4_data(TravState<LDS, STACK>& st,
+                                                int (&spill)[TravState<LDS, STACK>::SPILL],
+                                                float4 a, float4 b, float4 c, float4 d, int* s_stack) {
+    if (STATS) ++st.nodes_visited;
+    const uint32_t meta = __float_as_uint(a.w);
+    const int nc = (int)(meta >> 24);
+    const float sax = __uint_as_float((meta & 0xffu) << 23) * st.ix;
+    const float say = __uint_as_float(((meta >> 8) & 0xffu) << 23) * st.iy;
+    const float saz = __uint_as_float(((meta >> 16) & 0xffu) << 23) * st.iz;
+    const float bx = fmaf(a.x, st.ix, -st.oix), by = fmaf(a.y, st.iy, -st.oiy),
+                bz = fmaf(a.z, st.iz, -st.oiz);
+    const uint32_t qlx = __float_as_uint(b.x), qly = __float_as_uint(b.y),
+                   qlz = __float_as_uint(b.z), qhx = __float_as_uint(b.w),
+                   qhy = __float_as_uint(c.x), qhz = __float_as_uint(c.y);
+    float tn[4];
+    int ref[4] = {__float_as_int(c.z), __float_as_int(c.w), __float_as_int(d.x), __float_as_int(d.y)};
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int sh = 8 * k;
-            const float t0x = fmaf((float)((qlx >> sh) & 0xffu), sax, bx);
-            const float t1x = fmaf((float)((qhx >> sh) & 0xffu), sax, bx);
-            const float t0y = fmaf((float)((qly >> sh) & 0xffu), say, by);
-            const float t1y = fmaf((float)((qhy >> sh) & 0xffu), say, by);
-            const float t0z = fmaf((float)((qlz >> sh) & 0xffu), saz, bz);
-            const float t1z = fmaf((float)((qhz >> sh) & 0xffu), saz, bz);
-            const float nn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), st.tmin));
-            const float ff = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), st.best_t));
-            tn[k] = (k < nc && nn <= ff) ? nn : INFINITY;
-        }
-        cswap(tn[0], ref[0], tn[1], ref[1]);
-        cswap(tn[2], ref[2], tn[3], ref[3]);
-        cswap(tn[0], ref[0], tn[2], ref[2]);
-        cswap(tn[1], ref[1], tn[3], ref[3]);
-        cswap(tn[1], ref[1], tn[2], ref[2]);
-        if (tn[3] < INFINITY) stack_push<LDS, BLOCK, STACK>(st, spill, s_stack, ref[3]);
-        if (tn[2] < INFINITY) stack_push<LDS, BLOCK, STACK>(st, spill, s_stack, ref[2]);
-        if (tn[1] < INFINITY) stack_push<LDS, BLOCK, STACK>(st, spill, s_stack, ref[1]);
-        if (tn[0] < INFINITY) {
-            st.node = ref[0];
-        } else {
-            if (st.sp == 0) return true;
-            st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
-        }
-        return false;
+    for (int k = 0; k < 4; ++k) {
+        const int sh = 8 * k;
+        const float t0x = fmaf((float)((qlx >> sh) & 0xffu), sax, bx);
+        const float t1x = fmaf((float)((qhx >> sh) & 0xffu), sax, bx);
+        const float t0y = fmaf((float)((qly >> sh) & 0xffu), say, by);
+        const float t1y = fmaf((float)((qhy >> sh) & 0xffu), say, by);
+        const float t0z = fmaf((float)((qlz >> sh) & 0xffu), saz, bz);
+        const float t1z = fmaf((float)((qhz >> sh) & 0xffu), saz, bz);
+        const float nn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), st.tmin));
+        const float ff = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), st.best_t));
+        tn[k] = (k < nc && nn <= ff) ? nn : INFINITY;
     }
-    if (trav_leaf<CLOSEST, LDS, STACK, STATS>(st, tris, wr)) return true;
+    cswap(tn[0], ref[0], tn[1], ref[1]);
+    cswap(tn[2], ref[2], tn[3], ref[3]);
+    cswap(tn[0], ref[0], tn[2], ref[2]);
+    cswap(tn[1], ref[1], tn[3], ref[3]);
+    cswap(tn[1], ref[1], tn[2], ref[2]);
+    if (tn[3] < INFINITY) stack_push<LDS, BLOCK, STACK>(st, spill, s_stack, ref[3]);
+    if (tn[2] < INFINITY) stack_push<LDS, BLOCK, STACK>(st, spill, s_stack, ref[2]);
+    if (tn[1] < INFINITY) stack_push<LDS, BLOCK, STACK>(st, spill, s_stack, ref[1]);
+    if (tn[0] < INFINITY) {
+        st.node = ref[0];
+        return true;
+    }
+    if (st.sp == 0) return false;
+    st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
+    return true;
+}
+
+template <int BLOCK, bool LDS, bool STATS, int STACK>
+__device__ __forceinline__ bool trav_node4(TravState<LDS, STACK>& st,
+                                           int (&spill)[TravState<LDS, STACK>::SPILL],
+                                           const float4* __restrict__ nodes4, int* s_stack) {
+    const float4* nd = nodes4 + 4 * (int64_t)st.node;
+    return trav_node4_data<BLOCK, LDS, STATS>(st, spill, nd[0], nd[1], nd[2], nd[3], s_stack);
+}
+
+// One traversal step of the 4-wide hierarchy that issues a single load per
+// lane whatever the lane visits: the node's 64 bytes, or the first triangle
+// of a leaf (48 bytes), so a wave whose lanes are split between nodes and
+// leaves waits for memory once per step instead of once per branch.
+// Returns true when the ray is finished.
+template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, int STACK>
+__device__ __forceinline__ bool trav_step4_fused(TravState<LDS, STACK>& st,
+                                                 int (&spill)[TravState<LDS, STACK>::SPILL],
+                                                 const float4* __restrict__ nodes4,
+                                                 const float4* __restrict__ tris, const WtRay& wr,
+                                                 int* s_stack) {
+    const bool inner = st.node >= 0;
+    const float4* p = inner ? nodes4 + 4 * (int64_t)st.node : tris + 3 * (int64_t)leaf_first(st.node);
+    const float4 a = p[0], b = p[1], c = p[2];
+    float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (inner) d = p[3];
+    if (inner) return !trav_node4_data<BLOCK, LDS, STATS>(st, spill, a, b, c, d, s_stack);
+    const int first = leaf_first(st.node), cnt = leaf_count(st.node);
+    if (trav_tri<CLOSEST, LDS, STACK, STATS>(st, first, a, b, c, wr)) return true;
+    for (int k = 1; k < cnt; ++k) {
+        const float4* tp = tris + 3 * (int64_t)(first + k);
+        if (trav_tri<CLOSEST, LDS, STACK, STATS>(st, first + k, tp[0], tp[1], tp[2], wr)) return true;
+    }
     if (st.sp == 0) return true;
     st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
     return false;
 }
 
+// One traversal step of the 4-wide hierarchy: an interior node or a whole
+// leaf.  Returns true when the ray is finished (stack empty, or any-hit
+// found).
+template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, int STACK, bool RECORD = false>
+__device__ __forceinline__ bool trav_step4(TravState<LDS, STACK>& st,
+                                           int (&spill)[TravState<LDS, STACK>::SPILL],
+                                           const float4* __restrict__ nodes4,
+                                           const float4* __restrict__ tris, const WtRay& wr,
+                                           int* s_stack) {
+    if (st.node >= 0) return !trav_node4<BLOCK, LDS, STATS>(st, spill, nodes4, s_stack);
+    if (trav_leaf<CLOSEST, LDS, STACK, STATS, RECORD>(st, st.node, tris, wr)) return true;
+    if (st.sp == 0) return true;
+    st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
+    return false;
+}
+
+// 4-wide traversal with postponed leaves (Aila & Laine, "Understanding the
+// Efficiency of Ray Traversal on GPUs", HPG 2009, speculative traversal): a
+// lane that reaches a leaf parks it and keeps descending interior nodes; the
+// wave tests triangles only once every lane still descending has a parked
+// leaf, so the triangle test is not issued for one or two lanes on most
+// steps.  Order-independent result: the closest hit (ties to the smaller
+// original triangle index) or any hit.
+template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, int STACK>
+__device__ __forceinline__ void trav4_postponed(TravState<LDS, STACK>& st,
+                                                int (&spill)[TravState<LDS, STACK>::SPILL],
+                                                const float4* __restrict__ nodes4,
+                                                const float4* __restrict__ tris, const WtRay& wr,
+                                                int* s_stack, bool live) {
+    bool have = live;  // st.node holds a node (interior or leaf) still to visit
+    int leaf = 0;      // parked leaf reference (< 0), 0 for none
+    while (have || leaf != 0) {
+        while (have && (st.node >= 0 || leaf == 0)) {
+            if (st.node < 0) {
+                leaf = st.node;
+                have = st.sp > 0;
+                if (have) st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
+            } else {
+                have = trav_node4<BLOCK, LDS, STATS>(st, spill, nodes4, s_stack);
+            }
+            if (__all(!have || leaf != 0)) break;
+        }
+        while (leaf != 0) {
+            if (trav_leaf<CLOSEST, LDS, STACK, STATS>(st, leaf, tris, wr)) {
+                have = false;
+                leaf = 0;
+                break;
+            }
+            if (have && st.node < 0) {
+                leaf = st.node;
+                have = st.sp > 0;
+                if (have) st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
+            } else {
+                leaf = 0;
+            }
+        }
+    }
+}
+
+// Loop structures of the 4-wide traversal (the trace.loop knob).
+enum TraceLoop { LOOP_BRANCH = 0, LOOP_FUSED = 1, LOOP_POSTPONED = 2 };
+constexpr int TRACE_LOOP_DEFAULT = LOOP_FUSED;
+
 // s_stack: this block's LDS stack (LDS_DEPTH * BLOCK ints) when LDS is set.
-// WIDE selects the 4-wide quantized hierarchy (nodes = scene nodes4).
-template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, bool WIDE = false>
+// WIDE selects the 4-wide quantized hierarchy (nodes = scene nodes4) and
+// LOOP its loop structure.
+template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, bool WIDE = false,
+          int LOOP = TRACE_LOOP_DEFAULT>
 __device__ __forceinline__ TraceOut traverse(const float4* __restrict__ nodes,
                                              const float4* __restrict__ tris, float4 ro, float4 rd,
                                              const WtRay& wr, bool live, int* s_stack) {
@@ -237,8 +350,13 @@ __device__ __forceinline__ TraceOut traverse(const float4* __restrict__ nodes,
     TravState<LDS, STACK> st;
     int spill[TravState<LDS, STACK>::SPILL];
     trav_begin(st, ro, rd);
-    if (live) {
-        if (WIDE) {
+    if (WIDE && LOOP == LOOP_POSTPONED) {
+        trav4_postponed<CLOSEST, BLOCK, LDS, STATS>(st, spill, nodes, tris, wr, s_stack, live);
+    } else if (live) {
+        if (WIDE && LOOP == LOOP_FUSED) {
+            while (!trav_step4_fused<CLOSEST, BLOCK, LDS, STATS>(st, spill, nodes, tris, wr, s_stack)) {
+            }
+        } else if (WIDE) {
             while (!trav_step4<CLOSEST, BLOCK, LDS, STATS>(st, spill, nodes, tris, wr, s_stack)) {
             }
         } else {
