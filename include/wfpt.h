@@ -288,6 +288,11 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
  *   "trace.refill"     1-64 idle lanes of a wave that trigger a refill (default 24)
  *   "trace.width"      2/4  traverse the 2-wide or the 4-wide quantized hierarchy
  *                           (both are built; default 4)
+ *   "sort.onesweep_tiles" >= 0  sorts of more than 8192 pairs in at most this
+ *                           many 4096-pair tiles, with digits of at most 8
+ *                           bits, run one kernel per pass with decoupled
+ *                           look-back (onesweep); the others run
+ *                           reduce-then-scan passes (default 64)
  *   "trace.loop"       0-2  4-wide traversal loop: 0 branch on node/leaf, then
  *                           load; 1 one load per step for node or leaf
  *                           (default); 2 leaves postponed until every lane of

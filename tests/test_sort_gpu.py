@@ -98,12 +98,54 @@ def test_sort_bit_exact(dev, orc, dtype, bits, digit, n):
     assert np.array_equal(tk.cpu().numpy().view(dtype), rk)
 
 
+@pytest.fixture
+def onesweep_tiles():
+    """Sets sort.onesweep_tiles for one test and restores it."""
+    import ctypes as C
+
+    from vulkanwavefrontpathtracer_amd import _capi
+
+    cur = C.c_int()
+    _capi.call("wf_get_tuning", b"sort.onesweep_tiles", C.byref(cur))
+    yield lambda v: _capi.call("wf_set_tuning", b"sort.onesweep_tiles", v)
+    _capi.call("wf_set_tuning", b"sort.onesweep_tiles", cur.value)
+
+
+@pytest.mark.parametrize("tiles", [0, 1 << 20])
+@pytest.mark.parametrize("dtype,bits,digit", [(np.uint32, 30, 8), (np.uint64, 60, 8),
+                                              (np.uint64, 64, 4), (np.uint32, 32, 11)])
+@pytest.mark.parametrize("n", [8193, 70_001, 300_017])
+def test_sort_paths_bit_exact(dev, orc, onesweep_tiles, tiles, dtype, bits, digit, n):
+    """Onesweep passes (tiles = 2^20: every size with digits of at most 8
+    bits) and reduce-then-scan passes (tiles = 0) both give the oracle's
+    stable order."""
+    from vulkanwavefrontpathtracer_amd import stages
+
+    onesweep_tiles(tiles)
+    rng = np.random.default_rng(n + bits + digit)
+    keys = rng.integers(0, np.iinfo(dtype).max, size=n, dtype=dtype, endpoint=True)
+    keys[: n // 5] = keys[1]
+    keys[n // 5: n // 3] &= dtype(0xF0F)
+    vals = rng.permutation(n).astype(np.uint32)
+    tk = torch.from_numpy(keys.view(np.int32 if dtype == np.uint32 else np.int64)).to(dev)
+    tv = torch.from_numpy(vals.view(np.int32)).to(dev)
+    stages.sort_pairs(tk, tv, bits, digit)
+    torch.cuda.synchronize()
+    rk, rv = orc.sort_pairs(keys, vals, bits)
+    assert np.array_equal(tv.cpu().numpy().view(np.uint32), rv)
+    assert np.array_equal(tk.cpu().numpy().view(dtype), rk)
+
+
+@pytest.mark.parametrize("tiles", [0, 1 << 20])
 @pytest.mark.parametrize("dtype,bits,digit", [(np.uint32, 32, 8), (np.uint64, 60, 8),
                                               (np.uint64, 60, 11)])
-def test_sort_full_size_properties(dev, dtype, bits, digit):
-    """At a 1080p-bounce size (2,073,600): sorted order, permutation, and a
-    stable tie order, checked on the GPU without the oracle."""
+def test_sort_full_size_properties(dev, onesweep_tiles, tiles, dtype, bits, digit):
+    """At a 1080p-bounce size (2,073,600), through reduce-then-scan and through
+    onesweep passes: sorted order, permutation, and a stable tie order,
+    checked on the GPU without the oracle."""
     from vulkanwavefrontpathtracer_amd import stages
+
+    onesweep_tiles(tiles)
 
     n = 2_073_600
     g = torch.Generator(device="cpu").manual_seed(5)

@@ -34,6 +34,9 @@ struct SortPlan {
     // byte offsets inside scratch
     int64_t off_counts = 0, off_totals = 0;
     int64_t off_keys_a = 0, off_vals_a = 0, off_keys_b = 0, off_vals_b = 0;
+    // onesweep: digit histograms of all passes, per-pass tile counters and
+    // two look-back status arrays (tiles x radix), cleared as one region
+    int64_t off_hist = 0, off_ctr = 0, off_status = 0, status_words = 0, clear_words = 0;
     int64_t total = 0;
 };
 
@@ -59,6 +62,12 @@ static bool make_plan(int64_t n, int key_bytes, int key_bits, int digit_bits, So
     o += align256((int64_t)key_bytes * n);
     p.off_vals_b = o;
     o += align256(sizeof(uint32_t) * n);
+    p.off_hist = o;  // hist | counters | status 0 are contiguous (one clear)
+    p.off_ctr = o + sizeof(uint32_t) * (int64_t)p.passes * p.radix;
+    p.off_status = p.off_ctr + sizeof(uint32_t) * SORT_MAX_PASSES;
+    p.status_words = p.tiles * p.radix;
+    p.clear_words = ((int64_t)p.passes * p.radix + SORT_MAX_PASSES) + p.status_words;
+    o = align256(p.off_status + sizeof(uint32_t) * 2 * p.status_words);
     p.total = o;
     return true;
 }
@@ -299,6 +308,216 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
     }
 }
 
+// ---------------------------------------------------------------- onesweep
+// Onesweep (Adinets & Merrill, "Onesweep: A Faster Least Significant Digit
+// Radix Sort for GPUs", 2022): the digit histograms of every pass come from
+// one read of the keys, then each pass is a single kernel.  A tile takes its
+// index from a counter (so every earlier tile is already running), ranks its
+// keys as the downsweep does, publishes its per-digit counts, and adds up the
+// counts of the earlier tiles by decoupled look-back over their published
+// status words (each digit's thread reads four predecessors per round).  One
+// launch per pass instead of three; used where the passes are latency-bound
+// (few tiles).
+
+// Digit histograms of every pass (hist[pass * RADIX + digit]) in one read.
+template <typename K, int DBITS>
+__global__ __launch_bounds__(SORT_BLOCK) void sort_hist_all_kernel(const K* __restrict__ keys,
+                                                                   int64_t n, int key_bits,
+                                                                   int passes,
+                                                                   uint32_t* __restrict__ hist,
+                                                                   const uint32_t* __restrict__ d_n) {
+    constexpr int RADIX = 1 << DBITS;
+    constexpr int MAXP = (64 + DBITS - 1) / DBITS;
+    __shared__ uint32_t s_h[MAXP * RADIX];
+    const unsigned tid = threadIdx.x;
+    for (int i = tid; i < passes * RADIX; i += SORT_BLOCK) s_h[i] = 0;
+    __syncthreads();
+    if (d_n) n = min(n, (int64_t)*d_n);
+    const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
+#pragma unroll 4
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        const int64_t idx = base + (int64_t)it * SORT_BLOCK + tid;
+        if (idx < n) {
+            const K k = keys[idx];
+            for (int p = 0; p < passes; ++p) {
+                const int shift = p * DBITS;
+                const int width = min(DBITS, key_bits - shift);
+                atomicAdd(&s_h[p * RADIX + digit_of(k, shift, (1u << width) - 1u)], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < passes * RADIX; i += SORT_BLOCK)
+        if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+}
+
+// Exclusive count of digit d over the tiles before `tile`: walks back over
+// the status words (AGG: that tile's count, keep walking; INC: the inclusive
+// count up to that tile, stop), four words per round, waiting on a word not
+// yet published.
+__device__ __forceinline__ uint32_t onesweep_lookback(const uint32_t* __restrict__ status,
+                                                      int64_t tile, int radix, int d) {
+    uint32_t excl = 0;
+    int64_t k = tile - 1;
+    while (true) {
+        uint32_t v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            v[j] = (k - j >= 0) ? lb_load(&status[(k - j) * radix + d]) : LB_FLAG_INC;
+        int j = 0;
+        for (; j < 4; ++j) {
+            if ((v[j] & ~LB_MASK) == 0u) break;
+            excl += v[j] & LB_MASK;
+            if (v[j] & LB_FLAG_INC) return excl;
+        }
+        k -= j;
+    }
+}
+
+template <typename K, int DBITS>
+__global__ __launch_bounds__(SORT_BLOCK) void sort_onesweep_kernel(
+    const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, K* __restrict__ keys_out,
+    uint32_t* __restrict__ vals_out, int64_t n, int shift, int width,
+    const uint32_t* __restrict__ hist, uint32_t* __restrict__ status,
+    uint32_t* __restrict__ status_next, uint32_t* __restrict__ tile_counter,
+    const uint32_t* __restrict__ d_n) {
+    constexpr int RADIX = 1 << DBITS;
+    constexpr int DPT = RADIX >= SORT_BLOCK ? RADIX / SORT_BLOCK : 1;  // digits per thread
+    __shared__ uint32_t s_wave[SORT_WAVES][RADIX];
+    __shared__ uint32_t s_lstart[RADIX];
+    __shared__ uint32_t s_goff[RADIX];
+    __shared__ K s_stage[SORT_TILE];
+    __shared__ uint32_t s_tmp[SORT_WAVES];
+    __shared__ uint32_t s_tile;
+
+    const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+    const uint32_t mask = (1u << width) - 1u;
+    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+    for (int i = tid; i < SORT_WAVES * RADIX; i += SORT_BLOCK) (&s_wave[0][0])[i] = 0;
+    __syncthreads();
+    const int64_t tile = s_tile;
+    // the next pass's status words of this tile index start at zero
+    if (status_next)
+        for (int d = tid; d < RADIX; d += SORT_BLOCK) status_next[tile * RADIX + d] = 0u;
+    if (d_n) n = min(n, (int64_t)*d_n);
+    const int64_t base = tile * SORT_TILE;
+    if (base >= n) return;  // uniform: tiles past the count
+
+    K key[SORT_ITEMS];
+    uint32_t val[SORT_ITEMS];
+    uint32_t off[SORT_ITEMS];
+    const int64_t wbase = base + (int64_t)wave * 64 * SORT_ITEMS;
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        const int64_t idx = wbase + it * 64 + lane;
+        const bool ok = idx < n;
+        key[it] = ok ? keys_in[idx] : K(0);
+        val[it] = ok ? vals_in[idx] : 0u;
+    }
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        const bool ok = wbase + it * 64 + lane < n;
+        const uint32_t d = digit_of(key[it], shift, mask);
+        uint64_t peers = ballot64(ok);
+#pragma unroll
+        for (int b = 0; b < DBITS; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = ballot64(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t rank = popc_below(peers);
+        const uint32_t cnt = (uint32_t)__popcll(peers);
+        uint32_t pre = 0;
+        if (ok) pre = s_wave[wave][d];
+        off[it] = pre + rank;
+        if (ok && rank == 0) s_wave[wave][d] = pre + cnt;
+    }
+    __syncthreads();
+
+    uint32_t cnt[DPT];
+    uint32_t my_total = 0;
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        const int d = tid * DPT + k;
+        uint32_t sum = 0;
+        if (d < RADIX) {
+#pragma unroll
+            for (int w = 0; w < SORT_WAVES; ++w) {
+                const uint32_t c = s_wave[w][d];
+                s_wave[w][d] = sum;
+                sum += c;
+            }
+            // publish this tile's count early so later tiles can walk past it
+            lb_store(&status[tile * RADIX + d], (tile == 0 ? LB_FLAG_INC : LB_FLAG_AGG) | sum);
+        }
+        cnt[k] = sum;
+        my_total += sum;
+    }
+    uint32_t tile_total;
+    uint32_t lstart = block_excl_scan(my_total, s_tmp, &tile_total);
+    (void)tile_total;
+    uint32_t tot[DPT];
+    uint32_t my_tot = 0;
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        const int d = tid * DPT + k;
+        tot[k] = d < RADIX ? hist[d] : 0u;
+        my_tot += tot[k];
+    }
+    uint32_t all_total;
+    uint32_t dstart = block_excl_scan(my_tot, s_tmp, &all_total);
+    (void)all_total;
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        const int d = tid * DPT + k;
+        if (d < RADIX) {
+            uint32_t excl = 0;
+            if (tile > 0) {
+                excl = onesweep_lookback(status, tile, RADIX, d);
+                lb_store(&status[tile * RADIX + d], LB_FLAG_INC | (excl + cnt[k]));
+            }
+            s_lstart[d] = lstart;
+            s_goff[d] = dstart + excl - lstart;
+            lstart += cnt[k];
+            dstart += tot[k];
+        }
+    }
+    __syncthreads();
+
+    uint32_t pos[SORT_ITEMS];
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        const uint32_t d = digit_of(key[it], shift, mask);
+        pos[it] = s_lstart[d] + s_wave[wave][d] + off[it];
+    }
+    const int count = (int)min((int64_t)SORT_TILE, n - base);
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it)
+        if (wbase + it * 64 + lane < n) s_stage[pos[it]] = key[it];
+    __syncthreads();
+    uint32_t gpos[SORT_ITEMS];
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        const int j = it * SORT_BLOCK + (int)tid;
+        if (j < count) {
+            const K k = s_stage[j];
+            gpos[it] = s_goff[digit_of(k, shift, mask)] + (uint32_t)j;
+            keys_out[gpos[it]] = k;
+        }
+    }
+    __syncthreads();
+    uint32_t* s_vstage = reinterpret_cast<uint32_t*>(s_stage);
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it)
+        if (wbase + it * 64 + lane < n) s_vstage[pos[it]] = val[it];
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < SORT_ITEMS; ++it) {
+        const int j = it * SORT_BLOCK + (int)tid;
+        if (j < count) vals_out[gpos[it]] = s_vstage[j];
+    }
+}
+
 // ------------------------------------------------------------ small sorts
 // Up to SMALL_N pairs: one workgroup of 1024 threads holds 8 pairs per thread
 // in registers and runs every pass in LDS (rank by wave64 ballot matching,
@@ -489,11 +708,39 @@ static int sort_impl(void* keys, uint32_t* vals, int64_t n, int key_bits, int di
         for (int i = 3; i < p.passes; ++i) seq[len++] = (i % 2 == 1) ? A : C;
     }
 
+    // onesweep where the passes are latency-bound: few tiles, and digits of
+    // at most 8 bits (an 11-bit look-back walks 8 digits per thread)
+    const bool onesweep = p.tiles <= tuning().sort_onesweep_tiles && digit_bits <= 8;
+    uint32_t* hist = reinterpret_cast<uint32_t*>(scratch + p.off_hist);
+    uint32_t* ctr = reinterpret_cast<uint32_t*>(scratch + p.off_ctr);
+    uint32_t* status = reinterpret_cast<uint32_t*>(scratch + p.off_status);
+    if (onesweep) {
+        WF_TRY(fill_u32(hist, p.clear_words, 0u, s));
+        switch (digit_bits) {
+            case 4: sort_hist_all_kernel<K, 4><<<(unsigned)p.tiles, SORT_BLOCK, 0, s>>>(static_cast<const K*>(keys), n, key_bits, p.passes, hist, d_n); break;
+            case 8: sort_hist_all_kernel<K, 8><<<(unsigned)p.tiles, SORT_BLOCK, 0, s>>>(static_cast<const K*>(keys), n, key_bits, p.passes, hist, d_n); break;
+            default: sort_hist_all_kernel<K, 11><<<(unsigned)p.tiles, SORT_BLOCK, 0, s>>>(static_cast<const K*>(keys), n, key_bits, p.passes, hist, d_n); break;
+        }
+        WF_TRY(launch_check("sort_hist_all_kernel"));
+    }
     for (int pass = 0; pass < p.passes; ++pass) {
         const int shift = pass * digit_bits;
         const int width = std::min(digit_bits, key_bits - shift);
         const Buf& in = seq[pass];
         const Buf& out = seq[pass + 1];
+        if (onesweep) {
+            uint32_t* st = status + (pass & 1) * p.status_words;
+            uint32_t* st_next = pass + 1 < p.passes ? status + ((pass + 1) & 1) * p.status_words : nullptr;
+            const uint32_t* h = hist + (int64_t)pass * p.radix;
+            const unsigned g = (unsigned)p.tiles;
+            switch (digit_bits) {
+                case 4: sort_onesweep_kernel<K, 4><<<g, SORT_BLOCK, 0, s>>>(static_cast<const K*>(in.k), in.v, static_cast<K*>(out.k), out.v, n, shift, width, h, st, st_next, ctr + pass, d_n); break;
+                case 8: sort_onesweep_kernel<K, 8><<<g, SORT_BLOCK, 0, s>>>(static_cast<const K*>(in.k), in.v, static_cast<K*>(out.k), out.v, n, shift, width, h, st, st_next, ctr + pass, d_n); break;
+                default: sort_onesweep_kernel<K, 11><<<g, SORT_BLOCK, 0, s>>>(static_cast<const K*>(in.k), in.v, static_cast<K*>(out.k), out.v, n, shift, width, h, st, st_next, ctr + pass, d_n); break;
+            }
+            WF_TRY(launch_check("sort_onesweep_kernel"));
+            continue;
+        }
         switch (digit_bits) {
             case 4: WF_TRY((launch_pass<K, 4>(in.k, in.v, out.k, out.v, n, shift, width, counts, totals, p.tiles, d_n, s))); break;
             case 8: WF_TRY((launch_pass<K, 8>(in.k, in.v, out.k, out.v, n, shift, width, counts, totals, p.tiles, d_n, s))); break;
