@@ -613,7 +613,7 @@ def tuning_state():
     out = {}
     for k in ("bvh.presplit", "bvh.split_depth", "bvh.leaf_max", "bvh.builder", "bvh.collapse",
               "trace.lds_stack", "trace.block", "trace.persistent", "trace.refill", "trace.width",
-              "trace.loop"):
+              "trace.loop", "trace.group64", "trace.group16", "trace.group4", "sort.onesweep_tiles"):
         v = C.c_int()
         stages.call("wf_get_tuning", k.encode(), C.byref(v))
         out[k] = v.value

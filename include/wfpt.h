@@ -293,6 +293,11 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
  *                           bits, run one kernel per pass with decoupled
  *                           look-back (onesweep); the others run
  *                           reduce-then-scan passes (default 64)
+ *   "trace.group64"    >= 0 4-wide traces of at most this many rays run one ray
+ *   "trace.group16"         per group of 64 / 16 / 4 lanes: the group splits
+ *   "trace.group4"          the top of the hierarchy into subtrees walked in
+ *                           parallel with a shared closest distance (latency
+ *                           of small wavefronts; defaults 5000 / 24000 / 64000)
  *   "trace.loop"       0-2  4-wide traversal loop: 0 branch on node/leaf, then
  *                           load; 1 one load per step for node or leaf
  *                           (default); 2 leaves postponed until every lane of

@@ -315,3 +315,52 @@ def test_tuning_variants_bit_exact(dev, orc, presplit, leaf, lds, block, persist
     finally:
         for k, v in saved.items():
             _capi.call("wf_set_tuning", k.encode(), v)
+
+
+@pytest.mark.parametrize("group", [4, 16, 64])
+def test_group_traversal_bit_exact(dev, orc, group):
+    """Small wavefronts traced by groups of 4/16/64 lanes per ray (subtrees
+    walked in parallel with a shared closest distance) return the oracle's
+    closest-hit records bit for bit, including dead rays, indirect order and
+    any-hit visibility; coincident triangles tie to the smaller index."""
+    import ctypes as C
+
+    from vulkanwavefrontpathtracer_amd import _capi, stages
+
+    key = f"trace.group{group}".encode()
+    cur = C.c_int()
+    _capi.call("wf_get_tuning", key, C.byref(cur))
+    _capi.call("wf_set_tuning", key, 1 << 20)
+    try:
+        for kind, seed in ((stages.SCENE_INTERIOR, 1001), (stages.SCENE_BOXES, 1000)):
+            hs = stages.HostScene(kind, seed)
+            sc = stages.Scene.from_host(hs, dev)
+            rays = scene_rays(hs, 3000, 90 + group, tmin=1e-4)
+            rays[::53, 7] = -1.0
+            tr = torch.from_numpy(rays).to(dev)
+            got = stages.trace(sc, tr).cpu().numpy()
+            exp = expected_hits(orc, hs, rays)
+            bad = np.nonzero(np.any(got != exp, axis=1))[0]
+            assert len(bad) == 0, f"{len(bad)} mismatches, e.g. {bad[:3]}: {got[bad[:3]]} vs {exp[bad[:3]]}"
+            anyh = stages.trace(sc, tr, closest=False).cpu().numpy()
+            assert np.array_equal(anyh[:, 0] >= 0, got[:, 0] >= 0)
+            perm = torch.randperm(len(rays), device=dev).to(torch.int32)
+            assert np.array_equal(stages.trace(sc, tr, ray_index=perm).cpu().numpy(), got)
+        # coincident triangles: every hit is a tie resolved to index 0
+        base = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+        tris = np.repeat(base[None], 300, axis=0)
+        verts = np.zeros((900, 6), np.float32)
+        verts[:, :3] = tris.reshape(-1, 3)
+        idx = np.arange(900, dtype=np.uint32).reshape(-1, 3)
+        sc = stages.Scene(verts, idx, np.array([0, 300], np.int32), device=dev)
+        rays = np.zeros((500, 8), np.float32)
+        rng = np.random.default_rng(3)
+        rays[:, :2] = rng.uniform(0.01, 0.45, (500, 2))
+        rays[:, 2] = 1.0
+        rays[:, 6] = -1.0
+        rays[:, 7] = 10.0
+        got = stages.trace(sc, torch.from_numpy(rays).to(dev)).cpu().numpy()
+        exp = orc.trace_f32(rays, tris.reshape(-1, 9))
+        assert np.all(got[:, 0] == 0) and np.array_equal(got[:, 2], exp[:, 1])
+    finally:
+        _capi.call("wf_set_tuning", key, cur.value)

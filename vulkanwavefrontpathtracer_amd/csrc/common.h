@@ -144,6 +144,9 @@ struct Tuning {
     int trace_refill = 24;      // idle lanes of a wave that trigger a refill
     int trace_width = 4;        // traverse the 2-wide or the 4-wide hierarchy
     int sort_onesweep_tiles = 64;  // sorts of at most this many 4096-element tiles use onesweep
+    int trace_group64 = 5000;   // wavefronts of at most this many rays: 64 lanes per ray
+    int trace_group16 = 24000;  // ... 16 lanes per ray
+    int trace_group4 = 64000;   // ... 4 lanes per ray
     int trace_loop = 1;         // 4-wide loop: 0 branch then load, 1 one load per step, 2 postponed leaves
 };
 Tuning& tuning();

@@ -111,6 +111,137 @@ __global__ __launch_bounds__(BLOCK) void trace_persistent_kernel(
     }
 }
 
+// Small wavefronts: a group of G lanes (4, 16 or 64) traces one ray of the
+// 4-wide hierarchy.  The group expands the top levels breadth first (while
+// the next level fits in G lanes) into at most G subtrees, then each lane
+// walks one subtree with its own stack, and after every step the lanes take
+// the group's closest hit distance for culling.  The last bounces of a frame
+// have a few thousand rays, and a launch then lasts as long as the chain of
+// dependent node fetches of its longest ray (about 150 steps for a ray that
+// crosses a whole room); spreading the walk over G lanes shortens that
+// chain.  The record is the group's minimum of (t, original triangle index)
+// over its lanes' hits, the same as a one-lane traversal's closest hit; an
+// any-hit group stops at the first hit of any lane.  A lane that adopts a
+// strictly closer group distance drops its own hit (it cannot win), so a
+// later tie at that distance is decided by the original index alone.
+template <bool CLOSEST, int G>
+__global__ __launch_bounds__(64) void trace_group_kernel(const float4* __restrict__ nodes4,
+                                                         const float4* __restrict__ tris,
+                                                         const wf_ray* __restrict__ rays,
+                                                         const uint32_t* __restrict__ ray_index,
+                                                         wf_hit* __restrict__ hits, int64_t n) {
+    static_assert(G == 4 || G == 16 || G == 64, "group of 4, 16 or 64 lanes");
+    constexpr int STACK = TRACE_STACK4;
+    __shared__ int s_stack[LDS_DEPTH * 64];
+    __shared__ int s_front[2][64];
+    const unsigned lane = lane_id();
+    const int gl = (int)(lane & (G - 1));
+    const int gbase = (int)(lane & ~(unsigned)(G - 1));
+    const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << gbase);
+    const int64_t ray = ((int64_t)blockIdx.x * 64 + lane) / G;
+    const bool valid = ray < n;
+    int64_t r = 0;
+    float4 ro = make_float4(0.f, 0.f, 0.f, 0.f), rd = make_float4(0.f, 0.f, 1.f, -1.f);
+    if (valid) {
+        r = ray_index ? (int64_t)ray_index[ray] : ray;
+        load_ray(rays, r, ro, rd);
+    }
+    WtRay wr;
+    const bool live = valid && (rd.w >= ro.w) && wt_setup(ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, wr);
+    TravState<true, STACK> st;
+    int spill[TravState<true, STACK>::SPILL];
+    trav_begin(st, ro, rd);
+
+    // Breadth-first expansion: before each level the frontier holds at most
+    // G/4 nodes, so the next one (at most 4 children each) fits in G lanes.
+    int cur = 0, fsize = live ? 1 : 0;
+    if (gl == 0) s_front[0][gbase] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int level = 0; (4 << (2 * level)) <= G; ++level) {
+        int c = 0;
+        int ch[4] = {0, 0, 0, 0};
+        if (gl < fsize) {
+            const int node = s_front[cur][gbase + gl];
+            if (node >= 0) {
+                const float4* nd = nodes4 + 4 * (int64_t)node;
+                float tn[4];
+                node4_hits(st, nd[0], nd[1], nd[2], nd[3], tn, ch);
+                c = (tn[0] < INFINITY) + (tn[1] < INFINITY) + (tn[2] < INFINITY) + (tn[3] < INFINITY);
+            } else {
+                ch[0] = node;  // a leaf stays in the frontier
+                c = 1;
+            }
+        }
+        int incl = c;
+#pragma unroll
+        for (int o = 1; o < G; o <<= 1) {
+            const int y = __shfl_up(incl, o, G);
+            if (gl >= o) incl += y;
+        }
+        const int total = __shfl(incl, G - 1, G);
+        for (int k = 0; k < c; ++k) s_front[cur ^ 1][gbase + incl - c + k] = ch[k];
+        __syncthreads();
+        cur ^= 1;
+        fsize = total;
+    }
+
+    bool active = gl < fsize;
+    if (active) st.node = s_front[cur][gbase + gl];
+    while (__any(active)) {
+        if (active) active = !trav_step4<CLOSEST, 64, true, false, STACK, true>(st, spill, nodes4, tris, wr, s_stack);
+        float bt = st.best_t;
+#pragma unroll
+        for (int o = 1; o < G; o <<= 1) bt = fminf(bt, __shfl_xor(bt, o, G));
+        if (bt < st.best_t) {
+            st.best_t = bt;
+            st.best_slot = -1;
+            st.best_orig = 0x7fffffff;
+        }
+        if (!CLOSEST && (ballot64(st.best_slot >= 0) & gmask)) active = false;
+    }
+
+    // group minimum of (t, original index) over the lanes that hold a hit
+    const bool has = st.best_slot >= 0;
+    const float t = has ? st.best_t : INFINITY;
+    const int o = has ? st.best_orig : 0x7fffffff;
+    float tm = t;
+    int om = o;
+#pragma unroll
+    for (int sft = 1; sft < G; sft <<= 1) {
+        const float t2 = __shfl_xor(tm, sft, G);
+        const int o2 = __shfl_xor(om, sft, G);
+        if (t2 < tm || (t2 == tm && o2 < om)) {
+            tm = t2;
+            om = o2;
+        }
+    }
+    const uint64_t winners = ballot64(has && t == tm && o == om) & gmask;
+    if (valid) {
+        if (winners == 0) {
+            if (gl == 0) *reinterpret_cast<int4*>(hits + r) = make_int4(-1, -1, __float_as_int(-1.0f), 0);
+        } else if ((int)lane == __ffsll((unsigned long long)winners) - 1) {
+            *reinterpret_cast<int4*>(hits + r) =
+                make_int4(st.best_prim, st.best_geom, __float_as_int(st.best_t), (int)st.best_bary);
+        }
+    }
+}
+
+template <bool CLOSEST>
+static bool launch_group(const Tuning& tu, const float4* nodes4, const float4* tris,
+                         const wf_ray* rays, const uint32_t* ray_index, wf_hit* hits, int64_t n,
+                         hipStream_t s) {
+    if (n <= tu.trace_group64)
+        trace_group_kernel<CLOSEST, 64><<<grid_for(n * 64, 64), 64, 0, s>>>(nodes4, tris, rays, ray_index, hits, n);
+    else if (n <= tu.trace_group16)
+        trace_group_kernel<CLOSEST, 16><<<grid_for(n * 16, 64), 64, 0, s>>>(nodes4, tris, rays, ray_index, hits, n);
+    else if (n <= tu.trace_group4)
+        trace_group_kernel<CLOSEST, 4><<<grid_for(n * 4, 64), 64, 0, s>>>(nodes4, tris, rays, ray_index, hits, n);
+    else
+        return false;
+    return true;
+}
+
 __global__ __launch_bounds__(256) void miss_kernel(const uint32_t* __restrict__ ray_index,
                                                    wf_hit* __restrict__ hits, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -188,6 +319,10 @@ int trace_launch(wf_scene scene, const wf_ray* rays, const uint32_t* ray_index, 
         return launch_check("miss_kernel");
     }
     const Tuning& tu = tuning();
+    if (tu.trace_width == 4 && !tu.trace_persistent &&
+        (closest ? launch_group<true>(tu, scene->nodes4, scene->tris, rays, ray_index, hits, n, s)
+                 : launch_group<false>(tu, scene->nodes4, scene->tris, rays, ray_index, hits, n, s)))
+        return launch_check("trace_group_kernel");
     if (tu.trace_width == 4) {
         if (closest)
             WF_TRY((launch_trace_block<true, true>(tu, scene->nodes4, scene->tris, rays, ray_index, hits, n, s)));

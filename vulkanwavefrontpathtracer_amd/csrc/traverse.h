@@ -192,11 +192,11 @@ __device__ __forceinline__ void cswap(float& ta, int& ra, float& tb, int& rb) {
 // boxes decoded from 8-bit planes (t = q * 2^e / d + (origin - o) / d, one
 // fused multiply-add per plane), the hit children visited nearest first.
 // Returns false when the lane has no node left (no child hit, stack empty).
-template <int BLOCK, bool LDS, bool STATS, int STACK>
-__device__ __forceinline__ bool trav_node4_data(TravState<LDS, STACK>& st,
-                                                int (&spill)[TravState<LDS, STACK>::SPILL],
-                                                float4 a, float4 b, float4 c, float4 d, int* s_stack) {
-    if (STATS) ++st.nodes_visited;
+// Child boxes of a 4-wide node (a..d) hit within [tmin, best_t], nearest
+// first: tn[k] the entry distance (INFINITY for a miss), ref[k] the child.
+template <bool LDS, int STACK>
+__device__ __forceinline__ void node4_hits(const TravState<LDS, STACK>& st, float4 a, float4 b,
+                                           float4 c, float4 d, float (&tn)[4], int (&ref)[4]) {
     const uint32_t meta = __float_as_uint(a.w);
     const int nc = (int)(meta >> 24);
     const float sax = __uint_as_float((meta & 0xffu) << 23) * st.ix;
@@ -207,8 +207,10 @@ __device__ __forceinline__ bool trav_node4_data(TravState<LDS, STACK>& st,
     const uint32_t qlx = __float_as_uint(b.x), qly = __float_as_uint(b.y),
                    qlz = __float_as_uint(b.z), qhx = __float_as_uint(b.w),
                    qhy = __float_as_uint(c.x), qhz = __float_as_uint(c.y);
-    float tn[4];
-    int ref[4] = {__float_as_int(c.z), __float_as_int(c.w), __float_as_int(d.x), __float_as_int(d.y)};
+    ref[0] = __float_as_int(c.z);
+    ref[1] = __float_as_int(c.w);
+    ref[2] = __float_as_int(d.x);
+    ref[3] = __float_as_int(d.y);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int sh = 8 * k;
@@ -227,6 +229,16 @@ __device__ __forceinline__ bool trav_node4_data(TravState<LDS, STACK>& st,
     cswap(tn[0], ref[0], tn[2], ref[2]);
     cswap(tn[1], ref[1], tn[3], ref[3]);
     cswap(tn[1], ref[1], tn[2], ref[2]);
+}
+
+template <int BLOCK, bool LDS, bool STATS, int STACK>
+__device__ __forceinline__ bool trav_node4_data(TravState<LDS, STACK>& st,
+                                                int (&spill)[TravState<LDS, STACK>::SPILL],
+                                                float4 a, float4 b, float4 c, float4 d, int* s_stack) {
+    if (STATS) ++st.nodes_visited;
+    float tn[4];
+    int ref[4];
+    node4_hits(st, a, b, c, d, tn, ref);
     if (tn[3] < INFINITY) stack_push<LDS, BLOCK, STACK>(st, spill, s_stack, ref[3]);
     if (tn[2] < INFINITY) stack_push<LDS, BLOCK, STACK>(st, spill, s_stack, ref[2]);
     if (tn[1] < INFINITY) stack_push<LDS, BLOCK, STACK>(st, spill, s_stack, ref[1]);
