@@ -99,7 +99,7 @@ def main():
     r.set_capture(True)
     r.render()                       # untimed: captures the bounce wavefronts
     frame_stats = r.stats()
-    mode = {"aila32": 0, "origin30": 1, "origin30_oct30": 2}[cfg["Renderer"]["keyMode"]]
+    mode = KEY_MODES[cfg["Renderer"]["keyMode"]]
     kbytes = 4 if mode < 2 else 8
     kbits = stages.key_bits(mode)
     smin, smax = (C.c_float * 3)(), (C.c_float * 3)()
@@ -146,8 +146,9 @@ def main():
     names = ["keys", "sort", "reorder", "trace"]
     ev = {k: [] for k in names}
 
-    def step(record=False, sp=None):
+    def step(record=False, sp=None, key_mode=None):
         s_ptr = sp if sp is not None else C.c_void_p(stream.cuda_stream)
+        km = mode if key_mode is None else key_mode
         for x in bounces:
             n = x["n"]
             marks = []
@@ -156,7 +157,7 @@ def main():
                 marks[-1].record(stream)
             stages._capi.check("wf_morton_keys", lib.wf_morton_keys(
                 C.c_void_p(x["rays"].data_ptr()), C.c_void_p(x["keys"].data_ptr()),
-                C.c_void_p(x["idx"].data_ptr()), n, mode, mn3, mx3, s_ptr, None))
+                C.c_void_p(x["idx"].data_ptr()), n, km, mn3, mx3, s_ptr, None))
             if record:
                 marks.append(torch.cuda.Event(enable_timing=True))
                 marks[-1].record(stream)
@@ -271,6 +272,23 @@ def main():
     same = all(torch.equal(x["hits"][torch.argsort(x["out_s2i"])], x["hits_u"][torch.argsort(x["s2i"])])
                for x in bounces)
 
+    # Key layout ablation: the same 60 origin + direction bits concatenated
+    # (origin30 << 30 | oct30) or interleaved as a 5-D Morton code.
+    key_layout = None
+    if mode in (2, 4):
+        key_layout = {}
+        for km, name in ((2, "origin30_oct30"), (4, "origin_dir_morton60")):
+            for k in names:
+                ev[k].clear()
+            for _ in range(3):
+                step(record=True, key_mode=km)
+            torch.cuda.synchronize()
+            kms = {k: sum(a.elapsed_time(b_) for a, b_, _ in ev[k]) / 3 for k in names}
+            key_layout[name] = {
+                "stage_ms": {k: round(v, 4) for k, v in kms.items()},
+                "mrays_s": round(total_rays / (sum(kms.values()) / 1e3) / 1e6, 1),
+                "trace_speedup_vs_unsorted": round(unsorted_ms / kms["trace"], 3)}
+
     # Reference-style full frames (Benchmark::run): path Mrays/s includes shade.
     frame_walls, fstats = [], []
     for _ in range(args.frames):
@@ -314,7 +332,8 @@ def main():
         "data": "synthetic (seeded procedural interior scene, captured bounce wavefronts)",
         "config": {
             "workload": "medium plain: interior 263,250 tris, 1920x1080, 1 spp, 8 bounces, RR after "
-                        "bounce 3, 60-bit keys (30 origin + 30 octahedral), u64 key sort + u32 payload",
+                        "bounce 3, 60-bit keys (30 origin + 30 octahedral bits, " + cfg["Renderer"]["keyMode"]
+                        + "), u64 key sort + u32 payload",
             "rays_per_step": total_rays, "bounces": len(bounces),
             "rays_per_bounce": [x["n"] for x in bounces], "digit_bits": args.digit_bits,
             "parallelism": "1 GPU",
@@ -332,6 +351,7 @@ def main():
         "unsorted_trace_mrays_s": round(unsorted_mrays, 1),
         "sort_speedup_trace": round(unsorted_ms / stage_ms["trace"], 3),
         "sorted_vs_unsorted_hits_equal": bool(same),
+        "key_layout_ablation": key_layout,
         "reference_style_frame": ref_style,
         "untimed_capture_frame": {"path_rays": frame_stats["path_rays"]},
         "large_ablation": large,
@@ -399,8 +419,9 @@ def adjoint_section(dev, reps=3):
 
 def large_section(dev, reps=3):
     """Large scene reorder ablation: the city (~2.8M triangles), 1920x1080,
-    4 spp, 8 bounces.  For each key width (30-bit origin keys in u32, 60-bit
-    origin + octahedral keys in u64) and sort period (every bounce, every 2nd,
+    4 spp, 8 bounces.  For each key (30-bit origin keys in u32, 60-bit
+    origin + octahedral keys in u64, concatenated or interleaved as a 5-D
+    Morton code) and sort period (every bounce, every 2nd,
     never) a frame of the C++ frame loop with those settings captures every
     pass's path wavefronts; the 4 passes of each bounce are concatenated into
     one wavefront (8.3M rays, all samples in flight) and the timed step runs
@@ -426,7 +447,8 @@ def large_section(dev, reps=3):
     lib = stages._capi.load()
     stream = torch.cuda.current_stream()
     s_ptr = C.c_void_p(stream.cuda_stream)
-    modes = [("32-bit", "origin30", 1, 4, 30), ("60-bit", "origin30_oct30", 2, 8, 60)]
+    modes = [("32-bit", "origin30", 1, 4, 30), ("60-bit", "origin30_oct30", 2, 8, 60),
+             ("60-bit 5-D", "origin_dir_morton60", 4, 8, 60)]
     rows, sort_rates = [], {}
     for name, key_mode, mode, kbytes, kbits in modes:
         for period in (1, 2, 0):
@@ -525,7 +547,7 @@ def large_section(dev, reps=3):
             log(f"large ablation {name} period {period}: {best['mrays_s']} Mrays/s")
     out = {"workload": "city ~2.8M tris, 1920x1080, 4 spp in flight (8.3M rays/bounce), 8 bounces",
            "rows": rows}
-    for name in ("32-bit", "60-bit"):
+    for name, *_ in modes:
         every = next(x for x in rows if x["keys"] == name and x["period"] == 1)
         never = next(x for x in rows if x["keys"] == name and x["period"] == 0)
         out[f"speedup_every_vs_never_{name}"] = round(every["mrays_s"] / never["mrays_s"], 3)
@@ -620,6 +642,7 @@ def tuning_state():
     return out
 
 
+KEY_MODES = {"aila32": 0, "origin30": 1, "origin30_oct30": 2, "origin_dir_morton60": 4}
 TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_interior_1080p.txt"
 STAGE_KERNEL = {"trace": "trace_kernel", "sort": "sort_bin_kernel", "reorder": "reorder_kernel",
                 "keys": "keys_kernel"}

@@ -74,12 +74,20 @@ const char* wf_version(void);
  *                            of the 15+15-bit octahedral direction, u64.
  *   WF_KEY_SCENE6_ORIGIN30_OCT28 : batched key = scene_id << 58 |
  *                            origin30 << 28 | 14+14-bit octahedral, u64.
+ *   WF_KEY_ORIGIN_DIR_MORTON60 : the 30 origin and 30 direction bits of
+ *                            ORIGIN30_OCT30 as one 5-D Morton code: from the
+ *                            top, 10 levels of (u, v, x, y, z) bits (direction
+ *                            bits 14..5 with origin bits 9..0), then the
+ *                            direction's low 5 levels as (u, v) pairs, u64;
+ *                            in the spirit of the reference's interleaved
+ *                            Aila32 key (origin and direction bits alternate).
  * keys receive u32 (AILA32, ORIGIN30) or u64 (others); indices[i] = i. */
 enum {
     WF_KEY_AILA32 = 0,
     WF_KEY_ORIGIN30 = 1,
     WF_KEY_ORIGIN30_OCT30 = 2,
-    WF_KEY_SCENE6_ORIGIN30_OCT28 = 3
+    WF_KEY_SCENE6_ORIGIN30_OCT28 = 3,
+    WF_KEY_ORIGIN_DIR_MORTON60 = 4
 };
 int wf_key_bytes(int mode);  /* 4 or 8, or WF_ERR_INVALID_ARG */
 int wf_key_bits(int mode);   /* significant bits: 32, 30, 60, 64 */

@@ -60,6 +60,13 @@ def keys_oct60(rays, mn, mx):
     return out
 
 
+def keys_morton5d60(rays, mn, mx):
+    r = _rays(rays)
+    out = np.empty(len(r), np.uint64)
+    load().or_keys_morton5d60(_p(r), C.c_int64(len(r)), _f3(mn), _f3(mx), _p(out))
+    return out
+
+
 def keys_scene64(rays, scene, mins, maxs):
     r = _rays(rays)
     s = np.ascontiguousarray(scene, np.uint8)

@@ -69,10 +69,15 @@ OR_API void or_keys_aila32(const float* rays, int64_t n, const float mn[3], cons
     }
 }
 
+/* 10-bit origin coordinates within the scene box. */
+static void origin_q10(const float* ray, const float mn[3], const float sz[3], uint32_t q[3]) {
+    for (int k = 0; k < 3; ++k) q[k] = (uint32_t)(clamp01f((ray[k] - mn[k]) / sz[k]) * 1023.0f);
+}
+
 /* 10-bit-per-axis origin Morton code: bit 3k+2 = x bit k, 3k+1 = y, 3k = z. */
 static uint32_t origin30(const float* ray, const float mn[3], const float sz[3]) {
     uint32_t q[3];
-    for (int k = 0; k < 3; ++k) q[k] = (uint32_t)(clamp01f((ray[k] - mn[k]) / sz[k]) * 1023.0f);
+    origin_q10(ray, mn, sz, q);
     uint32_t key = 0;
     for (int b = 0; b < 10; ++b)
         for (int k = 0; k < 3; ++k) key |= ((q[k] >> b) & 1u) << (3 * b + (2 - k));
@@ -80,8 +85,8 @@ static uint32_t origin30(const float* ray, const float mn[3], const float sz[3])
 }
 
 /* Octahedral map (Cigolle et al. 2014) of the direction, `bits` bits per
- * axis, 2-D Morton with u above v. */
-static uint32_t oct_code(const float* ray, int bits) {
+ * axis. */
+static void oct_q(const float* ray, int bits, uint32_t* qu_out, uint32_t* qv_out) {
     float dx = ray[4], dy = ray[5], dz = ray[6];
     float l1 = (fabsf(dx) + fabsf(dy)) + fabsf(dz);
     float px = dx / l1, py = dy / l1;
@@ -93,7 +98,14 @@ static uint32_t oct_code(const float* ray, int bits) {
     }
     float u = px * 0.5f + 0.5f, v = py * 0.5f + 0.5f;
     float scale = (float)((1u << bits) - 1u);
-    uint32_t qu = (uint32_t)(clamp01f(u) * scale), qv = (uint32_t)(clamp01f(v) * scale);
+    *qu_out = (uint32_t)(clamp01f(u) * scale);
+    *qv_out = (uint32_t)(clamp01f(v) * scale);
+}
+
+/* ... as a 2-D Morton code with u above v. */
+static uint32_t oct_code(const float* ray, int bits) {
+    uint32_t qu, qv;
+    oct_q(ray, bits, &qu, &qv);
     uint32_t code = 0;
     for (int b = 0; b < bits; ++b) code |= (((qu >> b) & 1u) << (2 * b + 1)) | (((qv >> b) & 1u) << (2 * b));
     return code;
@@ -110,6 +122,32 @@ OR_API void or_keys_oct60(const float* rays, int64_t n, const float mn[3], const
     float sz[3] = {mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
     for (int64_t i = 0; i < n; ++i)
         keys[i] = ((uint64_t)origin30(rays + 8 * i, mn, sz) << 30) | oct_code(rays + 8 * i, 15);
+}
+
+/* The same 30 origin and 30 direction bits as one 5-D Morton code (this
+ * library's WF_KEY_ORIGIN_DIR_MORTON60): level b = 9..0 holds, from its top,
+ * u bit b+5, v bit b+5, x bit b, y bit b, z bit b at key bits 10+5b+4..0;
+ * below them the direction's low bits, u bit b at 2b+1 and v bit b at 2b. */
+OR_API void or_keys_morton5d60(const float* rays, int64_t n, const float mn[3], const float mx[3],
+                               uint64_t* keys) {
+    float sz[3] = {mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
+    for (int64_t i = 0; i < n; ++i) {
+        uint32_t q[3], qu, qv;
+        origin_q10(rays + 8 * i, mn, sz, q);
+        oct_q(rays + 8 * i, 15, &qu, &qv);
+        uint64_t key = 0;
+        for (int b = 0; b < 10; ++b) {
+            const int at = 10 + 5 * b;
+            key |= (uint64_t)((qu >> (b + 5)) & 1u) << (at + 4);
+            key |= (uint64_t)((qv >> (b + 5)) & 1u) << (at + 3);
+            key |= (uint64_t)((q[0] >> b) & 1u) << (at + 2);
+            key |= (uint64_t)((q[1] >> b) & 1u) << (at + 1);
+            key |= (uint64_t)((q[2] >> b) & 1u) << at;
+        }
+        for (int b = 0; b < 5; ++b)
+            key |= ((uint64_t)((qu >> b) & 1u) << (2 * b + 1)) | ((uint64_t)((qv >> b) & 1u) << (2 * b));
+        keys[i] = key;
+    }
 }
 
 OR_API void or_keys_scene64(const float* rays, const uint8_t* scene, int64_t n, const float* mn,

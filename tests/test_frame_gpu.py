@@ -188,7 +188,8 @@ def test_frame_deterministic_and_sort_invariant(dev):
     again = render({"Renderer.sortPathRays": False}).pixels()
     assert np.array_equal(base, again)
     for sort, reorder, mode in [(True, True, "aila32"), (True, False, "origin30"),
-                                (True, True, "origin30_oct30")]:
+                                (True, True, "origin30_oct30"), (True, True, "origin_dir_morton60"),
+                                (True, False, "origin_dir_morton60")]:
         img = render({"Renderer.sortPathRays": sort, "Renderer.reorderPathRays": reorder,
                       "Renderer.keyMode": mode, "Renderer.sortShadowRays": True,
                       "Renderer.reorderShadowRays": reorder}).pixels()

@@ -77,6 +77,56 @@ def test_oct60_known_answers(orc):
     assert int(k[3]) == (0x3FFFFFFF << 30) | 0x0FFFFFFF
 
 
+def test_morton5d60_known_answers(orc):
+    """Bit layout of the 5-D key, derived by hand: level b (0..9) holds u, v
+    bits b+5 and x, y, z bits b at key bits 10+5b+4 .. 10+5b; the low 10
+    bits hold u, v bits 4..0 as (u, v) pairs."""
+    rays = np.array([ray((0, 0, 0), (0, 0, 1)),     # q = 0; u = v = 0.5 -> 16383 (bit 14 clear)
+                     ray((0, 0, 0), (0, 0, -1)),    # folded: u = v = 1 -> 32767
+                     ray((1, 1, 1), (0, 0, 1)),     # q = 1023 on every axis
+                     ray((1, 0, 0), (-1, 0, 0))],   # x = 1023; u = 0, v = 16383
+                    np.float32)
+    k = [int(x) for x in orc.keys_morton5d60(rays, UNIT_MIN, UNIT_MAX)]
+    # u, v both set on levels 0..8 (direction bits 5..13), all 10 low bits set
+    uv_lo9 = sum(0b11000 << (10 + 5 * b) for b in range(9))
+    assert k[0] == uv_lo9 | 0x3FF
+    assert k[1] == sum(0b11000 << (10 + 5 * b) for b in range(10)) | 0x3FF
+    assert k[2] == k[0] | sum(0b00111 << (10 + 5 * b) for b in range(10))
+    v_lo9 = sum(0b01000 << (10 + 5 * b) for b in range(9))
+    x_all = sum(0b00100 << (10 + 5 * b) for b in range(10))
+    assert k[3] == v_lo9 | x_all | 0b0101010101
+
+
+def test_morton5d60_same_bits_as_oct60(orc):
+    """Both 60-bit keys carry the same coordinates: decoding x, y, z (10 bits)
+    and u, v (15 bits) from each layout gives equal values on random rays."""
+    rng = np.random.default_rng(17)
+    n = 5000
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = rng.uniform(-0.2, 1.2, (n, 3))
+    d = rng.standard_normal((n, 3))
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    k60 = orc.keys_oct60(rays, UNIT_MIN, UNIT_MAX).astype(np.uint64)
+    k5 = orc.keys_morton5d60(rays, UNIT_MIN, UNIT_MAX).astype(np.uint64)
+
+    def bits(k, positions):
+        out = np.zeros(n, np.uint64)
+        for i, p in enumerate(positions):
+            out |= ((k >> np.uint64(p)) & np.uint64(1)) << np.uint64(i)
+        return out
+
+    # concatenated layout: origin Morton above bit 30 (x at 3i+2), direction below (u at 2i+1)
+    c = [bits(k60, [30 + 3 * i + 2 - a for i in range(10)]) for a in range(3)]
+    c += [bits(k60, [2 * i + 1 - a for i in range(15)]) for a in range(2)]
+    # 5-D layout: low 5 direction bits as pairs, then levels of (u, v, x, y, z)
+    m = [bits(k5, [10 + 5 * i + 2 - a for i in range(10)]) for a in range(3)]
+    m += [bits(k5, [2 * i + 1 - a for i in range(5)] + [10 + 5 * i + 4 - a for i in range(10)])
+          for a in range(2)]
+    for a in range(5):
+        assert np.array_equal(c[a], m[a]), a
+    assert len(np.unique(c[3])) > 1000 and len(np.unique(c[0])) > 500
+
+
 def test_scene64_prefix(orc):
     rays = np.array([ray((0, 0, 0), (0, 0, 1))] * 3, np.float32)
     scene = np.array([0, 5, 63], np.uint8)

@@ -43,16 +43,24 @@ __device__ __forceinline__ void origin_unit(const KeyFrame& f, float4 o, float& 
     az = clamp01((o.z - f.minz) / f.sz);
 }
 
-__device__ __forceinline__ uint32_t origin30(const KeyFrame& f, float4 o) {
+// 10-bit origin coordinates within the scene box.
+__device__ __forceinline__ void origin_q10(const KeyFrame& f, float4 o, uint32_t& ix, uint32_t& iy,
+                                           uint32_t& iz) {
     float ax, ay, az;
     origin_unit(f, o, ax, ay, az);
-    uint32_t ix = (uint32_t)(ax * 1023.0f), iy = (uint32_t)(ay * 1023.0f),
-             iz = (uint32_t)(az * 1023.0f);
+    ix = (uint32_t)(ax * 1023.0f);
+    iy = (uint32_t)(ay * 1023.0f);
+    iz = (uint32_t)(az * 1023.0f);
+}
+
+__device__ __forceinline__ uint32_t origin30(const KeyFrame& f, float4 o) {
+    uint32_t ix, iy, iz;
+    origin_q10(f, o, ix, iy, iz);
     return (spread3_10(ix) << 2) | (spread3_10(iy) << 1) | spread3_10(iz);
 }
 
-// Octahedral direction code with `bits` bits per axis, u-major 2-D Morton.
-__device__ __forceinline__ uint32_t oct_code(float4 d, int bits) {
+// Octahedral direction coordinates (Cigolle et al. 2014), `bits` bits each.
+__device__ __forceinline__ void oct_q(float4 d, int bits, uint32_t& qu, uint32_t& qv) {
     float l1 = (fabsf(d.x) + fabsf(d.y)) + fabsf(d.z);
     float px = d.x / l1, py = d.y / l1;
     if (d.z < 0.0f) {
@@ -63,8 +71,39 @@ __device__ __forceinline__ uint32_t oct_code(float4 d, int bits) {
     }
     float u = px * 0.5f + 0.5f, v = py * 0.5f + 0.5f;
     float scale = (float)((1u << bits) - 1u);
-    uint32_t qu = (uint32_t)(clamp01(u) * scale), qv = (uint32_t)(clamp01(v) * scale);
+    qu = (uint32_t)(clamp01(u) * scale);
+    qv = (uint32_t)(clamp01(v) * scale);
+}
+
+// Octahedral direction code with `bits` bits per axis, u-major 2-D Morton.
+__device__ __forceinline__ uint32_t oct_code(float4 d, int bits) {
+    uint32_t qu, qv;
+    oct_q(d, bits, qu, qv);
     return (spread2_16(qu) << 1) | spread2_16(qv);
+}
+
+// Spread the low 10 bits of v so that bit k lands on bit 5k.
+__device__ __forceinline__ uint64_t spread5_10(uint32_t v) {
+    uint64_t r = 0;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) r |= (uint64_t)((v >> k) & 1u) << (5 * k);
+    return r;
+}
+
+// 5-D Morton code of the same 30 origin and 30 octahedral-direction bits as
+// the 60-bit key: 10 levels of (u, v, x, y, z) bits from the top (direction
+// bits 14..5 beside origin bits 9..0), then the direction's low 5 levels as
+// (u, v) pairs.  Rays adjacent in this order share a coarse origin cell and
+// a coarse direction cone at every scale, which is what keeps a wave's
+// traversals together; the concatenated key groups by origin only.
+__device__ __forceinline__ uint64_t morton5d60(const KeyFrame& f, float4 o, float4 d) {
+    uint32_t ix, iy, iz, qu, qv;
+    origin_q10(f, o, ix, iy, iz);
+    oct_q(d, 15, qu, qv);
+    const uint64_t hi = (spread5_10(qu >> 5) << 4) | (spread5_10(qv >> 5) << 3) |
+                        (spread5_10(ix) << 2) | (spread5_10(iy) << 1) | spread5_10(iz);
+    const uint32_t lo = (spread2_16(qu & 31u) << 1) | spread2_16(qv & 31u);
+    return (hi << 10) | lo;
 }
 
 __device__ __forceinline__ uint32_t aila32(const KeyFrame& f, float4 o, float4 d) {

@@ -520,7 +520,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_onesweep_kernel(
 
 // ------------------------------------------------------------ small sorts
 // Up to SMALL_N pairs: one workgroup of 1024 threads holds 8 pairs per thread
-// in registers and runs every pass in LDS (rank by wave64 ballot matching,
+// (4 for up to 4096 pairs) in registers and runs every pass in LDS (rank by wave64 ballot matching,
 // per-digit prefixes over the 16 waves and over the digits, scatter to the
 // LDS copy in digit order, read back in index order), then writes the
 // result once.  One launch instead of three per pass: the tail bounces of a
@@ -528,8 +528,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_onesweep_kernel(
 // launch and memory latency, not bandwidth.
 constexpr int SMALL_BLOCK = 1024;
 constexpr int SMALL_WAVES = SMALL_BLOCK / 64;
-constexpr int SMALL_ITEMS = 8;
-constexpr int SMALL_N = SMALL_BLOCK * SMALL_ITEMS;
+constexpr int SMALL_N = SMALL_BLOCK * 8;
 
 // Block-wide (1024 threads) exclusive scan of one value per thread.
 __device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t* s_tmp) {
@@ -551,17 +550,18 @@ __device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t* s
     return wbase + x - v;
 }
 
-template <typename K, int DBITS>
+template <typename K, int DBITS, int SMALL_ITEMS>
 __global__ __launch_bounds__(SMALL_BLOCK) void sort_small_kernel(K* __restrict__ keys,
                                                                  uint32_t* __restrict__ vals,
                                                                  int64_t n, int key_bits,
                                                                  const uint32_t* __restrict__ d_n) {
     constexpr int RADIX = 1 << DBITS;
+    constexpr int CAP = SMALL_BLOCK * SMALL_ITEMS;
     static_assert(RADIX <= SMALL_BLOCK, "one thread per digit");
     __shared__ uint32_t s_wave[SMALL_WAVES][RADIX];  // per-wave counts -> wave prefixes
     __shared__ uint32_t s_start[RADIX];              // digit starts in the block
-    __shared__ K s_key[SMALL_N];
-    __shared__ uint32_t s_val[SMALL_N];
+    __shared__ K s_key[CAP];
+    __shared__ uint32_t s_val[CAP];
     __shared__ uint32_t s_tmp[SMALL_WAVES];
     if (d_n) n = min(n, (int64_t)*d_n);
     const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
@@ -755,6 +755,19 @@ static int sort_impl(void* keys, uint32_t* vals, int64_t n, int key_bits, int di
     return WF_OK;
 }
 
+template <int ITEMS>
+static void launch_small(void* keys, uint32_t* vals, int64_t n, int key_bytes, int key_bits,
+                         int digit_bits, const uint32_t* d_n, hipStream_t s) {
+    if (key_bytes == 4 && digit_bits == 8)
+        sort_small_kernel<uint32_t, 8, ITEMS><<<1, SMALL_BLOCK, 0, s>>>((uint32_t*)keys, vals, n, key_bits, d_n);
+    else if (key_bytes == 4)
+        sort_small_kernel<uint32_t, 4, ITEMS><<<1, SMALL_BLOCK, 0, s>>>((uint32_t*)keys, vals, n, key_bits, d_n);
+    else if (digit_bits == 8)
+        sort_small_kernel<uint64_t, 8, ITEMS><<<1, SMALL_BLOCK, 0, s>>>((uint64_t*)keys, vals, n, key_bits, d_n);
+    else
+        sort_small_kernel<uint64_t, 4, ITEMS><<<1, SMALL_BLOCK, 0, s>>>((uint64_t*)keys, vals, n, key_bits, d_n);
+}
+
 int sort_pairs_launch(void* keys, uint32_t* vals, int64_t n_cap, const uint32_t* d_n,
                       int key_bytes, int key_bits, int digit_bits, void* scratch,
                       int64_t scratch_bytes, hipStream_t s) {
@@ -768,14 +781,10 @@ int sort_pairs_launch(void* keys, uint32_t* vals, int64_t n_cap, const uint32_t*
     if (scratch_bytes < p.total)
         return fail(WF_ERR_SCRATCH_TOO_SMALL, "sort: scratch smaller than wf_sort_scratch_bytes");
     if (n_cap <= SMALL_N && digit_bits <= 8) {
-        if (key_bytes == 4 && digit_bits == 8)
-            sort_small_kernel<uint32_t, 8><<<1, SMALL_BLOCK, 0, s>>>((uint32_t*)keys, vals, n_cap, key_bits, d_n);
-        else if (key_bytes == 4)
-            sort_small_kernel<uint32_t, 4><<<1, SMALL_BLOCK, 0, s>>>((uint32_t*)keys, vals, n_cap, key_bits, d_n);
-        else if (digit_bits == 8)
-            sort_small_kernel<uint64_t, 8><<<1, SMALL_BLOCK, 0, s>>>((uint64_t*)keys, vals, n_cap, key_bits, d_n);
+        if (n_cap <= SMALL_N / 2)
+            launch_small<4>(keys, vals, n_cap, key_bytes, key_bits, digit_bits, d_n, s);
         else
-            sort_small_kernel<uint64_t, 4><<<1, SMALL_BLOCK, 0, s>>>((uint64_t*)keys, vals, n_cap, key_bits, d_n);
+            launch_small<8>(keys, vals, n_cap, key_bytes, key_bits, digit_bits, d_n, s);
         return launch_check("sort_small_kernel");
     }
     return key_bytes == 4
