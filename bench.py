@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable workload")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--no-large", action="store_true", help="skip the large-scene reorder ablation")
+    ap.add_argument("--no-ablations", action="store_true",
+                    help="skip the unsorted-trace and key-layout ablations of the main workload "
+                         "(profiling runs: every trace launch is then one of the measured steps)")
     ap.add_argument("--batch-views", type=int, default=64)
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="wf_set_tuning knob applied before the scene is built (repeatable)")
@@ -252,7 +255,7 @@ def main():
 
     # Unsorted trace of the same wavefronts (sort ablation).
     ts = []
-    for _ in range(3):
+    for _ in range(0 if args.no_ablations else 3):
         for x in bounces:
             a = torch.cuda.Event(enable_timing=True)
             b_ = torch.cuda.Event(enable_timing=True)
@@ -263,19 +266,19 @@ def main():
             b_.record(stream)
             ts.append((a, b_))
     torch.cuda.synchronize()
-    unsorted_ms = sum(a.elapsed_time(b_) for a, b_ in ts) / 3
-    unsorted_mrays = total_rays / (unsorted_ms / 1e3) / 1e6
+    unsorted_ms = sum(a.elapsed_time(b_) for a, b_ in ts) / 3 if ts else None
+    unsorted_mrays = total_rays / (unsorted_ms / 1e3) / 1e6 if ts else None
     sorted_ms = sum(stage_ms.values())
     sorted_mrays = total_rays / (sorted_ms / 1e3) / 1e6
     trace_only_sorted = total_rays / (stage_ms["trace"] / 1e3) / 1e6
     # parity of the sorted trace with the unsorted one (same hits per ray)
     same = all(torch.equal(x["hits"][torch.argsort(x["out_s2i"])], x["hits_u"][torch.argsort(x["s2i"])])
-               for x in bounces)
+               for x in bounces) if ts else None
 
     # Key layout ablation: the same 60 origin + direction bits concatenated
     # (origin30 << 30 | oct30) or interleaved as a 5-D Morton code.
     key_layout = None
-    if mode in (2, 4):
+    if mode in (2, 4) and not args.no_ablations:
         key_layout = {}
         for km, name in ((2, "origin30_oct30"), (4, "origin_dir_morton60")):
             for k in names:
@@ -348,9 +351,9 @@ def main():
         "graph_replay_equals_eager": bool(graph_equal),
         "sorted_pipeline_mrays_s": round(sorted_mrays, 1),
         "trace_only_sorted_mrays_s": round(trace_only_sorted, 1),
-        "unsorted_trace_mrays_s": round(unsorted_mrays, 1),
-        "sort_speedup_trace": round(unsorted_ms / stage_ms["trace"], 3),
-        "sorted_vs_unsorted_hits_equal": bool(same),
+        "unsorted_trace_mrays_s": round(unsorted_mrays, 1) if ts else None,
+        "sort_speedup_trace": round(unsorted_ms / stage_ms["trace"], 3) if ts else None,
+        "sorted_vs_unsorted_hits_equal": same,
         "key_layout_ablation": key_layout,
         "reference_style_frame": ref_style,
         "untimed_capture_frame": {"path_rays": frame_stats["path_rays"]},
