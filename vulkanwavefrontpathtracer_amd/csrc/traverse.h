@@ -204,9 +204,17 @@ __device__ __forceinline__ void node4_hits(const TravState<LDS, STACK>& st, floa
     const float saz = __uint_as_float(((meta >> 16) & 0xffu) << 23) * st.iz;
     const float bx = fmaf(a.x, st.ix, -st.oix), by = fmaf(a.y, st.iy, -st.oiy),
                 bz = fmaf(a.z, st.iz, -st.oiz);
+    // Near and far planes per axis by the sign of the inverse direction (a
+    // positive one meets the low plane first): the same floats as the min
+    // and max of the two plane distances, chosen once per node instead of
+    // per child.
     const uint32_t qlx = __float_as_uint(b.x), qly = __float_as_uint(b.y),
                    qlz = __float_as_uint(b.z), qhx = __float_as_uint(b.w),
                    qhy = __float_as_uint(c.x), qhz = __float_as_uint(c.y);
+    const bool fx = st.ix < 0.0f, fy = st.iy < 0.0f, fz = st.iz < 0.0f;
+    const uint32_t qnx = fx ? qhx : qlx, qfx = fx ? qlx : qhx;
+    const uint32_t qny = fy ? qhy : qly, qfy = fy ? qly : qhy;
+    const uint32_t qnz = fz ? qhz : qlz, qfz = fz ? qlz : qhz;
     ref[0] = __float_as_int(c.z);
     ref[1] = __float_as_int(c.w);
     ref[2] = __float_as_int(d.x);
@@ -214,14 +222,14 @@ __device__ __forceinline__ void node4_hits(const TravState<LDS, STACK>& st, floa
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int sh = 8 * k;
-        const float t0x = fmaf((float)((qlx >> sh) & 0xffu), sax, bx);
-        const float t1x = fmaf((float)((qhx >> sh) & 0xffu), sax, bx);
-        const float t0y = fmaf((float)((qly >> sh) & 0xffu), say, by);
-        const float t1y = fmaf((float)((qhy >> sh) & 0xffu), say, by);
-        const float t0z = fmaf((float)((qlz >> sh) & 0xffu), saz, bz);
-        const float t1z = fmaf((float)((qhz >> sh) & 0xffu), saz, bz);
-        const float nn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), st.tmin));
-        const float ff = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), st.best_t));
+        const float nx = fmaf((float)((qnx >> sh) & 0xffu), sax, bx);
+        const float fxv = fmaf((float)((qfx >> sh) & 0xffu), sax, bx);
+        const float ny = fmaf((float)((qny >> sh) & 0xffu), say, by);
+        const float fyv = fmaf((float)((qfy >> sh) & 0xffu), say, by);
+        const float nz = fmaf((float)((qnz >> sh) & 0xffu), saz, bz);
+        const float fzv = fmaf((float)((qfz >> sh) & 0xffu), saz, bz);
+        const float nn = fmaxf(fmaxf(nx, ny), fmaxf(nz, st.tmin));
+        const float ff = fminf(fminf(fxv, fyv), fminf(fzv, st.best_t));
         tn[k] = (k < nc && nn <= ff) ? nn : INFINITY;
     }
     cswap(tn[0], ref[0], tn[1], ref[1]);
