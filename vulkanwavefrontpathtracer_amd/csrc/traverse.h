@@ -247,9 +247,21 @@ __device__ __forceinline__ bool trav_node4_data(TravState<LDS, STACK>& st,
     float tn[4];
     int ref[4];
     node4_hits(st, a, b, c, d, tn, ref);
-    if (tn[3] < INFINITY) stack_push<LDS, BLOCK, STACK>(st, spill, s_stack, ref[3]);
-    if (tn[2] < INFINITY) stack_push<LDS, BLOCK, STACK>(st, spill, s_stack, ref[2]);
-    if (tn[1] < INFINITY) stack_push<LDS, BLOCK, STACK>(st, spill, s_stack, ref[1]);
+    const int h1 = tn[1] < INFINITY, h2 = tn[2] < INFINITY, h3 = tn[3] < INFINITY;
+    if (LDS && __all(st.sp <= LDS_DEPTH - 3)) {
+        // The hit children come first (nearest first), so the pushes can be
+        // three unconditional LDS writes: a missed child's slot is
+        // overwritten by the next write or left above the new top.
+        lds_int* col = (lds_int*)s_stack + threadIdx.x;
+        col[st.sp * BLOCK] = ref[3];
+        col[(st.sp + h3) * BLOCK] = ref[2];
+        col[(st.sp + h3 + h2) * BLOCK] = ref[1];
+        st.sp += h1 + h2 + h3;
+    } else {
+        if (h3) stack_push<LDS, BLOCK, STACK>(st, spill, s_stack, ref[3]);
+        if (h2) stack_push<LDS, BLOCK, STACK>(st, spill, s_stack, ref[2]);
+        if (h1) stack_push<LDS, BLOCK, STACK>(st, spill, s_stack, ref[1]);
+    }
     if (tn[0] < INFINITY) {
         st.node = ref[0];
         return true;
