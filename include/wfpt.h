@@ -306,6 +306,11 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
  *   "trace.group4"          the top of the hierarchy into subtrees walked in
  *                           parallel with a shared closest distance (latency
  *                           of small wavefronts; defaults 5000 / 24000 / 64000)
+ *   "batch.key_layout" 0/1  batched queue sort key below the 6-bit scene prefix:
+ *                           0 origin30 << 28 | oct28, 1 the same bits as a 5-D
+ *                           Morton code (default 0: on the batched boxes
+ *                           scenes, sorted every 4th bounce, both run alike;
+ *                           read when a pass is enqueued or captured)
  *   "trace.loop"       0-2  4-wide traversal loop: 0 branch on node/leaf, then
  *                           load; 1 one load per step for node or leaf
  *                           (default); 2 leaves postponed until every lane of

@@ -289,8 +289,12 @@ __global__ __launch_bounds__(BSHADE_BLOCK) void batch_shade_kernel(
 }
 
 // ------------------------------------------------------ keys and reorder
-// 64-bit key: 6-bit scene index | 30-bit origin Morton code in the scene's
-// AABB | 28-bit octahedral direction code.
+// 64-bit key: 6-bit scene index, then the 30-bit origin Morton code in the
+// scene's AABB and the 28-bit octahedral direction code, concatenated
+// (MORTON5D = false) or interleaved as a 5-D Morton code (true: a wave then
+// holds rays of a coarse origin cell and direction cone, as for the
+// single-view 60-bit key).
+template <bool MORTON5D>
 __global__ __launch_bounds__(256) void batch_keys_kernel(const BatchTables t,
                                                          const uint32_t* __restrict__ q, int b,
                                                          const wf_ray* __restrict__ rays,
@@ -304,7 +308,9 @@ __global__ __launch_bounds__(256) void batch_keys_kernel(const BatchTables t,
     const KeyFrame f{sc.bmin[0], sc.bmin[1], sc.bmin[2], sc.bsize[0], sc.bsize[1], sc.bsize[2]};
     float4 o, d;
     load_ray(rays, i, o, d);
-    keys[i] = ((uint64_t)(sid & 63) << 58) | ((uint64_t)origin30(f, o) << 28) | (uint64_t)oct_code(d, 14);
+    keys[i] = ((uint64_t)(sid & 63) << 58) |
+              (MORTON5D ? morton5d<14>(f, o, d)
+                        : (((uint64_t)origin30(f, o) << 28) | (uint64_t)oct_code(d, 14)));
     idx[i] = (uint32_t)i;
 }
 
@@ -403,8 +409,12 @@ int batch_enqueue_pass(const BatchTables& t, const BatchBuffers& bb, const Batch
         WF_TRY(launch_check("batch_shade_kernel"));
         cur = 1 - cur;
         if (tu.reorder_period > 0 && b % tu.reorder_period == 0) {
-            batch_keys_kernel<<<grid_for(cap, 256), 256, 0, s>>>(t, bb.q, b + 1, bb.rays[cur],
-                                                                bb.s2i[cur], bb.keys, bb.idx);
+            if (tuning().batch_key_layout == 1)
+                batch_keys_kernel<true><<<grid_for(cap, 256), 256, 0, s>>>(t, bb.q, b + 1, bb.rays[cur],
+                                                                           bb.s2i[cur], bb.keys, bb.idx);
+            else
+                batch_keys_kernel<false><<<grid_for(cap, 256), 256, 0, s>>>(t, bb.q, b + 1, bb.rays[cur],
+                                                                            bb.s2i[cur], bb.keys, bb.idx);
             WF_TRY(launch_check("batch_keys_kernel"));
             WF_TRY(sort_pairs_launch(bb.keys, bb.idx, cap, bb.q + BQ_COUNT0 + b + 1, 8, 64,
                                      tu.digit_bits, bb.sort_scratch, bb.sort_scratch_bytes, s));

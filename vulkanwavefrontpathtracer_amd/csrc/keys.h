@@ -96,14 +96,22 @@ __device__ __forceinline__ uint64_t spread5_10(uint32_t v) {
 // (u, v) pairs.  Rays adjacent in this order share a coarse origin cell and
 // a coarse direction cone at every scale, which is what keeps a wave's
 // traversals together; the concatenated key groups by origin only.
-__device__ __forceinline__ uint64_t morton5d60(const KeyFrame& f, float4 o, float4 d) {
+// DBITS = 15 gives the 60-bit key, 14 the 58 bits below the batched key's
+// 6-bit scene prefix (4 low direction levels).
+template <int DBITS>
+__device__ __forceinline__ uint64_t morton5d(const KeyFrame& f, float4 o, float4 d) {
+    constexpr int LOW = DBITS - 10;  // direction levels below the origin's
     uint32_t ix, iy, iz, qu, qv;
     origin_q10(f, o, ix, iy, iz);
-    oct_q(d, 15, qu, qv);
-    const uint64_t hi = (spread5_10(qu >> 5) << 4) | (spread5_10(qv >> 5) << 3) |
+    oct_q(d, DBITS, qu, qv);
+    const uint64_t hi = (spread5_10(qu >> LOW) << 4) | (spread5_10(qv >> LOW) << 3) |
                         (spread5_10(ix) << 2) | (spread5_10(iy) << 1) | spread5_10(iz);
-    const uint32_t lo = (spread2_16(qu & 31u) << 1) | spread2_16(qv & 31u);
-    return (hi << 10) | lo;
+    const uint32_t m = (1u << LOW) - 1u;
+    const uint32_t lo = (spread2_16(qu & m) << 1) | spread2_16(qv & m);
+    return (hi << (2 * LOW)) | lo;
+}
+__device__ __forceinline__ uint64_t morton5d60(const KeyFrame& f, float4 o, float4 d) {
+    return morton5d<15>(f, o, d);
 }
 
 __device__ __forceinline__ uint32_t aila32(const KeyFrame& f, float4 o, float4 d) {
