@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable workload")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--no-large", action="store_true", help="skip the large-scene reorder ablation")
+    ap.add_argument("--no-fuse", action="store_true",
+                    help="reorder and trace as separate kernels (wf_reorder + wf_trace) instead of "
+                         "the fused wf_reorder_trace")
     ap.add_argument("--no-ablations", action="store_true",
                     help="skip the unsorted-trace and key-layout ablations of the main workload "
                          "(profiling runs: every trace launch is then one of the measured steps)")
@@ -149,6 +152,8 @@ def main():
     names = ["keys", "sort", "reorder", "trace"]
     ev = {k: [] for k in names}
 
+    fused = not args.no_fuse
+
     def step(record=False, sp=None, key_mode=None):
         s_ptr = sp if sp is not None else C.c_void_p(stream.cuda_stream)
         km = mode if key_mode is None else key_mode
@@ -171,17 +176,26 @@ def main():
             if record:
                 marks.append(torch.cuda.Event(enable_timing=True))
                 marks[-1].record(stream)
-            stages._capi.check("wf_reorder", lib.wf_reorder(
-                C.c_void_p(x["idx"].data_ptr()), C.c_void_p(x["rays"].data_ptr()),
-                C.c_void_p(x["out_rays"].data_ptr()), C.c_void_p(x["s2i"].data_ptr()),
-                C.c_void_p(x["out_s2i"].data_ptr()), C.c_void_p(x["out_i2s"].data_ptr()), n, s_ptr,
-                None))
+            if not fused:
+                stages._capi.check("wf_reorder", lib.wf_reorder(
+                    C.c_void_p(x["idx"].data_ptr()), C.c_void_p(x["rays"].data_ptr()),
+                    C.c_void_p(x["out_rays"].data_ptr()), C.c_void_p(x["s2i"].data_ptr()),
+                    C.c_void_p(x["out_s2i"].data_ptr()), C.c_void_p(x["out_i2s"].data_ptr()), n,
+                    s_ptr, None))
             if record:
                 marks.append(torch.cuda.Event(enable_timing=True))
                 marks[-1].record(stream)
-            stages._capi.check("wf_trace", lib.wf_trace(
-                sc.handle, C.c_void_p(x["out_rays"].data_ptr()), None,
-                C.c_void_p(x["hits"].data_ptr()), n, 1, s_ptr, None))
+            if fused:
+                # reorder + trace in one kernel: the same outputs as the two calls
+                stages._capi.check("wf_reorder_trace", lib.wf_reorder_trace(
+                    sc.handle, C.c_void_p(x["idx"].data_ptr()), C.c_void_p(x["rays"].data_ptr()),
+                    C.c_void_p(x["out_rays"].data_ptr()), C.c_void_p(x["s2i"].data_ptr()),
+                    C.c_void_p(x["out_s2i"].data_ptr()), C.c_void_p(x["out_i2s"].data_ptr()),
+                    C.c_void_p(x["hits"].data_ptr()), n, 1, s_ptr, None))
+            else:
+                stages._capi.check("wf_trace", lib.wf_trace(
+                    sc.handle, C.c_void_p(x["out_rays"].data_ptr()), None,
+                    C.c_void_p(x["hits"].data_ptr()), n, 1, s_ptr, None))
             if record:
                 marks.append(torch.cuda.Event(enable_timing=True))
                 marks[-1].record(stream)
@@ -233,14 +247,21 @@ def main():
     # Algorithmic bytes per element: keys 32+K+4, sort P*2*(K+4)+K,
     # reorder 80, trace 32 read + 16 write.
     passes = -(-kbits // args.digit_bits)
+    # Fused reorder + trace: 4 (index) + 32 (ray gathered once) + 32 (reordered
+    # ray) + 4 + 4 + 4 (LUTs) + 16 (hit) = 96 B, all under "trace".
     bpe = {"keys": 32 + kbytes + 4, "sort": passes * 2 * (kbytes + 4) + kbytes, "reorder": 80,
-           "trace": 48}
+           "trace": 96 if fused else 48}
     stages_roof = {}
     for k in names:
         t = stage_ms[k] / 1e3
+        if fused and k == "reorder":
+            stages_roof[k] = {"ms_per_frame": 0.0, "fused_into": "trace"}
+            continue
         gbs = total_rays * bpe[k] / t / 1e9 if t > 0 else 0.0
         stages_roof[k] = {"ms_per_frame": round(stage_ms[k], 4), "bytes_per_ray": bpe[k],
                           "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    if fused:
+        stages_roof["trace"]["fused"] = "reorder + trace (wf_reorder_trace)"
     dominant = max(names, key=lambda k: stage_ms[k])
     dom = stages_roof[dominant]
     # per-launch average duration of the dominant kernel at the largest bounce

@@ -182,6 +182,16 @@ int wf_scene_download(wf_scene scene, float* nodes_host, int32_t* tri_order_host
  * traversal order.  tmax < tmin always misses. */
 int wf_trace(wf_scene scene, const wf_ray* rays, const uint32_t* ray_index,
              wf_hit* hits, int64_t n, int closest, wf_stream_t stream, float* ms);
+/* Tracer::traceSort's reorder + trace (Tracer/Tracer.cpp:80-98 then
+ * :185-207, as composed at :209-225 with reorder on) in one pass: the
+ * outputs of wf_reorder(idx, in_rays, out_rays, in_s2i, out_s2i, out_i2s)
+ * followed by wf_trace(scene, out_rays, NULL, hits, n, closest), bit for bit,
+ * with each ray gathered once and traced from registers (configurations
+ * without the fused kernel run the two stages). */
+int wf_reorder_trace(wf_scene scene, const uint32_t* idx, const wf_ray* in_rays,
+                     wf_ray* out_rays, const int32_t* in_s2i, int32_t* out_s2i,
+                     int32_t* out_i2s, wf_hit* hits, int64_t n, int closest,
+                     wf_stream_t stream, float* ms);
 
 /* Diagnostic variant of wf_trace (same results) that also writes, per ray,
  * the number of BVH nodes visited and triangles tested (u32 arrays indexed
@@ -311,6 +321,8 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
  *                           Morton code (default 0: on the batched boxes
  *                           scenes, sorted every 4th bounce, both run alike;
  *                           read when a pass is enqueued or captured)
+ *   "trace.fuse_reorder" 0/1 wf_reorder_trace as one fused kernel (1, default)
+ *                           or as the reorder and trace kernels (0)
  *   "trace.loop"       0-2  4-wide traversal loop: 0 branch on node/leaf, then
  *                           load; 1 one load per step for node or leaf
  *                           (default); 2 leaves postponed until every lane of

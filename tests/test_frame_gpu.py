@@ -259,7 +259,21 @@ def test_frame_stats_counts(dev):
     assert 0 < st["path_rays"] <= 4 * 256 * 256
     assert st["sorted_path_rays"] == st["path_rays"]
     sm = st["stage_ms"]
-    assert sm["keys"] > 0 and sm["sort"] > 0 and sm["reorder"] > 0 and sm["trace_sorted"] > 0
+    # reorder fused into the trace by default: its time counts as trace
+    assert sm["keys"] > 0 and sm["sort"] > 0 and sm["reorder"] == 0 and sm["trace_sorted"] > 0
+    import ctypes as C
+
+    from vulkanwavefrontpathtracer_amd import _capi
+
+    _capi.call("wf_set_tuning", b"trace.fuse_reorder", 0)
+    try:
+        r2 = render({"Renderer.sortPathRays": True, "Benchmark.printSortLogs": True})
+        sm2 = r2.stats()["stage_ms"]
+        assert sm2["reorder"] > 0 and sm2["trace_sorted"] > 0
+        assert np.array_equal(r2.pixels(), r.pixels())
+    finally:
+        _capi.call("wf_set_tuning", b"trace.fuse_reorder", 1)
+    del C
     assert sm["trace_unsorted"] > 0
 
 

@@ -364,3 +364,52 @@ def test_group_traversal_bit_exact(dev, orc, group):
         assert np.all(got[:, 0] == 0) and np.array_equal(got[:, 2], exp[:, 1])
     finally:
         _capi.call("wf_set_tuning", key, cur.value)
+
+
+@pytest.mark.parametrize("knobs", [{}, {"trace.loop": 0}, {"trace.loop": 2}, {"trace.block": 128},
+                                   {"trace.fuse_reorder": 0}, {"trace.width": 2}])
+def test_reorder_trace_fused_bit_exact(dev, knobs):
+    """wf_reorder_trace writes exactly what wf_reorder followed by wf_trace on
+    the reordered rays writes (rays, both LUTs, hit records), on wavefronts
+    that take the fused kernel and ones that fall back (grouped sizes)."""
+    import ctypes as C
+
+    from vulkanwavefrontpathtracer_amd import _capi, stages
+
+    saved = {}
+    for k, v in knobs.items():
+        cur = C.c_int()
+        _capi.call("wf_get_tuning", k.encode(), C.byref(cur))
+        saved[k] = cur.value
+        _capi.call("wf_set_tuning", k.encode(), v)
+    try:
+        hs = stages.HostScene(stages.SCENE_INTERIOR, 1001)
+        sc = stages.Scene.from_host(hs, dev)
+        mn, mx = sc.bounds()
+        for n, closest in ((1000, 1), (70_001, 1), (70_001, 0), (150_000, 1)):
+            rays = torch.from_numpy(scene_rays(hs, n, n + closest, tmin=1e-4)).to(dev)
+            rays[::89, 7] = -1.0
+            keys, idx = stages.morton_keys(rays, stages.KEY_ORIGIN_DIR_MORTON60, mn, mx)
+            stages.sort_pairs(keys, idx, 60, 8)
+            s2i = torch.randperm(n + 17, device=dev)[:n].to(torch.int32)
+            outs = []
+            for fused in (True, False):
+                out_rays = torch.full_like(rays, 7.0)
+                out_s2i = torch.full_like(s2i, -5)
+                out_i2s = torch.full((n + 17,), -3, dtype=torch.int32, device=dev)
+                hits = torch.full((n, 4), -9, dtype=torch.int32, device=dev)
+                ptrs = [C.c_void_p(t.data_ptr()) for t in (idx, rays, out_rays, s2i, out_s2i, out_i2s)]
+                if fused:
+                    _capi.call("wf_reorder_trace", sc.handle, *ptrs, C.c_void_p(hits.data_ptr()), n,
+                               closest, None, None)
+                else:
+                    _capi.call("wf_reorder", *ptrs, n, None, None)
+                    _capi.call("wf_trace", sc.handle, C.c_void_p(out_rays.data_ptr()), None,
+                               C.c_void_p(hits.data_ptr()), n, closest, None, None)
+                torch.cuda.synchronize()
+                outs.append((out_rays, out_s2i, out_i2s, hits))
+            for a, b in zip(*outs):
+                assert torch.equal(a, b), (n, closest)
+    finally:
+        for k, v in saved.items():
+            _capi.call("wf_set_tuning", k.encode(), v)

@@ -66,6 +66,7 @@ SIGNATURES: dict[str, tuple] = {
     "wf_scene_info": (i32, [vp, pi64, pi64, pi64, C.POINTER(i32)]),
     "wf_scene_download": (i32, [vp, pf, pi32]),
     "wf_trace": (i32, [vp, vp, vp, vp, i64, i32, vp, pf]),
+    "wf_reorder_trace": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, i64, i32, vp, pf]),
     "wf_trace_stats": (i32, [vp, vp, vp, i64, i32, vp, vp, vp]),
     "wf_trace_sort": (i32, [vp, C.POINTER(WfRayBuffer), i32, i32, pf, pf, i32, vp, pf]),
     "wf_init_seeds": (i32, [vp, i64, i32, vp, pf]),

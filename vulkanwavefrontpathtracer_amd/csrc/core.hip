@@ -89,6 +89,7 @@ extern "C" int wf_set_tuning(const char* key, int value) {
     else if (k == "trace.refill" && value >= 1 && value <= 64) t.trace_refill = value;
     else if (k == "trace.width" && (value == 2 || value == 4)) t.trace_width = value;
     else if (k == "trace.loop" && value >= 0 && value <= 2) t.trace_loop = value;
+    else if (k == "trace.fuse_reorder" && (value == 0 || value == 1)) t.trace_fuse_reorder = value;
     else if (k == "batch.key_layout" && (value == 0 || value == 1)) t.batch_key_layout = value;
     else if (k == "trace.group64" && value >= 0) t.trace_group64 = value;
     else if (k == "trace.group16" && value >= 0) t.trace_group16 = value;
@@ -114,6 +115,7 @@ extern "C" int wf_get_tuning(const char* key, int* value) {
     else if (k == "trace.refill") *value = t.trace_refill;
     else if (k == "trace.width") *value = t.trace_width;
     else if (k == "trace.loop") *value = t.trace_loop;
+    else if (k == "trace.fuse_reorder") *value = t.trace_fuse_reorder;
     else if (k == "batch.key_layout") *value = t.batch_key_layout;
     else if (k == "trace.group64") *value = t.trace_group64;
     else if (k == "trace.group16") *value = t.trace_group16;

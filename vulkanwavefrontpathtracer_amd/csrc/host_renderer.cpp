@@ -292,7 +292,10 @@ struct Frame {
         WF_TRY(wf_sort_pairs(rb.keys.p, rb.indices.as<uint32_t>(), n, kb, bits, r->digit_bits,
                              rb.scratch.p, (int64_t)rb.scratch.bytes, ws, nullptr));
         tend();
-        if (reorder) {
+        int fuse = 1;
+        (void)wf_get_tuning("trace.fuse_reorder", &fuse);
+        if (reorder && !fuse) {
+            // Tracer::reorderRays then Tracer::trace, timed as two stages
             int o = 1 - rb.swap;
             tbegin(ST_REORDER, kind);
             WF_TRY(wf_reorder(rb.indices.as<uint32_t>(), rb.ray_buf(), rb.rays[o].as<wf_ray>(),
@@ -300,11 +303,25 @@ struct Frame {
                               ws, nullptr));
             tend();
             rb.swap = o;
+            tbegin(ST_TRACE, kind);
+            WF_TRY(wf_trace(r->scene, rb.ray_buf(), nullptr, rb.hit_buf(), n, rb.closest, ws, nullptr));
+            tend();
+        } else if (reorder) {
+            // reorder fused into the trace (its time counts as trace)
+            int o = 1 - rb.swap;
+            tbegin(ST_TRACE, kind);
+            WF_TRY(wf_reorder_trace(r->scene, rb.indices.as<uint32_t>(), rb.ray_buf(),
+                                    rb.rays[o].as<wf_ray>(), rb.s2i_buf(), rb.s2i[o].as<int32_t>(),
+                                    rb.i2s[o].as<int32_t>(), rb.hit_buf(), n, rb.closest, ws,
+                                    nullptr));
+            tend();
+            rb.swap = o;
+        } else {
+            tbegin(ST_TRACE, kind);
+            WF_TRY(wf_trace(r->scene, rb.ray_buf(), rb.indices.as<uint32_t>(), rb.hit_buf(), n,
+                            rb.closest, ws, nullptr));
+            tend();
         }
-        tbegin(ST_TRACE, kind);
-        WF_TRY(wf_trace(r->scene, rb.ray_buf(), reorder ? nullptr : rb.indices.as<uint32_t>(),
-                        rb.hit_buf(), n, rb.closest, ws, nullptr));
-        tend();
         if (kind == RK_PATH) r->sorted_rays += (uint64_t)n;
         return WF_OK;
     }
@@ -784,12 +801,15 @@ extern "C" int wf_trace_sort(wf_scene scene, wf_ray_buffer* rb, int key_mode, in
     if (reorder) {
         WF_CHECK_ARG(rb->rays[o] && rb->slot_to_index[cur] && rb->slot_to_index[o] && rb->index_to_slot[o],
                      "wf_trace_sort: reorder buffers missing");
-        WF_TRY(wf_reorder(rb->indices, rb->rays[cur], rb->rays[o], rb->slot_to_index[cur],
-                          rb->slot_to_index[o], rb->index_to_slot[o], n, stream, tp[2]));
+        // reorder and trace fused: the time is reported as trace, reorder 0
+        WF_TRY(wf_reorder_trace(scene, rb->indices, rb->rays[cur], rb->rays[o], rb->slot_to_index[cur],
+                                rb->slot_to_index[o], rb->index_to_slot[o], rb->results, n,
+                                rb->closest_hit, stream, tp[3]));
         rb->swap = o;
+    } else {
+        WF_TRY(wf_trace(scene, rb->rays[cur], rb->indices, rb->results, n, rb->closest_hit, stream,
+                        tp[3]));
     }
-    WF_TRY(wf_trace(scene, rb->rays[rb->swap ? 1 : 0], reorder ? nullptr : rb->indices, rb->results,
-                    n, rb->closest_hit, stream, tp[3]));
     if (times)
         for (int k = 0; k < 4; ++k) times[k] = t[k];
     return WF_OK;

@@ -20,21 +20,41 @@
 
 namespace wfpt {
 
+// REORDER: the reorder gather fused in front of the trace (Tracer::traceSort
+// with reorder, Tracer.cpp:209-225): ray_index is the sorted permutation,
+// out ray i = rays[ray_index[i]] is written with its LUT entries exactly as
+// reorder_kernel writes them, and traced from registers; its hit record
+// goes to the reordered slot i.
 template <bool CLOSEST, int BLOCK, bool LDS, bool STATS = false, bool WIDE = false,
-          int LOOP = TRACE_LOOP_DEFAULT>
+          int LOOP = TRACE_LOOP_DEFAULT, bool REORDER = false>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__ nodes,
                                                       const float4* __restrict__ tris,
                                                       const wf_ray* __restrict__ rays,
                                                       const uint32_t* __restrict__ ray_index,
                                                       wf_hit* __restrict__ hits, int64_t n,
                                                       uint32_t* __restrict__ stat_nodes = nullptr,
-                                                      uint32_t* __restrict__ stat_tris = nullptr) {
+                                                      uint32_t* __restrict__ stat_tris = nullptr,
+                                                      const int32_t* __restrict__ in_s2i = nullptr,
+                                                      wf_ray* __restrict__ out_rays = nullptr,
+                                                      int32_t* __restrict__ out_s2i = nullptr,
+                                                      int32_t* __restrict__ out_i2s = nullptr) {
     __shared__ int s_stack[LDS ? LDS_DEPTH * BLOCK : 1];
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
-    const int64_t r = ray_index ? (int64_t)ray_index[i] : i;
+    int64_t r;
     float4 ro, rd;
-    load_ray(rays, r, ro, rd);
+    if (REORDER) {
+        const int64_t src = ray_index[i];
+        load_ray(rays, src, ro, rd);
+        store_ray(out_rays, i, ro, rd);
+        const int32_t index = in_s2i[src];
+        out_s2i[i] = index;
+        out_i2s[index] = (int32_t)i;
+        r = i;
+    } else {
+        r = ray_index ? (int64_t)ray_index[i] : i;
+        load_ray(rays, r, ro, rd);
+    }
     WtRay wr;
     const bool live = (rd.w >= ro.w) && wt_setup(ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, wr);
     const TraceOut tr =
@@ -311,6 +331,51 @@ static int launch_trace_block(const Tuning& tu, const float4* nodes, const float
     }
 }
 
+template <bool CLOSEST, int BLOCK>
+static void launch_reorder_trace_block(int loop, const float4* nodes4, const float4* tris,
+                                       const uint32_t* idx, const wf_ray* in_rays,
+                                       const int32_t* in_s2i, wf_ray* out_rays, int32_t* out_s2i,
+                                       int32_t* out_i2s, wf_hit* hits, int64_t n, hipStream_t s) {
+    const unsigned g = grid_for(n, BLOCK);
+    if (loop == LOOP_FUSED)
+        trace_kernel<CLOSEST, BLOCK, true, false, true, LOOP_FUSED, true><<<g, BLOCK, 0, s>>>(
+            nodes4, tris, in_rays, idx, hits, n, nullptr, nullptr, in_s2i, out_rays, out_s2i, out_i2s);
+    else
+        trace_kernel<CLOSEST, BLOCK, true, false, true, LOOP_BRANCH, true><<<g, BLOCK, 0, s>>>(
+            nodes4, tris, in_rays, idx, hits, n, nullptr, nullptr, in_s2i, out_rays, out_s2i, out_i2s);
+}
+
+// The fused reorder + trace kernel serves the one-lane, LDS-stack, 4-wide
+// traversal (branch or single-load loop); other configurations run the
+// reorder kernel and then the trace.  Returns false when not fused.
+static bool launch_reorder_trace(wf_scene scene, const uint32_t* idx, const wf_ray* in_rays,
+                                 const int32_t* in_s2i, wf_ray* out_rays, int32_t* out_s2i,
+                                 int32_t* out_i2s, wf_hit* hits, int64_t n, int closest,
+                                 hipStream_t s) {
+    const Tuning& tu = tuning();
+    const bool grouped = n <= tu.trace_group64 || n <= tu.trace_group16 || n <= tu.trace_group4;
+    if (!tu.trace_fuse_reorder || scene->num_tris == 0 || tu.trace_width != 4 ||
+        tu.trace_persistent || !tu.trace_lds_stack ||
+        grouped || tu.trace_loop == LOOP_POSTPONED)
+        return false;
+    const float4 *nodes4 = scene->nodes4, *tris = scene->tris;
+    switch (tu.trace_block) {
+        case 64:
+            if (closest) launch_reorder_trace_block<true, 64>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s);
+            else launch_reorder_trace_block<false, 64>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s);
+            break;
+        case 128:
+            if (closest) launch_reorder_trace_block<true, 128>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s);
+            else launch_reorder_trace_block<false, 128>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s);
+            break;
+        default:
+            if (closest) launch_reorder_trace_block<true, 256>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s);
+            else launch_reorder_trace_block<false, 256>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s);
+            break;
+    }
+    return true;
+}
+
 int trace_launch(wf_scene scene, const wf_ray* rays, const uint32_t* ray_index, wf_hit* hits,
                  int64_t n, int closest, hipStream_t s) {
     if (n == 0) return WF_OK;
@@ -385,5 +450,30 @@ extern "C" int wf_trace(wf_scene scene, const wf_ray* rays, const uint32_t* ray_
     hipStream_t s = as_stream(stream);
     StageTimer t(s, ms);
     WF_TRY(trace_launch(scene, rays, ray_index, hits, n, closest, s));
+    return t.finish();
+}
+
+extern "C" int wf_reorder_trace(wf_scene scene, const uint32_t* idx, const wf_ray* in_rays,
+                                wf_ray* out_rays, const int32_t* in_s2i, int32_t* out_s2i,
+                                int32_t* out_i2s, wf_hit* hits, int64_t n, int closest,
+                                wf_stream_t stream, float* ms) {
+    WF_CHECK_ARG(scene, "wf_reorder_trace: null scene");
+    WF_CHECK_ARG(n >= 0, "wf_reorder_trace: n < 0");
+    if (n == 0) {
+        if (ms) *ms = 0.f;
+        return WF_OK;
+    }
+    WF_CHECK_ARG(idx && in_rays && out_rays && in_s2i && out_s2i && out_i2s && hits,
+                 "wf_reorder_trace: null buffer");
+    WF_CHECK_ARG((const void*)in_rays != (const void*)out_rays, "wf_reorder_trace: in_rays == out_rays");
+    hipStream_t s = as_stream(stream);
+    StageTimer t(s, ms);
+    if (launch_reorder_trace(scene, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n,
+                             closest, s)) {
+        WF_TRY(launch_check("trace_kernel<reorder>"));
+    } else {
+        WF_TRY(wf_reorder(idx, in_rays, out_rays, in_s2i, out_s2i, out_i2s, n, stream, nullptr));
+        WF_TRY(trace_launch(scene, out_rays, nullptr, hits, n, closest, s));
+    }
     return t.finish();
 }
