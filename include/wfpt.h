@@ -323,10 +323,12 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
  *                           read when a pass is enqueued or captured)
  *   "trace.fuse_reorder" 0/1 wf_reorder_trace as one fused kernel (1, default)
  *                           or as the reorder and trace kernels (0)
- *   "trace.loop"       0-2  4-wide traversal loop: 0 branch on node/leaf, then
+ *   "trace.loop"       0-3  4-wide traversal loop: 0 branch on node/leaf, then
  *                           load; 1 one load per step for node or leaf
  *                           (default); 2 leaves postponed until every lane of
- *                           the wave holds one (Aila & Laine 2009)
+ *                           the wave holds one (Aila & Laine 2009); 3 lanes at a
+ *                           leaf wait until "trace.leaf_gate" lanes (1-64,
+ *                           default 16) wait, then test their triangles
  * Returns WF_ERR_INVALID_ARG for an unknown key or value out of range. */
 int wf_set_tuning(const char* key, int value);
 int wf_get_tuning(const char* key, int* value);

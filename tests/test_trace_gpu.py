@@ -366,7 +366,8 @@ def test_group_traversal_bit_exact(dev, orc, group):
         _capi.call("wf_set_tuning", key, cur.value)
 
 
-@pytest.mark.parametrize("knobs", [{}, {"trace.loop": 0}, {"trace.loop": 2}, {"trace.block": 128},
+@pytest.mark.parametrize("knobs", [{}, {"trace.loop": 0}, {"trace.loop": 2}, {"trace.loop": 3},
+                                   {"trace.block": 128},
                                    {"trace.fuse_reorder": 0}, {"trace.width": 2}])
 def test_reorder_trace_fused_bit_exact(dev, knobs):
     """wf_reorder_trace writes exactly what wf_reorder followed by wf_trace on
@@ -410,6 +411,37 @@ def test_reorder_trace_fused_bit_exact(dev, knobs):
                 outs.append((out_rays, out_s2i, out_i2s, hits))
             for a, b in zip(*outs):
                 assert torch.equal(a, b), (n, closest)
+    finally:
+        for k, v in saved.items():
+            _capi.call("wf_set_tuning", k.encode(), v)
+
+
+@pytest.mark.parametrize("gate", [1, 16, 64])
+def test_gated_leaf_loop_bit_exact(dev, orc, gate):
+    """trace.loop 3 (lanes at a leaf wait for `gate` waiting lanes) returns the
+    oracle's closest-hit records and any-hit visibility."""
+    import ctypes as C
+
+    from vulkanwavefrontpathtracer_amd import _capi, stages
+
+    saved = {}
+    for k, v in {"trace.loop": 3, "trace.leaf_gate": gate}.items():
+        cur = C.c_int()
+        _capi.call("wf_get_tuning", k.encode(), C.byref(cur))
+        saved[k] = cur.value
+        _capi.call("wf_set_tuning", k.encode(), v)
+    try:
+        hs = stages.HostScene(stages.SCENE_INTERIOR, 1001)
+        sc = stages.Scene.from_host(hs, dev)
+        rays = scene_rays(hs, 70_001, 300 + gate, tmin=1e-4)
+        rays[::61, 7] = -1.0
+        tr = torch.from_numpy(rays).to(dev)
+        got = stages.trace(sc, tr).cpu().numpy()
+        sub = np.arange(0, len(rays), 23)
+        exp = expected_hits(orc, hs, rays[sub])
+        assert np.array_equal(got[sub], exp)
+        anyh = stages.trace(sc, tr, closest=False).cpu().numpy()
+        assert np.array_equal(anyh[:, 0] >= 0, got[:, 0] >= 0)
     finally:
         for k, v in saved.items():
             _capi.call("wf_set_tuning", k.encode(), v)

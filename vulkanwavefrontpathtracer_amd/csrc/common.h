@@ -149,6 +149,7 @@ struct Tuning {
     int trace_group4 = 64000;   // ... 4 lanes per ray
     int batch_key_layout = 0;   // batched queue key below the scene prefix: 0 concatenated, 1 5-D Morton
     int trace_fuse_reorder = 1;  // wf_reorder_trace: one fused kernel where available
+    int trace_leaf_gate = 16;   // trace.loop 3: waiting lanes that trigger the triangle test
     int trace_loop = 1;         // 4-wide loop: 0 branch then load, 1 one load per step, 2 postponed leaves
 };
 Tuning& tuning();

@@ -88,7 +88,8 @@ extern "C" int wf_set_tuning(const char* key, int value) {
     else if (k == "trace.persistent" && (value == 0 || value == 1)) t.trace_persistent = value;
     else if (k == "trace.refill" && value >= 1 && value <= 64) t.trace_refill = value;
     else if (k == "trace.width" && (value == 2 || value == 4)) t.trace_width = value;
-    else if (k == "trace.loop" && value >= 0 && value <= 2) t.trace_loop = value;
+    else if (k == "trace.loop" && value >= 0 && value <= 3) t.trace_loop = value;
+    else if (k == "trace.leaf_gate" && value >= 1 && value <= 64) t.trace_leaf_gate = value;
     else if (k == "trace.fuse_reorder" && (value == 0 || value == 1)) t.trace_fuse_reorder = value;
     else if (k == "batch.key_layout" && (value == 0 || value == 1)) t.batch_key_layout = value;
     else if (k == "trace.group64" && value >= 0) t.trace_group64 = value;
@@ -115,6 +116,7 @@ extern "C" int wf_get_tuning(const char* key, int* value) {
     else if (k == "trace.refill") *value = t.trace_refill;
     else if (k == "trace.width") *value = t.trace_width;
     else if (k == "trace.loop") *value = t.trace_loop;
+    else if (k == "trace.leaf_gate") *value = t.trace_leaf_gate;
     else if (k == "trace.fuse_reorder") *value = t.trace_fuse_reorder;
     else if (k == "batch.key_layout") *value = t.batch_key_layout;
     else if (k == "trace.group64") *value = t.trace_group64;

@@ -37,7 +37,8 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__
                                                       const int32_t* __restrict__ in_s2i = nullptr,
                                                       wf_ray* __restrict__ out_rays = nullptr,
                                                       int32_t* __restrict__ out_s2i = nullptr,
-                                                      int32_t* __restrict__ out_i2s = nullptr) {
+                                                      int32_t* __restrict__ out_i2s = nullptr,
+                                                      int leaf_gate = 1) {
     __shared__ int s_stack[LDS ? LDS_DEPTH * BLOCK : 1];
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
@@ -58,7 +59,8 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__
     WtRay wr;
     const bool live = (rd.w >= ro.w) && wt_setup(ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, wr);
     const TraceOut tr =
-        traverse<CLOSEST, BLOCK, LDS, STATS, WIDE, LOOP>(nodes, tris, ro, rd, wr, live, s_stack);
+        traverse<CLOSEST, BLOCK, LDS, STATS, WIDE, LOOP>(nodes, tris, ro, rd, wr, live, s_stack,
+                                                         leaf_gate);
     *reinterpret_cast<int4*>(hits + r) = hit_record(tris, wr, tr);
     if (STATS) {
         stat_nodes[r] = tr.nodes_visited;
@@ -297,7 +299,11 @@ template <bool CLOSEST, int BLOCK, bool LDS, bool WIDE>
 static void launch_trace_loop(int loop, unsigned g, const float4* nodes, const float4* tris,
                               const wf_ray* rays, const uint32_t* ray_index, wf_hit* hits,
                               int64_t n, hipStream_t s) {
-    if (WIDE && loop == LOOP_FUSED)
+    if (WIDE && loop == LOOP_GATED)
+        trace_kernel<CLOSEST, BLOCK, LDS, false, WIDE, LOOP_GATED><<<g, BLOCK, 0, s>>>(
+            nodes, tris, rays, ray_index, hits, n, nullptr, nullptr, nullptr, nullptr, nullptr,
+            nullptr, tuning().trace_leaf_gate);
+    else if (WIDE && loop == LOOP_FUSED)
         trace_kernel<CLOSEST, BLOCK, LDS, false, WIDE, LOOP_FUSED><<<g, BLOCK, 0, s>>>(nodes, tris, rays, ray_index, hits, n);
     else if (WIDE && loop == LOOP_POSTPONED)
         trace_kernel<CLOSEST, BLOCK, LDS, false, WIDE, LOOP_POSTPONED><<<g, BLOCK, 0, s>>>(nodes, tris, rays, ray_index, hits, n);
@@ -337,7 +343,11 @@ static void launch_reorder_trace_block(int loop, const float4* nodes4, const flo
                                        const int32_t* in_s2i, wf_ray* out_rays, int32_t* out_s2i,
                                        int32_t* out_i2s, wf_hit* hits, int64_t n, hipStream_t s) {
     const unsigned g = grid_for(n, BLOCK);
-    if (loop == LOOP_FUSED)
+    if (loop == LOOP_GATED)
+        trace_kernel<CLOSEST, BLOCK, true, false, true, LOOP_GATED, true><<<g, BLOCK, 0, s>>>(
+            nodes4, tris, in_rays, idx, hits, n, nullptr, nullptr, in_s2i, out_rays, out_s2i, out_i2s,
+            tuning().trace_leaf_gate);
+    else if (loop == LOOP_FUSED)
         trace_kernel<CLOSEST, BLOCK, true, false, true, LOOP_FUSED, true><<<g, BLOCK, 0, s>>>(
             nodes4, tris, in_rays, idx, hits, n, nullptr, nullptr, in_s2i, out_rays, out_s2i, out_i2s);
     else

@@ -366,8 +366,43 @@ __device__ __forceinline__ void trav4_postponed(TravState<LDS, STACK>& st,
     }
 }
 
+// 4-wide traversal with gated leaves: a lane whose next entry is a leaf
+// waits (it neither tests nor moves) while the wave steps its interior
+// nodes, and the wave runs the triangle test only once `gate` or more lanes
+// wait, or when no lane has an interior node to step.  With one-triangle
+// leaves some lane sits at a leaf on most steps, so without the gate the
+// triangle test is issued nearly every step for one or two lanes; a waiting
+// lane keeps its place, so no traversal is done before its closest hit is
+// known (unlike postponed leaves).  gate = 1 is the plain loop.
+template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, int STACK>
+__device__ __forceinline__ void trav4_gated(TravState<LDS, STACK>& st,
+                                            int (&spill)[TravState<LDS, STACK>::SPILL],
+                                            const float4* __restrict__ nodes4,
+                                            const float4* __restrict__ tris, const WtRay& wr,
+                                            int* s_stack, bool live, int gate) {
+    bool active = live;
+    while (true) {
+        const bool inner = active && st.node >= 0;
+        const uint64_t at_inner = ballot64(inner);
+        const uint64_t at_leaf = ballot64(active && st.node < 0);
+        if ((at_inner | at_leaf) == 0) break;
+        if (inner) active = trav_node4<BLOCK, LDS, STATS>(st, spill, nodes4, s_stack);
+        if (at_leaf != 0 && (__popcll(at_leaf) >= gate || at_inner == 0)) {
+            if (active && st.node < 0 && ((at_leaf >> lane_id()) & 1ull)) {
+                if (trav_leaf<CLOSEST, LDS, STACK, STATS>(st, st.node, tris, wr)) {
+                    active = false;
+                } else if (st.sp == 0) {
+                    active = false;
+                } else {
+                    st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
+                }
+            }
+        }
+    }
+}
+
 // Loop structures of the 4-wide traversal (the trace.loop knob).
-enum TraceLoop { LOOP_BRANCH = 0, LOOP_FUSED = 1, LOOP_POSTPONED = 2 };
+enum TraceLoop { LOOP_BRANCH = 0, LOOP_FUSED = 1, LOOP_POSTPONED = 2, LOOP_GATED = 3 };
 constexpr int TRACE_LOOP_DEFAULT = LOOP_FUSED;
 
 // s_stack: this block's LDS stack (LDS_DEPTH * BLOCK ints) when LDS is set.
@@ -377,12 +412,15 @@ template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, bool WIDE = false,
           int LOOP = TRACE_LOOP_DEFAULT>
 __device__ __forceinline__ TraceOut traverse(const float4* __restrict__ nodes,
                                              const float4* __restrict__ tris, float4 ro, float4 rd,
-                                             const WtRay& wr, bool live, int* s_stack) {
+                                             const WtRay& wr, bool live, int* s_stack,
+                                             int leaf_gate = 1) {
     constexpr int STACK = WIDE ? TRACE_STACK4 : TRACE_STACK;
     TravState<LDS, STACK> st;
     int spill[TravState<LDS, STACK>::SPILL];
     trav_begin(st, ro, rd);
-    if (WIDE && LOOP == LOOP_POSTPONED) {
+    if (WIDE && LOOP == LOOP_GATED) {
+        trav4_gated<CLOSEST, BLOCK, LDS, STATS>(st, spill, nodes, tris, wr, s_stack, live, leaf_gate);
+    } else if (WIDE && LOOP == LOOP_POSTPONED) {
         trav4_postponed<CLOSEST, BLOCK, LDS, STATS>(st, spill, nodes, tris, wr, s_stack, live);
     } else if (live) {
         if (WIDE && LOOP == LOOP_FUSED) {
