@@ -177,15 +177,16 @@ __device__ __forceinline__ bool trav_step(TravState<LDS, STACK>& st,
     return false;
 }
 
+// Compare-exchange of (distance, reference) pairs, written as selects so the
+// sorting network stays branch-free (one compare and four selects).
 __device__ __forceinline__ void cswap(float& ta, int& ra, float& tb, int& rb) {
-    if (tb < ta) {
-        const float t = ta;
-        ta = tb;
-        tb = t;
-        const int r = ra;
-        ra = rb;
-        rb = r;
-    }
+    const bool c = tb < ta;
+    const float lo = c ? tb : ta, hi = c ? ta : tb;
+    const int rl = c ? rb : ra, rh = c ? ra : rb;
+    ta = lo;
+    tb = hi;
+    ra = rl;
+    rb = rh;
 }
 
 // Interior node st.node of the 4-wide quantized hierarchy: up to four child
