@@ -329,6 +329,16 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
  *                           the wave holds one (Aila & Laine 2009); 3 lanes at a
  *                           leaf wait until "trace.leaf_gate" lanes (1-64,
  *                           default 16) wait, then test their triangles
+ *   "trace.compact"    0-64 tail hand-over of the loop 1 walk (wavefronts of at
+ *                           least "trace.compact_min" rays, default 262144):
+ *                           a wave with fewer walking lanes writes their
+ *                           traversal state to a per-scene queue and a tail
+ *                           pass finishes them ("trace.tail_refill" idle
+ *                           lanes, default 16, take more; queue of
+ *                           "trace.tail_cap" records at the scene's first
+ *                           use, default max(2^20, n/2)); same results; 0 off
+ *                           (default).  Traces of one scene on different
+ *                           streams must not overlap while it is on.
  * Returns WF_ERR_INVALID_ARG for an unknown key or value out of range. */
 int wf_set_tuning(const char* key, int value);
 int wf_get_tuning(const char* key, int* value);

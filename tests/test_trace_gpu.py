@@ -445,3 +445,82 @@ def test_gated_leaf_loop_bit_exact(dev, orc, gate):
     finally:
         for k, v in saved.items():
             _capi.call("wf_set_tuning", k.encode(), v)
+
+
+@pytest.mark.parametrize("knobs", [
+    {"trace.compact": 16, "trace.compact_min": 0},
+    {"trace.compact": 64, "trace.compact_min": 0},
+    {"trace.compact": 48, "trace.compact_min": 0, "trace.tail_refill": 1},
+    {"trace.compact": 64, "trace.compact_min": 0, "trace.tail_cap": 1000},
+    {"trace.compact": 32, "trace.compact_min": 0, "trace.block": 256},
+    {},
+])
+def test_tail_handover_bit_exact(dev, orc, knobs):
+    """Waves that hand their last rays to the tail pass (trace.compact) give
+    the same hit records as walks without hand-over, for direct, indirect and
+    fused reorder traces, closest and any hit (the tail pass continues the
+    same step sequence), including a queue too small for the reservations
+    (trace.tail_cap, voided records) and the default knobs at a wavefront
+    above trace.compact_min; sampled rays also equal the oracle."""
+    import ctypes as C
+
+    from vulkanwavefrontpathtracer_amd import _capi, stages
+
+    def get(k):
+        cur = C.c_int()
+        _capi.call("wf_get_tuning", k.encode(), C.byref(cur))
+        return cur.value
+
+    keys_all = set(knobs) | {"trace.compact"}
+    saved = {k: get(k) for k in keys_all}
+    on = {k: v for k, v in knobs.items()}
+    on.setdefault("trace.compact", saved["trace.compact"])
+
+    def apply(d):
+        for k, v in d.items():
+            _capi.call("wf_set_tuning", k.encode(), v)
+
+    try:
+        apply(on)  # the scene's queue is allocated with these knobs
+        hs = stages.HostScene(stages.SCENE_INTERIOR, 1001)
+        sc = stages.Scene.from_host(hs, dev)
+        mn, mx = sc.bounds()
+        sizes = ((300_000, 1),) if not knobs else ((70_001, 1), (150_000, 0))
+        for n, closest in sizes:
+            rays = torch.from_numpy(scene_rays(hs, n, 7 * n + closest, tmin=1e-4)).to(dev)
+            rays[::97, 7] = -1.0
+            perm = torch.randperm(n, device=dev).to(torch.int32)
+            keys, idx = stages.morton_keys(rays, stages.KEY_ORIGIN_DIR_MORTON60, mn, mx)
+            stages.sort_pairs(keys, idx, 60, 8)
+            s2i = torch.arange(n, dtype=torch.int32, device=dev)
+            res = []
+            for setting in (on, {"trace.compact": 0}):
+                apply(setting)
+                direct = torch.full((n, 4), -9, dtype=torch.int32, device=dev)
+                _capi.call("wf_trace", sc.handle, C.c_void_p(rays.data_ptr()), None,
+                           C.c_void_p(direct.data_ptr()), n, closest, None, None)
+                indirect = torch.full((n, 4), -9, dtype=torch.int32, device=dev)
+                _capi.call("wf_trace", sc.handle, C.c_void_p(rays.data_ptr()),
+                           C.c_void_p(perm.data_ptr()), C.c_void_p(indirect.data_ptr()), n,
+                           closest, None, None)
+                out_rays = torch.empty_like(rays)
+                out_s2i = torch.empty_like(s2i)
+                out_i2s = torch.empty(n, dtype=torch.int32, device=dev)
+                fused = torch.full((n, 4), -9, dtype=torch.int32, device=dev)
+                ptrs = [C.c_void_p(t.data_ptr()) for t in (idx, rays, out_rays, s2i, out_s2i, out_i2s)]
+                _capi.call("wf_reorder_trace", sc.handle, *ptrs, C.c_void_p(fused.data_ptr()), n,
+                           closest, None, None)
+                torch.cuda.synchronize()
+                res.append((direct, indirect, fused, out_rays))
+            for a, b in zip(*res):
+                assert torch.equal(a, b), (n, closest, knobs)
+            got = res[0][0].cpu().numpy()
+            sub = np.arange(0, n, 211)
+            exp = expected_hits(orc, hs, rays.cpu().numpy()[sub], closest=bool(closest))
+            if closest:
+                assert np.array_equal(got[sub], exp)
+            else:
+                assert np.array_equal(got[sub, 0] >= 0, exp[:, 0] >= 0)
+            apply(on)
+    finally:
+        apply(saved)

@@ -878,6 +878,8 @@ static void free_scene(wf_scene_s* s) {
     if (s->geom_first) (void)hipFree(s->geom_first);
     if (s->albedo) (void)hipFree(s->albedo);
     if (s->emission) (void)hipFree(s->emission);
+    if (s->tailq) (void)hipFree(s->tailq);
+    if (s->tailq_ctr) (void)hipFree(s->tailq_ctr);
     delete s;
 }
 

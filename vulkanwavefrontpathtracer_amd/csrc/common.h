@@ -151,6 +151,10 @@ struct Tuning {
     int trace_fuse_reorder = 1;  // wf_reorder_trace: one fused kernel where available
     int trace_leaf_gate = 16;   // trace.loop 3: waiting lanes that trigger the triangle test
     int trace_loop = 1;         // 4-wide loop: 0 branch then load, 1 one load per step, 2 postponed leaves
+    int trace_compact = 0;      // waves with fewer active lanes hand their rays to the tail pass (0 off)
+    int trace_compact_min = 262144;  // smallest wavefront traced with hand-over
+    int trace_tail_refill = 16;  // tail pass: idle lanes that trigger taking more handed-over rays
+    int trace_tail_cap = 0;     // tail queue records at a scene's first use (0: max(2^20, n/2))
 };
 Tuning& tuning();
 }  // namespace wfpt

@@ -90,6 +90,10 @@ extern "C" int wf_set_tuning(const char* key, int value) {
     else if (k == "trace.width" && (value == 2 || value == 4)) t.trace_width = value;
     else if (k == "trace.loop" && value >= 0 && value <= 3) t.trace_loop = value;
     else if (k == "trace.leaf_gate" && value >= 1 && value <= 64) t.trace_leaf_gate = value;
+    else if (k == "trace.compact" && value >= 0 && value <= 64) t.trace_compact = value;
+    else if (k == "trace.compact_min" && value >= 0) t.trace_compact_min = value;
+    else if (k == "trace.tail_refill" && value >= 1 && value <= 64) t.trace_tail_refill = value;
+    else if (k == "trace.tail_cap" && value >= 0) t.trace_tail_cap = value;
     else if (k == "trace.fuse_reorder" && (value == 0 || value == 1)) t.trace_fuse_reorder = value;
     else if (k == "batch.key_layout" && (value == 0 || value == 1)) t.batch_key_layout = value;
     else if (k == "trace.group64" && value >= 0) t.trace_group64 = value;
@@ -117,6 +121,10 @@ extern "C" int wf_get_tuning(const char* key, int* value) {
     else if (k == "trace.width") *value = t.trace_width;
     else if (k == "trace.loop") *value = t.trace_loop;
     else if (k == "trace.leaf_gate") *value = t.trace_leaf_gate;
+    else if (k == "trace.compact") *value = t.trace_compact;
+    else if (k == "trace.compact_min") *value = t.trace_compact_min;
+    else if (k == "trace.tail_refill") *value = t.trace_tail_refill;
+    else if (k == "trace.tail_cap") *value = t.trace_tail_cap;
     else if (k == "trace.fuse_reorder") *value = t.trace_fuse_reorder;
     else if (k == "batch.key_layout") *value = t.batch_key_layout;
     else if (k == "trace.group64") *value = t.trace_group64;

@@ -46,6 +46,11 @@ struct wf_scene_s {
     float* albedo = nullptr;
     float* emission = nullptr;
     int device = 0;
+    // Tail queue of the compacting trace (rays handed over by waves with few
+    // lanes left; allocated on first use, stream-ordered with the traces).
+    int4* tailq = nullptr;
+    uint32_t* tailq_ctr = nullptr;
+    int64_t tailq_cap = 0;
 };
 
 namespace wfpt {

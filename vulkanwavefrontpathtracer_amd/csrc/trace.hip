@@ -17,6 +17,7 @@
 #include "traverse.h"
 
 #include <algorithm>
+#include <mutex>
 
 namespace wfpt {
 
@@ -25,8 +26,11 @@ namespace wfpt {
 // out ray i = rays[ray_index[i]] is written with its LUT entries exactly as
 // reorder_kernel writes them, and traced from registers; its hit record
 // goes to the reordered slot i.
+// HANDOVER: the 4-wide single-load walk that hands the last rays of a wave
+// to the tail queue `tq` (traverse4_handover); trace_tail_kernel finishes
+// them.
 template <bool CLOSEST, int BLOCK, bool LDS, bool STATS = false, bool WIDE = false,
-          int LOOP = TRACE_LOOP_DEFAULT, bool REORDER = false>
+          int LOOP = TRACE_LOOP_DEFAULT, bool REORDER = false, bool HANDOVER = false>
 __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__ nodes,
                                                       const float4* __restrict__ tris,
                                                       const wf_ray* __restrict__ rays,
@@ -38,7 +42,9 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__
                                                       wf_ray* __restrict__ out_rays = nullptr,
                                                       int32_t* __restrict__ out_s2i = nullptr,
                                                       int32_t* __restrict__ out_i2s = nullptr,
-                                                      int leaf_gate = 1) {
+                                                      int leaf_gate = 1,
+                                                      TailQueue tq = TailQueue()) {
+    static_assert(!HANDOVER || (LDS && WIDE && !STATS), "hand-over needs the 4-wide LDS walk");
     __shared__ int s_stack[LDS ? LDS_DEPTH * BLOCK : 1];
     int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
     if (i >= n) return;
@@ -58,6 +64,16 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__
     }
     WtRay wr;
     const bool live = (rd.w >= ro.w) && wt_setup(ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, wr);
+    if constexpr (HANDOVER) {
+        TravState<true, TRACE_STACK4> st;
+        int spill[TravState<true, TRACE_STACK4>::SPILL];
+        trav_begin(st, ro, rd);
+        if (traverse4_handover<CLOSEST, BLOCK>(st, spill, nodes, tris, wr, live, s_stack, r, tq))
+            return;
+        *reinterpret_cast<int4*>(hits + r) =
+            hit_record(tris, wr, TraceOut{st.best_slot, st.best_t, 0u, 0u});
+        return;
+    }
     const TraceOut tr =
         traverse<CLOSEST, BLOCK, LDS, STATS, WIDE, LOOP>(nodes, tris, ro, rd, wr, live, s_stack,
                                                          leaf_gate);
@@ -65,6 +81,83 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__
     if (STATS) {
         stat_nodes[r] = tr.nodes_visited;
         stat_tris[r] = tr.tris_tested;
+    }
+}
+
+// Tail pass of the hand-over: a grid of resident waves finishes the rays in
+// the tail queue.  Records are dealt statically (wave w of W takes the
+// 64-record chunks w, w + W, ...) and a wave takes the next records of its
+// sequence whenever `refill` or more of its lanes are idle.  A record
+// restores the ray's traversal state (node, LDS stack, closest hit so far)
+// and the walk continues with the same steps as in trace_kernel; the hit
+// record goes to the slot the record names (rays[slot], hits[slot]).
+// Voided records (slot -1, from a reservation that overran the queue) are
+// skipped.
+template <bool CLOSEST>
+__global__ __launch_bounds__(64) void trace_tail_kernel(const float4* __restrict__ nodes4,
+                                                        const float4* __restrict__ tris,
+                                                        const wf_ray* __restrict__ rays,
+                                                        wf_hit* __restrict__ hits,
+                                                        const int4* __restrict__ q,
+                                                        const uint32_t* __restrict__ ctr,
+                                                        int64_t cap, int refill) {
+    __shared__ int s_stack[LDS_DEPTH * 64];
+    constexpr int STACK = TRACE_STACK4;
+    const int64_t count = std::min<int64_t>((int64_t)ctr[0], cap);
+    const int64_t num_waves = gridDim.x, wave = blockIdx.x;
+    auto rec_of = [&](int64_t j) { return (((j >> 6) * num_waves + wave) << 6) + (j & 63); };
+    if (rec_of(0) >= count) return;
+    TravState<true, STACK> st;
+    int spill[TravState<true, STACK>::SPILL];
+    WtRay wr;
+    int64_t r = 0, next = 0;
+    bool active = false, exhausted = false;
+    lds_int* col = (lds_int*)s_stack + threadIdx.x;
+    while (true) {
+        const uint64_t idle = ballot64(!active);
+        const int nidle = __popcll(idle);
+        if (!exhausted && (nidle >= refill || nidle == 64)) {
+            if (!active) {
+                const int64_t k = rec_of(next + popc_below(idle));
+                if (k < count) {
+                    const int4* rec = q + k * TAIL_INT4;
+                    const int4 h0 = rec[0];
+                    if (h0.x >= 0) {
+                        r = h0.x;
+                        float4 ro, rd;
+                        load_ray(rays, r, ro, rd);
+                        (void)wt_setup(ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, wr);
+                        trav_begin(st, ro, rd);
+                        const int4 h1 = rec[1];
+                        st.node = h0.y;
+                        st.sp = h0.z;
+                        st.best_slot = h0.w;
+                        st.best_orig = h1.x;
+                        st.best_t = __int_as_float(h1.y);
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            const int4 v = rec[2 + e];
+                            col[(4 * e) * 64] = v.x;
+                            col[(4 * e + 1) * 64] = v.y;
+                            col[(4 * e + 2) * 64] = v.z;
+                            col[(4 * e + 3) * 64] = v.w;
+                        }
+                        active = true;
+                    }
+                }
+            }
+            next += nidle;
+            exhausted = rec_of(next) >= count;
+        }
+        if (ballot64(active) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (active && trav_step4_fused<CLOSEST, 64, true, false>(st, spill, nodes4, tris, wr, s_stack)) {
+            *reinterpret_cast<int4*>(hits + r) =
+                hit_record(tris, wr, TraceOut{st.best_slot, st.best_t, 0u, 0u});
+            active = false;
+        }
     }
 }
 
@@ -298,7 +391,15 @@ static int launch_persistent(const float4* nodes, const float4* tris, const wf_r
 template <bool CLOSEST, int BLOCK, bool LDS, bool WIDE>
 static void launch_trace_loop(int loop, unsigned g, const float4* nodes, const float4* tris,
                               const wf_ray* rays, const uint32_t* ray_index, wf_hit* hits,
-                              int64_t n, hipStream_t s) {
+                              int64_t n, hipStream_t s, const TailQueue& tq) {
+    if constexpr (WIDE && LDS) {
+        if (loop == LOOP_FUSED && tq.q) {
+            trace_kernel<CLOSEST, BLOCK, true, false, true, LOOP_FUSED, false, true><<<g, BLOCK, 0, s>>>(
+                nodes, tris, rays, ray_index, hits, n, nullptr, nullptr, nullptr, nullptr, nullptr,
+                nullptr, 1, tq);
+            return;
+        }
+    }
     if (WIDE && loop == LOOP_GATED)
         trace_kernel<CLOSEST, BLOCK, LDS, false, WIDE, LOOP_GATED><<<g, BLOCK, 0, s>>>(
             nodes, tris, rays, ray_index, hits, n, nullptr, nullptr, nullptr, nullptr, nullptr,
@@ -314,36 +415,104 @@ static void launch_trace_loop(int loop, unsigned g, const float4* nodes, const f
 template <bool CLOSEST, int BLOCK, bool WIDE>
 static int launch_trace(bool lds, bool persistent, int loop, int refill, const float4* nodes,
                         const float4* tris, const wf_ray* rays, const uint32_t* ray_index,
-                        wf_hit* hits, int64_t n, hipStream_t s) {
+                        wf_hit* hits, int64_t n, hipStream_t s, const TailQueue& tq) {
     if (persistent)
         return lds ? launch_persistent<CLOSEST, BLOCK, true, WIDE>(nodes, tris, rays, ray_index, hits, n, refill, s)
                    : launch_persistent<CLOSEST, BLOCK, false, WIDE>(nodes, tris, rays, ray_index, hits, n, refill, s);
     const unsigned g = grid_for(n, BLOCK);
-    if (lds) launch_trace_loop<CLOSEST, BLOCK, true, WIDE>(loop, g, nodes, tris, rays, ray_index, hits, n, s);
-    else launch_trace_loop<CLOSEST, BLOCK, false, WIDE>(loop, g, nodes, tris, rays, ray_index, hits, n, s);
+    if (lds) launch_trace_loop<CLOSEST, BLOCK, true, WIDE>(loop, g, nodes, tris, rays, ray_index, hits, n, s, tq);
+    else launch_trace_loop<CLOSEST, BLOCK, false, WIDE>(loop, g, nodes, tris, rays, ray_index, hits, n, s, tq);
     return WF_OK;
 }
 
 template <bool CLOSEST, bool WIDE>
 static int launch_trace_block(const Tuning& tu, const float4* nodes, const float4* tris,
                               const wf_ray* rays, const uint32_t* ray_index, wf_hit* hits,
-                              int64_t n, hipStream_t s) {
+                              int64_t n, hipStream_t s, const TailQueue& tq) {
     const bool lds = tu.trace_lds_stack != 0, pers = tu.trace_persistent != 0;
     const int loop = tu.trace_loop;
     switch (tu.trace_block) {
-        case 64: return launch_trace<CLOSEST, 64, WIDE>(lds, pers, loop, tu.trace_refill, nodes, tris, rays, ray_index, hits, n, s);
-        case 128: return launch_trace<CLOSEST, 128, WIDE>(lds, pers, loop, tu.trace_refill, nodes, tris, rays, ray_index, hits, n, s);
-        default: return launch_trace<CLOSEST, 256, WIDE>(lds, pers, loop, tu.trace_refill, nodes, tris, rays, ray_index, hits, n, s);
+        case 64: return launch_trace<CLOSEST, 64, WIDE>(lds, pers, loop, tu.trace_refill, nodes, tris, rays, ray_index, hits, n, s, tq);
+        case 128: return launch_trace<CLOSEST, 128, WIDE>(lds, pers, loop, tu.trace_refill, nodes, tris, rays, ray_index, hits, n, s, tq);
+        default: return launch_trace<CLOSEST, 256, WIDE>(lds, pers, loop, tu.trace_refill, nodes, tris, rays, ray_index, hits, n, s, tq);
     }
+}
+
+// Tail queue of a trace of n rays on `scene`: set (q non-null) when the
+// hand-over is on and applies (4-wide single-load LDS walk, n at least
+// trace.compact_min), with its counter cleared on stream s.  The queue is
+// allocated on first use (trace.tail_cap records of 96 bytes, by default
+// max(2^20, n/2), kept with the scene); a first use under stream capture
+// walks without hand-over instead of allocating.
+static std::mutex g_tailq_mutex;
+
+static int tail_queue(wf_scene sc, int64_t n, hipStream_t s, TailQueue& tq) {
+    const Tuning& tu = tuning();
+    tq = TailQueue();
+    if (tu.trace_compact <= 0 || n < tu.trace_compact_min || n > INT32_MAX || tu.trace_width != 4 ||
+        !tu.trace_lds_stack || tu.trace_persistent || tu.trace_loop != LOOP_FUSED)
+        return WF_OK;
+    {
+        std::lock_guard<std::mutex> lock(g_tailq_mutex);
+        if (!sc->tailq) {
+            hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+            WF_HIP(hipStreamIsCapturing(s, &cs));
+            if (cs != hipStreamCaptureStatusNone) return WF_OK;
+            const int64_t cap = tu.trace_tail_cap > 0 ? (int64_t)tu.trace_tail_cap
+                                                      : std::max<int64_t>(int64_t(1) << 20, n / 2);
+            int4* q = nullptr;
+            uint32_t* c = nullptr;
+            if (hipMalloc((void**)&q, sizeof(int4) * TAIL_INT4 * cap) != hipSuccess)
+                return fail(WF_ERR_OUT_OF_MEMORY, "trace: tail queue allocation failed");
+            if (hipMalloc((void**)&c, 4 * sizeof(uint32_t)) != hipSuccess) {
+                (void)hipFree(q);
+                return fail(WF_ERR_OUT_OF_MEMORY, "trace: tail queue allocation failed");
+            }
+            sc->tailq = q;
+            sc->tailq_ctr = c;
+            sc->tailq_cap = cap;
+        }
+    }
+    WF_TRY(fill_u32(sc->tailq_ctr, 1, 0u, s));
+    tq.q = sc->tailq;
+    tq.ctr = sc->tailq_ctr;
+    tq.cap = sc->tailq_cap;
+    tq.compact = tu.trace_compact;
+    return WF_OK;
+}
+
+// The tail pass over the queue filled by the preceding hand-over trace.
+template <bool CLOSEST>
+static int launch_tail(const float4* nodes4, const float4* tris, const wf_ray* rays, wf_hit* hits,
+                       const TailQueue& tq, hipStream_t s) {
+    static int resident[TRACE_MAX_DEVICES] = {};
+    int dev = 0;
+    WF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= TRACE_MAX_DEVICES) return fail(WF_ERR_HIP, "trace: device index out of range");
+    if (resident[dev] == 0) {
+        int per_cu = 0, cus = 0;
+        WF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, trace_tail_kernel<CLOSEST>, 64, 0));
+        WF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        resident[dev] = std::max(1, per_cu) * std::max(1, cus);
+    }
+    const int64_t waves = std::min<int64_t>(resident[dev], (tq.cap + 63) / 64);
+    trace_tail_kernel<CLOSEST><<<(unsigned)waves, 64, 0, s>>>(nodes4, tris, rays, hits, tq.q, tq.ctr,
+                                                              tq.cap, tuning().trace_tail_refill);
+    return launch_check("trace_tail_kernel");
 }
 
 template <bool CLOSEST, int BLOCK>
 static void launch_reorder_trace_block(int loop, const float4* nodes4, const float4* tris,
                                        const uint32_t* idx, const wf_ray* in_rays,
                                        const int32_t* in_s2i, wf_ray* out_rays, int32_t* out_s2i,
-                                       int32_t* out_i2s, wf_hit* hits, int64_t n, hipStream_t s) {
+                                       int32_t* out_i2s, wf_hit* hits, int64_t n, hipStream_t s,
+                                       const TailQueue& tq) {
     const unsigned g = grid_for(n, BLOCK);
-    if (loop == LOOP_GATED)
+    if (loop == LOOP_FUSED && tq.q)
+        trace_kernel<CLOSEST, BLOCK, true, false, true, LOOP_FUSED, true, true><<<g, BLOCK, 0, s>>>(
+            nodes4, tris, in_rays, idx, hits, n, nullptr, nullptr, in_s2i, out_rays, out_s2i, out_i2s,
+            1, tq);
+    else if (loop == LOOP_GATED)
         trace_kernel<CLOSEST, BLOCK, true, false, true, LOOP_GATED, true><<<g, BLOCK, 0, s>>>(
             nodes4, tris, in_rays, idx, hits, n, nullptr, nullptr, in_s2i, out_rays, out_s2i, out_i2s,
             tuning().trace_leaf_gate);
@@ -357,33 +526,43 @@ static void launch_reorder_trace_block(int loop, const float4* nodes4, const flo
 
 // The fused reorder + trace kernel serves the one-lane, LDS-stack, 4-wide
 // traversal (branch or single-load loop); other configurations run the
-// reorder kernel and then the trace.  Returns false when not fused.
-static bool launch_reorder_trace(wf_scene scene, const uint32_t* idx, const wf_ray* in_rays,
-                                 const int32_t* in_s2i, wf_ray* out_rays, int32_t* out_s2i,
-                                 int32_t* out_i2s, wf_hit* hits, int64_t n, int closest,
-                                 hipStream_t s) {
+// reorder kernel and then the trace.  *fused is false when not fused.
+static int launch_reorder_trace(wf_scene scene, const uint32_t* idx, const wf_ray* in_rays,
+                                const int32_t* in_s2i, wf_ray* out_rays, int32_t* out_s2i,
+                                int32_t* out_i2s, wf_hit* hits, int64_t n, int closest,
+                                hipStream_t s, bool* fused) {
     const Tuning& tu = tuning();
     const bool grouped = n <= tu.trace_group64 || n <= tu.trace_group16 || n <= tu.trace_group4;
+    *fused = false;
     if (!tu.trace_fuse_reorder || scene->num_tris == 0 || tu.trace_width != 4 ||
         tu.trace_persistent || !tu.trace_lds_stack ||
         grouped || tu.trace_loop == LOOP_POSTPONED)
-        return false;
+        return WF_OK;
+    *fused = true;
+    TailQueue tq;
+    WF_TRY(tail_queue(scene, n, s, tq));
     const float4 *nodes4 = scene->nodes4, *tris = scene->tris;
     switch (tu.trace_block) {
         case 64:
-            if (closest) launch_reorder_trace_block<true, 64>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s);
-            else launch_reorder_trace_block<false, 64>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s);
+            if (closest) launch_reorder_trace_block<true, 64>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s, tq);
+            else launch_reorder_trace_block<false, 64>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s, tq);
             break;
         case 128:
-            if (closest) launch_reorder_trace_block<true, 128>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s);
-            else launch_reorder_trace_block<false, 128>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s);
+            if (closest) launch_reorder_trace_block<true, 128>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s, tq);
+            else launch_reorder_trace_block<false, 128>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s, tq);
             break;
         default:
-            if (closest) launch_reorder_trace_block<true, 256>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s);
-            else launch_reorder_trace_block<false, 256>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s);
+            if (closest) launch_reorder_trace_block<true, 256>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s, tq);
+            else launch_reorder_trace_block<false, 256>(tu.trace_loop, nodes4, tris, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n, s, tq);
             break;
     }
-    return true;
+    WF_TRY(launch_check("trace_kernel<reorder>"));
+    if (tq.q) {
+        // the handed-over rays were written to out_rays[i] before their walk
+        if (closest) WF_TRY(launch_tail<true>(nodes4, tris, out_rays, hits, tq, s));
+        else WF_TRY(launch_tail<false>(nodes4, tris, out_rays, hits, tq, s));
+    }
+    return WF_OK;
 }
 
 int trace_launch(wf_scene scene, const wf_ray* rays, const uint32_t* ray_index, wf_hit* hits,
@@ -398,18 +577,26 @@ int trace_launch(wf_scene scene, const wf_ray* rays, const uint32_t* ray_index, 
         (closest ? launch_group<true>(tu, scene->nodes4, scene->tris, rays, ray_index, hits, n, s)
                  : launch_group<false>(tu, scene->nodes4, scene->tris, rays, ray_index, hits, n, s)))
         return launch_check("trace_group_kernel");
+    TailQueue tq;
+    WF_TRY(tail_queue(scene, n, s, tq));
     if (tu.trace_width == 4) {
         if (closest)
-            WF_TRY((launch_trace_block<true, true>(tu, scene->nodes4, scene->tris, rays, ray_index, hits, n, s)));
+            WF_TRY((launch_trace_block<true, true>(tu, scene->nodes4, scene->tris, rays, ray_index, hits, n, s, tq)));
         else
-            WF_TRY((launch_trace_block<false, true>(tu, scene->nodes4, scene->tris, rays, ray_index, hits, n, s)));
+            WF_TRY((launch_trace_block<false, true>(tu, scene->nodes4, scene->tris, rays, ray_index, hits, n, s, tq)));
     } else {
         if (closest)
-            WF_TRY((launch_trace_block<true, false>(tu, scene->nodes, scene->tris, rays, ray_index, hits, n, s)));
+            WF_TRY((launch_trace_block<true, false>(tu, scene->nodes, scene->tris, rays, ray_index, hits, n, s, tq)));
         else
-            WF_TRY((launch_trace_block<false, false>(tu, scene->nodes, scene->tris, rays, ray_index, hits, n, s)));
+            WF_TRY((launch_trace_block<false, false>(tu, scene->nodes, scene->tris, rays, ray_index, hits, n, s, tq)));
     }
-    return launch_check("trace_kernel");
+    WF_TRY(launch_check("trace_kernel"));
+    if (tq.q) {
+        // handed-over records name the slot r of rays[r] / hits[r]
+        if (closest) WF_TRY(launch_tail<true>(scene->nodes4, scene->tris, rays, hits, tq, s));
+        else WF_TRY(launch_tail<false>(scene->nodes4, scene->tris, rays, hits, tq, s));
+    }
+    return WF_OK;
 }
 
 }  // namespace wfpt
@@ -478,10 +665,10 @@ extern "C" int wf_reorder_trace(wf_scene scene, const uint32_t* idx, const wf_ra
     WF_CHECK_ARG((const void*)in_rays != (const void*)out_rays, "wf_reorder_trace: in_rays == out_rays");
     hipStream_t s = as_stream(stream);
     StageTimer t(s, ms);
-    if (launch_reorder_trace(scene, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n,
-                             closest, s)) {
-        WF_TRY(launch_check("trace_kernel<reorder>"));
-    } else {
+    bool fused = false;
+    WF_TRY(launch_reorder_trace(scene, idx, in_rays, in_s2i, out_rays, out_s2i, out_i2s, hits, n,
+                                closest, s, &fused));
+    if (!fused) {
         WF_TRY(wf_reorder(idx, in_rays, out_rays, in_s2i, out_s2i, out_i2s, n, stream, nullptr));
         WF_TRY(trace_launch(scene, out_rays, nullptr, hits, n, closest, s));
     }

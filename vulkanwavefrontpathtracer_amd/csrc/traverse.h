@@ -438,6 +438,74 @@ __device__ __forceinline__ TraceOut traverse(const float4* __restrict__ nodes,
     return TraceOut{st.best_slot, st.best_t, st.nodes_visited, st.tris_tested};
 }
 
+// Tail hand-over.  A wave holds its SIMD slot until its slowest ray is done,
+// and on sorted path-ray wavefronts the last ~15 of 64 rays take about a
+// quarter of the wave's steps.  The compacting loop below therefore stops a
+// wave once fewer than `compact` of its lanes are still walking, writes each
+// walking ray's whole traversal state to the tail queue, and lets a tail
+// pass (trace_tail_kernel) finish those rays in full waves.  The state is
+// the ray slot, the current node, the stack (at most LDS_DEPTH entries, all
+// in LDS; a wave with a deeper stack keeps walking) and the closest hit so
+// far, so the tail pass continues exactly the same step sequence and the
+// results do not change.  Record: 6 int4 = {slot, node, sp, best slot},
+// {best original index, best t, 0, 0}, then the LDS_DEPTH stack entries.
+constexpr int TAIL_INT4 = 6;
+static_assert(LDS_DEPTH == 16, "tail record holds 16 stack entries");
+
+struct TailQueue {
+    int4* q = nullptr;          // cap records
+    uint32_t* ctr = nullptr;    // [0]: records reserved (may exceed cap)
+    int64_t cap = 0;
+    int compact = 0;            // hand over below this many walking lanes
+};
+
+// Closest-hit or any-hit walk of the 4-wide hierarchy with the single-load
+// loop, handing the wave's walking rays over to `tq` once fewer than
+// tq.compact lanes walk (one attempt per wave).  Returns true when this
+// lane's ray was handed over (its record is then written by the tail pass).
+template <bool CLOSEST, int BLOCK, int STACK>
+__device__ __forceinline__ bool traverse4_handover(TravState<true, STACK>& st,
+                                                   int (&spill)[TravState<true, STACK>::SPILL],
+                                                   const float4* __restrict__ nodes4,
+                                                   const float4* __restrict__ tris,
+                                                   const WtRay& wr, bool live, int* s_stack,
+                                                   int64_t slot, const TailQueue& tq) {
+    bool active = live;
+    bool tried = false;
+    while (true) {
+        const uint64_t act = ballot64(active);
+        if (act == 0) return false;
+        const int walking = __popcll(act);
+        if (!tried && walking < tq.compact) {
+            tried = true;
+            if (__all(!active || st.sp <= LDS_DEPTH)) {
+                const int leader = __ffsll((unsigned long long)act) - 1;
+                uint32_t base = 0;
+                if ((int)lane_id() == leader) base = atomicAdd(tq.ctr, (uint32_t)walking);
+                base = __shfl(base, leader, 64);
+                const int64_t rec = (int64_t)base + popc_below(act);
+                if ((int64_t)base + walking <= tq.cap) {
+                    if (!active) return false;
+                    const lds_int* col = (const lds_int*)s_stack + threadIdx.x;
+                    int4* o = tq.q + rec * TAIL_INT4;
+                    o[0] = make_int4((int)slot, st.node, st.sp, st.best_slot);
+                    o[1] = make_int4(st.best_orig, __float_as_int(st.best_t), 0, 0);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        o[2 + k] = make_int4(col[(4 * k) * BLOCK], col[(4 * k + 1) * BLOCK],
+                                             col[(4 * k + 2) * BLOCK], col[(4 * k + 3) * BLOCK]);
+                    return true;
+                }
+                // The queue is full: void the reserved records below the
+                // capacity and keep walking.
+                if (active && rec < tq.cap) tq.q[rec * TAIL_INT4] = make_int4(-1, 0, 0, 0);
+            }
+        }
+        if (active)
+            active = !trav_step4_fused<CLOSEST, BLOCK, true, false>(st, spill, nodes4, tris, wr, s_stack);
+    }
+}
+
 // Hit record {prim in geometry, geometry, t, packUnorm2x16(bary)} of an
 // accepted slot (trace.comp:70-76), or the miss record {-1, -1, -1, 0}
 // (trace.comp:78-81).  *orig_tri receives the original triangle index.
