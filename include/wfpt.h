@@ -339,6 +339,9 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
  *                           use, default max(2^20, n/2)); same results; 0 off
  *                           (default).  Traces of one scene on different
  *                           streams must not overlap while it is on.
+ *   "trace.xcd"        >= 0 loop 1 walk: the 8 XCDs take runs of this many
+ *                           consecutive workgroups instead of one each
+ *                           (0 off, default)
  * Returns WF_ERR_INVALID_ARG for an unknown key or value out of range. */
 int wf_set_tuning(const char* key, int value);
 int wf_get_tuning(const char* key, int* value);

@@ -367,7 +367,7 @@ def test_group_traversal_bit_exact(dev, orc, group):
 
 
 @pytest.mark.parametrize("knobs", [{}, {"trace.loop": 0}, {"trace.loop": 2}, {"trace.loop": 3},
-                                   {"trace.block": 128},
+                                   {"trace.block": 128}, {"trace.xcd": 8},
                                    {"trace.fuse_reorder": 0}, {"trace.width": 2}])
 def test_reorder_trace_fused_bit_exact(dev, knobs):
     """wf_reorder_trace writes exactly what wf_reorder followed by wf_trace on
@@ -524,3 +524,45 @@ def test_tail_handover_bit_exact(dev, orc, knobs):
             apply(on)
     finally:
         apply(saved)
+
+
+@pytest.mark.parametrize("n,k", [(64 * 8 * 500 + 3 * 64 + 5, 1), (64 * 8 * 500 + 3 * 64 + 5, 16),
+                                 (300_000, 4), (300_000, 4096)])
+def test_xcd_block_order_bit_exact(dev, n, k):
+    """The XCD run order of workgroups (trace.xcd = k) traces every ray exactly
+    once: direct and fused reorder traces equal the default order, for grids
+    with and without a partial last round of 8k workgroups."""
+    import ctypes as C
+
+    from vulkanwavefrontpathtracer_amd import _capi, stages
+
+    cur = C.c_int()
+    _capi.call("wf_get_tuning", b"trace.xcd", C.byref(cur))
+    try:
+        hs = stages.HostScene(stages.SCENE_INTERIOR, 1001)
+        sc = stages.Scene.from_host(hs, dev)
+        mn, mx = sc.bounds()
+        rays = torch.from_numpy(scene_rays(hs, n, n, tmin=1e-4)).to(dev)
+        keys, idx = stages.morton_keys(rays, stages.KEY_ORIGIN_DIR_MORTON60, mn, mx)
+        stages.sort_pairs(keys, idx, 60, 8)
+        s2i = torch.arange(n, dtype=torch.int32, device=dev)
+        res = []
+        for xcd in (0, k):
+            _capi.call("wf_set_tuning", b"trace.xcd", xcd)
+            direct = torch.full((n, 4), -9, dtype=torch.int32, device=dev)
+            _capi.call("wf_trace", sc.handle, C.c_void_p(rays.data_ptr()), None,
+                       C.c_void_p(direct.data_ptr()), n, 1, None, None)
+            out_rays = torch.full_like(rays, 7.0)
+            out_s2i = torch.full_like(s2i, -5)
+            out_i2s = torch.full((n,), -3, dtype=torch.int32, device=dev)
+            fused = torch.full((n, 4), -9, dtype=torch.int32, device=dev)
+            ptrs = [C.c_void_p(t.data_ptr()) for t in (idx, rays, out_rays, s2i, out_s2i, out_i2s)]
+            _capi.call("wf_reorder_trace", sc.handle, *ptrs, C.c_void_p(fused.data_ptr()), n, 1,
+                       None, None)
+            torch.cuda.synchronize()
+            res.append((direct, fused, out_rays, out_s2i, out_i2s))
+        for a, b in zip(*res):
+            assert torch.equal(a, b)
+        assert int((res[1][0][:, 2] == -9).sum()) == 0
+    finally:
+        _capi.call("wf_set_tuning", b"trace.xcd", cur.value)

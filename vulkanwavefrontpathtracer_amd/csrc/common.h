@@ -105,6 +105,20 @@ __device__ __forceinline__ void store_ray(wf_ray* rays, int64_t i, float4 a, flo
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
 // Number of set bits of `mask` below the calling lane.
+// Workgroup b of a g-workgroup grid renumbered so that the 8 XCDs, which
+// receive workgroups round-robin (b mod 8), each take runs of k consecutive
+// workgroups: run c of XCD x is the grid's run 8c + x.  Consecutive
+// workgroups of a sorted wavefront then share an L2 while every XCD still
+// samples the whole wavefront.  Workgroups past the last whole round of
+// 8k keep their index.
+__device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t g, int64_t k) {
+    constexpr int64_t NX = 8;
+    const int64_t whole = (g / (NX * k)) * (NX * k);
+    if (b >= whole) return b;
+    const int64_t x = b % NX, p = b / NX;
+    return ((p / k) * NX + x) * k + (p % k);
+}
+
 __device__ __forceinline__ unsigned popc_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                      __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
@@ -155,6 +169,7 @@ struct Tuning {
     int trace_compact_min = 262144;  // smallest wavefront traced with hand-over
     int trace_tail_refill = 16;  // tail pass: idle lanes that trigger taking more handed-over rays
     int trace_tail_cap = 0;     // tail queue records at a scene's first use (0: max(2^20, n/2))
+    int trace_xcd = 0;          // loop 1 walk: runs of this many workgroups per XCD (0 off)
 };
 Tuning& tuning();
 }  // namespace wfpt

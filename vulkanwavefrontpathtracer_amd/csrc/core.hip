@@ -94,6 +94,7 @@ extern "C" int wf_set_tuning(const char* key, int value) {
     else if (k == "trace.compact_min" && value >= 0) t.trace_compact_min = value;
     else if (k == "trace.tail_refill" && value >= 1 && value <= 64) t.trace_tail_refill = value;
     else if (k == "trace.tail_cap" && value >= 0) t.trace_tail_cap = value;
+    else if (k == "trace.xcd" && value >= 0 && value <= 65536) t.trace_xcd = value;
     else if (k == "trace.fuse_reorder" && (value == 0 || value == 1)) t.trace_fuse_reorder = value;
     else if (k == "batch.key_layout" && (value == 0 || value == 1)) t.batch_key_layout = value;
     else if (k == "trace.group64" && value >= 0) t.trace_group64 = value;
@@ -125,6 +126,7 @@ extern "C" int wf_get_tuning(const char* key, int* value) {
     else if (k == "trace.compact_min") *value = t.trace_compact_min;
     else if (k == "trace.tail_refill") *value = t.trace_tail_refill;
     else if (k == "trace.tail_cap") *value = t.trace_tail_cap;
+    else if (k == "trace.xcd") *value = t.trace_xcd;
     else if (k == "trace.fuse_reorder") *value = t.trace_fuse_reorder;
     else if (k == "batch.key_layout") *value = t.batch_key_layout;
     else if (k == "trace.group64") *value = t.trace_group64;

@@ -43,10 +43,11 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__
                                                       int32_t* __restrict__ out_s2i = nullptr,
                                                       int32_t* __restrict__ out_i2s = nullptr,
                                                       int leaf_gate = 1,
-                                                      TailQueue tq = TailQueue()) {
+                                                      TailQueue tq = TailQueue(), int xcd = 0) {
     static_assert(!HANDOVER || (LDS && WIDE && !STATS), "hand-over needs the 4-wide LDS walk");
     __shared__ int s_stack[LDS ? LDS_DEPTH * BLOCK : 1];
-    int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    const int64_t blk = xcd ? xcd_block(blockIdx.x, gridDim.x, xcd) : (int64_t)blockIdx.x;
+    int64_t i = blk * BLOCK + threadIdx.x;
     if (i >= n) return;
     int64_t r;
     float4 ro, rd;
@@ -396,7 +397,7 @@ static void launch_trace_loop(int loop, unsigned g, const float4* nodes, const f
         if (loop == LOOP_FUSED && tq.q) {
             trace_kernel<CLOSEST, BLOCK, true, false, true, LOOP_FUSED, false, true><<<g, BLOCK, 0, s>>>(
                 nodes, tris, rays, ray_index, hits, n, nullptr, nullptr, nullptr, nullptr, nullptr,
-                nullptr, 1, tq);
+                nullptr, 1, tq, tuning().trace_xcd);
             return;
         }
     }
@@ -405,7 +406,9 @@ static void launch_trace_loop(int loop, unsigned g, const float4* nodes, const f
             nodes, tris, rays, ray_index, hits, n, nullptr, nullptr, nullptr, nullptr, nullptr,
             nullptr, tuning().trace_leaf_gate);
     else if (WIDE && loop == LOOP_FUSED)
-        trace_kernel<CLOSEST, BLOCK, LDS, false, WIDE, LOOP_FUSED><<<g, BLOCK, 0, s>>>(nodes, tris, rays, ray_index, hits, n);
+        trace_kernel<CLOSEST, BLOCK, LDS, false, WIDE, LOOP_FUSED><<<g, BLOCK, 0, s>>>(
+            nodes, tris, rays, ray_index, hits, n, nullptr, nullptr, nullptr, nullptr, nullptr,
+            nullptr, 1, TailQueue(), tuning().trace_xcd);
     else if (WIDE && loop == LOOP_POSTPONED)
         trace_kernel<CLOSEST, BLOCK, LDS, false, WIDE, LOOP_POSTPONED><<<g, BLOCK, 0, s>>>(nodes, tris, rays, ray_index, hits, n);
     else
@@ -511,14 +514,15 @@ static void launch_reorder_trace_block(int loop, const float4* nodes4, const flo
     if (loop == LOOP_FUSED && tq.q)
         trace_kernel<CLOSEST, BLOCK, true, false, true, LOOP_FUSED, true, true><<<g, BLOCK, 0, s>>>(
             nodes4, tris, in_rays, idx, hits, n, nullptr, nullptr, in_s2i, out_rays, out_s2i, out_i2s,
-            1, tq);
+            1, tq, tuning().trace_xcd);
     else if (loop == LOOP_GATED)
         trace_kernel<CLOSEST, BLOCK, true, false, true, LOOP_GATED, true><<<g, BLOCK, 0, s>>>(
             nodes4, tris, in_rays, idx, hits, n, nullptr, nullptr, in_s2i, out_rays, out_s2i, out_i2s,
             tuning().trace_leaf_gate);
     else if (loop == LOOP_FUSED)
         trace_kernel<CLOSEST, BLOCK, true, false, true, LOOP_FUSED, true><<<g, BLOCK, 0, s>>>(
-            nodes4, tris, in_rays, idx, hits, n, nullptr, nullptr, in_s2i, out_rays, out_s2i, out_i2s);
+            nodes4, tris, in_rays, idx, hits, n, nullptr, nullptr, in_s2i, out_rays, out_s2i, out_i2s,
+            1, TailQueue(), tuning().trace_xcd);
     else
         trace_kernel<CLOSEST, BLOCK, true, false, true, LOOP_BRANCH, true><<<g, BLOCK, 0, s>>>(
             nodes4, tris, in_rays, idx, hits, n, nullptr, nullptr, in_s2i, out_rays, out_s2i, out_i2s);
