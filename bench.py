@@ -449,8 +449,8 @@ def large_section(dev, reps=3):
     never) a frame of the C++ frame loop with those settings captures every
     pass's path wavefronts; the 4 passes of each bounce are concatenated into
     one wavefront (8.3M rays, all samples in flight) and the timed step runs
-    keys -> sort -> reorder -> closest-hit trace on the bounces the period
-    sorts and the trace alone on the others."""
+    keys -> sort -> reorder + closest-hit trace (fused, wf_reorder_trace) on
+    the bounces the period sorts and the trace alone on the others."""
     import torch
 
     from vulkanwavefrontpathtracer_amd import stages
@@ -529,17 +529,19 @@ def large_section(dev, reps=3):
                                 C.c_void_p(keys.data_ptr()), C.c_void_p(idx.data_ptr()), n, kbytes,
                                 kbits, digit_bits, C.c_void_p(scratch.data_ptr()), need, s_ptr, None))
                         ev[2].record(stream)
-                        if srt:
-                            stages._capi.check("wf_reorder", lib.wf_reorder(
-                                C.c_void_p(idx.data_ptr()), C.c_void_p(w["rays"].data_ptr()),
-                                C.c_void_p(out_rays.data_ptr()), C.c_void_p(w["s2i"].data_ptr()),
-                                C.c_void_p(out_s2i.data_ptr()), C.c_void_p(out_i2s.data_ptr()), n,
-                                s_ptr, None))
                         ev[3].record(stream)
-                        src = out_rays if srt else w["rays"]
-                        stages._capi.check("wf_trace", lib.wf_trace(
-                            SceneRef.handle, C.c_void_p(src.data_ptr()), None,
-                            C.c_void_p(hits.data_ptr()), n, 1, s_ptr, None))
+                        if srt:
+                            # reorder gather fused into the trace (its time counts as trace)
+                            stages._capi.check("wf_reorder_trace", lib.wf_reorder_trace(
+                                SceneRef.handle, C.c_void_p(idx.data_ptr()),
+                                C.c_void_p(w["rays"].data_ptr()), C.c_void_p(out_rays.data_ptr()),
+                                C.c_void_p(w["s2i"].data_ptr()), C.c_void_p(out_s2i.data_ptr()),
+                                C.c_void_p(out_i2s.data_ptr()), C.c_void_p(hits.data_ptr()), n, 1,
+                                s_ptr, None))
+                        else:
+                            stages._capi.check("wf_trace", lib.wf_trace(
+                                SceneRef.handle, C.c_void_p(w["rays"].data_ptr()), None,
+                                C.c_void_p(hits.data_ptr()), n, 1, s_ptr, None))
                         ev[4].record(stream)
                         evs.append((ev, n, srt))
                     torch.cuda.synchronize()
