@@ -104,7 +104,6 @@ __device__ __forceinline__ void store_ray(wf_ray* rays, int64_t i, float4 a, flo
 // Wave64 primitives.
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __ballot(p); }
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
-// Number of set bits of `mask` below the calling lane.
 // Workgroup b of a g-workgroup grid renumbered so that the 8 XCDs, which
 // receive workgroups round-robin (b mod 8), each take runs of k consecutive
 // workgroups: run c of XCD x is the grid's run 8c + x.  Consecutive
@@ -119,6 +118,7 @@ __device__ __forceinline__ int64_t xcd_block(int64_t b, int64_t g, int64_t k) {
     return ((p / k) * NX + x) * k + (p % k);
 }
 
+// Number of set bits of `mask` below the calling lane.
 __device__ __forceinline__ unsigned popc_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                      __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));

@@ -107,6 +107,25 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp,
 // Upsweep: digit counts of one tile, written digit-major (counts[d * tiles
 // + tile]) so that the spine scans contiguous rows.  Tiles past the element
 // count write zeros.
+// Lanes of the wave whose DBITS-bit digit equals the calling lane's d (among
+// lanes with ok set): one ballot per digit bit, of which each lane keeps the
+// ballot where its own bit is set and the complement where it is clear,
+// i.e. peers &= ~(m ^ ballot) with m the lane's bit spread to all 32 bits
+// (one three-input logic operation per 32-bit half).
+template <int DBITS>
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, bool ok) {
+    const uint64_t okm = ballot64(ok);
+    uint32_t lo = (uint32_t)okm, hi = (uint32_t)(okm >> 32);
+#pragma unroll
+    for (int b = 0; b < DBITS; ++b) {
+        const uint32_t m = 0u - ((d >> b) & 1u);
+        const uint64_t bal = ballot64(m != 0u);
+        lo &= ~(m ^ (uint32_t)bal);
+        hi &= ~(m ^ (uint32_t)(bal >> 32));
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
 template <typename K, int DBITS>
 __global__ __launch_bounds__(SORT_BLOCK) void sort_upsweep_kernel(const K* __restrict__ keys,
                                                                   int64_t n, int shift, int width,
@@ -209,13 +228,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
         int64_t idx = wbase + it * 64 + lane;
         bool ok = idx < n;
         uint32_t d = digit_of(key[it], shift, mask);
-        uint64_t peers = ballot64(ok);
-#pragma unroll
-        for (int b = 0; b < DBITS; ++b) {
-            bool bit = (d >> b) & 1u;
-            uint64_t bal = ballot64(bit);
-            peers &= bit ? bal : ~bal;
-        }
+        const uint64_t peers = match_digit<DBITS>(d, ok);
         uint32_t rank = popc_below(peers);
         uint32_t cnt = (uint32_t)__popcll(peers);
         uint32_t pre = 0;
@@ -418,13 +431,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_onesweep_kernel(
     for (int it = 0; it < SORT_ITEMS; ++it) {
         const bool ok = wbase + it * 64 + lane < n;
         const uint32_t d = digit_of(key[it], shift, mask);
-        uint64_t peers = ballot64(ok);
-#pragma unroll
-        for (int b = 0; b < DBITS; ++b) {
-            const bool bit = (d >> b) & 1u;
-            const uint64_t bal = ballot64(bit);
-            peers &= bit ? bal : ~bal;
-        }
+        const uint64_t peers = match_digit<DBITS>(d, ok);
         const uint32_t rank = popc_below(peers);
         const uint32_t cnt = (uint32_t)__popcll(peers);
         uint32_t pre = 0;
@@ -589,13 +596,7 @@ __global__ __launch_bounds__(SMALL_BLOCK) void sort_small_kernel(K* __restrict__
         for (int it = 0; it < SMALL_ITEMS; ++it) {
             const bool ok = wbase + it * 64 + (int)lane < n;
             const uint32_t d = digit_of(key[it], shift, mask);
-            uint64_t peers = ballot64(ok);
-#pragma unroll
-            for (int b = 0; b < DBITS; ++b) {
-                const bool bit = (d >> b) & 1u;
-                const uint64_t bal = ballot64(bit);
-                peers &= bit ? bal : ~bal;
-            }
+            const uint64_t peers = match_digit<DBITS>(d, ok);
             const uint32_t rank = popc_below(peers);
             const uint32_t cnt = (uint32_t)__popcll(peers);
             uint32_t pre = 0;
