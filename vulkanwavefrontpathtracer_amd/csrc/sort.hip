@@ -118,8 +118,8 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool ok) {
     uint32_t lo = (uint32_t)okm, hi = (uint32_t)(okm >> 32);
 #pragma unroll
     for (int b = 0; b < DBITS; ++b) {
-        const uint32_t m = 0u - ((d >> b) & 1u);
-        const uint64_t bal = ballot64(m != 0u);
+        const uint32_t m = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+        const uint64_t bal = ballot64((d >> b) & 1u);
         lo &= ~(m ^ (uint32_t)bal);
         hi &= ~(m ^ (uint32_t)(bal >> 32));
     }
@@ -207,6 +207,15 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
     const int64_t base = tile * SORT_TILE;
     if (base >= n) return;  // uniform: surplus tiles of a capacity-sized grid
     for (int i = tid; i < SORT_WAVES * RADIX; i += SORT_BLOCK) (&s_wave[0][0])[i] = 0;
+    // the pass's digit totals and this tile's scanned offsets, fetched
+    // beside the keys
+    uint32_t tot[DPT], toff[DPT];
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        const int d = tid * DPT + k;
+        tot[k] = d < RADIX ? totals[d] : 0u;
+        toff[k] = d < RADIX ? tile_offsets[(int64_t)d * tiles + tile] : 0u;
+    }
     __syncthreads();
 
     // Wave-striped load: element index = base + wave*64*ITEMS + it*64 + lane.
@@ -261,14 +270,9 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
     uint32_t lstart = block_excl_scan(my_total, s_tmp, &tile_total);
     (void)tile_total;
     // Digit starts in the output: exclusive scan of the pass's digit totals.
-    uint32_t tot[DPT];
     uint32_t my_tot = 0;
 #pragma unroll
-    for (int k = 0; k < DPT; ++k) {
-        const int d = tid * DPT + k;
-        tot[k] = d < RADIX ? totals[d] : 0u;
-        my_tot += tot[k];
-    }
+    for (int k = 0; k < DPT; ++k) my_tot += tot[k];
     uint32_t all_total;
     uint32_t dstart = block_excl_scan(my_tot, s_tmp, &all_total);
     (void)all_total;
@@ -277,7 +281,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
         int d = tid * DPT + k;
         if (d < RADIX) {
             s_lstart[d] = lstart;
-            s_goff[d] = dstart + tile_offsets[(int64_t)d * tiles + tile] - lstart;
+            s_goff[d] = dstart + toff[k] - lstart;
             lstart += cnt[k];
             dstart += tot[k];
         }
@@ -416,6 +420,13 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_onesweep_kernel(
     const int64_t base = tile * SORT_TILE;
     if (base >= n) return;  // uniform: tiles past the count
 
+    // the pass's digit totals, fetched beside the keys
+    uint32_t tot[DPT];
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        const int d = tid * DPT + k;
+        tot[k] = d < RADIX ? hist[d] : 0u;
+    }
     K key[SORT_ITEMS];
     uint32_t val[SORT_ITEMS];
     uint32_t off[SORT_ITEMS];
@@ -463,14 +474,9 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_onesweep_kernel(
     uint32_t tile_total;
     uint32_t lstart = block_excl_scan(my_total, s_tmp, &tile_total);
     (void)tile_total;
-    uint32_t tot[DPT];
     uint32_t my_tot = 0;
 #pragma unroll
-    for (int k = 0; k < DPT; ++k) {
-        const int d = tid * DPT + k;
-        tot[k] = d < RADIX ? hist[d] : 0u;
-        my_tot += tot[k];
-    }
+    for (int k = 0; k < DPT; ++k) my_tot += tot[k];
     uint32_t all_total;
     uint32_t dstart = block_excl_scan(my_tot, s_tmp, &all_total);
     (void)all_total;
