@@ -113,6 +113,14 @@ def test_wide_hierarchy_structure_and_conservative_boxes(dev, boxes):
     # clipped boxes, not whole; containment is checked for the others
     whole = np.bincount(order, minlength=hs.num_tris)[order] == 1
     assert np.all((nc >= 2) | (n4.value == 1)) and np.all(nc <= 4)
+    # empty slots: inverted boxes (low byte 255, high byte 0 on every axis) and
+    # the harmless reference leaf slot 0, so traversal tests all four slots
+    raw = nodes4.view(np.uint32).reshape(-1, 16)
+    for k in range(4):
+        empty = nc <= k
+        qb = (raw[empty][:, 4:10] >> (8 * k)) & 255
+        assert np.all(qb[:, :3] == 255) and np.all(qb[:, 3:] == 0)
+        assert np.all(refs[empty, k] == -1)
     tv = hs.tri_verts().reshape(-1, 3, 3).astype(np.float64)
     seen = np.zeros(n4.value, np.int32)
     covered = np.zeros(slots, np.int32)

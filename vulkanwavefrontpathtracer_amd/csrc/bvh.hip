@@ -703,11 +703,14 @@ __global__ void wide_single_kernel(const float4* __restrict__ nodes, const uint3
         qhi[k] = (uint32_t)fminf(fmaxf(ceilf((hi[k] - origin[k]) / scale) + 1.0f, 0.0f), 255.0f);
     }
     nodes4[0] = make_float4(origin[0], origin[1], origin[2], __uint_as_float(meta));
+    // children 1-3 empty: inverted boxes and the harmless leaf slot 0, as
+    // wide_emit_kernel writes them
+    for (int ax = 0; ax < 3; ++ax) qlo[ax] |= 0xffffff00u;
     nodes4[1] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qlo[1]), __uint_as_float(qlo[2]),
                             __uint_as_float(qhi[0]));
     nodes4[2] = make_float4(__uint_as_float(qhi[1]), __uint_as_float(qhi[2]),
-                            __int_as_float(leaf_ref(0, 1)), 0.f);
-    nodes4[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+                            __int_as_float(leaf_ref(0, 1)), __int_as_float(leaf_ref(0, 1)));
+    nodes4[3] = make_float4(__int_as_float(leaf_ref(0, 1)), __int_as_float(leaf_ref(0, 1)), 0.f, 0.f);
 }
 
 // Child c (0/1) of 2-wide node `node`: its box and reference.
@@ -844,7 +847,15 @@ __global__ __launch_bounds__(256) void wide_emit_kernel(int n, const uint32_t* _
             qlo[a] |= l << (8 * k);
             qhi[a] |= h << (8 * k);
         }
-    for (int k = nc; k < 4; ++k) ref[k] = 0;
+    // Empty child slots: an inverted box (low plane 255 above high plane 0 on
+    // every axis), which no slab test with a nonzero scale meets, and a
+    // reference that is harmless if a degenerate ray meets it anyway (leaf
+    // slot 0: one extra test of a scene triangle changes no closest hit), so
+    // the traversal needs no child count.
+    for (int k = nc; k < 4; ++k) {
+        for (int a = 0; a < 3; ++a) qlo[a] |= 255u << (8 * k);
+        ref[k] = leaf_ref(0, 1);
+    }
     float4* o = nodes4 + 4 * (int64_t)idx4[i];
     o[0] = make_float4(origin[0], origin[1], origin[2], __uint_as_float(meta));
     o[1] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qlo[1]), __uint_as_float(qlo[2]),

@@ -198,8 +198,9 @@ __device__ __forceinline__ void cswap(float& ta, int& ra, float& tb, int& rb) {
 template <bool LDS, int STACK>
 __device__ __forceinline__ void node4_hits(const TravState<LDS, STACK>& st, float4 a, float4 b,
                                            float4 c, float4 d, float (&tn)[4], int (&ref)[4]) {
+    // empty child slots hold inverted boxes and a harmless reference (see
+    // wide_emit_kernel), so every slot is tested without the child count
     const uint32_t meta = __float_as_uint(a.w);
-    const int nc = (int)(meta >> 24);
     const float sax = __uint_as_float((meta & 0xffu) << 23) * st.ix;
     const float say = __uint_as_float(((meta >> 8) & 0xffu) << 23) * st.iy;
     const float saz = __uint_as_float(((meta >> 16) & 0xffu) << 23) * st.iz;
@@ -231,7 +232,7 @@ __device__ __forceinline__ void node4_hits(const TravState<LDS, STACK>& st, floa
         const float fzv = fmaf((float)((qfz >> sh) & 0xffu), saz, bz);
         const float nn = fmaxf(fmaxf(nx, ny), fmaxf(nz, st.tmin));
         const float ff = fminf(fminf(fxv, fyv), fminf(fzv, st.best_t));
-        tn[k] = (k < nc && nn <= ff) ? nn : INFINITY;
+        tn[k] = nn <= ff ? nn : INFINITY;
     }
     cswap(tn[0], ref[0], tn[1], ref[1]);
     cswap(tn[2], ref[2], tn[3], ref[3]);
