@@ -64,7 +64,9 @@ __global__ __launch_bounds__(BLOCK) void trace_kernel(const float4* __restrict__
         load_ray(rays, r, ro, rd);
     }
     WtRay wr;
+    wr.uperm = -1;
     const bool live = (rd.w >= ro.w) && wt_setup(ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, wr);
+    wt_uniform_perm(wr, live);
     if constexpr (HANDOVER) {
         TravState<true, TRACE_STACK4> st;
         int spill[TravState<true, TRACE_STACK4>::SPILL];
