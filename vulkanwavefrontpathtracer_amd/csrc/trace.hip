@@ -265,7 +265,9 @@ __global__ __launch_bounds__(64) void trace_group_kernel(const float4* __restric
         load_ray(rays, r, ro, rd);
     }
     WtRay wr;
+    wr.uperm = -1;
     const bool live = valid && (rd.w >= ro.w) && wt_setup(ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, wr);
+    wt_uniform_perm(wr, live);
     TravState<true, STACK> st;
     int spill[TravState<true, STACK>::SPILL];
     trav_begin(st, ro, rd);
