@@ -241,10 +241,14 @@ __device__ __forceinline__ void node4_hits(const TravState<LDS, STACK>& st, floa
     cswap(tn[1], ref[1], tn[2], ref[2]);
 }
 
-template <int BLOCK, bool LDS, bool STATS, int STACK>
+// TOP: `top` holds the stack's top entry read before this node's pushes (the
+// fused step reads it beside the node fetch); a node that pushes nothing and
+// pops takes it instead of reading LDS again.
+template <int BLOCK, bool LDS, bool STATS, int STACK, bool TOP = false>
 __device__ __forceinline__ bool trav_node4_data(TravState<LDS, STACK>& st,
                                                 int (&spill)[TravState<LDS, STACK>::SPILL],
-                                                float4 a, float4 b, float4 c, float4 d, int* s_stack) {
+                                                float4 a, float4 b, float4 c, float4 d, int* s_stack,
+                                                int top = 0) {
     if (STATS) ++st.nodes_visited;
     float tn[4];
     int ref[4];
@@ -268,8 +272,14 @@ __device__ __forceinline__ bool trav_node4_data(TravState<LDS, STACK>& st,
         st.node = ref[0];
         return true;
     }
+    // no child hit, so nothing was pushed and `top` is still the top entry
     if (st.sp == 0) return false;
-    st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
+    if (TOP && LDS && st.sp <= LDS_DEPTH) {
+        --st.sp;
+        st.node = top;
+    } else {
+        st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
+    }
     return true;
 }
 
@@ -297,7 +307,12 @@ __device__ __forceinline__ bool trav_step4_fused(TravState<LDS, STACK>& st,
     const float4 a = p[0], b = p[1], c = p[2];
     float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
     if (inner) d = p[3];
-    if (inner) return !trav_node4_data<BLOCK, LDS, STATS>(st, spill, a, b, c, d, s_stack);
+    // The stack's top entry, read while the fetch is in flight: the next node
+    // of a step that pushes nothing (a leaf, or a node with no child hit),
+    // which then waits for neither LDS nor the fetch's dependents to pop.
+    int top = 0;
+    if (LDS && st.sp > 0) top = ((const lds_int*)s_stack)[min(st.sp - 1, LDS_DEPTH - 1) * BLOCK + threadIdx.x];
+    if (inner) return !trav_node4_data<BLOCK, LDS, STATS, STACK, true>(st, spill, a, b, c, d, s_stack, top);
     const int first = leaf_first(st.node), cnt = leaf_count(st.node);
     if (trav_tri<CLOSEST, LDS, STACK, STATS>(st, first, a, b, c, wr)) return true;
     for (int k = 1; k < cnt; ++k) {
@@ -305,7 +320,12 @@ __device__ __forceinline__ bool trav_step4_fused(TravState<LDS, STACK>& st,
         if (trav_tri<CLOSEST, LDS, STACK, STATS>(st, first + k, tp[0], tp[1], tp[2], wr)) return true;
     }
     if (st.sp == 0) return true;
-    st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
+    if (LDS && st.sp <= LDS_DEPTH) {
+        --st.sp;
+        st.node = top;
+    } else {
+        st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
+    }
     return false;
 }
 
