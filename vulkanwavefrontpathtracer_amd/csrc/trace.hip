@@ -309,7 +309,9 @@ __global__ __launch_bounds__(64) void trace_group_kernel(const float4* __restric
     bool active = gl < fsize;
     if (active) st.node = s_front[cur][gbase + gl];
     while (__any(active)) {
-        if (active) active = !trav_step4<CLOSEST, 64, true, false, STACK, true>(st, spill, nodes4, tris, wr, s_stack);
+        if (active)
+            active = !trav_step4_fused<CLOSEST, 64, true, false, STACK, true>(st, spill, nodes4, tris, wr,
+                                                                          s_stack);
         float bt = st.best_t;
 #pragma unroll
         for (int o = 1; o < G; o <<= 1) bt = fminf(bt, __shfl_xor(bt, o, G));

@@ -296,7 +296,7 @@ __device__ __forceinline__ bool trav_node4(TravState<LDS, STACK>& st,
 // of a leaf (48 bytes), so a wave whose lanes are split between nodes and
 // leaves waits for memory once per step instead of once per branch.
 // Returns true when the ray is finished.
-template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, int STACK>
+template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, int STACK, bool RECORD = false>
 __device__ __forceinline__ bool trav_step4_fused(TravState<LDS, STACK>& st,
                                                  int (&spill)[TravState<LDS, STACK>::SPILL],
                                                  const float4* __restrict__ nodes4,
@@ -314,10 +314,11 @@ __device__ __forceinline__ bool trav_step4_fused(TravState<LDS, STACK>& st,
     if (LDS && st.sp > 0) top = ((const lds_int*)s_stack)[min(st.sp - 1, LDS_DEPTH - 1) * BLOCK + threadIdx.x];
     if (inner) return !trav_node4_data<BLOCK, LDS, STATS, STACK, true>(st, spill, a, b, c, d, s_stack, top);
     const int first = leaf_first(st.node), cnt = leaf_count(st.node);
-    if (trav_tri<CLOSEST, LDS, STACK, STATS>(st, first, a, b, c, wr)) return true;
+    if (trav_tri<CLOSEST, LDS, STACK, STATS, RECORD>(st, first, a, b, c, wr)) return true;
     for (int k = 1; k < cnt; ++k) {
         const float4* tp = tris + 3 * (int64_t)(first + k);
-        if (trav_tri<CLOSEST, LDS, STACK, STATS>(st, first + k, tp[0], tp[1], tp[2], wr)) return true;
+        if (trav_tri<CLOSEST, LDS, STACK, STATS, RECORD>(st, first + k, tp[0], tp[1], tp[2], wr))
+            return true;
     }
     if (st.sp == 0) return true;
     if (LDS && st.sp <= LDS_DEPTH) {
