@@ -104,9 +104,6 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp,
     return wbase + x - v;
 }
 
-// Upsweep: digit counts of one tile, written digit-major (counts[d * tiles
-// + tile]) so that the spine scans contiguous rows.  Tiles past the element
-// count write zeros.
 // Lanes of the wave whose DBITS-bit digit equals the calling lane's d (among
 // lanes with ok set): one ballot per digit bit, of which each lane keeps the
 // ballot where its own bit is set and the complement where it is clear,
@@ -126,6 +123,9 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool ok) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// Upsweep: digit counts of one tile, written digit-major (counts[d * tiles
+// + tile]) so that the spine scans contiguous rows.  Tiles past the element
+// count write zeros.
 template <typename K, int DBITS>
 __global__ __launch_bounds__(SORT_BLOCK) void sort_upsweep_kernel(const K* __restrict__ keys,
                                                                   int64_t n, int shift, int width,
