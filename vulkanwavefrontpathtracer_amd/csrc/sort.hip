@@ -78,8 +78,10 @@ __device__ __forceinline__ uint32_t digit_of(K key, int shift, uint32_t mask) {
 }
 
 // ---------------------------------------------------------------- binning
-// Block-wide exclusive scan of one value per thread; returns the exclusive
-// prefix and writes the block total to *total.
+// Exclusive scan of one value per thread over a workgroup of WAVES waves (the
+// launch's block size / 64); returns the exclusive prefix and writes the
+// block total to *total.
+template <int WAVES>
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp, uint32_t* total) {
     const unsigned lane = lane_id();
     const unsigned wave = threadIdx.x >> 6;
@@ -94,7 +96,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp,
     __syncthreads();
     uint32_t wbase = 0, all = 0;
 #pragma unroll
-    for (int w = 0; w < SORT_WAVES; ++w) {
+    for (int w = 0; w < WAVES; ++w) {
         uint32_t t = s_tmp[w];
         if (w < (int)wave) wbase += t;
         all += t;
@@ -165,17 +167,18 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_upsweep_kernel(const K* __res
 
 // Spine: one workgroup per digit scans its row of tile counts in place
 // (exclusive) and writes the row total, the digit's count in the pass.
-__global__ __launch_bounds__(256) void sort_spine_kernel(uint32_t* __restrict__ counts,
-                                                         int64_t tiles,
-                                                         uint32_t* __restrict__ totals) {
-    __shared__ uint32_t s_tmp[SORT_WAVES];
+constexpr int SPINE_BLOCK = 256;
+__global__ __launch_bounds__(SPINE_BLOCK) void sort_spine_kernel(uint32_t* __restrict__ counts,
+                                                                 int64_t tiles,
+                                                                 uint32_t* __restrict__ totals) {
+    __shared__ uint32_t s_tmp[SPINE_BLOCK / 64];
     uint32_t* row = counts + (int64_t)blockIdx.x * tiles;
     uint32_t carry = 0;
-    for (int64_t base = 0; base < tiles; base += 256) {
+    for (int64_t base = 0; base < tiles; base += SPINE_BLOCK) {
         const int64_t i = base + threadIdx.x;
         const uint32_t v = i < tiles ? row[i] : 0u;
         uint32_t total;
-        const uint32_t ex = block_excl_scan(v, s_tmp, &total);
+        const uint32_t ex = block_excl_scan<SPINE_BLOCK / 64>(v, s_tmp, &total);
         if (i < tiles) row[i] = carry + ex;
         carry += total;
     }
@@ -267,14 +270,14 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
     }
     // Tile-local digit starts (block scan over digits in thread order).
     uint32_t tile_total;
-    uint32_t lstart = block_excl_scan(my_total, s_tmp, &tile_total);
+    uint32_t lstart = block_excl_scan<SORT_WAVES>(my_total, s_tmp, &tile_total);
     (void)tile_total;
     // Digit starts in the output: exclusive scan of the pass's digit totals.
     uint32_t my_tot = 0;
 #pragma unroll
     for (int k = 0; k < DPT; ++k) my_tot += tot[k];
     uint32_t all_total;
-    uint32_t dstart = block_excl_scan(my_tot, s_tmp, &all_total);
+    uint32_t dstart = block_excl_scan<SORT_WAVES>(my_tot, s_tmp, &all_total);
     (void)all_total;
 #pragma unroll
     for (int k = 0; k < DPT; ++k) {
@@ -472,13 +475,13 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_onesweep_kernel(
         my_total += sum;
     }
     uint32_t tile_total;
-    uint32_t lstart = block_excl_scan(my_total, s_tmp, &tile_total);
+    uint32_t lstart = block_excl_scan<SORT_WAVES>(my_total, s_tmp, &tile_total);
     (void)tile_total;
     uint32_t my_tot = 0;
 #pragma unroll
     for (int k = 0; k < DPT; ++k) my_tot += tot[k];
     uint32_t all_total;
-    uint32_t dstart = block_excl_scan(my_tot, s_tmp, &all_total);
+    uint32_t dstart = block_excl_scan<SORT_WAVES>(my_tot, s_tmp, &all_total);
     (void)all_total;
 #pragma unroll
     for (int k = 0; k < DPT; ++k) {
@@ -675,7 +678,7 @@ static int launch_pass(const void* ki, const uint32_t* vi, void* ko, uint32_t* v
     sort_upsweep_kernel<K, DBITS><<<(unsigned)tiles, SORT_BLOCK, 0, s>>>(
         static_cast<const K*>(ki), n, shift, width, counts, tiles, d_n);
     WF_TRY(launch_check("sort_upsweep_kernel"));
-    sort_spine_kernel<<<RADIX, 256, 0, s>>>(counts, tiles, totals);
+    sort_spine_kernel<<<RADIX, SPINE_BLOCK, 0, s>>>(counts, tiles, totals);
     WF_TRY(launch_check("sort_spine_kernel"));
     sort_bin_kernel<K, DBITS><<<(unsigned)tiles, SORT_BLOCK, 0, s>>>(
         static_cast<const K*>(ki), vi, static_cast<K*>(ko), vo, n, shift, width, counts, tiles,
