@@ -19,9 +19,9 @@
 
 namespace wfpt {
 
-constexpr int SORT_BLOCK = 256;
+constexpr int SORT_BLOCK = 512;
 constexpr int SORT_WAVES = SORT_BLOCK / 64;
-constexpr int SORT_ITEMS = 16;
+constexpr int SORT_ITEMS = 8;
 constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
 constexpr int SORT_MAX_PASSES = 16;
 
