@@ -82,12 +82,17 @@ __device__ __forceinline__ uint32_t oct_code(float4 d, int bits) {
     return (spread2_16(qu) << 1) | spread2_16(qv);
 }
 
-// Spread the low 10 bits of v so that bit k lands on bit 5k.
+// Spread the low 10 bits of v so that bit k lands on bit 5k.  Bit k must move
+// left by 4k = 32a + 16b + 8c + 4d for k = 8a + 4b + 2c + d, so four
+// shift-and-mask steps (by 32, 16, 8, 4) move every bit whose k has that
+// binary digit set.
 __device__ __forceinline__ uint64_t spread5_10(uint32_t v) {
-    uint64_t r = 0;
-#pragma unroll
-    for (int k = 0; k < 10; ++k) r |= (uint64_t)((v >> k) & 1u) << (5 * k);
-    return r;
+    uint64_t x = v & 0x3ffu;
+    x = (x | (x << 32)) & 0x00000300000000ffull;
+    x = (x | (x << 16)) & 0x0000030000f0000full;
+    x = (x | (x << 8)) & 0x00000300c0300c03ull;
+    x = (x | (x << 4)) & 0x0000210842108421ull;
+    return x;
 }
 
 // 5-D Morton code of the same 30 origin and 30 octahedral-direction bits as
