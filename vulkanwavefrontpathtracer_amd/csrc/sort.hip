@@ -167,7 +167,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_upsweep_kernel(const K* __res
 
 // Spine: one workgroup per digit scans its row of tile counts in place
 // (exclusive) and writes the row total, the digit's count in the pass.
-constexpr int SPINE_BLOCK = 256;
+constexpr int SPINE_BLOCK = 512;
 __global__ __launch_bounds__(SPINE_BLOCK) void sort_spine_kernel(uint32_t* __restrict__ counts,
                                                                  int64_t tiles,
                                                                  uint32_t* __restrict__ totals) {
