@@ -138,7 +138,7 @@ __global__ __launch_bounds__(64) void trace_tail_kernel(const float4* __restrict
                         st.best_orig = h1.x;
                         st.best_t = __int_as_float(h1.y);
 #pragma unroll
-                        for (int e = 0; e < 4; ++e) {
+                        for (int e = 0; e < LDS_DEPTH / 4; ++e) {
                             const int4 v = rec[2 + e];
                             col[(4 * e) * 64] = v.x;
                             col[(4 * e + 1) * 64] = v.y;
@@ -450,7 +450,7 @@ static int launch_trace_block(const Tuning& tu, const float4* nodes, const float
 // Tail queue of a trace of n rays on `scene`: set (q non-null) when the
 // hand-over is on and applies (4-wide single-load LDS walk, n at least
 // trace.compact_min), with its counter cleared on stream s.  The queue is
-// allocated on first use (trace.tail_cap records of 96 bytes, by default
+// allocated on first use (trace.tail_cap records of 16 * TAIL_INT4 bytes, by default
 // max(2^20, n/2), kept with the scene); a first use under stream capture
 // walks without hand-over instead of allocating.
 static std::mutex g_tailq_mutex;

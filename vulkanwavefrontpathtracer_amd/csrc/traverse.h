@@ -469,10 +469,11 @@ __device__ __forceinline__ TraceOut traverse(const float4* __restrict__ nodes,
 // the ray slot, the current node, the stack (at most LDS_DEPTH entries, all
 // in LDS; a wave with a deeper stack keeps walking) and the closest hit so
 // far, so the tail pass continues exactly the same step sequence and the
-// results do not change.  Record: 6 int4 = {slot, node, sp, best slot},
-// {best original index, best t, 0, 0}, then the LDS_DEPTH stack entries.
-constexpr int TAIL_INT4 = 6;
-static_assert(LDS_DEPTH == 16, "tail record holds 16 stack entries");
+// results do not change.  Record: 2 + LDS_DEPTH / 4 int4 = {slot, node, sp,
+// best slot}, {best original index, best t, 0, 0}, then the LDS_DEPTH stack
+// entries.
+static_assert(LDS_DEPTH % 4 == 0, "tail record holds the LDS stack as int4");
+constexpr int TAIL_INT4 = 2 + LDS_DEPTH / 4;
 
 struct TailQueue {
     int4* q = nullptr;          // cap records
@@ -513,7 +514,7 @@ __device__ __forceinline__ bool traverse4_handover(TravState<true, STACK>& st,
                     o[0] = make_int4((int)slot, st.node, st.sp, st.best_slot);
                     o[1] = make_int4(st.best_orig, __float_as_int(st.best_t), 0, 0);
 #pragma unroll
-                    for (int k = 0; k < 4; ++k)
+                    for (int k = 0; k < LDS_DEPTH / 4; ++k)
                         o[2 + k] = make_int4(col[(4 * k) * BLOCK], col[(4 * k + 1) * BLOCK],
                                              col[(4 * k + 2) * BLOCK], col[(4 * k + 3) * BLOCK]);
                     return true;
