@@ -289,6 +289,9 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
 /* Process-wide knobs read by later calls (the autotuner sets them):
  *   "bvh.presplit"     0/1  split references of large triangles (default 1)
  *   "bvh.split_depth"  0-8  max binary splits per triangle (default 6)
+ *   "bvh.split_denom"  1-4096 split triangles whose box exceeds the scene
+ *                           diagonal / this (default 64; 32-128 measure
+ *                           alike on the interior scene, 256+ slower)
  *   "bvh.leaf_max"     1-8  triangles per collapsed leaf (default 1)
  *   "bvh.builder"      0/1  binary hierarchy: 0 Karras LBVH over centroid Morton
  *                           codes (default), 1 PLOC clustering of the

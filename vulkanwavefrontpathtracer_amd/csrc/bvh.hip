@@ -9,7 +9,7 @@
 //                   (wave reductions + one ordered-int atomic per wave),
 //                   shading records in original order.
 //   2. references: a triangle whose box is larger than 1/64 of the scene
-//      diagonal is split (Ernst & Greiner's early split clipping: recursive
+//      diagonal (bvh.split_denom) is split (Ernst & Greiner's early split clipping: recursive
 //      midpoint splits of the clipped polygon, up to 2^split_depth pieces),
 //      so that large triangles (walls, floors) do not inflate the boxes near
 //      the root; small triangles are one reference each.
@@ -1123,7 +1123,7 @@ extern "C" int wf_scene_create(const void* vertices, int64_t num_vertices, int v
     const float4* refs = (const float4*)tri_box.p;
     const uint32_t* ref_tri = nullptr;
     if (tu.bvh_presplit && tu.bvh_split_depth > 0) {
-        const float frac = 1.0f / 64.0f;
+        const float frac = 1.0f / (float)tu.bvh_split_denom;
         WF_TRY(dev_alloc(counts, sizeof(uint32_t) * n));
         split_kernel<false><<<grid_for(n, 128), 128, 0, s>>>(
             (const char*)vertices, vertex_stride, indices, (const float4*)tri_box.p, n, b, frac,

@@ -148,6 +148,7 @@ namespace wfpt {
 struct Tuning {
     int bvh_presplit = 1;
     int bvh_split_depth = 6;
+    int bvh_split_denom = 64;   // split triangles whose box exceeds diagonal / this
     int bvh_leaf_max = 1;
     int bvh_builder = 0;        // 0: Karras LBVH over Morton codes; 1: PLOC clustering
     int bvh_ploc_radius = 16;   // PLOC nearest-neighbour search radius (positions)

@@ -79,6 +79,7 @@ extern "C" int wf_set_tuning(const char* key, int value) {
     Tuning& t = tuning();
     if (k == "bvh.presplit" && (value == 0 || value == 1)) t.bvh_presplit = value;
     else if (k == "bvh.split_depth" && value >= 0 && value <= 8) t.bvh_split_depth = value;
+    else if (k == "bvh.split_denom" && value >= 1 && value <= 4096) t.bvh_split_denom = value;
     else if (k == "bvh.leaf_max" && value >= 1 && value <= 8) t.bvh_leaf_max = value;
     else if (k == "bvh.builder" && (value == 0 || value == 1)) t.bvh_builder = value;
     else if (k == "bvh.ploc_radius" && value >= 1 && value <= 32) t.bvh_ploc_radius = value;
@@ -111,6 +112,7 @@ extern "C" int wf_get_tuning(const char* key, int* value) {
     const Tuning& t = tuning();
     if (k == "bvh.presplit") *value = t.bvh_presplit;
     else if (k == "bvh.split_depth") *value = t.bvh_split_depth;
+    else if (k == "bvh.split_denom") *value = t.bvh_split_denom;
     else if (k == "bvh.leaf_max") *value = t.bvh_leaf_max;
     else if (k == "bvh.builder") *value = t.bvh_builder;
     else if (k == "bvh.ploc_radius") *value = t.bvh_ploc_radius;
