@@ -251,7 +251,9 @@ __global__ __launch_bounds__(64) void trace_group_kernel(const float4* __restric
     static_assert(G == 4 || G == 16 || G == 64, "group of 4, 16 or 64 lanes");
     constexpr int STACK = TRACE_STACK4;
     __shared__ int s_stack[LDS_DEPTH * 64];
-    __shared__ int s_front[2][64];
+    // frontier of each group: up to FRONT nodes at group offset gbase * FRONT / G
+    constexpr int FRONT = 4 * G;
+    __shared__ int s_front[2][64 * 4];
     const unsigned lane = lane_id();
     const int gl = (int)(lane & (G - 1));
     const int gbase = (int)(lane & ~(unsigned)(G - 1));
@@ -273,16 +275,19 @@ __global__ __launch_bounds__(64) void trace_group_kernel(const float4* __restric
     trav_begin(st, ro, rd);
 
     // Breadth-first expansion: before each level the frontier holds at most
-    // G/4 nodes, so the next one (at most 4 children each) fits in G lanes.
+    // G nodes (one per lane), so the next one (at most 4 children each) fits
+    // in the group's 4G frontier slots; the last level leaves up to 4G
+    // subtrees, which the lanes then take from the frontier one at a time.
+    const int fbase = gbase * 4;
     int cur = 0, fsize = live ? 1 : 0;
-    if (gl == 0) s_front[0][gbase] = 0;
+    if (gl == 0) s_front[0][fbase] = 0;
     __syncthreads();
 #pragma unroll
-    for (int level = 0; (4 << (2 * level)) <= G; ++level) {
+    for (int level = 0; (1 << (2 * level)) <= G; ++level) {
         int c = 0;
         int ch[4] = {0, 0, 0, 0};
         if (gl < fsize) {
-            const int node = s_front[cur][gbase + gl];
+            const int node = s_front[cur][fbase + gl];
             if (node >= 0) {
                 const float4* nd = nodes4 + 4 * (int64_t)node;
                 float tn[4];
@@ -300,15 +305,27 @@ __global__ __launch_bounds__(64) void trace_group_kernel(const float4* __restric
             if (gl >= o) incl += y;
         }
         const int total = __shfl(incl, G - 1, G);
-        for (int k = 0; k < c; ++k) s_front[cur ^ 1][gbase + incl - c + k] = ch[k];
+        for (int k = 0; k < c; ++k) s_front[cur ^ 1][fbase + incl - c + k] = ch[k];
         __syncthreads();
         cur ^= 1;
         fsize = total;
     }
+    static_assert(FRONT <= 256, "frontier slots of a wave");
 
-    bool active = gl < fsize;
-    if (active) st.node = s_front[cur][gbase + gl];
-    while (__any(active)) {
+    // Each lane walks one frontier subtree with its own stack; a lane whose
+    // walk ends takes the group's next subtree, so the group's time is its
+    // total work over G lanes plus one subtree, not its largest lane's share.
+    bool active = false;
+    int head = 0;  // next frontier subtree of the group (same on every lane)
+    while (true) {
+        const uint64_t want = ballot64(!active) & gmask;
+        const int k = (int)popc_below(want);
+        if (!active && head + k < fsize) {
+            st.node = s_front[cur][fbase + head + k];
+            active = true;
+        }
+        head = min(fsize, head + __popcll(want));
+        if (!__any(active)) break;
         if (active)
             active = !trav_step4_fused<CLOSEST, 64, true, false, STACK, true>(st, spill, nodes4, tris, wr,
                                                                           s_stack);
@@ -320,7 +337,10 @@ __global__ __launch_bounds__(64) void trace_group_kernel(const float4* __restric
             st.best_slot = -1;
             st.best_orig = 0x7fffffff;
         }
-        if (!CLOSEST && (ballot64(st.best_slot >= 0) & gmask)) active = false;
+        if (!CLOSEST && (ballot64(st.best_slot >= 0) & gmask)) {
+            active = false;
+            head = fsize;
+        }
     }
 
     // group minimum of (t, original index) over the lanes that hold a hit
