@@ -314,6 +314,13 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
  *                           bits, run one kernel per pass with decoupled
  *                           look-back (onesweep); the others run
  *                           reduce-then-scan passes (default 64)
+ *   "sort.rank_max"    0-1048576  sorts of at most rank_max x passes / 8
+ *                           pairs (passes = digit passes the key needs) place
+ *                           each pair at its counted rank (keys below it plus
+ *                           equal keys before it) in one pass instead of
+ *                           digit passes; the order is the same (default
+ *                           28000: 60-bit keys in 8-bit digits up to 28,000
+ *                           pairs, 30-bit keys up to 14,000; 0 off)
  *   "trace.group64"    >= 0 4-wide traces of at most this many rays run one ray
  *   "trace.group16"         per group of 64 / 16 / 4 lanes: the group splits
  *   "trace.group4"          the top of the hierarchy into subtrees walked in

@@ -79,11 +79,29 @@ SORT_CASES = [
 ]
 
 
+@pytest.fixture
+def rank_max():
+    """Sets sort.rank_max for one test and restores it."""
+    import ctypes as C
+
+    from vulkanwavefrontpathtracer_amd import _capi
+
+    cur = C.c_int()
+    _capi.call("wf_get_tuning", b"sort.rank_max", C.byref(cur))
+    yield lambda v: _capi.call("wf_set_tuning", b"sort.rank_max", v)
+    _capi.call("wf_set_tuning", b"sort.rank_max", cur.value)
+
+
+@pytest.mark.parametrize("rank", [0, 1 << 20])
 @pytest.mark.parametrize("dtype,bits,digit", SORT_CASES)
-@pytest.mark.parametrize("n", [0, 1, 2, 4095, 4096, 4097, 8191, 8192, 8193, 70_001])
-def test_sort_bit_exact(dev, orc, dtype, bits, digit, n):
+@pytest.mark.parametrize("n", [0, 1, 2, 63, 64, 65, 4095, 4096, 4097, 8191, 8192, 8193, 70_001])
+def test_sort_bit_exact(dev, orc, rank_max, rank, dtype, bits, digit, n):
+    """Digit passes (rank = 0: single workgroup, onesweep) and the counted
+    rank (rank = 2^20: every size here) give the oracle's stable order; keys
+    carry bits above the sorted ones, which the order ignores."""
     from vulkanwavefrontpathtracer_amd import stages
 
+    rank_max(rank)
     rng = np.random.default_rng(n * 7 + bits + digit)
     keys = rng.integers(0, np.iinfo(dtype).max, size=n, dtype=dtype, endpoint=True)
     if n > 10:
@@ -116,13 +134,14 @@ def onesweep_tiles():
 @pytest.mark.parametrize("dtype,bits,digit", [(np.uint32, 30, 8), (np.uint64, 60, 8),
                                               (np.uint64, 64, 4), (np.uint32, 32, 11)])
 @pytest.mark.parametrize("n", [8193, 70_001, 300_017])
-def test_sort_paths_bit_exact(dev, orc, onesweep_tiles, tiles, dtype, bits, digit, n):
+def test_sort_paths_bit_exact(dev, orc, onesweep_tiles, rank_max, tiles, dtype, bits, digit, n):
     """Onesweep passes (tiles = 2^20: every size with digits of at most 8
     bits) and reduce-then-scan passes (tiles = 0) both give the oracle's
     stable order."""
     from vulkanwavefrontpathtracer_amd import stages
 
     onesweep_tiles(tiles)
+    rank_max(0)
     rng = np.random.default_rng(n + bits + digit)
     keys = rng.integers(0, np.iinfo(dtype).max, size=n, dtype=dtype, endpoint=True)
     keys[: n // 5] = keys[1]
@@ -171,12 +190,17 @@ def test_sort_full_size_properties(dev, onesweep_tiles, tiles, dtype, bits, digi
                                               (np.uint64, 64, 4), (np.uint32, 32, 11)])
 @pytest.mark.parametrize("capacity,count", [(8192, 5000), (8192, 8192), (8192, 0),
                                             (20_000, 8000), (70_001, 9000), (70_001, 70_001)])
-def test_sort_count_on_device_bit_exact(dev, orc, dtype, bits, digit, capacity, count):
+@pytest.mark.parametrize("rank", [0, 1 << 20])
+def test_sort_count_on_device_bit_exact(dev, orc, rank_max, rank, dtype, bits, digit, capacity,
+                                        count):
     """wf_sort_pairs_count sorts the first *d_count pairs of a capacity-sized
-    buffer (the count stays on the device) and leaves the rest untouched."""
+    buffer (the count stays on the device) and leaves the rest untouched,
+    through digit passes (rank = 0) and through the counted rank."""
     import ctypes as C
 
     from vulkanwavefrontpathtracer_amd import _capi
+
+    rank_max(rank)
 
     rng = np.random.default_rng(capacity + count + bits)
     keys = rng.integers(0, np.iinfo(dtype).max, size=capacity, dtype=dtype, endpoint=True)

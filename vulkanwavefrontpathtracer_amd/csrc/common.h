@@ -159,6 +159,7 @@ struct Tuning {
     int trace_refill = 24;      // idle lanes of a wave that trigger a refill
     int trace_width = 4;        // traverse the 2-wide or the 4-wide hierarchy
     int sort_onesweep_tiles = 64;  // sorts of at most this many 4096-element tiles use onesweep
+    int sort_rank_max = 28000;  // sorts of at most this many pairs x passes / 8 count each rank directly (0 off)
     int trace_group64 = 5000;   // wavefronts of at most this many rays: 64 lanes per ray
     int trace_group16 = 24000;  // ... 16 lanes per ray
     int trace_group4 = 64000;   // ... 4 lanes per ray

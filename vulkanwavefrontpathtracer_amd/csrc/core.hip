@@ -102,6 +102,7 @@ extern "C" int wf_set_tuning(const char* key, int value) {
     else if (k == "trace.group16" && value >= 0) t.trace_group16 = value;
     else if (k == "trace.group4" && value >= 0) t.trace_group4 = value;
     else if (k == "sort.onesweep_tiles" && value >= 0) t.sort_onesweep_tiles = value;
+    else if (k == "sort.rank_max" && value >= 0 && value <= 1048576) t.sort_rank_max = value;
     else return fail(WF_ERR_INVALID_ARG, "wf_set_tuning: unknown key or value out of range: " + k);
     return WF_OK;
 }
@@ -135,6 +136,7 @@ extern "C" int wf_get_tuning(const char* key, int* value) {
     else if (k == "trace.group16") *value = t.trace_group16;
     else if (k == "trace.group4") *value = t.trace_group4;
     else if (k == "sort.onesweep_tiles") *value = t.sort_onesweep_tiles;
+    else if (k == "sort.rank_max") *value = t.sort_rank_max;
     else return fail(WF_ERR_INVALID_ARG, "wf_get_tuning: unknown key " + k);
     return WF_OK;
 }

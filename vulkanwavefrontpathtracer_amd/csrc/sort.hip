@@ -656,6 +656,78 @@ __global__ __launch_bounds__(SMALL_BLOCK) void sort_small_kernel(K* __restrict__
     }
 }
 
+// ------------------------------------------------------------------- rank
+// Rank sort for small pair counts.  A stable sort puts element i at
+//   #{j : key_j < key_i} + #{j < i : key_j == key_i}
+// (keys compared on their low key_bits bits, as the digit passes do), so
+// that position is counted directly: n^2 compares in one pass, spread over
+// the whole GPU, where the digit passes of a few thousand pairs are each
+// bound by launch and memory latency (several passes) or by one CU (the
+// single-workgroup sort).  A workgroup takes 64 elements, one per lane;
+// its 16 waves split the compared keys, which are wave-uniform (scalar
+// loads), and add their partial counts in LDS.
+constexpr int RANK_BLOCK = 1024;
+constexpr int RANK_WAVES = RANK_BLOCK / 64;
+
+// Keys of [j0, j1) below k (LE: at most k), counted per lane.
+template <typename K, bool LE>
+__device__ __forceinline__ uint32_t rank_count(const K* __restrict__ keys, int64_t j0,
+                                               int64_t j1, K k, K kmask) {
+    constexpr int U = 16;  // keys per block: two 64-byte scalar loads, one wait
+    uint32_t c = 0;
+    int64_t j = j0;
+    for (; j + U <= j1; j += U) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const K kj = keys[j + u] & kmask;
+            c += LE ? (kj <= k) : (kj < k);
+        }
+    }
+    for (; j < j1; ++j) {
+        const K kj = keys[j] & kmask;
+        c += LE ? (kj <= k) : (kj < k);
+    }
+    return c;
+}
+
+template <typename K>
+__global__ __launch_bounds__(RANK_BLOCK) void sort_rank_kernel(
+    const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, K* __restrict__ keys_out,
+    uint32_t* __restrict__ vals_out, int64_t n, int key_bits, const uint32_t* __restrict__ d_n) {
+    __shared__ uint32_t s_cnt[RANK_WAVES][64];
+    if (d_n) n = min(n, (int64_t)*d_n);
+    const int64_t e0 = (int64_t)blockIdx.x * 64;
+    if (e0 >= n) return;  // uniform
+    const unsigned lane = lane_id();
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const K kmask = key_bits >= (int)(8 * sizeof(K)) ? ~K(0) : (K(1) << key_bits) - K(1);
+    const int64_t e = e0 + lane;
+    const bool ok = e < n;
+    const K kraw = ok ? keys_in[e] : K(0);
+    const K k = kraw & kmask;
+    // this wave's keys [j0, j1): those before the workgroup's elements
+    // count when at most k, those after when below k, and the 64 of the
+    // workgroup by the full rule
+    const int64_t per = (n + RANK_WAVES - 1) / RANK_WAVES;
+    const int64_t j0 = min(n, (int64_t)wave * per), j1 = min(n, j0 + per);
+    const int64_t m0 = max(j0, e0), m1 = min(j1, e0 + 64);
+    uint32_t c = rank_count<K, true>(keys_in, j0, min(j1, e0), k, kmask) +
+                 rank_count<K, false>(keys_in, max(j0, e0 + 64), j1, k, kmask);
+    for (int64_t j = m0; j < m1; ++j) {
+        const K kj = keys_in[j] & kmask;
+        c += (kj < k) || (kj == k && j < e);
+    }
+    s_cnt[wave][lane] = c;
+    __syncthreads();
+    if (wave == 0 && ok) {
+        uint32_t r = 0;
+#pragma unroll
+        for (int w = 0; w < RANK_WAVES; ++w) r += s_cnt[w][lane];
+        keys_out[r] = kraw;
+        vals_out[r] = vals_in[e];
+    }
+}
+
 template <typename K>
 __global__ __launch_bounds__(256) void copy_pairs_kernel(const K* __restrict__ ks,
                                                          const uint32_t* __restrict__ vs,
@@ -790,6 +862,25 @@ int sort_pairs_launch(void* keys, uint32_t* vals, int64_t n_cap, const uint32_t*
     WF_CHECK_ARG(keys && vals && scratch, "sort: null buffer");
     if (scratch_bytes < p.total)
         return fail(WF_ERR_SCRATCH_TOO_SMALL, "sort: scratch smaller than wf_sort_scratch_bytes");
+    // the counted rank costs ~n^2, the digit passes ~passes: the crossover
+    // scales with the pass count (28k pairs at 8 passes, 14k at 4)
+    if (n_cap <= (int64_t)tuning().sort_rank_max * p.passes / 8) {
+        // rank into the first scratch pair, then back into the caller's
+        char* sc = static_cast<char*>(scratch);
+        void* ka = sc + p.off_keys_a;
+        uint32_t* va = reinterpret_cast<uint32_t*>(sc + p.off_vals_a);
+        const unsigned g = (unsigned)((n_cap + 63) / 64);
+        if (key_bytes == 4) {
+            sort_rank_kernel<uint32_t><<<g, RANK_BLOCK, 0, s>>>((const uint32_t*)keys, vals, (uint32_t*)ka, va, n_cap, key_bits, d_n);
+            WF_TRY(launch_check("sort_rank_kernel"));
+            copy_pairs_kernel<uint32_t><<<grid_for(n_cap, 256), 256, 0, s>>>((const uint32_t*)ka, va, (uint32_t*)keys, vals, n_cap, d_n);
+        } else {
+            sort_rank_kernel<uint64_t><<<g, RANK_BLOCK, 0, s>>>((const uint64_t*)keys, vals, (uint64_t*)ka, va, n_cap, key_bits, d_n);
+            WF_TRY(launch_check("sort_rank_kernel"));
+            copy_pairs_kernel<uint64_t><<<grid_for(n_cap, 256), 256, 0, s>>>((const uint64_t*)ka, va, (uint64_t*)keys, vals, n_cap, d_n);
+        }
+        return launch_check("copy_pairs_kernel");
+    }
     if (n_cap <= SMALL_N && digit_bits <= 8) {
         if (n_cap <= SMALL_N / 2)
             launch_small<4>(keys, vals, n_cap, key_bytes, key_bits, digit_bits, d_n, s);
