@@ -3,18 +3,22 @@
 // Replaces the vulkan_radix_sort dependency called by Tracer::radixSort
 // (projects/WaveFrontPathTracer/Tracer/Tracer.cpp:39-78 ->
 // external/vulkan_radix_sort/vk_radix_sort.h:2579-2585), which runs 4 fixed
-// 8-bit passes on u32 keys.  Here:
-// per pass, reduce-then-scan with no dependency between the workgroups of a
-// kernel: an upsweep counts each 4096-element tile's digits, a spine scans
-// every digit's row of tile counts (one workgroup per digit) and writes the
-// digit's total, and a downsweep scans the digit totals into digit starts,
-// ranks its tile stably with wave64 ballot matching, stages it in LDS in
-// digit order and writes each digit run contiguously to its position.
-// Decoupled look-back (single-pass) was measured slower here: its chained
-// status reads cross the XCDs' separate L2 caches and, with ~1000 tiles
-// resident at once, walk hundreds of predecessors.
-// HBM traffic per element and pass: a key read (upsweep) and one read and
-// one write of key and value (downsweep).
+// 8-bit passes on u32 keys.  The structure follows the pair count (any
+// structure gives the same stable order):
+// - large sorts, per pass reduce-then-scan with no dependency between the
+//   workgroups of a kernel: an upsweep counts each 4096-element tile's
+//   digits, a spine scans every digit's row of tile counts (one workgroup
+//   per digit) and writes the digit's total, and a downsweep scans the digit
+//   totals into digit starts, ranks its tile stably with wave64 ballot
+//   matching, stages it in LDS in digit order and writes each digit run
+//   contiguously to its position.  HBM traffic per element and pass: a key
+//   read (upsweep) and one read and one write of key and value (downsweep).
+//   Decoupled look-back measured slower at these sizes: its chained status
+//   reads cross the XCDs' separate L2 caches and, with ~500-2000 tiles
+//   resident at once, walk hundreds of predecessors;
+// - up to 64 tiles, onesweep: one kernel per pass with decoupled look-back;
+// - up to 8192 pairs, one workgroup runs every pass in LDS;
+// - up to ~28k pairs (scaled by the pass count), a one-pass counted rank.
 #include "common.h"
 
 namespace wfpt {
