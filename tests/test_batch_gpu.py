@@ -89,3 +89,31 @@ def test_batch_autotune_keeps_results(dev, small_batch, expected):
     b.render()
     for k in range(len(views)):
         assert np.array_equal(b.pixels(k)[:, :3], expected[k][0][:, :3])
+
+
+def test_batch_graph_recaptured_after_tuning_changes(dev, small_batch, expected):
+    """A captured pass points at the sort scratch and reads the process-wide
+    knobs at capture time: growing the scratch (11-bit digits) and going
+    back, or changing a knob with wf_set_tuning, must recapture rather than
+    replay a graph that points at freed memory or stale knobs."""
+    from vulkanwavefrontpathtracer_amd import stages
+    from vulkanwavefrontpathtracer_amd.batch import Batch
+
+    hosts, scenes, views = small_batch
+    b = Batch(scenes, views, depth=DEPTH, shadow_samples=S)
+    b.set_tuning(8, 64, 1)
+    b.set_graph(True)
+    b.render()
+    b.set_tuning(11, 64, 1)
+    b.set_tuning(8, 64, 1)
+    b.render()
+    for k in range(len(views)):
+        assert np.array_equal(b.pixels(k)[:, :3], expected[k][0][:, :3]), k
+    old = stages.get_tuning("sort.onesweep_tiles")
+    try:
+        stages.set_tuning("sort.onesweep_tiles", 0)
+        b.render()
+        for k in range(len(views)):
+            assert np.array_equal(b.pixels(k)[:, :3], expected[k][0][:, :3]), k
+    finally:
+        stages.set_tuning("sort.onesweep_tiles", old)

@@ -210,6 +210,39 @@ def test_errors(tmp_path):
     p.write_text(json.dumps(d))
     assert _status(lambda: load(p)) == -6
 
+    # Malformed offsets, lengths, strides and counts: each would address
+    # memory outside the buffer if the bounds sums could wrap.
+    def tri_file(name, edit):
+        b = GltfBuilder()
+        pos = b.vec3(np.eye(3))
+        m = b.mesh([{"attributes": {"POSITION": pos}, "indices": b.indices([0, 1, 2], np.uint16)}])
+        b.doc["scenes"] = [{"nodes": [b.node(mesh=m)]}]
+        q = b.write(tmp_path / name)
+        d = json.loads(q.read_text())
+        edit(d, d["bufferViews"][d["accessors"][pos]["bufferView"]], d["accessors"][pos])
+        q.write_text(json.dumps(d))
+        return q
+
+    cases = {
+        "neg_view_offset": lambda d, v, a: v.update(byteOffset=-16, byteLength=v["byteLength"] + 16),
+        "neg_view_length": lambda d, v, a: v.update(byteLength=-1),
+        "neg_acc_offset": lambda d, v, a: a.update(byteOffset=-4),
+        "neg_stride": lambda d, v, a: v.update(byteStride=-12),
+        "short_stride": lambda d, v, a: v.update(byteStride=4),
+        "huge_count": lambda d, v, a: a.update(count=(1 << 62) // 3),
+        "wrap_count": lambda d, v, a: a.update(count=(1 << 64) // 12 + 1),
+        "view_past_buffer": lambda d, v, a: v.update(byteOffset=(1 << 63) - 8),
+        "acc_past_view": lambda d, v, a: a.update(byteOffset=v["byteLength"]),
+    }
+    for name, edit in cases.items():
+        # -5: a count too large to index is caught by the size pass first
+        assert _status(lambda: load(tri_file(name + ".gltf", edit))) in (-5, -6), name
+    assert load(tri_file("ok.gltf", lambda d, v, a: None)).tri_verts().shape[0] == 1
+
+    deep = tmp_path / "deep.gltf"
+    deep.write_text("[" * 100_000 + "]" * 100_000)
+    assert _status(lambda: load(deep)) == -6
+
 
 def test_procedural_scene_round_trip(tmp_path):
     """A procedural scene exported one primitive per geometry loads back with

@@ -174,4 +174,7 @@ struct Tuning {
     int trace_xcd = 0;          // loop 1 walk: runs of this many workgroups per XCD (0 off)
 };
 Tuning& tuning();
+// Incremented by every successful wf_set_tuning: a captured graph records the
+// knobs read at capture time, so its owner recaptures when this changes.
+uint64_t tuning_generation();
 }  // namespace wfpt

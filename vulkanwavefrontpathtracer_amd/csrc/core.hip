@@ -71,6 +71,8 @@ Tuning& tuning() {
     static Tuning t;
     return t;
 }
+static uint64_t g_tuning_generation = 0;
+uint64_t tuning_generation() { return g_tuning_generation; }
 }  // namespace wfpt
 
 extern "C" int wf_set_tuning(const char* key, int value) {
@@ -104,6 +106,7 @@ extern "C" int wf_set_tuning(const char* key, int value) {
     else if (k == "sort.onesweep_tiles" && value >= 0) t.sort_onesweep_tiles = value;
     else if (k == "sort.rank_max" && value >= 0 && value <= 1048576) t.sort_rank_max = value;
     else return fail(WF_ERR_INVALID_ARG, "wf_set_tuning: unknown key or value out of range: " + k);
+    ++g_tuning_generation;
     return WF_OK;
 }
 

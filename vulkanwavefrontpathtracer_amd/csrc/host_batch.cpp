@@ -34,7 +34,11 @@ struct wf_batch_s {
     wfpt::BatchTuning tuning;
     bool use_graph = true;
     hipGraphExec_t graph = nullptr;
+    // What the captured graph was recorded with: the batch tuning, the sort
+    // scratch it points at and the process-wide knobs' generation.
     wfpt::BatchTuning graph_tuning;
+    void* graph_scratch = nullptr;
+    uint64_t graph_generation = 0;
     uint64_t totals[3] = {0, 0, 0};
 
     ~wf_batch_s() {
@@ -78,18 +82,21 @@ static BatchBuffers buffers(wf_batch_s* b) {
     return bb;
 }
 
-static int ensure_sort_scratch(wf_batch_s* b, int digit_bits) {
-    const int64_t need = wf_sort_scratch_bytes(b->total_pixels, 8, 64, digit_bits);
-    if (need < 0) return (int)need;
-    return b->sort_scratch.ensure((size_t)need);
-}
-
 static int drop_graph(wf_batch_s* b) {
     if (b->graph) {
         WF_HIP(hipGraphExecDestroy(b->graph));
         b->graph = nullptr;
     }
     return WF_OK;
+}
+
+// Growing the scratch frees the old allocation, which a captured graph may
+// still point at: the graph goes with it.
+static int ensure_sort_scratch(wf_batch_s* b, int digit_bits) {
+    const int64_t need = wf_sort_scratch_bytes(b->total_pixels, 8, 64, digit_bits);
+    if (need < 0) return (int)need;
+    if ((size_t)need > b->sort_scratch.bytes) WF_TRY(drop_graph(b));
+    return b->sort_scratch.ensure((size_t)need);
 }
 
 // Capture one pass for the current tuning (replayed for every pass).
@@ -113,6 +120,8 @@ static int capture_pass(wf_batch_s* b) {
         return fail(WF_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     }
     b->graph_tuning = b->tuning;
+    b->graph_scratch = b->sort_scratch.p;
+    b->graph_generation = tuning_generation();
     return WF_OK;
 }
 
@@ -126,7 +135,9 @@ static int run_pass(wf_batch_s* b, int frame, int pass) {
     b->tuning.width = tuning().trace_width;
     WF_TRY(batch_set_pass(bb, frame, pass, b->pass_count[pass], b->depth, b->stream));
     if (b->use_graph) {
-        if (!b->graph || !same_tuning(b->graph_tuning, b->tuning)) WF_TRY(capture_pass(b));
+        if (!b->graph || !same_tuning(b->graph_tuning, b->tuning) ||
+            b->graph_scratch != b->sort_scratch.p || b->graph_generation != tuning_generation())
+            WF_TRY(capture_pass(b));
         WF_HIP(hipGraphLaunch(b->graph, b->stream));
         return WF_OK;
     }

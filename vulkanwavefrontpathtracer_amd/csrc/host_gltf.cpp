@@ -301,15 +301,23 @@ class Loader {
         WF_TRY(buffer(inum(bv->find("buffer"), -1), &buf));
         const size_t comp = ct == GL_UNSIGNED_BYTE ? 1 : ct == GL_UNSIGNED_SHORT ? 2 : 4;
         const size_t elem = comp * (size_t)want_type_size;
-        const size_t view_off = (size_t)inum(bv->find("byteOffset"), 0);
-        const size_t view_len = (size_t)inum(bv->find("byteLength"), 0);
-        const size_t acc_off = (size_t)inum(a->find("byteOffset"), 0);
-        const size_t stride = (size_t)inum(bv->find("byteStride"), 0);
-        v.stride = stride ? stride : elem;
-        const size_t last = acc_off + (size_t)(v.count - 1) * v.stride + elem;
-        if (view_off + view_len > buf->size() || last > view_len)
-            return fail(WF_ERR_PARSE, path_ + ": " + what + " accessor runs past its buffer view");
-        v.data = buf->data() + view_off + acc_off;
+        const long long view_off = inum(bv->find("byteOffset"), 0);
+        const long long view_len = inum(bv->find("byteLength"), -1);
+        const long long acc_off = inum(a->find("byteOffset"), 0);
+        const long long stride = inum(bv->find("byteStride"), 0);
+        if (view_off < 0 || view_len < 0 || acc_off < 0 || stride < 0)
+            return fail(WF_ERR_PARSE, path_ + ": " + what + " accessor or view has a negative offset, length or stride");
+        v.stride = stride ? (size_t)stride : elem;
+        if (v.stride < elem)
+            return fail(WF_ERR_PARSE, path_ + ": " + what + " view's byteStride is shorter than an element");
+        // Every comparison below is free of overflow: each right-hand side is
+        // a difference of sizes already known to be ordered.
+        const size_t size = buf->size(), off = (size_t)view_off, len = (size_t)view_len,
+                     aoff = (size_t)acc_off;
+        const bool fits = off <= size && len <= size - off && aoff <= len && elem <= len - aoff &&
+                          (size_t)(v.count - 1) <= (len - aoff - elem) / v.stride;
+        if (!fits) return fail(WF_ERR_PARSE, path_ + ": " + what + " accessor runs past its buffer view");
+        v.data = buf->data() + off + aoff;
         return WF_OK;
     }
 

@@ -34,11 +34,19 @@ class JParser {
         ws();
         return i_ == s_.size();
     }
-    std::string error() const { return "JSON parse error at offset " + std::to_string(i_); }
+    std::string error() const {
+        return depth_ > MAX_DEPTH ? "JSON nesting deeper than " + std::to_string(MAX_DEPTH) + " levels"
+                                  : "JSON parse error at offset " + std::to_string(i_);
+    }
+
+    // Arrays and objects nest at most this deep (the parser recurses once
+    // per level; deeper input is rejected instead of exhausting the stack).
+    static constexpr int MAX_DEPTH = 256;
 
   private:
     const std::string& s_;
     size_t i_ = 0;
+    int depth_ = 0;
     void ws() {
         while (i_ < s_.size() && std::isspace((unsigned char)s_[i_])) ++i_;
     }
@@ -51,8 +59,12 @@ class JParser {
     bool value(JValue& v) {
         if (i_ >= s_.size()) return false;
         char c = s_[i_];
-        if (c == '{') return object(v);
-        if (c == '[') return array(v);
+        if (c == '{' || c == '[') {
+            if (++depth_ > MAX_DEPTH) return false;
+            const bool ok = c == '{' ? object(v) : array(v);
+            --depth_;
+            return ok;
+        }
         if (c == '"') {
             v.kind = JValue::STR;
             return string(v.text);

@@ -339,3 +339,15 @@ def count_hits(hits):
     c = torch.zeros(1, dtype=torch.int32, device=hits.device)
     call("wf_count_hits", _ptr(hits), hits.shape[0], _ptr(c), _stream(), None)
     return int(c.item())
+
+
+def set_tuning(key: str, value: int) -> None:
+    """wf_set_tuning: set one process-wide knob (e.g. "sort.onesweep_tiles")."""
+    call("wf_set_tuning", key.encode(), int(value))
+
+
+def get_tuning(key: str) -> int:
+    """wf_get_tuning: read one process-wide knob."""
+    v = C.c_int()
+    call("wf_get_tuning", key.encode(), C.byref(v))
+    return v.value
