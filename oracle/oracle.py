@@ -306,3 +306,58 @@ def render_frame_f64(tri_verts, tri_n, tri_mat, albedo, emission, *, w, h, spp, 
     load().or_render_frame_f64(_p(tv), _p(tn), _p(tm), C.c_int64(len(tv)), _p(al), _p(em),
                                C.c_int(len(al)), C.byref(cfg), _p(out), _p(amb), _p(dp), _p(ga), _p(ge))
     return out, amb.astype(bool), ga, ge
+
+
+def trace_bvh_f32(rays, tri_verts, closest=True):
+    """fp32 watertight trace over a BVH (large scenes); same records as
+    trace_f32."""
+    r = _rays(rays)
+    tv = np.ascontiguousarray(tri_verts, np.float32).reshape(-1, 9)
+    out = np.zeros((len(r), 4), np.int32)
+    load().or_trace_bvh_f32(_p(r), C.c_int64(len(r)), _p(tv), C.c_int64(len(tv)),
+                            C.c_int(int(closest)), _p(out))
+    return out
+
+
+def lcg(seed, n):
+    """n draws of raygenPath's LCG from seed: (states uint32, floats)."""
+    st = np.zeros(n, np.uint32)
+    r = np.zeros(n, np.float32)
+    load().or_lcg(C.c_uint32(seed), C.c_int64(n), _p(st), _p(r))
+    return st, r
+
+
+def duff_basis(n, f64=False):
+    """(b1, b2) of the Duff et al. basis around unit normal n."""
+    dt = np.float64 if f64 else np.float32
+    nn = np.ascontiguousarray(n, dt)
+    b1 = np.zeros(3, dt)
+    b2 = np.zeros(3, dt)
+    (load().or_duff_basis_f64 if f64 else load().or_duff_basis)(_p(nn), _p(b1), _p(b2))
+    return b1, b2
+
+
+def cosine_dir(r0, r1, n, f64=False):
+    """raygenPath's cosine-weighted bounce direction about unit normal n."""
+    dt = np.float64 if f64 else np.float32
+    nn = np.ascontiguousarray(n, dt)
+    out = np.zeros(3, dt)
+    if f64:
+        load().or_cosine_dir_f64(C.c_double(r0), C.c_double(r1), _p(nn), _p(out))
+    else:
+        load().or_cosine_dir(C.c_float(r0), C.c_float(r1), _p(nn), _p(out))
+    return out
+
+
+def shadow_point(seed, pixel, i, S):
+    """Shadow sample i of S of a pixel in [-1,1]^3 (Sobol x/y, Hammersley z,
+    Cranley-Patterson offset from the Jenkins-mixed pixel seed)."""
+    p = np.zeros(3, np.float32)
+    load().or_shadow_point(C.c_uint32(seed), C.c_int(pixel), C.c_int(i), C.c_int(S), _p(p))
+    return p
+
+
+def jenkins(a, b, c):
+    abc = np.array([a, b, c], np.uint32)
+    load().or_jenkins(_p(abc))
+    return tuple(int(x) for x in abc)

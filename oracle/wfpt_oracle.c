@@ -402,6 +402,14 @@ static float randf(uint32_t* seed) {
     *seed = 1103515245u * *seed + 12345u;
     return (float)(*seed & 0x00FFFFFFu) / 16777216.0f;
 }
+/* n successive draws from seed: the generator states and the floats. */
+OR_API void or_lcg(uint32_t seed, int64_t n, uint32_t* states, float* r) {
+    for (int64_t i = 0; i < n; ++i) {
+        float x = randf(&seed);
+        if (states) states[i] = seed;
+        if (r) r[i] = x;
+    }
+}
 
 /* raygenPrimary.comp:11-22 */
 static float halton(int i, int base) {
@@ -481,6 +489,10 @@ static void shadow_point(uint32_t seed, int pixel, int i, int S, float p[3]) {
     }
 }
 
+OR_API void or_shadow_point(uint32_t seed, int pixel, int i, int S, float p[3]) {
+    shadow_point(seed, pixel, i, S, p);
+}
+
 OR_API void or_raygen_shadow(const uint32_t* seeds, const float* in_rays, const int32_t* in_hits,
                              const int32_t* in_s2i, int64_t begin, int64_t end, int S,
                              const float* light, float radius, float* out_rays, int32_t* out_s2i,
@@ -528,16 +540,47 @@ static int shading_normal(const float* tri_n, const int32_t* tri_mat, int64_t tr
     return tri_mat[tri];
 }
 
-/* raygenPath.comp:28-49 (fp32; sinf/cosf of the host libm) */
+/* raygenPath.comp:28-37: the branchless orthonormal basis of Duff et al.
+ * 2017 around the unit normal n (fp32). */
+static void duff_basis(const float n[3], float b1[3], float b2[3]) {
+    float s = n[2] >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (s + n[2]), b = n[0] * n[1] * a;
+    b1[0] = 1.0f + s * n[0] * n[0] * a; b1[1] = s * b; b1[2] = -s * n[0];
+    b2[0] = b; b2[1] = s + n[1] * n[1] * a; b2[2] = -n[1];
+}
+OR_API void or_duff_basis(const float n[3], float b1[3], float b2[3]) { duff_basis(n, b1, b2); }
+
+/* raygenPath.comp:39-49 (fp32; sinf/cosf of the host libm): cosine-weighted
+ * direction about n from two uniform draws. */
 static void cosine_dir(float r0, float r1, const float n[3], float out[3]) {
     const float two_pi = 2.0f * 3.14159265358979323846f;
     float theta = two_pi * r1, radius = sqrtf(r0);
     float x = radius * sinf(theta), z = radius * cosf(theta), y = sqrtf(1.0f - r0);
-    float s = n[2] >= 0.0f ? 1.0f : -1.0f;
-    float a = -1.0f / (s + n[2]), b = n[0] * n[1] * a;
-    float b1[3] = {1.0f + s * n[0] * n[0] * a, s * b, -s * n[0]};
-    float b2[3] = {b, s + n[1] * n[1] * a, -n[1]};
+    float b1[3], b2[3];
+    duff_basis(n, b1, b2);
     for (int k = 0; k < 3; ++k) out[k] = x * b1[k] + z * b2[k] + y * n[k];
+}
+OR_API void or_cosine_dir(float r0, float r1, const float n[3], float out[3]) {
+    cosine_dir(r0, r1, n, out);
+}
+
+/* The same basis and direction in fp64 (the fp64 frame oracle's bounce). */
+static void duff_basis_f64(const double n[3], double b1[3], double b2[3]) {
+    double s = n[2] >= 0.0 ? 1.0 : -1.0;
+    double a = -1.0 / (s + n[2]), b = n[0] * n[1] * a;
+    b1[0] = 1.0 + s * n[0] * n[0] * a; b1[1] = s * b; b1[2] = -s * n[0];
+    b2[0] = b; b2[1] = s + n[1] * n[1] * a; b2[2] = -n[1];
+}
+static void cosine_dir_f64(double r0, double r1, const double n[3], double out[3]) {
+    double theta = 2.0 * 3.14159265358979323846 * r1, rad = sqrt(r0);
+    double x = rad * sin(theta), z = rad * cos(theta), y = sqrt(1.0 - r0);
+    double b1[3], b2[3];
+    duff_basis_f64(n, b1, b2);
+    for (int k = 0; k < 3; ++k) out[k] = x * b1[k] + z * b2[k] + y * n[k];
+}
+OR_API void or_duff_basis_f64(const double n[3], double b1[3], double b2[3]) { duff_basis_f64(n, b1, b2); }
+OR_API void or_cosine_dir_f64(double r0, double r1, const double n[3], double out[3]) {
+    cosine_dir_f64(r0, r1, n, out);
 }
 
 /* raygenPath.comp:78-170 with the deterministic (input-order) compaction
@@ -1095,14 +1138,8 @@ OR_API void or_render_frame_f64(const float* tv, const float* tri_n, const int32
                     if (mt < 0 || mt >= nmat) mt = 0;
                     double r0 = randf(&seed), r1 = randf(&seed);
                     double p[3] = {o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t};
-                    double theta = 2.0 * 3.14159265358979323846 * r1, rad = sqrt(r0);
-                    double xx = rad * sin(theta), zz = rad * cos(theta), yy = sqrt(1.0 - r0);
-                    double s = n[2] >= 0.0 ? 1.0 : -1.0;
-                    double a = -1.0 / (s + n[2]), bb = n[0] * n[1] * a;
-                    double b1[3] = {1.0 + s * n[0] * n[0] * a, s * bb, -s * n[0]};
-                    double b2[3] = {bb, s + n[1] * n[1] * a, -n[1]};
                     double nd[3];
-                    for (int k = 0; k < 3; ++k) nd[k] = xx * b1[k] + zz * b2[k] + yy * n[k];
+                    cosine_dir_f64(r0, r1, n, nd);
                     double w[3] = {albedo[3 * mt], albedo[3 * mt + 1], albedo[3 * mt + 2]};
                     if (c->rr && bounce >= c->rr_start) {
                         double q = fmax(fmax(dec[0] * w[0], dec[1] * w[1]), dec[2] * w[2]);
@@ -1165,6 +1202,96 @@ OR_API void or_trace_bvh_f64(const float* rays, int64_t n, const float* tri_vert
         int hit = ntri > 0 ? cb_trace(&b, o, d, r[3], r[7], !closest, &t, &u, &v, NULL, NULL, NULL) : -1;
         out_t[i] = hit >= 0 ? t : -1.0;
         out_tri[i] = hit;
+    }
+    free(b.nodes);
+    free(b.order);
+}
+
+/* fp32 watertight trace over the median-split BVH, for scenes too large for
+ * the brute-force scan of or_trace_f32 (trace.comp:27-88 semantics): each
+ * candidate triangle goes through the same wt_intersect and the same
+ * interval and tie rules, so the records equal or_trace_f32's exactly,
+ * provided no box rejects a triangle the fp32 test would accept.  The boxes
+ * are therefore widened by 1e-5 of the coordinate magnitudes in play (the
+ * fp32 test's effective triangle and t deviate from the exact ones by a few
+ * ulps, ~5e-7 relative) and the slab interval is tested in fp64 with slack.
+ * Any hit: the smallest triangle index hit within the interval, which is
+ * what the brute-force scan returns.  Output layout as or_trace_f32. */
+OR_API void or_trace_bvh_f32(const float* rays, int64_t n, const float* tri_verts, int64_t ntri,
+                             int closest, int32_t* out) {
+    cbvh b;
+    b.tv = tri_verts;
+    b.nodes = (bnode*)malloc(sizeof(bnode) * (2 * ntri + 1));
+    b.order = (int*)malloc(sizeof(int) * (ntri + 1));
+    b.nn = 0;
+    for (int64_t i = 0; i < ntri; ++i) b.order[i] = (int)i;
+    if (ntri > 0) cb_build(&b, 0, (int)ntri);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i) {
+        const float* ry = rays + 8 * i;
+        const float tmin = ry[3], tmax = ry[7];
+        int32_t best = -1;
+        float best_t = tmax, bv = 0, bw = 0, bd = 1;
+        wt_ray w;
+        if (ntri > 0 && tmax >= tmin && wt_setup(ry, ry + 4, &w)) {
+            const double o[3] = {ry[0], ry[1], ry[2]};
+            const double d[3] = {ry[4], ry[5], ry[6]};
+            double inv[3];
+            for (int a = 0; a < 3; ++a)
+                inv[a] = 1.0 / (fabs(d[a]) > 1e-300 ? d[a] : copysign(1e-300, d[a]));
+            const double omag = fmax(fmax(fabs(o[0]), fabs(o[1])), fabs(o[2]));
+            int stack[128], sp = 0;
+            stack[sp++] = 0;
+            while (sp) {
+                const bnode* nd = &b.nodes[stack[--sp]];
+                double m = omag;
+                for (int a = 0; a < 3; ++a) m = fmax(m, fmax(fabs(nd->lo[a]), fabs(nd->hi[a])));
+                const double grow = 1e-5 * (m + 1.0);
+                double tn = (double)tmin, tf = closest ? (double)best_t : (double)tmax;
+                tn -= 1e-5 * (fabs(tn) + 1.0);
+                tf += 1e-5 * (fabs(tf) + 1.0);
+                int miss = 0;
+                for (int a = 0; a < 3; ++a) {
+                    double t0 = (nd->lo[a] - grow - o[a]) * inv[a], t1 = (nd->hi[a] + grow - o[a]) * inv[a];
+                    tn = fmax(tn, fmin(t0, t1));
+                    tf = fmin(tf, fmax(t0, t1));
+                    if (tn > tf) { miss = 1; break; }
+                }
+                if (miss) continue;
+                if (nd->left >= 0) {
+                    if (sp + 2 > 128) abort();
+                    stack[sp++] = nd->right;
+                    stack[sp++] = nd->left;
+                    continue;
+                }
+                for (int k = nd->first; k < nd->first + nd->count; ++k) {
+                    const int t_ = b.order[k];
+                    const float* tv = tri_verts + 9 * (int64_t)t_;
+                    float t, U, V, W, det;
+                    if (!wt_intersect(&w, tv, tv + 3, tv + 6, &t, &U, &V, &W, &det)) continue;
+                    if (closest) {
+                        if (!(t >= tmin && t <= best_t)) continue;
+                        if (t == best_t && best >= 0 && t_ > best) continue;
+                    } else {
+                        if (!(t >= tmin && t <= tmax)) continue;
+                        if (best >= 0 && t_ > best) continue;
+                    }
+                    best = t_; best_t = t; bv = V; bw = W; bd = det;
+                }
+            }
+        }
+        int32_t* q = out + 4 * i;
+        if (best >= 0) {
+            q[0] = best;
+            memcpy(&q[1], &best_t, 4);
+            q[2] = (int32_t)pack_unorm2x16(bv / bd, bw / bd);
+        } else {
+            const float m1 = -1.0f;
+            q[0] = -1;
+            memcpy(&q[1], &m1, 4);
+            q[2] = 0;
+        }
+        q[3] = 0;
     }
     free(b.nodes);
     free(b.order);
