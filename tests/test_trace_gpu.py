@@ -241,17 +241,25 @@ def test_interior_scene_sampled_bit_exact(dev, orc):
     assert np.array_equal(got, expected_hits(orc, hs, rays))
 
 
-def test_trace_sort_composite(dev, boxes):
-    """Tracer::traceSort with and without reorder gives the per-ray hits of
-    the unsorted trace (reordered results mapped back through the LUT)."""
+def test_trace_sort_composite(dev, orc, boxes):
+    """Tracer::traceSort with and without reorder: the keys, sort, reorder
+    and trace of the oracle (the hit records of the oracle's fp32 trace,
+    the reordered rays and LUTs of the oracle's reorder of the oracle's
+    stable sort), and the per-ray hits of the unsorted trace."""
     import ctypes as C
 
     from vulkanwavefrontpathtracer_amd import _capi, stages
 
     hs, sc = boxes
     n = 50_000
-    rays = torch.from_numpy(scene_rays(hs, n, 30)).to(dev)
+    rays_h = scene_rays(hs, n, 30)
+    rays = torch.from_numpy(rays_h).to(dev)
     ref = stages.trace(sc, rays)
+    exp = expected_hits(orc, hs, rays_h)
+    assert np.array_equal(ref.cpu().numpy(), exp)
+    mn_, mx_ = sc.bounds()
+    order = np.argsort(orc.keys_oct60(rays_h, mn_, mx_), kind="stable").astype(np.uint32)
+    r_rays, r_s2i, r_i2s = orc.reorder(order, rays_h, np.arange(n, dtype=np.int32), n)
     for reorder in (0, 1):
         bufs = dict(
             r0=rays.clone(), r1=torch.empty_like(rays),
@@ -274,7 +282,12 @@ def test_trace_sort_composite(dev, boxes):
                    stages._stream(), times)
         torch.cuda.synchronize()
         assert rb.swap == reorder
+        assert np.array_equal(bufs["idx"].cpu().numpy().view(np.uint32), order)
         if reorder:
+            assert np.array_equal(bufs["r1"].cpu().numpy(), r_rays)
+            assert np.array_equal(bufs["s2i1"].cpu().numpy(), r_s2i)
+            assert np.array_equal(bufs["i2s1"].cpu().numpy(), r_i2s)
+            assert np.array_equal(bufs["res"].cpu().numpy(), exp[order])
             # slot i holds original ray s2i1[i]
             back = torch.empty_like(ref)
             back[bufs["s2i1"].long()] = bufs["res"]
@@ -374,13 +387,34 @@ def test_group_traversal_bit_exact(dev, orc, group):
         _capi.call("wf_set_tuning", key, cur.value)
 
 
+_FUSED_ORACLE = {}
+
+
+def _fused_oracle(orc, hs, n, closest, idx, rays, s2i):
+    """Oracle reorder of (idx, rays, s2i) with an inverse LUT of n + 17
+    entries (-3 where unwritten), and the oracle's fp32 trace of the
+    reordered rays (BVH form, pinned to the brute-force scan); cached per
+    case, since the inputs are seeded."""
+    key = (n, closest)
+    if key not in _FUSED_ORACLE:
+        r_rays, r_s2i, r_i2s = orc.reorder(idx.cpu().numpy().view(np.uint32), rays.cpu().numpy(),
+                                           s2i.cpu().numpy(), n + 17)
+        r_i2s[r_i2s < 0] = -3
+        ref = orc.trace_bvh_f32(r_rays, hs.tri_verts(), bool(closest))
+        prim, geom = orc.hits_to_prim_geom(ref[:, 0], hs.geom_first)
+        hits = np.stack([prim, geom, ref[:, 1], ref[:, 2]], axis=1).astype(np.int32)
+        _FUSED_ORACLE[key] = (r_rays, r_s2i, r_i2s, hits)
+    return _FUSED_ORACLE[key]
+
+
 @pytest.mark.parametrize("knobs", [{}, {"trace.loop": 0}, {"trace.loop": 2}, {"trace.loop": 3},
                                    {"trace.block": 128}, {"trace.xcd": 8},
                                    {"trace.fuse_reorder": 0}, {"trace.width": 2}])
-def test_reorder_trace_fused_bit_exact(dev, knobs):
+def test_reorder_trace_fused_bit_exact(dev, orc, knobs):
     """wf_reorder_trace writes exactly what wf_reorder followed by wf_trace on
     the reordered rays writes (rays, both LUTs, hit records), on wavefronts
-    that take the fused kernel and ones that fall back (grouped sizes)."""
+    that take the fused kernel and ones that fall back (grouped sizes), and
+    that is the oracle's reorder and the oracle's trace of its output."""
     import ctypes as C
 
     from vulkanwavefrontpathtracer_amd import _capi, stages
@@ -400,7 +434,7 @@ def test_reorder_trace_fused_bit_exact(dev, knobs):
             rays[::89, 7] = -1.0
             keys, idx = stages.morton_keys(rays, stages.KEY_ORIGIN_DIR_MORTON60, mn, mx)
             stages.sort_pairs(keys, idx, 60, 8)
-            s2i = torch.randperm(n + 17, device=dev)[:n].to(torch.int32)
+            s2i = torch.from_numpy(np.random.default_rng(n).permutation(n + 17)[:n].astype(np.int32)).to(dev)
             outs = []
             for fused in (True, False):
                 out_rays = torch.full_like(rays, 7.0)
@@ -419,6 +453,16 @@ def test_reorder_trace_fused_bit_exact(dev, knobs):
                 outs.append((out_rays, out_s2i, out_i2s, hits))
             for a, b in zip(*outs):
                 assert torch.equal(a, b), (n, closest)
+            # and both equal the oracle: its reorder of the sorted indices,
+            # and its fp32 trace of the reordered rays
+            r_rays, r_s2i, r_i2s, r_hits = _fused_oracle(orc, hs, n, closest, idx, rays, s2i)
+            out_rays, out_s2i, out_i2s, hits = (t.cpu().numpy() for t in outs[0])
+            assert np.array_equal(out_rays, r_rays), (n, closest)
+            assert np.array_equal(out_s2i, r_s2i) and np.array_equal(out_i2s[:len(r_i2s)], r_i2s)
+            if closest:
+                assert np.array_equal(hits, r_hits), (n, closest)
+            else:
+                assert np.array_equal(hits[:, 0] >= 0, r_hits[:, 0] >= 0), (n, closest)
     finally:
         for k, v in saved.items():
             _capi.call("wf_set_tuning", k.encode(), v)

@@ -56,6 +56,8 @@ SIGNATURES: dict[str, tuple] = {
     "wf_sort_scratch_bytes": (i64, [i64, i32, i32, i32]),
     "wf_sort_pairs": (i32, [vp, vp, i64, i32, i32, i32, vp, i64, vp, pf]),
     "wf_sort_pairs_count": (i32, [vp, vp, i64, vp, i32, i32, i32, vp, i64, vp]),
+    "wf_sort_noalloc_max_pairs": (i64, []),
+    "wf_sort_pairs_noalloc": (i32, [vp, vp, i64, i32, i32, i32, vp]),
     "wf_reorder": (i32, [vp, vp, vp, vp, vp, vp, i64, vp, pf]),
     "wf_scene_create": (i32, [vp, i64, i32, vp, i64, pi32, i32, vp, vp, C.POINTER(vp)]),
     "wf_scene_destroy": (i32, [vp]),
@@ -182,7 +184,7 @@ def memcpy_d2d(dst: int, src: int, nbytes: int) -> None:
         _hip.hipMemcpy.restype = C.c_int
     rc = _hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), 3)
     if rc != 0:
-        raise WfError(f"hipMemcpy failed ({rc})")
+        raise RuntimeError(f"hipMemcpy failed ({rc})")
 
 
 def f3(v) -> C.Array:
