@@ -120,6 +120,17 @@ int wf_sort_pairs_count(void* keys, uint32_t* vals, int64_t capacity,
                         const uint32_t* d_count, int key_bytes, int key_bits,
                         int digit_bits, void* scratch, int64_t scratch_bytes,
                         wf_stream_t stream);
+/* No counterpart in the reference: wf_sort_pairs for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The scratch is a 32 MiB device array inside the
+ * library, one per GPU, held until the process ends; the call allocates
+ * nothing and only enqueues work (no timing).  n is at most
+ * wf_sort_noalloc_max_pairs() = 1,048,576 pairs (WF_ERR_TOO_LARGE above).
+ * All calls on one GPU share its array: calls that overlap on one GPU
+ * (different streams or threads) race and must be serialised by the caller. */
+int64_t wf_sort_noalloc_max_pairs(void);
+int wf_sort_pairs_noalloc(void* keys, uint32_t* vals, int64_t n, int key_bytes, int key_bits,
+                          int digit_bits, wf_stream_t stream);
 
 /* ----------------------------------------------------------------- reorder */
 /* Replaces Tracer::reorderRays (Tracer/Tracer.cpp:80-98), reorderRays.comp:

@@ -19,6 +19,8 @@
 // - up to 64 tiles, onesweep: one kernel per pass with decoupled look-back;
 // - up to 8192 pairs, one workgroup runs every pass in LDS;
 // - up to ~28k pairs (scaled by the pass count), a one-pass counted rank.
+#include <mutex>
+
 #include "common.h"
 
 namespace wfpt {
@@ -927,4 +929,61 @@ extern "C" int wf_sort_pairs_count(void* keys, uint32_t* vals, int64_t capacity,
     WF_CHECK_ARG(d_count, "wf_sort_pairs_count: null count");
     return sort_pairs_launch(keys, vals, capacity, d_count, key_bytes, key_bits, digit_bits,
                              scratch, scratch_bytes, as_stream(stream));
+}
+
+// ------------------------------------------------------- allocation-free sort
+// wf_sort_pairs with its scratch in a file-scope device array instead of a
+// caller buffer: for callers that cannot allocate (graph capture, serving
+// loops).  Every call on a GPU shares that GPU's copy of the array.
+namespace wfpt {
+constexpr int64_t NOALLOC_SORT_MAX_PAIRS = int64_t(1) << 20;
+// wf_sort_scratch_bytes at 2^20 pairs peaks at 31,514,880 bytes (u64 keys,
+// 11-bit digits); the array rounds that up to 32 MiB.
+constexpr int64_t NOALLOC_SORT_SCRATCH_BYTES = int64_t(32) << 20;
+}  // namespace wfpt
+
+static __device__ __align__(256) unsigned char scratch[wfpt::NOALLOC_SORT_SCRATCH_BYTES];
+
+namespace wfpt {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mutex;
+static void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The device address of this GPU's `scratch` (the array's name is only a
+// host-side stand-in): looked up once per device, with the device current.
+static int noalloc_scratch(void** out) {
+    int dev = -1;
+    WF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(WF_ERR_INVALID_ARG, "wf_sort_pairs_noalloc: device index beyond 64");
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(WF_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return WF_OK;
+}
+}  // namespace wfpt
+
+extern "C" int64_t wf_sort_noalloc_max_pairs(void) { return NOALLOC_SORT_MAX_PAIRS; }
+
+extern "C" int wf_sort_pairs_noalloc(void* keys, uint32_t* vals, int64_t n, int key_bytes,
+                                     int key_bits, int digit_bits, wf_stream_t stream) {
+    SortPlan p;
+    WF_CHECK_ARG(make_plan(n, key_bytes, key_bits, digit_bits, p),
+                 "wf_sort_pairs_noalloc: key_bytes must be 4 or 8, 1 <= key_bits <= 8*key_bytes, "
+                 "digit_bits in {4, 8, 11}, n >= 0");
+    if (n > NOALLOC_SORT_MAX_PAIRS || p.total > NOALLOC_SORT_SCRATCH_BYTES)
+        return fail(WF_ERR_TOO_LARGE, "wf_sort_pairs_noalloc: n = " + std::to_string(n) +
+                                          " exceeds the allocation-free limit of " +
+                                          std::to_string(NOALLOC_SORT_MAX_PAIRS) + " pairs");
+    if (n <= 1) return WF_OK;
+    void* sp = nullptr;
+    WF_TRY(noalloc_scratch(&sp));
+    return sort_pairs_launch(keys, vals, n, nullptr, key_bytes, key_bits, digit_bits, sp,
+                             NOALLOC_SORT_SCRATCH_BYTES, as_stream(stream));
 }
