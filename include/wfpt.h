@@ -440,6 +440,42 @@ int wf_renderer_size(wf_renderer r, int* width, int* height);
 int wf_renderer_stats(wf_renderer r, uint64_t counts[3], float times[3], float stage_ms[8],
                       uint64_t* sorted_rays);
 /* Environment lookups (Environment::get*Value). */
+/* SortLog (Renderer/Renderer.h:29-46, filled at Renderer.cpp:280-289 and
+ * printed per depth by Benchmark.cpp:159-195): per path bounce (kind 0,
+ * recursionDepth entries) or per shadow vertex (kind 1, recursionDepth + 1
+ * entries, 0 = shadow rays of the primary hits), the last frame's ray count
+ * and stage times summed over its passes.  trace_sort_ms is the trace of
+ * the sorted wavefront (with the reorder when it is fused into the trace),
+ * trace_ms the trace of an unsorted one (the comparison trace that
+ * Benchmark.printSortLogs adds, or the only trace when not sorting). */
+typedef struct wf_sort_log {
+    int64_t ray_count;      /* rays traced                      (rayCount)        */
+    int64_t sorted_rays;    /* of which sorted first                              */
+    float morton_ms;        /* key generation                   (mortonCodesTime) */
+    float sort_ms;          /* radix sort                       (sortTime)        */
+    float reorder_ms;       /* reorder, when not fused          (reorderTime)     */
+    float trace_sort_ms;    /* trace of the sorted wavefront    (traceSortTime)   */
+    float trace_ms;         /* trace of the unsorted wavefront  (traceTime)       */
+    float reserved;
+} wf_sort_log;
+int wf_renderer_sort_logs(wf_renderer r, int kind, wf_sort_log* out, int max_entries,
+                          int* num_entries);
+
+/* Decision log (verification, no reference counterpart): when enabled, a
+ * frame records per pass p, path vertex v (0 = primary hit, b + 1 = the hit
+ * of path bounce b) and pixel the discrete choices its path made there: the
+ * global triangle its ray hit (-1 miss, -2 no ray: the path had ended), per
+ * shadow sample 1 if the light was visible (0 occluded, 255 no shadow ray),
+ * and the sign decisions of the bounce leaving the vertex (bit 0: shading
+ * normal flipped to face the ray, bit 1: basis branch of a normal with
+ * negative z; 255 no bounce).  wf_renderer_decisions copies them to host
+ * arrays (any may be NULL) of spp * (recursionDepth + 1) * width * height
+ * int32, that many times numberOfShadowSamples bytes, and that many bytes,
+ * indexed [p][v][pixel], [p][v][pixel][sample] and [p][v][pixel]. */
+int wf_renderer_set_decision_log(wf_renderer r, int enable);
+int wf_renderer_decisions(wf_renderer r, int32_t* hit_tri, uint8_t* shadow_visible,
+                          uint8_t* bounce_signs);
+
 int wf_env_get_int(wf_renderer r, const char* name, int* value);
 int wf_env_get_float(wf_renderer r, const char* name, float* value);
 int wf_env_get_string(wf_renderer r, const char* name, char* buf, int buf_len);

@@ -275,13 +275,27 @@ class FrameCfg(C.Structure):
 
 def render_frame_f64(tri_verts, tri_n, tri_mat, albedo, emission, *, w, h, spp, depth, S, frame,
                      light, bg, origin, radius, max_dist, s2w_colmajor, rr=False, rr_start=3,
-                     d_pixels=None):
+                     d_pixels=None, hints=None, first_disagreement=None):
     """Frame 1 of Renderer::renderPath, primary rays in fp32 (as raygenPrimary
     makes them) and the paths followed in fp64.  Returns ([w*h, 4] float32
     linear RGBA, [w*h] bool ambiguous): a pixel is ambiguous when some vertex
     of its paths grazes a triangle edge, ties two hits or sits on an interval
     end within the fp32 error bound propagated along the path, where an fp32
     pipeline may follow another path.
+
+    hints = (hit_tri [spp, depth+1, w*h] int32, shadow_visible [spp, depth+1,
+    w*h, S] uint8[, bounce_signs [spp, depth+1, w*h] uint8]), the fp32
+    pipeline's discrete decisions (the renderer's decision log): where one
+    differs from the fp64 decision the replay takes it, and a third array
+    [w*h] uint8 is returned with bit 0 set where that happened at an
+    ambiguous decision, bit 1 where it happened at an unambiguous one (a real
+    disagreement), bit 2 where a bounce sign decision (faceforward, basis
+    branch) lies within the fp32 bound and no bounce_signs were given, and
+    bit 3 where the path's propagated fp32 uncertainty passes 1e-3 world
+    units at some vertex (beyond it the two paths part continuously).  first_disagreement (optional
+    [w*h, 4] int32 array) receives, per pixel, the first unambiguous
+    disagreement as (vertex, kind: 1 hit / 2+i shadow sample i / -1
+    roulette, fp64 choice, fp32 choice), vertex -1 where there is none.
 
     With d_pixels ([w*h, 4] float32, dLoss/dpixel), also returns the fp64
     adjoint: (image, ambiguous, grad_albedo [nmat, 3], grad_emission [nmat, 3])
@@ -296,15 +310,27 @@ def render_frame_f64(tri_verts, tri_n, tri_mat, albedo, emission, *, w, h, spp, 
     em = np.ascontiguousarray(emission, np.float32)
     out = np.zeros((w * h, 4), np.float32)
     amb = np.zeros(w * h, np.uint8)
+    ht = hv = hs = fl = None
+    if hints is not None:
+        ht = np.ascontiguousarray(hints[0], np.int32).reshape(spp, depth + 1, w * h)
+        hv = np.ascontiguousarray(hints[1], np.uint8).reshape(spp, depth + 1, w * h, S)
+        if len(hints) > 2 and hints[2] is not None:
+            hs = np.ascontiguousarray(hints[2], np.uint8).reshape(spp, depth + 1, w * h)
+        fl = np.zeros(w * h, np.uint8)
+    fd = first_disagreement
+    pp = lambda a: _p(a) if a is not None else None  # noqa: E731
+    hint_args = (pp(ht), pp(hv), pp(hs), pp(fl), pp(fd))
     if d_pixels is None:
         load().or_render_frame_f64(_p(tv), _p(tn), _p(tm), C.c_int64(len(tv)), _p(al), _p(em),
-                                   C.c_int(len(al)), C.byref(cfg), _p(out), _p(amb), None, None, None)
-        return out, amb.astype(bool)
+                                   C.c_int(len(al)), C.byref(cfg), _p(out), _p(amb), None, None, None,
+                                   *hint_args)
+        return (out, amb.astype(bool)) if fl is None else (out, amb.astype(bool), fl)
     dp = np.ascontiguousarray(d_pixels, np.float32).reshape(w * h, 4)
     ga = np.zeros((len(al), 3), np.float64)
     ge = np.zeros((len(al), 3), np.float64)
     load().or_render_frame_f64(_p(tv), _p(tn), _p(tm), C.c_int64(len(tv)), _p(al), _p(em),
-                               C.c_int(len(al)), C.byref(cfg), _p(out), _p(amb), _p(dp), _p(ga), _p(ge))
+                               C.c_int(len(al)), C.byref(cfg), _p(out), _p(amb), _p(dp), _p(ga), _p(ge),
+                               *hint_args)
     return out, amb.astype(bool), ga, ge
 
 

@@ -578,6 +578,17 @@ static void cosine_dir_f64(double r0, double r1, const double n[3], double out[3
     duff_basis_f64(n, b1, b2);
     for (int k = 0; k < 3; ++k) out[k] = x * b1[k] + z * b2[k] + y * n[k];
 }
+/* the same with the basis branch s (+1 or -1) given instead of taken from
+ * the sign of n.z */
+static void cosine_dir_f64_branch(double r0, double r1, const double n[3], double s,
+                                  double out[3]) {
+    double theta = 2.0 * 3.14159265358979323846 * r1, rad = sqrt(r0);
+    double x = rad * sin(theta), z = rad * cos(theta), y = sqrt(1.0 - r0);
+    double a = -1.0 / (s + n[2]), bb = n[0] * n[1] * a;
+    double b1[3] = {1.0 + s * n[0] * n[0] * a, s * bb, -s * n[0]};
+    double b2[3] = {bb, s + n[1] * n[1] * a, -n[1]};
+    for (int k = 0; k < 3; ++k) out[k] = x * b1[k] + z * b2[k] + y * n[k];
+}
 OR_API void or_duff_basis_f64(const double n[3], double b1[3], double b2[3]) { duff_basis_f64(n, b1, b2); }
 OR_API void or_cosine_dir_f64(double r0, double r1, const double n[3], double out[3]) {
     cosine_dir_f64(r0, r1, n, out);
@@ -824,6 +835,7 @@ typedef struct {
     double cos;    /* |cos| of the angle between the ray and the plane's normal */
     double edge;   /* signed distance (world units, in the plane) to the nearest edge
                       line: >= 0 inside the triangle */
+    double n[3];   /* unit normal of the plane */
 } line_hit;
 
 /* The ray's line against the triangle's plane, without inside test; returns
@@ -854,11 +866,19 @@ static int mt_line(const float* tv, const double o[3], const double d[3], line_h
     double l3 = sqrt(e3[0] * e3[0] + e3[1] * e3[1] + e3[2] * e3[2]);
     double w = 1.0 - h->u - h->v;
     h->edge = fmin(fmin(w * area2 / l3, h->u * area2 / l2), h->v * area2 / l1);
+    h->n[0] = nx / area2; h->n[1] = ny / area2; h->n[2] = nz / area2;
     return 1;
 }
 
 typedef struct {
     double tol0, tol_eps;   /* lateral uncertainty of the fp32 ray: tol0 + tol_eps * t */
+    double along;           /* how far the fp32 origin may sit from the exact one in any
+                               direction by rounding (of the previous hit distance and the
+                               point built from it): it moves the distance to a plane at
+                               the interval's start */
+    double on[3];           /* unit normal of the surface the ray leaves (0 for a camera
+                               ray): the lateral uncertainty runs within that surface, so it
+                               moves the distance to a plane by sin(dihedral) times itself */
 } ray_tol;
 
 /* closest (any = 0) or any hit in [tmin, tmax]; returns triangle or -1.
@@ -873,9 +893,11 @@ static int cb_trace(const cbvh* b, const double o[3], const double d[3], double 
     double best_t = tmax, bu = 0, bv = 0, bcos = 1;
     double near_t = INFINITY, second_t = INFINITY;
     int certain_any = 0;
+    double any_t = 0.0;
     double inv[3];
     const double dl = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     const double t0_ = tl ? tl->tol0 : 0.0, te_ = tl ? tl->tol_eps : 0.0;
+    const double along_ = tl ? tl->along : 0.0;
     /* lateral uncertainty (world units) at parameter t, and the same in t units */
 #define LAT(t) (t0_ + te_ * fabs(t) * dl)
 #define TTOL(t) fmax(EPS_T * fmax(fabs(t), 1.0), LAT(t) / dl)
@@ -903,33 +925,47 @@ static int cb_trace(const cbvh* b, const double o[3], const double d[3], double 
                 const double u = lh.u, v = lh.v, th = lh.t;
                 double margin = fmin(fmin(u, v), 1.0 - u - v);
                 int inside = u >= 0.0 && u <= 1.0 && v >= 0.0 && u + v <= 1.0;
+                int at_end = 0;   /* within the fp32 bound of an interval end */
                 if (amb) {
                     double tol = TTOL(th);
                     double lat = LAT(th) / fmax(lh.cos, 1e-3);
                     /* the fp32 origin also sits a few ulps of its coordinates
-                     * away, seen along the ray through this plane */
+                     * away, by the rounding of the point it was built from
+                     * (along, any direction) and anywhere within the previous
+                     * hit point's lateral uncertainty (within the surface it
+                     * left), which move its distance to this plane, seen
+                     * along the ray through it */
                     double mag = fmax(fmax(fabs(o[0]), fabs(o[1])), fabs(o[2])) + 1.0;
-                    double tol_end = fmax(EPS_T * fabs(th), 1.2e-7 * mag / fmax(lh.cos, 1e-4) / dl);
+                    double cn = tl ? lh.n[0] * tl->on[0] + lh.n[1] * tl->on[1] + lh.n[2] * tl->on[2] : 0.0;
+                    double sin_dihedral = sqrt(fmax(0.0, 1.0 - cn * cn));
+                    double tol_end = fmax(EPS_T * fabs(th),
+                                          (1.2e-7 * mag + along_ + LAT(th) * sin_dihedral) /
+                                              fmax(lh.cos, 1e-4) / dl);
                     if ((fabs(margin) < EPS_BARY || fabs(lh.edge) < lat) && th >= tmin - tol &&
                         th <= tmax + tol)
                         near_t = fmin(near_t, th);
-                    if (inside && (fabs(th - tmin) <= tol_end ||
-                                   (isfinite(tmax) && tmax < 1e30 && fabs(th - tmax) <= tol_end)))
-                        near_t = fmin(near_t, th);
+                    at_end = fabs(th - tmin) <= tol_end ||
+                             (isfinite(tmax) && tmax < 1e30 && fabs(th - tmax) <= tol_end);
+                    if (inside && at_end) near_t = fmin(near_t, th);
                 }
                 if (!inside || !(th >= tmin && th <= best_t)) {
                     if (inside && th >= tmin && th < second_t && best >= 0 && t != best) second_t = fmin(second_t, th);
                     continue;
                 }
-                if (th == best_t && best >= 0 && t > best) { second_t = fmin(second_t, th); continue; }
-                if (best >= 0) second_t = fmin(second_t, best_t);
-                best = t; best_t = th; bu = u; bv = v; bcos = lh.cos;
                 if (any) {
-                    if (!amb || (margin >= EPS_BARY && lh.edge >= LAT(th) / fmax(lh.cos, 1e-3))) {
+                    /* any hit: the interval stays whole, so that a certain
+                     * blocker further along settles an uncertain first one */
+                    if (best < 0) { best = t; any_t = th; bu = u; bv = v; bcos = lh.cos; }
+                    if (!amb || (margin >= EPS_BARY && lh.edge >= LAT(th) / fmax(lh.cos, 1e-3) &&
+                                 !at_end)) {
                         certain_any = 1;
                         goto done;
                     }
+                    continue;
                 }
+                if (th == best_t && best >= 0 && t > best) { second_t = fmin(second_t, th); continue; }
+                if (best >= 0) second_t = fmin(second_t, best_t);
+                best = t; best_t = th; bu = u; bv = v; bcos = lh.cos;
             }
         } else {
             stack[sp++] = nd->right;
@@ -948,7 +984,7 @@ done:
     }
 #undef LAT
 #undef TTOL
-    if (best >= 0) { *t_out = best_t; *u_out = bu; *v_out = bv; }
+    if (best >= 0) { *t_out = any ? any_t : best_t; *u_out = bu; *v_out = bv; }
     if (cos_out) *cos_out = bcos;
     return best;
 }
@@ -965,12 +1001,14 @@ static void dnorm(double v[3]) {
 
 /* shading normal (exact barycentrics), faceforwarded against d */
 static int d_normal(const float* tri_n, const int32_t* tri_mat, int t, double u, double v,
-                    const double d[3], double n[3]) {
+                    const double d[3], double n[3], double* dot_out) {
     const float* a = tri_n + 9 * (int64_t)t;
     double w = 1.0 - u - v;
     for (int k = 0; k < 3; ++k) n[k] = a[k] * w + a[3 + k] * u + a[6 + k] * v;
     dnorm(n);
-    if (!(n[0] * d[0] + n[1] * d[1] + n[2] * d[2] < 0.0)) { n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2]; }
+    const double dn = n[0] * d[0] + n[1] * d[1] + n[2] * d[2];
+    if (dot_out) *dot_out = dn / sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (!(dn < 0.0)) { n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2]; }
     return tri_mat[t];
 }
 
@@ -992,21 +1030,83 @@ static double hit_uncertainty(const ray_tol* tl, const double o[3], const double
     return (tl->tol0 + tl->tol_eps * fabs(t) * dl) / fmax(cs, 1e-3) + FP32_POS * mag;
 }
 
-/* direction uncertainty of a bounce: fp32 arithmetic plus the 16-bit
- * barycentrics (packUnorm2x16) that interpolate the shading normal */
-static double bounce_dir_uncertainty(const float* tri_n, int t) {
+/* How far the fp32 shading normal of a hit on triangle t may turn from the
+ * exact one: the 16-bit barycentrics (packUnorm2x16) that interpolate it, and
+ * the hit point's own uncertainty lat (world units in the surface), over
+ * which the interpolated normal turns by up to var per smallest triangle
+ * height (var: the largest difference of its vertex normals, or of one
+ * component of them, comp 0-2; a component equal at the three vertices
+ * interpolates exactly). */
+static double normal_turn_of(const float* tri_n, const float* tv, int t, double lat, int comp) {
     const float* a = tri_n + 9 * (int64_t)t;
     double var = 0.0;
     for (int i = 0; i < 3; ++i)
         for (int j = i + 1; j < 3; ++j)
-            for (int k = 0; k < 3; ++k) var = fmax(var, fabs((double)a[3 * i + k] - a[3 * j + k]));
-    return FP32_DIR + 2.0 * var / 65535.0;
+            for (int k = 0; k < 3; ++k)
+                if (comp < 0 || k == comp) var = fmax(var, fabs((double)a[3 * i + k] - a[3 * j + k]));
+    if (comp >= 0 && var == 0.0) return 0.0;   /* equal components interpolate exactly */
+    const float* p = tv + 9 * (int64_t)t;
+    double e[3][3], len[3];
+    for (int k = 0; k < 3; ++k) {
+        const float* q0 = p + 3 * k;
+        const float* q1 = p + 3 * ((k + 1) % 3);
+        for (int c = 0; c < 3; ++c) e[k][c] = (double)q1[c] - q0[c];
+        len[k] = sqrt(e[k][0] * e[k][0] + e[k][1] * e[k][1] + e[k][2] * e[k][2]);
+    }
+    double cx = e[0][1] * e[1][2] - e[0][2] * e[1][1], cy = e[0][2] * e[1][0] - e[0][0] * e[1][2],
+           cz = e[0][0] * e[1][1] - e[0][1] * e[1][0];
+    double area2 = sqrt(cx * cx + cy * cy + cz * cz);
+    double height = area2 / fmax(fmax(len[0], len[1]), fmax(len[2], 1e-30));
+    return FP32_DIR + 2.0 * var / 65535.0 + (height > 0 ? lat * var / height : var);
+}
+
+static double normal_turn(const float* tri_n, const float* tv, int t, double lat) {
+    return normal_turn_of(tri_n, tv, t, lat, -1);
+}
+
+/* direction uncertainty of a bounce: fp32 arithmetic plus the shading
+ * normal's turn, which the basis built on it turns the bounce by at most
+ * about three times */
+static double bounce_dir_uncertainty(const float* tri_n, const float* tv, int t, double lat) {
+    return FP32_DIR + 3.0 * normal_turn(tri_n, tv, t, lat);
+}
+
+/* unit geometric normal of triangle t */
+static void tri_unit_normal(const float* tv, int t, double n[3]) {
+    const float* a = tv + 9 * (int64_t)t;
+    double e1[3] = {a[3] - a[0], a[4] - a[1], a[5] - a[2]};
+    double e2[3] = {a[6] - a[0], a[7] - a[1], a[8] - a[2]};
+    n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    double l = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (l > 0) { n[0] /= l; n[1] /= l; n[2] /= l; }
 }
 
 /* visibility fraction of the S shadow rays from the hit of (o, d, t) */
+/* A decision of the fp32 pipeline (hint) that differs from the fp64 one is
+ * followed; *follow records whether the fp64 decision was ambiguous there
+ * (bit 0: within the fp32 error bound) or not (bit 1: a real disagreement). */
+/* lateral uncertainty (world units) beyond which a replayed path is no
+ * longer pinned by the fp64 replay (bit 3 of the follow flags) */
+#define OR_LAT_DIVERGED 1e-3
+
+typedef struct {
+    uint8_t flags;
+    int vertex;                 /* path vertex being decided (set by the caller) */
+    int32_t first[4];           /* first unambiguous disagreement: vertex, kind
+                                   (1 hit, 2 + sample shadow, -1 roulette), fp64, fp32 */
+} follow_rec;
+static void note_follow(follow_rec* f, int ambiguous, int kind, int mine, int theirs) {
+    f->flags |= ambiguous ? 1 : 2;
+    if (!ambiguous && f->first[0] < 0) {
+        f->first[0] = f->vertex; f->first[1] = kind; f->first[2] = mine; f->first[3] = theirs;
+    }
+}
+
 static double d_shadow(const cbvh* b, const or_frame_cfg* c, int hit, const double o[3],
                        const double d[3], double t, uint32_t seed, int pixel, int* amb,
-                       double lat_hit) {
+                       double lat_hit, const uint8_t* hint_vis, follow_rec* follow) {
     if (hit < 0) return 1.0;
     double adv = fmax(t - 1.0e-5, 0.0);
     double p[3] = {o[0] + d[0] * adv, o[1] + d[1] * adv, o[2] + d[2] * adv};
@@ -1022,8 +1122,17 @@ static double d_shadow(const cbvh* b, const or_frame_cfg* c, int hit, const doub
         /* the fp32 shadow ray runs from a point lat_hit away to a light
          * sample rounded to fp32 */
         double lmag = fabs(c->light[0]) + fabs(c->light[1]) + fabs(c->light[2]) + c->radius;
-        ray_tol tl = {lat_hit + FP32_POS * lmag, 2 * FP32_DIR};
-        if (cb_trace(b, p, dir, 1.0e-5, len, 1, &tt, &uu, &vv, amb, &tl, NULL) < 0) ++vis;
+        double pmag = fmax(fmax(fabs(p[0]), fabs(p[1])), fabs(p[2]));
+        ray_tol tl = {lat_hit + FP32_POS * lmag, 2 * FP32_DIR, 2.4e-7 * (fabs(t) + pmag), {0, 0, 0}};
+        tri_unit_normal(b->tv, hit, tl.on);
+        int a = 0;
+        int seen = cb_trace(b, p, dir, 1.0e-5, len, 1, &tt, &uu, &vv, &a, &tl, NULL) < 0;
+        if (hint_vis && hint_vis[i] <= 1 && hint_vis[i] != seen) {
+            note_follow(follow, a, 2 + i, seen, hint_vis[i]);
+            seen = hint_vis[i];
+        }
+        *amb |= a;
+        vis += seen;
     }
     return (double)vis / c->S;
 }
@@ -1043,7 +1152,7 @@ static int d_smooth(const cbvh* b, const or_frame_cfg* c, const float* tri_n, co
         return -1;
     }
     double n[3];
-    int m = d_normal(tri_n, tri_mat, hit, u, v, d, n);
+    int m = d_normal(tri_n, tri_mat, hit, u, v, d, n, NULL);
     if (m < 0 || m >= nmat) m = 0;
     double p[3] = {o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t};
     double L[3] = {c->light[0] - p[0], c->light[1] - p[1], c->light[2] - p[2]};
@@ -1082,10 +1191,28 @@ static void grad_vertex(const path_grad* pg, int m, const double dec[3], const d
  * With d_pixels (w*h float4, rgb used) non-NULL, also the fp64 adjoint:
  * grad_albedo / grad_emission (3*nmat doubles, overwritten) receive
  * sum_pixels d_pixels . dpixel/dparameter. */
+/* Take the fp32 pipeline's hit (hint: triangle, or -1 for a miss) where it
+ * differs from the fp64 one: the hinted triangle's plane is intersected in
+ * fp64 (barycentrics may fall a rounding error outside it). */
+static int follow_hit(const cbvh* b, int hit, int hint, int ambiguous, const double o[3],
+                      const double d[3], double* t, double* u, double* v, double* cs,
+                      follow_rec* follow) {
+    if (hint == hit || hint < -1) return hit;
+    note_follow(follow, ambiguous, 1, hit, hint);
+    if (hint == -1) return -1;
+    line_hit lh;
+    if (!mt_line(b->tv + 9 * (int64_t)hint, o, d, &lh)) return hit;
+    *t = lh.t; *u = lh.u; *v = lh.v; *cs = lh.cos;
+    return hint;
+}
+
 OR_API void or_render_frame_f64(const float* tv, const float* tri_n, const int32_t* tri_mat,
                                 int64_t ntri, const float* albedo, const float* emission, int nmat,
                                 const or_frame_cfg* c, float* out_rgba, uint8_t* out_amb,
-                                const float* d_pixels, double* grad_albedo, double* grad_emission) {
+                                const float* d_pixels, double* grad_albedo, double* grad_emission,
+                                const int32_t* hint_tri, const uint8_t* hint_vis,
+                                const uint8_t* hint_sign, uint8_t* out_follow,
+                                int32_t* out_first) {
     cbvh b;
     b.tv = tv;
     b.nodes = (bnode*)malloc(sizeof(bnode) * (2 * ntri + 1));
@@ -1109,6 +1236,12 @@ OR_API void or_render_frame_f64(const float* tv, const float* tri_n, const int32
             if (ga)
                 for (int k = 0; k < 3; ++k) g[k] = d_pixels[4 * pixel + k];
             int amb = 0;
+            follow_rec follow = {0, 0, {-1, 0, 0, 0}};
+            const int V = c->depth + 1;
+            /* the fp32 pipeline's decisions of (pass, vertex) for this pixel */
+#define HINT_TRI(ps, vx) (hint_tri ? hint_tri[((int64_t)(ps) * V + (vx)) * np + pixel] : -3)
+#define HINT_VIS(ps, vx) (hint_vis ? hint_vis + (((int64_t)(ps) * V + (vx)) * np + pixel) * c->S : NULL)
+#define HINT_SIGN(ps, vx) (hint_sign ? hint_sign[((int64_t)(ps) * V + (vx)) * np + pixel] : 255)
             for (int pass = 0; pass < c->spp; ++pass) {
                 int si = pass + c->spp * (c->frame - 1);
                 float hx = halton(si, 2), hy = halton(si, 3);
@@ -1121,32 +1254,74 @@ OR_API void or_render_frame_f64(const float* tv, const float* tri_n, const int32
                 double d[3] = {df[0], df[1], df[2]};
                 double dec[3] = {1, 1, 1};
                 double t = 0, u = 0, v = 0, cs = 1;
-                ray_tol tl = {0.0, FP32_POS};
+                ray_tol tl = {0.0, FP32_POS, 0.0, {0, 0, 0}};
                 path_grad pg;
                 pg.nv = 0;
-                int hit = ntri > 0 ? cb_trace(&b, o, d, 0.0, c->max_dist, 0, &t, &u, &v, &amb, &tl, &cs) : -1;
+                int a0 = 0;
+                follow.vertex = 0;
+                int hit = ntri > 0 ? cb_trace(&b, o, d, 0.0, c->max_dist, 0, &t, &u, &v, &a0, &tl, &cs) : -1;
+                hit = follow_hit(&b, hit, HINT_TRI(pass, 0), a0, o, d, &t, &u, &v, &cs, &follow);
+                amb |= a0;
                 double lat = hit >= 0 ? hit_uncertainty(&tl, o, d, t, cs) : 0.0;
                 double aux[3], y[3];
                 int m = d_smooth(&b, c, tri_n, tri_mat, emission, nmat, hit, o, d, t, u, v, dec, aux, y);
-                double vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel, &amb, lat);
+                double vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel, &amb, lat,
+                                      HINT_VIS(pass, 0), &follow);
                 for (int k = 0; k < 3; ++k) acc[k] += aux[k] * vis;
                 grad_vertex(&pg, m, dec, y, vis, g, c->spp, ga, ge);
                 for (int bounce = 0; bounce < c->depth; ++bounce) {
                     if (hit < 0) break;
                     double n[3];
-                    int mt = d_normal(tri_n, tri_mat, hit, u, v, d, n);
+                    double ndot = 0.0;
+                    int mt = d_normal(tri_n, tri_mat, hit, u, v, d, n, &ndot);
+                    /* two sign decisions of the bounce: faceforward (the
+                     * normal flipped against the ray) and the basis branch on
+                     * the normal's z sign (raygenPath.comp:28-37); within the
+                     * fp32 bound either sends the path elsewhere, so the fp32
+                     * pipeline's are followed where given, else the pixel is
+                     * marked (bit 2) */
+                    const double nt = normal_turn(tri_n, b.tv, hit, lat);
+                    const double ntz = normal_turn_of(tri_n, b.tv, hit, lat, 2);
+                    const int amb_flip = fabs(ndot) <= nt + tl.tol_eps;
+                    const int amb_branch = ntz > 0 && fabs(n[2]) <= ntz;
+                    double branch = n[2] >= 0.0 ? 1.0 : -1.0;
+                    const int hs = HINT_SIGN(pass, bounce);
+                    follow.vertex = bounce;
+                    if (hs <= 3) {
+                        const int flipped = !(ndot < 0.0);
+                        if ((hs & 1) != flipped) {
+                            note_follow(&follow, amb_flip, 5, flipped, hs & 1);
+                            n[0] = -n[0]; n[1] = -n[1]; n[2] = -n[2];
+                            branch = n[2] >= 0.0 ? 1.0 : -1.0;
+                        }
+                        const double want = (hs & 2) ? -1.0 : 1.0;
+                        if (want != branch) {
+                            note_follow(&follow, amb_branch, 6, branch < 0, want < 0);
+                            branch = want;
+                        }
+                    } else if (amb_flip || amb_branch) {
+                        follow.flags |= 4;
+                    }
                     if (mt < 0 || mt >= nmat) mt = 0;
                     double r0 = randf(&seed), r1 = randf(&seed);
                     double p[3] = {o[0] + d[0] * t, o[1] + d[1] * t, o[2] + d[2] * t};
                     double nd[3];
-                    cosine_dir_f64(r0, r1, n, nd);
+                    cosine_dir_f64_branch(r0, r1, n, branch, nd);
                     double w[3] = {albedo[3 * mt], albedo[3 * mt + 1], albedo[3 * mt + 2]};
                     if (c->rr && bounce >= c->rr_start) {
                         double q = fmax(fmax(dec[0] * w[0], dec[1] * w[1]), dec[2] * w[2]);
                         q = fmin(fmax(q, 0.05), 1.0);
                         double r2 = randf(&seed);
-                        if (fabs(r2 - q) < 1e-5) amb = 1;   /* roulette decision on a knife edge */
-                        if (r2 >= q) break;
+                        const int edge = fabs(r2 - q) < 1e-5;   /* roulette decision on a knife edge */
+                        int survive = r2 < q;
+                        const int h = HINT_TRI(pass, bounce + 1);
+                        follow.vertex = bounce + 1;
+                        if (h >= -2 && (h != -2) != survive) {
+                            note_follow(&follow, edge, -1, survive, h != -2);
+                            survive = h != -2;
+                        }
+                        amb |= edge;
+                        if (!survive) break;
                         for (int k = 0; k < 3; ++k) w[k] /= q;
                     }
                     if (pg.nv < OR_MAX_VERTS) {
@@ -1157,11 +1332,23 @@ OR_API void or_render_frame_f64(const float* tv, const float* tri_n, const int32
                     }
                     for (int k = 0; k < 3; ++k) { dec[k] *= w[k]; o[k] = p[k] + nd[k] * 1.0e-4; d[k] = nd[k]; }
                     tl.tol0 = lat;
-                    tl.tol_eps = bounce_dir_uncertainty(tri_n, hit);
-                    hit = cb_trace(&b, o, d, 1.0e-4, 3.402823466e+38, 0, &t, &u, &v, &amb, &tl, &cs);
+                    tl.tol_eps = bounce_dir_uncertainty(tri_n, b.tv, hit, lat);
+                    tl.along = 2.4e-7 * (fabs(t) + fmax(fmax(fabs(p[0]), fabs(p[1])), fabs(p[2])));
+                    tri_unit_normal(b.tv, hit, tl.on);
+                    int ab = 0;
+                    follow.vertex = bounce + 1;
+                    hit = cb_trace(&b, o, d, 1.0e-4, 3.402823466e+38, 0, &t, &u, &v, &ab, &tl, &cs);
+                    hit = follow_hit(&b, hit, HINT_TRI(pass, bounce + 1), ab, o, d, &t, &u, &v, &cs,
+                                     &follow);
+                    amb |= ab;
                     lat = hit >= 0 ? hit_uncertainty(&tl, o, d, t, cs) : 0.0;
+                    /* past this bound the fp32 path and the fp64 one part ways
+                     * continuously (each bounce off a curved, interpolated
+                     * normal multiplies the deviation), whatever the choices */
+                    if (lat > OR_LAT_DIVERGED) follow.flags |= 8;
                     m = d_smooth(&b, c, tri_n, tri_mat, emission, nmat, hit, o, d, t, u, v, dec, aux, y);
-                    vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel, &amb, lat);
+                    vis = d_shadow(&b, c, hit, o, d, t, seed, (int)pixel, &amb, lat,
+                                   HINT_VIS(pass, bounce + 1), &follow);
                     for (int k = 0; k < 3; ++k) acc[k] += aux[k] * vis;
                     grad_vertex(&pg, m, dec, y, vis, g, c->spp, ga, ge);
                 }
@@ -1169,6 +1356,11 @@ OR_API void or_render_frame_f64(const float* tv, const float* tri_n, const int32
             float* px = out_rgba + 4 * pixel;
             px[0] = (float)acc[0]; px[1] = (float)acc[1]; px[2] = (float)acc[2]; px[3] = 1.0f;
             if (out_amb) out_amb[pixel] = (uint8_t)amb;
+            if (out_follow) out_follow[pixel] = follow.flags;
+            if (out_first) memcpy(out_first + 4 * pixel, follow.first, sizeof follow.first);
+#undef HINT_TRI
+#undef HINT_VIS
+#undef HINT_SIGN
         }
         if (ga) {
 #pragma omp critical

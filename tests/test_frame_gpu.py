@@ -172,11 +172,12 @@ def test_reconstruct_stages_bit_exact(dev, orc, boxes):
 
 
 # ------------------------------------------------------------------- frames
-def render(overrides, env="config0.json"):
+def render(overrides, env="config0.json", decisions=False):
     from vulkanwavefrontpathtracer_amd.renderer import Renderer, env_path
 
     r = Renderer(env_path(env), overrides)
     r.build_scene()
+    r.set_decision_log(decisions)
     r.render()
     return r
 
@@ -196,7 +197,7 @@ def test_frame_deterministic_and_sort_invariant(dev):
         assert np.array_equal(img, base), (sort, reorder, mode)
 
 
-def oracle_frame(orc, env_name, overrides=None):
+def oracle_frame(orc, env_name, overrides=None, hints=None, first=None):
     from vulkanwavefrontpathtracer_amd import stages
     from vulkanwavefrontpathtracer_amd.renderer import env_path
 
@@ -218,7 +219,8 @@ def oracle_frame(orc, env_name, overrides=None):
         S=R["numberOfShadowSamples"], frame=1, light=vec(cfg["Scene"]["light"]),
         bg=vec(cfg["Scene"]["backgroundcolor"]), origin=vec(cam["position"]),
         radius=R["shadowRadius"], max_dist=cam["farPlane"], s2w_colmajor=S2W.T.reshape(-1),
-        rr=R.get("russianRoulette", False), rr_start=R.get("russianRouletteStart", 3))
+        rr=R.get("russianRoulette", False), rr_start=R.get("russianRouletteStart", 3), hints=hints,
+        first_disagreement=first)
 
 
 def frame_rmse(gpu, ref, amb):
@@ -229,27 +231,111 @@ def frame_rmse(gpu, ref, amb):
     return rmse_clean, rmse_all, float(amb.mean()), float(np.abs(err[clean]).max())
 
 
-@pytest.mark.parametrize("env,overrides", [
+# Largest share of pixels that may differ by more than 1e-3 from the replay
+# (each must carry the divergence flag): none on the boxes (flat faces, the
+# image agrees over all pixels); the interior's columns and arches (8
+# bounces, roulette) and the city (8 bounces) scatter a few paths apart.
+MAX_DEVIATING = {"config0.json": 0.0, "config1.json": 0.01, "config2.json": 0.01}
+ALL_PIXELS = {"config0.json"}     # workloads held to RMSE <= 1e-3 over every pixel
+
+FRAME_CASES = [
     ("config0.json", {}),
     ("config0.json", {"Renderer.recursionDepth": 8, "Renderer.numberOfShadowSamples": 3}),
     ("config0.json", {"Renderer.russianRoulette": True, "Renderer.recursionDepth": 6}),
-])
+    # the interior at reduced resolution: 8 bounces, roulette from bounce 3
+    ("config1.json", {"Resolution.width": 240, "Resolution.height": 135}),
+    # the city at reduced resolution: 4 spp, 8 bounces
+    ("config2.json", {"Resolution.width": 128, "Resolution.height": 72}),
+]
+
+
+@pytest.mark.parametrize("env,overrides", FRAME_CASES)
 def test_frame_rmse_vs_fp64(dev, orc, env, overrides):
-    """Image RMSE <= 1e-3 against the fp64 oracle under common random numbers,
-    over the pixels whose paths pass no decision (edge, tie, interval end,
-    roulette) within the fp32 error bound the oracle propagates along the
-    path; those flagged pixels, where the two may legitimately part (a
-    primary ray along the emitter's outline), must stay under 1%, and every
-    other pixel must agree to 1e-2 on its own."""
+    """Image RMSE <= 1e-3 over ALL pixels against the fp64 oracle under
+    common random numbers, the replay taking the GPU frame's own discrete
+    choices (hit triangle, shadow visibility, roulette; the renderer's
+    decision log) wherever they differ from its own.  Each such difference
+    is classified by the fp32 error bound the oracle propagates along the
+    path (an edge crossing, a tie, an interval end, a roulette draw on the
+    threshold): on the primary hit and the first two bounces at most 0.1%
+    of the pixels, and over the whole path at most 0.5%, may carry one beyond
+    it.
+
+    Exclusion rule: a pixel whose path's propagated fp32 uncertainty passes
+    1e-3 world units at some vertex (oracle flag bit 3) is not pinned by the
+    replay, because the fp32 and fp64 paths then part continuously (every
+    bounce off a curved surface with interpolated normals multiplies the
+    deviation) even along the same choices.  RMSE <= 1e-3 is asserted over
+    every other pixel; every pixel that differs by more than 1e-3 must carry
+    the flag, and such pixels are capped per workload (none on the boxes,
+    where RMSE <= 1e-3 holds over all pixels).  The all-pixel RMSE is
+    reported, and so is the plain fp64 frame (its own choices everywhere)
+    with its flagged pixels."""
     flat = dict(overrides)
-    gpu = render(flat, env).pixels()
-    ref, amb = oracle_frame(orc, env, flat)
-    rmse_clean, rmse_all, amb_frac, max_clean = frame_rmse(gpu, ref, amb)
-    print(f"RMSE all pixels {rmse_all:.3e}, unflagged {rmse_clean:.3e} (max {max_clean:.3e}), "
-          f"flagged {100 * amb_frac:.3f}%")
-    assert rmse_clean <= RMSE_TOL and max_clean <= 1e-2, (rmse_clean, max_clean, rmse_all)
-    assert amb_frac < 0.01
+    r = render(flat, env, decisions=True)
+    gpu = r.pixels()
+    hints = r.decisions()
+    first = np.zeros((gpu.shape[0], 4), np.int32)
+    ref, amb, follow = oracle_frame(orc, env, flat, hints=hints, first=first)
+    plain, amb_plain = oracle_frame(orc, env, flat)
+    err = gpu[:, :3].astype(np.float64) - ref[:, :3]
+    rmse_all = float(np.sqrt(np.mean(err ** 2)))
+    rmse_clean, rmse_plain, amb_frac, _ = frame_rmse(gpu, plain, amb_plain)
+    followed = int(np.count_nonzero(follow & 1))
+    outside = first[:, 0] >= 0
+    diverged = (follow & 8) > 0
+    rmse_pinned = float(np.sqrt(np.mean(err[~diverged] ** 2)))
+    print(f"{env} {flat}: RMSE {rmse_pinned:.3e} over the {100 * (1 - diverged.mean()):.2f}% "
+          f"pinned pixels, {rmse_all:.3e} over all (max {np.abs(err).max():.3e}) "
+          f"with the GPU's choices taken at {followed} pixels within the fp32 bound and "
+          f"{int(outside.sum())} beyond it (first at vertices "
+          f"{np.bincount(first[outside, 0], minlength=1).tolist()}); plain fp64 frame: "
+          f"all pixels {rmse_plain:.3e}, unflagged {rmse_clean:.3e}, flagged {100 * amb_frac:.3f}%")
+    early = outside & (first[:, 0] <= 2)
+    assert early.mean() <= 0.001, f"disagreements beyond the fp32 bound: {first[early][:8]}"
+    assert outside.mean() <= 0.005
+    assert not (follow & 4).any()                  # every bounce sign was logged and taken
+    deviating = np.abs(err).max(axis=1) > RMSE_TOL
+    print(f"    {100 * diverged.mean():.2f}% of the pixels past the divergence bound, "
+          f"{100 * deviating.mean():.3f}% deviating by more than {RMSE_TOL}")
+    assert not (deviating & ~diverged).any(), np.nonzero(deviating & ~diverged)[0][:8]
+    assert deviating.mean() <= MAX_DEVIATING[env], deviating.mean()
+    assert rmse_pinned <= RMSE_TOL, rmse_pinned
+    if env in ALL_PIXELS:
+        assert rmse_all <= RMSE_TOL, rmse_all
     assert float(np.mean(ref[:, :3])) > 0.05      # a lit image, not a trivial match
+
+
+def test_frame_closed_sphere(dev):
+    """The GPU frame loop inside a closed sphere lit from its centre equals
+    the closed form sum_{b=0..depth} albedo^b per channel (every vertex has
+    N.L = 1 and sees the light), or the closed form of a path that left
+    through a facet, at >= 99% of the pixels."""
+    import json
+
+    from scenes import furnace_outcomes, match_outcomes, uv_sphere
+    from vulkanwavefrontpathtracer_amd import stages
+    from vulkanwavefrontpathtracer_amd.renderer import Renderer, env_path
+
+    v, idx = uv_sphere(10.0, 32, 64)
+    albedo = np.array([[0.6, 0.5, 0.3]], np.float32)
+    sc = stages.Scene(v, idx, np.array([0, len(idx)], np.int32), None, albedo,
+                      np.zeros((1, 3), np.float32), device=dev)
+    cfg = json.loads(env_path("config0.json").read_text())
+    cfg["Resolution"] = {"width": 96, "height": 64}
+    cfg["Scene"].update({"light": "0 0 0", "backgroundcolor": "0.2 0.4 0.6"})
+    cfg["Camera"].update({"position": "1 2 3", "rotation": "10 20 0"})
+    for depth in (0, 3, 8):
+        cfg["Renderer"].update({"recursionDepth": depth, "numberOfShadowSamples": 3,
+                                "shadowRadius": 0.5, "russianRoulette": False})
+        r = Renderer(cfg)
+        r.set_scene(sc, v[:, :3].min(0), v[:, :3].max(0))
+        r.render()
+        k = match_outcomes(r.pixels(), furnace_outcomes(albedo[0], (0.2, 0.4, 0.6), depth),
+                           rtol=2e-6)
+        print(f"depth {depth}: closed path {np.mean(k == 0):.4f}, escaped {np.mean(k > 0):.4f}")
+        assert np.mean(k >= 0) >= 0.99
+        assert np.mean(k == 0) > 0.97
 
 
 def test_frame_stats_counts(dev):
@@ -275,6 +361,21 @@ def test_frame_stats_counts(dev):
         _capi.call("wf_set_tuning", b"trace.fuse_reorder", 1)
     del C
     assert sm["trace_unsorted"] > 0
+    # SortLog per path bounce (Renderer.cpp:280-289): counts add up to the
+    # frame's path rays, every bounce with rays was sorted, timed sorted and
+    # timed unsorted; shadow logs per vertex add up to the shadow rays traced
+    logs = r.sort_logs("path")
+    assert len(logs) == 4
+    assert sum(x["ray_count"] for x in logs) == st["path_rays"]
+    for x in logs:
+        if x["ray_count"]:
+            assert x["sorted_rays"] == x["ray_count"]
+            assert x["morton_ms"] > 0 and x["sort_ms"] > 0
+            assert x["trace_sort_ms"] > 0 and x["trace_ms"] > 0
+    sh = r.sort_logs("shadow")
+    assert len(sh) == 5
+    assert sum(x["ray_count"] for x in sh) == 2 * (st["primary_rays"] + st["path_rays"])
+    assert all(x["sorted_rays"] == 0 and x["trace_ms"] > 0 for x in sh if x["ray_count"])
 
 
 def compose_frame(dev, orc, hs, sc, cfg, W, H, use_gpu=True):

@@ -103,6 +103,9 @@ SIGNATURES: dict[str, tuple] = {
     "wf_renderer_adjoint": (i32, [vp, i32, vp, vp, vp, vp, pf]),
     "wf_renderer_size": (i32, [vp, C.POINTER(i32), C.POINTER(i32)]),
     "wf_renderer_stats": (i32, [vp, pu64, pf, pf, pu64]),
+    "wf_renderer_sort_logs": (i32, [vp, i32, vp, i32, C.POINTER(i32)]),
+    "wf_renderer_set_decision_log": (i32, [vp, i32]),
+    "wf_renderer_decisions": (i32, [vp, vp, vp, vp]),
     "wf_env_get_int": (i32, [vp, cs, C.POINTER(i32)]),
     "wf_env_get_float": (i32, [vp, cs, pf]),
     "wf_env_get_string": (i32, [vp, cs, C.c_char_p, i32]),

@@ -140,6 +140,22 @@ __device__ __forceinline__ void lb_store(uint32_t* p, uint32_t v) {
 // inside captured graphs.
 __global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v);
 int fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
+// Decision log of the frame loop (reconstruct.hip): out[s2i[i]] = global
+// triangle hit by ray i (-1 miss); out[pixel * S + sample] = 1 where the
+// shadow ray reached the light.
+int log_hits(const wf_hit* hits, const int32_t* s2i, int64_t n, const int32_t* geom_first,
+             int32_t* out, hipStream_t s);
+int log_shadow(const wf_hit* shadow_hits, const int32_t* shadow_i2s, const int32_t* in_s2i,
+               int64_t begin, int64_t end, int S, uint8_t* out, hipStream_t s);
+// wf_shade_extend that also writes, per shaded pixel, its bounce's sign
+// decisions (bit 0: normal flipped to face the ray; bit 1: basis branch of a
+// negative z) to sign_log (nullptr: none).
+int shade_extend_logged(wf_scene scene, uint32_t* seeds, const wf_ray* in_rays,
+                        const wf_hit* in_hits, const int32_t* in_s2i, int64_t n, wf_ray* out_rays,
+                        int32_t* out_s2i, float* decreases, uint32_t* d_count,
+                        int russian_roulette, int bounce, int rr_start, void* scratch,
+                        int64_t scratch_bytes, uint32_t* host_count, wf_stream_t stream, float* ms,
+                        uint8_t* sign_log);
 
 }  // namespace wfpt
 

@@ -100,6 +100,9 @@ class EventLog {
     struct Rec {
         hipEvent_t a, b;
         int stage, kind;
+        int bounce;   // path bounce (-1 for the primary vertex)
+        int64_t n;    // rays the stage processed
+        bool sorted;  // a trace of a sorted wavefront
     };
     std::vector<Rec> recs;
     std::vector<hipEvent_t> pool;
@@ -112,8 +115,9 @@ class EventLog {
         }
         return pool[used++];
     }
-    void begin(hipStream_t s, int stage, int kind) {
-        Rec r{get(), get(), stage, kind};
+    void begin(hipStream_t s, int stage, int kind, int bounce = -1, int64_t n = 0,
+               bool sorted = false) {
+        Rec r{get(), get(), stage, kind, bounce, n, sorted};
         (void)hipEventRecord(r.a, s);
         recs.push_back(r);
     }
@@ -160,6 +164,13 @@ struct wf_renderer_s {
     uint64_t sorted_rays = 0;
     float trace_ms[3] = {0, 0, 0};
     float stage_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    // per-bounce sort logs of the last frame (SortLog, Renderer.h:29-46):
+    // path bounces 0..depth-1, shadow vertices 0..depth
+    std::vector<wf_sort_log> path_logs, shadow_logs;
+    // decision log: per pass and path vertex, the triangle hit by each
+    // pixel's ray (-2 = no ray) and per shadow sample the light's visibility
+    bool decision_log = false;
+    wfpt::DeviceBuffer dlog_tri, dlog_vis, dlog_sign;
     // capture of pre-sort path wavefronts
     bool capture = false;
     std::vector<std::unique_ptr<wfpt::DeviceBuffer>> cap;
@@ -258,14 +269,35 @@ struct Frame {
     hipStream_t s;
     wf_stream_t ws;
     int bounce = -1;
+    int pass = 0;
+    int64_t cur_n = 0;
+    bool cur_sorted = false;
 
-    void tbegin(int stage, int kind) { r->log.begin(s, stage, kind); }
+    void tbegin(int stage, int kind) { r->log.begin(s, stage, kind, bounce, cur_n, cur_sorted); }
+
+    // decision log slots of (pass, vertex): vertex 0 = primary hit, b + 1 =
+    // the hit of path bounce b
+    int32_t* dlog_tri(int vertex) {
+        const int64_t np = (int64_t)r->width * r->height;
+        return r->dlog_tri.as<int32_t>() + ((int64_t)pass * (r->depth + 1) + vertex) * np;
+    }
+    uint8_t* dlog_sign(int vertex) {
+        const int64_t np = (int64_t)r->width * r->height;
+        return r->dlog_sign.as<uint8_t>() + ((int64_t)pass * (r->depth + 1) + vertex) * np;
+    }
+    uint8_t* dlog_vis(int vertex) {
+        const int64_t np = (int64_t)r->width * r->height;
+        return r->dlog_vis.as<uint8_t>() +
+               ((int64_t)pass * (r->depth + 1) + vertex) * np * r->shadow_samples;
+    }
     void tend() { r->log.end(s); }
 
     // Tracer::trace / Tracer::traceSort (Tracer.cpp:185-225) on a RayBuffer.
     int trace_rays(RayBuffer& rb, int kind, bool sort, bool reorder) {
         int64_t n = rb.size;
         if (n == 0) return WF_OK;
+        cur_n = n;
+        cur_sorted = false;
         if (sort && kind == RK_PATH && r->reorder_period != 1) {
             // sort only every reorder_period-th bounce (0 = never)
             sort = r->reorder_period > 0 && (bounce % r->reorder_period) == 0;
@@ -284,6 +316,7 @@ struct Frame {
             WF_TRY(wf_trace(r->scene, rb.ray_buf(), nullptr, rb.hit_buf(), n, rb.closest, ws, nullptr));
             tend();
         }
+        cur_sorted = true;
         tbegin(ST_KEYS, kind);
         WF_TRY(wf_morton_keys(rb.ray_buf(), rb.keys.p, rb.indices.as<uint32_t>(), n, r->key_mode,
                               r->smin, r->smax, ws, nullptr));
@@ -352,6 +385,8 @@ struct Frame {
                                  ws, nullptr));
         tend();
         WF_TRY(trace_rays(rb, RK_PRIMARY, false, false));
+        if (r->decision_log)
+            WF_TRY(log_hits(rb.hit_buf(), rb.s2i_buf(), np, r->scene->geom_first, dlog_tri(0), s));
         tbegin(ST_RECONSTRUCT, -1);
         WF_TRY(wf_fill_f32(r->decreases.as<float>(), 4 * np, 1.0f, ws, nullptr));
         tend();
@@ -376,6 +411,9 @@ struct Frame {
                                     r->shadow_radius, ws, nullptr));
             tend();
             WF_TRY(trace_rays(sh, RK_SHADOW, r->sort_shadow, r->reorder_shadow));
+            if (r->decision_log)
+                WF_TRY(log_shadow(sh.hit_buf(), sh.i2s_buf(), in.s2i_buf(), begin, end, S,
+                                  dlog_vis(bounce < 0 ? 0 : bounce + 1), s));
             tbegin(ST_RECONSTRUCT, -1);
             WF_TRY(wf_reconstruct_shadow(sh.hit_buf(), in.s2i_buf(), sh.i2s_buf(), in_px, out_px,
                                          begin, end, S, 0, ws, nullptr));
@@ -388,8 +426,10 @@ struct Frame {
     }
 
     // Renderer::renderPath (Renderer.cpp:175-199).
-    int render_path(int pass) {
+    int render_path(int pass_index) {
         const int64_t np = (int64_t)r->width * r->height;
+        pass = pass_index;
+        bounce = -1;
         float* aux = r->aux_pixels.as<float>();
         float* fp = r->frame_pixels.as<float>();
         if (r->ray_type == "primary") return primary_pass(fp, pass);  // renderPrimary
@@ -409,11 +449,12 @@ struct Frame {
             out.closest = true;
             uint32_t count = 0;
             tbegin(ST_SHADE, RK_PATH);
-            WF_TRY(wf_shade_extend(r->scene, r->seeds.as<uint32_t>(), in.ray_buf(), in.hit_buf(),
-                                   in.s2i_buf(), in.size, out.ray_buf(), out.s2i_buf(),
-                                   r->decreases.as<float>(), r->counters.as<uint32_t>() + 1,
-                                   r->russian_roulette ? 1 : 0, bounce, r->rr_start, r->compact.p,
-                                   (int64_t)r->compact.bytes, nullptr, ws, nullptr));
+            WF_TRY(shade_extend_logged(r->scene, r->seeds.as<uint32_t>(), in.ray_buf(), in.hit_buf(),
+                                       in.s2i_buf(), in.size, out.ray_buf(), out.s2i_buf(),
+                                       r->decreases.as<float>(), r->counters.as<uint32_t>() + 1,
+                                       r->russian_roulette ? 1 : 0, bounce, r->rr_start,
+                                       r->compact.p, (int64_t)r->compact.bytes, nullptr, ws, nullptr,
+                                       r->decision_log ? dlog_sign(bounce) : nullptr));
             tend();
             WF_HIP(hipMemcpyAsync(&count, r->counters.as<uint32_t>() + 1, 4, hipMemcpyDeviceToHost, s));
             WF_HIP(hipStreamSynchronize(s));
@@ -433,6 +474,9 @@ struct Frame {
                 r->cap_n[slot] = count;
             }
             WF_TRY(trace_rays(out, RK_PATH, r->sort_path, r->reorder_path));
+            if (r->decision_log)
+                WF_TRY(log_hits(out.hit_buf(), out.s2i_buf(), count, r->scene->geom_first,
+                                dlog_tri(bounce + 1), s));
             WF_TRY(reconstruct_smooth(out, aux));
             r->n_rays[RK_PATH] += count;
             if (count == 0) break;
@@ -473,6 +517,16 @@ static int render_frame(wf_renderer_s* r, float* ms_out) {
     r->sorted_rays = 0;
     r->log.reset();
     r->cap_n.assign(r->cap_n.size(), 0);
+    if (r->decision_log) {
+        const int64_t slots = (int64_t)r->spp * (r->depth + 1) * np;
+        const int64_t vis_words = (slots * r->shadow_samples + 3) / 4;
+        WF_TRY(r->dlog_tri.ensure(4 * slots));
+        WF_TRY(r->dlog_vis.ensure(4 * vis_words));
+        WF_TRY(fill_u32(r->dlog_tri.as<uint32_t>(), slots, 0xFFFFFFFEu, s));   // -2: no ray
+        WF_TRY(fill_u32(r->dlog_vis.as<uint32_t>(), vis_words, 0xFFFFFFFFu, s));
+        WF_TRY(r->dlog_sign.ensure(4 * ((slots + 3) / 4)));
+        WF_TRY(fill_u32(r->dlog_sign.as<uint32_t>(), (slots + 3) / 4, 0xFFFFFFFFu, s));
+    }
     Frame f{r, s, reinterpret_cast<wf_stream_t>(s)};
     if (r->ray_type != "primary") {
         f.tbegin(ST_RAYGEN, -1);
@@ -490,11 +544,33 @@ static int render_frame(wf_renderer_s* r, float* ms_out) {
     WF_HIP(hipStreamSynchronize(s));
     r->n_rays[RK_SHADOW] = (uint64_t)r->shadow_samples * hits;
     for (int k = 0; k < 8; ++k) r->stage_ms[k] = 0.f;
+    r->path_logs.assign((size_t)r->depth, wf_sort_log{});
+    r->shadow_logs.assign((size_t)r->depth + 1, wf_sort_log{});
     float total = 0.f;
     for (auto& rec : r->log.recs) {
         float t = 0.f;
         WF_HIP(hipEventElapsedTime(&t, rec.a, rec.b));
         total += t;
+        // SortLog (Renderer.cpp:280-289): path logs by bounce, shadow logs by
+        // the vertex the shadow rays leave from (0 = primary hit)
+        wf_sort_log* sl = nullptr;
+        if (rec.kind == RK_PATH && rec.bounce >= 0 && rec.bounce < r->depth)
+            sl = &r->path_logs[(size_t)rec.bounce];
+        if (rec.kind == RK_SHADOW && rec.bounce + 1 >= 0 && rec.bounce + 1 <= r->depth)
+            sl = &r->shadow_logs[(size_t)(rec.bounce + 1)];
+        if (sl) {
+            switch (rec.stage) {
+                case ST_KEYS: sl->morton_ms += t; sl->sorted_rays += rec.n; break;
+                case ST_SORT: sl->sort_ms += t; break;
+                case ST_REORDER: sl->reorder_ms += t; break;
+                case ST_TRACE_UNSORTED: sl->trace_ms += t; break;
+                case ST_TRACE:
+                    sl->ray_count += rec.n;
+                    (rec.sorted ? sl->trace_sort_ms : sl->trace_ms) += t;
+                    break;
+                default: break;
+            }
+        }
         if (rec.kind == RK_PATH || rec.stage == ST_TRACE_UNSORTED) {
             // stage_ms: {keys, sort, reorder, trace_sorted, trace_unsorted, shade, raygen, reconstruct}
             if (rec.kind == RK_PATH) {
@@ -733,6 +809,38 @@ extern "C" int wf_renderer_stats(wf_renderer r, uint64_t counts[3], float times[
     if (stage_ms)
         for (int k = 0; k < 8; ++k) stage_ms[k] = r->stage_ms[k];
     if (sorted_rays) *sorted_rays = r->sorted_rays;
+    return WF_OK;
+}
+
+extern "C" int wf_renderer_sort_logs(wf_renderer r, int kind, wf_sort_log* out, int max_entries,
+                                     int* num_entries) {
+    WF_CHECK_ARG(r, "wf_renderer_sort_logs: null renderer");
+    WF_CHECK_ARG(kind == 0 || kind == 1, "wf_renderer_sort_logs: kind is 0 (path) or 1 (shadow)");
+    const std::vector<wf_sort_log>& v = kind == 0 ? r->path_logs : r->shadow_logs;
+    if (num_entries) *num_entries = (int)v.size();
+    if (out)
+        for (int k = 0; k < std::min(max_entries, (int)v.size()); ++k) out[k] = v[(size_t)k];
+    return WF_OK;
+}
+
+extern "C" int wf_renderer_set_decision_log(wf_renderer r, int enable) {
+    WF_CHECK_ARG(r, "wf_renderer_set_decision_log: null renderer");
+    r->decision_log = enable != 0;
+    return WF_OK;
+}
+
+extern "C" int wf_renderer_decisions(wf_renderer r, int32_t* hit_tri, uint8_t* shadow_visible,
+                                     uint8_t* bounce_signs) {
+    WF_CHECK_ARG(r, "wf_renderer_decisions: null renderer");
+    if (!r->decision_log || !r->dlog_tri.p)
+        return fail(WF_ERR_STATE, "wf_renderer_decisions: render a frame with the decision log on first");
+    const int64_t slots = (int64_t)r->spp * (r->depth + 1) * r->width * r->height;
+    WF_HIP(hipStreamSynchronize(r->stream));
+    if (hit_tri) WF_HIP(hipMemcpy(hit_tri, r->dlog_tri.p, 4 * slots, hipMemcpyDeviceToHost));
+    if (shadow_visible)
+        WF_HIP(hipMemcpy(shadow_visible, r->dlog_vis.p, (size_t)slots * r->shadow_samples,
+                         hipMemcpyDeviceToHost));
+    if (bounce_signs) WF_HIP(hipMemcpy(bounce_signs, r->dlog_sign.p, (size_t)slots, hipMemcpyDeviceToHost));
     return WF_OK;
 }
 

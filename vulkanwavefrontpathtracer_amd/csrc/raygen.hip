@@ -69,7 +69,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void shade_extend_kernel(
     const wf_ray* __restrict__ in_rays, const wf_hit* __restrict__ in_hits,
     const int32_t* __restrict__ in_s2i, int64_t n, wf_ray* __restrict__ out_rays,
     int32_t* __restrict__ out_s2i, float4* __restrict__ decreases, uint32_t* __restrict__ d_count,
-    uint32_t* __restrict__ tile_counter, uint32_t* __restrict__ status, int rr, int64_t ntiles) {
+    uint32_t* __restrict__ tile_counter, uint32_t* __restrict__ status, int rr, int64_t ntiles,
+    uint8_t* __restrict__ sign_log) {
     __shared__ uint32_t s_tile, s_excl;
     __shared__ uint32_t s_wave[SHADE_BLOCK / 64];
     const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
@@ -93,6 +94,11 @@ __global__ __launch_bounds__(SHADE_BLOCK) void shade_extend_kernel(
             V3 nrm;
             int m = clamp_material(shading_normal(shade, tri, (uint32_t)hit.w, nrm), nmat);
             V3 ff = faceforward3(nrm, v3(rd.x, rd.y, rd.z));
+            // decision log: the normal flipped to face the ray (bit 0) and the
+            // basis branch of a normal with negative z (bit 1)
+            if (sign_log)
+                sign_log[pixel] = (uint8_t)((dot3(nrm, v3(rd.x, rd.y, rd.z)) < 0.0f ? 0 : 1) |
+                                            (ff.z >= 0.0f ? 0 : 2));
             float r0 = randf(seed);
             float r1 = randf(seed);
             float t = __int_as_float(hit.z);
@@ -246,12 +252,34 @@ extern "C" int64_t wf_compact_scratch_bytes(int64_t n) {
     return 256 + 4 * (tiles + 1);
 }
 
+namespace wfpt {
+int shade_extend_logged(wf_scene scene, uint32_t* seeds, const wf_ray* in_rays,
+                        const wf_hit* in_hits, const int32_t* in_s2i, int64_t n, wf_ray* out_rays,
+                        int32_t* out_s2i, float* decreases, uint32_t* d_count,
+                        int russian_roulette, int bounce, int rr_start, void* scratch,
+                        int64_t scratch_bytes, uint32_t* host_count, wf_stream_t stream, float* ms,
+                        uint8_t* sign_log);
+}  // namespace wfpt
+
 extern "C" int wf_shade_extend(wf_scene scene, uint32_t* seeds, const wf_ray* in_rays,
                                const wf_hit* in_hits, const int32_t* in_s2i, int64_t n,
                                wf_ray* out_rays, int32_t* out_s2i, float* decreases,
                                uint32_t* d_count, int russian_roulette, int bounce, int rr_start,
                                void* scratch, int64_t scratch_bytes, uint32_t* host_count,
                                wf_stream_t stream, float* ms) {
+    return wfpt::shade_extend_logged(scene, seeds, in_rays, in_hits, in_s2i, n, out_rays, out_s2i,
+                                     decreases, d_count, russian_roulette, bounce, rr_start, scratch,
+                                     scratch_bytes, host_count, stream, ms, nullptr);
+}
+
+// wf_shade_extend with the frame's decision log of the bounce signs
+// (sign_log[pixel], or nullptr).
+int wfpt::shade_extend_logged(wf_scene scene, uint32_t* seeds, const wf_ray* in_rays,
+                              const wf_hit* in_hits, const int32_t* in_s2i, int64_t n,
+                              wf_ray* out_rays, int32_t* out_s2i, float* decreases,
+                              uint32_t* d_count, int russian_roulette, int bounce, int rr_start,
+                              void* scratch, int64_t scratch_bytes, uint32_t* host_count,
+                              wf_stream_t stream, float* ms, uint8_t* sign_log) {
     WF_CHECK_ARG(scene, "wf_shade_extend: null scene");
     WF_CHECK_ARG(n >= 0, "wf_shade_extend: n < 0");
     WF_CHECK_ARG(d_count, "wf_shade_extend: null d_count");
@@ -274,7 +302,7 @@ extern "C" int wf_shade_extend(wf_scene scene, uint32_t* seeds, const wf_ray* in
         shade_extend_kernel<<<(unsigned)tiles, SHADE_BLOCK, 0, s>>>(
             scene->shade, scene->geom_first, scene->albedo, scene->num_materials, seeds, in_rays,
             in_hits, in_s2i, n, out_rays, out_s2i, reinterpret_cast<float4*>(decreases), d_count,
-            ctr, status, rr, tiles);
+            ctr, status, rr, tiles, sign_log);
         WF_TRY(launch_check("shade_extend_kernel"));
     }
     if (host_count) {

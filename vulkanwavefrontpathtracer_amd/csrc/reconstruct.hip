@@ -210,3 +210,50 @@ extern "C" int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, w
     WF_TRY(launch_check("count_hits_kernel"));
     return t.finish();
 }
+
+// ------------------------------------------------------------ decision log
+// The frame loop's discrete choices per path vertex, recorded by pixel for
+// verification against an fp64 replay: the global triangle a ray hit (-1 for
+// a miss) and, per shadow sample, whether the light was visible.
+namespace wfpt {
+
+__global__ __launch_bounds__(256) void log_hits_kernel(const wf_hit* __restrict__ hits,
+                                                       const int32_t* __restrict__ s2i, int64_t n,
+                                                       const int32_t* __restrict__ geom_first,
+                                                       int32_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const wf_hit h = hits[i];
+    out[s2i[i]] = h.prim < 0 ? -1 : geom_first[h.geom] + h.prim;
+}
+
+__global__ __launch_bounds__(256) void log_shadow_kernel(const wf_hit* __restrict__ shadow_hits,
+                                                         const int32_t* __restrict__ shadow_i2s,
+                                                         const int32_t* __restrict__ in_s2i,
+                                                         int64_t begin, int64_t m, int S,
+                                                         uint8_t* __restrict__ out) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m * S) return;
+    const int64_t j = q / S;
+    const int i = (int)(q - j * S);
+    const int pixel = in_s2i[begin + j];
+    out[(int64_t)pixel * S + i] = shadow_hits[shadow_i2s[q]].prim < 0 ? 1 : 0;
+}
+
+int log_hits(const wf_hit* hits, const int32_t* s2i, int64_t n, const int32_t* geom_first,
+             int32_t* out, hipStream_t s) {
+    if (n <= 0) return WF_OK;
+    log_hits_kernel<<<grid_for(n, 256), 256, 0, s>>>(hits, s2i, n, geom_first, out);
+    return launch_check("log_hits_kernel");
+}
+
+int log_shadow(const wf_hit* shadow_hits, const int32_t* shadow_i2s, const int32_t* in_s2i,
+               int64_t begin, int64_t end, int S, uint8_t* out, hipStream_t s) {
+    const int64_t m = end - begin;
+    if (m <= 0) return WF_OK;
+    log_shadow_kernel<<<grid_for(m * S, 256), 256, 0, s>>>(shadow_hits, shadow_i2s, in_s2i, begin,
+                                                           m, S, out);
+    return launch_check("log_shadow_kernel");
+}
+
+}  // namespace wfpt

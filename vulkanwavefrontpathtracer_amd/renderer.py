@@ -23,6 +23,12 @@ def env_path(name: str) -> Path:
     return p
 
 
+class _SortLog(C.Structure):
+    _fields_ = [("ray_count", C.c_int64), ("sorted_rays", C.c_int64), ("morton_ms", C.c_float),
+                ("sort_ms", C.c_float), ("reorder_ms", C.c_float), ("trace_sort_ms", C.c_float),
+                ("trace_ms", C.c_float), ("reserved", C.c_float)]
+
+
 class Renderer:
     def __init__(self, env: str | Path | dict, overrides: dict | None = None,
                  asset_path: str | Path | None = None):
@@ -176,6 +182,34 @@ class Renderer:
         v = (C.c_float * 3)()
         call("wf_env_get_vector", self.handle, name.encode(), v)
         return list(v)
+
+    def sort_logs(self, kind="path") -> list[dict]:
+        """SortLog entries of the last frame (wf_renderer_sort_logs): per path
+        bounce, or per shadow vertex (kind="shadow")."""
+        k = 0 if kind == "path" else 1
+        n = C.c_int()
+        call("wf_renderer_sort_logs", self.handle, k, None, 0, C.byref(n))
+        arr = (_SortLog * max(n.value, 1))()
+        call("wf_renderer_sort_logs", self.handle, k, C.cast(arr, C.c_void_p), n.value, None)
+        return [{f: getattr(arr[i], f) for f, _ in _SortLog._fields_ if f != "reserved"}
+                for i in range(n.value)]
+
+    def set_decision_log(self, on=True):
+        call("wf_renderer_set_decision_log", self.handle, 1 if on else 0)
+
+    def decisions(self):
+        """(hit_tri [spp, depth+1, h*w] int32, shadow_visible [spp, depth+1, h*w, S] uint8,
+        bounce_signs [spp, depth+1, h*w] uint8) of the last frame rendered with the
+        decision log on."""
+        spp = self.env_int("Renderer.numberOfPrimarySamples")
+        depth = self.env_int("Renderer.recursionDepth")
+        S = self.env_int("Renderer.numberOfShadowSamples")
+        w, h = self.size
+        tri = np.zeros((spp, depth + 1, w * h), np.int32)
+        vis = np.zeros((spp, depth + 1, w * h, S), np.uint8)
+        sign = np.zeros((spp, depth + 1, w * h), np.uint8)
+        call("wf_renderer_decisions", self.handle, tri.ctypes.data, vis.ctypes.data, sign.ctypes.data)
+        return tri, vis, sign
 
     def set_capture(self, on=True):
         call("wf_renderer_set_capture", self.handle, 1 if on else 0)
