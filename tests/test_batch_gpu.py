@@ -117,3 +117,73 @@ def test_batch_graph_recaptured_after_tuning_changes(dev, small_batch, expected)
             assert np.array_equal(b.pixels(k)[:, :3], expected[k][0][:, :3]), k
     finally:
         stages.set_tuning("sort.onesweep_tiles", old)
+
+
+def test_batch_views_match_fp64_oracle(dev, orc, small_batch):
+    """Batched views against the fp64 frame oracle directly (common random
+    numbers, the view's own camera, scene and sample count): RMSE <= 1e-3
+    over the pixels whose paths pass no decision within the fp32 bound (a
+    pixel is flagged when any of its spp paths does: at most a quarter of a
+    4-spp view's pixels on these small, close views)."""
+    from vulkanwavefrontpathtracer_amd import stages
+    from vulkanwavefrontpathtracer_amd.batch import Batch
+
+    hosts, scenes, views = small_batch
+    b = Batch(scenes, views, depth=DEPTH, shadow_samples=S)
+    b.render()
+    for k in (0, 3):
+        v = views[k]
+        hs = hosts[v.scene]
+        _, _, s2w = stages.camera_matrices(list(v.position), list(v.rotation), v.fov,
+                                           v.width / v.height, v.znear, v.zfar)
+        ref, amb = orc.render_frame_f64(
+            hs.tri_verts(), orc.tri_normals(hs.vertices, hs.indices), hs.tri_material, hs.albedo,
+            hs.emission, w=v.width, h=v.height, spp=v.spp, depth=DEPTH, S=S, frame=1,
+            light=list(v.light), bg=list(v.background), origin=list(v.position),
+            radius=v.shadow_radius, max_dist=v.zfar, s2w_colmajor=s2w.T.reshape(-1))
+        got = b.pixels(k)
+        err = got[:, :3].astype(np.float64) - ref[:, :3]
+        rmse = float(np.sqrt(np.mean(err[~amb] ** 2)))
+        print(f"view {k} ({v.width}x{v.height}, {v.spp} spp): RMSE {rmse:.3e} over "
+              f"{100 * (1 - amb.mean()):.2f}% unflagged pixels")
+        assert rmse <= 1e-3 and amb.mean() < 0.25
+        assert float(ref[:, :3].mean()) > 0.05
+
+
+def test_batch_full_workload_equals_single_view_frames(dev):
+    """The whole batched workload (64 boxes scenes with 1k-500k triangles,
+    views of 128^2-1024^2 at 1-16 spp, 6 bounces, 2 shadow samples) in one
+    queue with the graph on: every view equals the single-view frame loop's
+    frame of its scene and camera bit for bit, and the ray counts add up."""
+    import json
+
+    from vulkanwavefrontpathtracer_amd import stages
+    from vulkanwavefrontpathtracer_amd.batch import Batch, boxes_env_view, workload
+    from vulkanwavefrontpathtracer_amd.renderer import Renderer, env_path
+
+    boxes, seeds, vspec = workload(num_views=64)
+    print(f"[batch] building 64 scenes ({sum(boxes) * 192:,} triangles)", flush=True)
+    hosts = [stages.HostScene(0, sd, num_boxes=nb) for nb, sd in zip(boxes, seeds)]
+    scenes = [stages.Scene.from_host(h, dev) for h in hosts]
+    env = env_path("config0.json")
+    views = [boxes_env_view(env, v["scene"], v["width"], v["height"], v["spp"]) for v in vspec]
+    depth, s = 6, 2
+    b = Batch(scenes, views, depth=depth, shadow_samples=s)
+    b.set_tuning(8, 128, 4)
+    b.set_graph(True)
+    print("[batch] rendering the batch", flush=True)
+    b.render()
+    st = b.stats()
+    assert st["pixels"] == sum(v.width * v.height for v in views)
+    assert st["primary_rays"] == sum(v.width * v.height * v.spp for v in views)
+    cfg = json.loads(env.read_text())
+    for k, v in enumerate(views):
+        c = json.loads(json.dumps(cfg))
+        c["Resolution"] = {"width": v.width, "height": v.height}
+        c["Renderer"].update({"numberOfPrimarySamples": v.spp, "numberOfShadowSamples": s,
+                              "recursionDepth": depth, "rayType": "path", "russianRoulette": False})
+        r = Renderer(c)
+        r.set_scene(scenes[v.scene], hosts[v.scene].bmin, hosts[v.scene].bmax)
+        r.render()
+        assert np.array_equal(b.pixels(k)[:, :3], r.pixels()[:, :3]), (k, v.width, v.spp)
+        del r
