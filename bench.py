@@ -1,21 +1,28 @@
 """Benchmark: Mrays/s (trace+sort+reorder) of the wavefront bounce loop.
 
-Workload (one GPU): the medium interior scene (263,250 triangles), 1920x1080,
-1 spp, 8 bounces, Russian roulette after bounce 3, 60-bit keys (30 origin
-bits + 30 octahedral direction bits) sorted as u64 keys with a u32 payload,
-8-bit digits.  One untimed frame of the C++ frame loop captures every
-bounce's path-ray wavefront (rays + slot-to-pixel LUT) in HBM.  A timed step
-is one pass of the hot path over all captured bounces of that frame:
-    keys -> sort -> reorder -> closest-hit trace          (Tracer::traceSort)
+Headline workload (one GPU): the large plain scene, a procedural city of
+2,821,186 triangles, 1920x1080 x 4 spp with all samples in flight (8.3M
+primary rays, 5.5M path rays at the first bounce), 8 bounces, 60-bit keys
+(30 origin + 30 octahedral direction bits as a 5-D Morton code) sorted as
+u64 keys with a u32 payload, 8-bit digits, reorder every bounce.  One untimed
+frame of the C++ frame loop captures every pass's path wavefronts in HBM;
+the 4 passes of each bounce are concatenated into one wavefront.  A timed
+step is one pass of the hot path over all captured bounces of that frame:
+    keys -> sort -> reorder + closest-hit trace    (Tracer::traceSort)
+with the reorder gather fused into the trace kernel (wf_reorder_trace).
 value = rays through the step / wall time of the timed steps, the step
 captured once in a HIP graph and replayed (the eager-launch rate is
 reported beside it).
 
 Also reported: per-stage GPU times (hipEvents on the stream the kernels run
-on), the roofline of every stage and of the dominant one, the unsorted-trace
-rate of the same wavefronts (the reference's sort ablation, Renderer.cpp:
-280-283), the reference-style full-frame figures of the C++ frame loop, and a
-CPU baseline (the oracle's port of the same stages on a bounded sample).
+on); the reference's per-bounce sort log (SortLog: rays, key, sort, sorted
+trace and unsorted trace times of each bounce, Renderer.cpp:280-289); the
+sort ablation (the same wavefronts traced unsorted, sorted every 2nd bounce,
+with 32-bit origin keys, with the concatenated 60-bit key); the dominant
+kernel's roofline; the medium interior workload by the same method; the
+reference-style full-frame figures of the C++ frame loop; the batched and
+differentiable workloads; and a CPU baseline (the oracle's port of the same
+stages on a bounded sample of the headline's first bounce).
 """
 from __future__ import annotations
 
@@ -31,40 +38,374 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-
-
-def d2d(dst: int, src: int, nbytes: int) -> None:
-    """Device-to-device copy out of renderer-owned buffers (hipMemcpy)."""
-    hip = C.CDLL("libamdhip64.so")
-    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-    rc = hip.hipMemcpy(C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes), 3)
-    if rc != 0:
-        raise RuntimeError(f"hipMemcpy failed: {rc}")
+KEY_MODES = {"aila32": 0, "origin30": 1, "origin30_oct30": 2, "origin_dir_morton60": 4}
+PROFILE_DIR = ROOT / "profiles"
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def d2d(dst: int, src: int, nbytes: int) -> None:
+    from vulkanwavefrontpathtracer_amd._capi import memcpy_d2d
+
+    memcpy_d2d(dst, src, nbytes)
+
+
+# --------------------------------------------------------------- wavefronts
+def capture_waves(dev, env_name, overrides=None):
+    """One untimed frame of the C++ frame loop with capture on; per bounce,
+    the passes' pre-sort wavefronts concatenated into one resident wavefront
+    (slot-to-pixel values offset by pass * pixels so that every slot has its
+    own inverse-LUT entry).  Returns (renderer, scene handle, bounds, waves)."""
+    import torch
+
+    from vulkanwavefrontpathtracer_amd.renderer import Renderer, env_path
+
+    env = env_path(env_name)
+    cfg = json.loads(env.read_text())
+    r = Renderer(env, overrides or {})
+    log(f"{env_name}: building the scene")
+    r.build_scene()
+    r.set_capture(True)
+    r.render()
+    depth = cfg["Renderer"]["recursionDepth"]
+    spp = cfg["Renderer"]["numberOfPrimarySamples"]
+    w, h = r.size
+    npx = w * h
+    waves = []
+    for b in range(depth):
+        parts = [r.captured(p * depth + b) for p in range(spp)]
+        n = sum(x[2] for x in parts)
+        if n == 0:
+            break
+        rays = torch.empty((n, 8), dtype=torch.float32, device=dev)
+        s2i = torch.empty(n, dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        o = 0
+        for p, (rp, sp, k) in enumerate(parts):
+            if k:
+                d2d(rays[o:].data_ptr(), rp, 32 * k)
+                d2d(s2i[o:].data_ptr(), sp, 4 * k)
+                s2i[o:o + k] += p * npx
+                o += k
+        waves.append(dict(n=n, rays=rays, s2i=s2i))
+    smin, smax = (C.c_float * 3)(), (C.c_float * 3)()
+    from vulkanwavefrontpathtracer_amd import stages
+
+    stages.call("wf_scene_bounds", r.scene_handle(), smin, smax)
+    log(f"{env_name}: {len(waves)} bounces, {sum(x['n'] for x in waves)} path rays "
+        f"{[x['n'] for x in waves]}")
+    return r, r.scene_handle(), (smin, smax), waves, spp * npx, cfg
+
+
+class Pipeline:
+    """Tracer::traceSort over captured wavefronts: per bounce keys -> sort ->
+    reorder + trace (fused), or the trace alone on bounces that are not
+    sorted; every stage call is stream-ordered, so a step can be captured in
+    a HIP graph."""
+
+    def __init__(self, dev, scene, bounds, waves, index_cap, key_mode, digit_bits, fused=True):
+        import torch
+
+        from vulkanwavefrontpathtracer_amd import stages
+
+        self.stages = stages
+        self.lib = stages._capi.load()
+        self.scene = scene
+        self.mn3, self.mx3 = bounds
+        self.waves = waves
+        self.fused = fused
+        self.set_keys(key_mode, digit_bits)
+        for x in waves:
+            n = x["n"]
+            x["keys"] = torch.empty(n, dtype=torch.int64, device=dev)
+            x["idx"] = torch.empty(n, dtype=torch.int32, device=dev)
+            x["out_rays"] = torch.empty_like(x["rays"])
+            x["out_s2i"] = torch.empty_like(x["s2i"])
+            x["out_i2s"] = torch.empty(index_cap, dtype=torch.int32, device=dev)
+            x["hits"] = torch.empty((n, 4), dtype=torch.int32, device=dev)
+            x["hits_u"] = torch.empty((n, 4), dtype=torch.int32, device=dev)
+        need = max(self.lib.wf_sort_scratch_bytes(x["n"], 8, 64, d) for x in waves for d in (8, 11))
+        self.scratch = torch.empty(need, dtype=torch.uint8, device=dev)
+        self.names = ["keys", "sort", "reorder", "trace"]
+
+    def set_keys(self, key_mode, digit_bits):
+        self.mode = key_mode
+        self.kbytes = 4 if key_mode < 2 else 8
+        self.kbits = self.stages.key_bits(key_mode)
+        self.digit_bits = digit_bits
+
+    def bytes_per_ray(self):
+        passes = -(-self.kbits // self.digit_bits)
+        return {"keys": 32 + self.kbytes + 4,
+                "sort": passes * 2 * (self.kbytes + 4) + self.kbytes,
+                "reorder": 80, "trace": 96 if self.fused else 48}
+
+    def step(self, stream_ptr, period=1, events=None, stream=None):
+        """One pass over all bounces; bounce b is sorted when period > 0 and
+        b % period == 0.  events: list receiving (bounce, stage, start, end)."""
+        import torch
+
+        st = self.stages._capi
+        lib = self.lib
+        V = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+        for b, x in enumerate(self.waves):
+            n = x["n"]
+            srt = period > 0 and b % period == 0
+
+            def mark():
+                if events is None:
+                    return None
+                e = torch.cuda.Event(enable_timing=True)
+                e.record(stream)
+                return e
+
+            m0 = mark()
+            if srt:
+                st.check("wf_morton_keys", lib.wf_morton_keys(
+                    V(x["rays"]), V(x["keys"]), V(x["idx"]), n, self.mode, self.mn3, self.mx3,
+                    stream_ptr, None))
+            m1 = mark()
+            if srt:
+                st.check("wf_sort_pairs", lib.wf_sort_pairs(
+                    V(x["keys"]), V(x["idx"]), n, self.kbytes, self.kbits, self.digit_bits,
+                    V(self.scratch), self.scratch.numel(), stream_ptr, None))
+            m2 = mark()
+            if srt and not self.fused:
+                st.check("wf_reorder", lib.wf_reorder(
+                    V(x["idx"]), V(x["rays"]), V(x["out_rays"]), V(x["s2i"]), V(x["out_s2i"]),
+                    V(x["out_i2s"]), n, stream_ptr, None))
+            m3 = mark()
+            if srt and self.fused:
+                st.check("wf_reorder_trace", lib.wf_reorder_trace(
+                    self.scene, V(x["idx"]), V(x["rays"]), V(x["out_rays"]), V(x["s2i"]),
+                    V(x["out_s2i"]), V(x["out_i2s"]), V(x["hits"]), n, 1, stream_ptr, None))
+            elif srt:
+                st.check("wf_trace", lib.wf_trace(self.scene, V(x["out_rays"]), None, V(x["hits"]),
+                                                  n, 1, stream_ptr, None))
+            else:
+                st.check("wf_trace", lib.wf_trace(self.scene, V(x["rays"]), None, V(x["hits"]), n,
+                                                  1, stream_ptr, None))
+            m4 = mark()
+            if events is not None:
+                for k, (a, e) in enumerate(((m0, m1), (m1, m2), (m2, m3), (m3, m4))):
+                    events.append((b, self.names[k], a, e))
+
+    def trace_unsorted(self, stream_ptr, events, stream):
+        import torch
+
+        for b, x in enumerate(self.waves):
+            a = torch.cuda.Event(enable_timing=True)
+            e = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            self.stages._capi.check("wf_trace", self.lib.wf_trace(
+                self.scene, C.c_void_p(x["rays"].data_ptr()), None, C.c_void_p(x["hits_u"].data_ptr()),
+                x["n"], 1, stream_ptr, None))
+            e.record(stream)
+            events.append((b, "trace_unsorted", a, e))
+
+    def timed(self, stream, reps=3, period=1):
+        """Per bounce and stage GPU ms (mean over reps), via hipEvents on the
+        kernels' stream."""
+        import torch
+
+        evs = []
+        for _ in range(reps):
+            self.step(C.c_void_p(stream.cuda_stream), period, evs, stream)
+        torch.cuda.synchronize()
+        out = {}
+        for b, k, a, e in evs:
+            out.setdefault(b, {}).setdefault(k, 0.0)
+            out[b][k] += a.elapsed_time(e) / reps
+        return out
+
+
+def run_workload(dev, args, env_name, ablations=True, overrides=None):
+    import torch
+
+    r, scene, bounds, waves, index_cap, cfg = capture_waves(dev, env_name, overrides)
+    mode = KEY_MODES[cfg["Renderer"]["keyMode"]]
+    p = Pipeline(dev, scene, bounds, waves, index_cap, mode, args.digit_bits, fused=not args.no_fuse)
+    total_rays = sum(x["n"] for x in waves)
+    stream = torch.cuda.current_stream()
+    sp = C.c_void_p(stream.cuda_stream)
+
+    log(f"{env_name}: warmup {args.warmup} steps")
+    for _ in range(args.warmup):
+        p.step(sp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        p.step(sp)
+    torch.cuda.synchronize()
+    wall_eager = time.perf_counter() - t0
+    value_eager = total_rays * args.steps / wall_eager / 1e6
+
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        p.step(C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    for _ in range(args.warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    log(f"{env_name}: timing {args.steps} graph-replayed steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    value = total_rays * args.steps / wall / 1e6
+    log(f"{env_name}: {value:.1f} Mrays/s graph-replayed, {value_eager:.1f} eager")
+    graph_hits = [x["hits"].clone() for x in waves]
+    p.step(sp)
+    torch.cuda.synchronize()
+    graph_equal = all(torch.equal(h, x["hits"]) for h, x in zip(graph_hits, waves))
+    del graph
+
+    per_b = p.timed(stream)
+    unsorted_ev = []
+    for _ in range(3):
+        p.trace_unsorted(sp, unsorted_ev, stream)
+    torch.cuda.synchronize()
+    for b, k, a, e in unsorted_ev:
+        per_b[b].setdefault(k, 0.0)
+        per_b[b][k] += a.elapsed_time(e) / 3
+    same = all(torch.equal(x["hits"][torch.argsort(x["out_s2i"])], x["hits_u"][torch.argsort(x["s2i"])])
+               for x in waves)
+
+    stage_ms = {k: sum(per_b[b][k] for b in per_b) for k in p.names}
+    unsorted_ms = sum(per_b[b]["trace_unsorted"] for b in per_b)
+    bpe = p.bytes_per_ray()
+    stages_roof = {}
+    for k in p.names:
+        if p.fused and k == "reorder":
+            stages_roof[k] = {"ms_per_step": 0.0, "fused_into": "trace"}
+            continue
+        t = stage_ms[k] / 1e3
+        gbs = total_rays * bpe[k] / t / 1e9 if t > 0 else 0.0
+        stages_roof[k] = {"ms_per_step": round(stage_ms[k], 4), "bytes_per_ray": bpe[k],
+                          "achieved_GBs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+    # SortLog per bounce (Renderer.h:29-46, Renderer.cpp:280-289, printed by
+    # Benchmark.cpp:159-195): the sorted pipeline's stages and the unsorted
+    # trace of the same wavefront
+    sort_log = []
+    for b, x in enumerate(waves):
+        d = per_b[b]
+        sort_log.append({"depth": b + 1, "ray_count": x["n"], "morton_ms": round(d["keys"], 4),
+                         "sort_ms": round(d["sort"], 4), "reorder_ms": round(d["reorder"], 4),
+                         "trace_sort_ms": round(d["trace"], 4),
+                         "trace_ms": round(d["trace_unsorted"], 4),
+                         "sorting_pays": d["keys"] + d["sort"] + d["reorder"] + d["trace"] <
+                         d["trace_unsorted"]})
+    sorted_ms = sum(stage_ms.values())
+    big = max(range(len(waves)), key=lambda i: waves[i]["n"])
+    nbig = waves[big]["n"]
+    sort_big_ms = per_b[big]["sort"]
+    sort_gbs = nbig * bpe["sort"] / (sort_big_ms / 1e3) / 1e9
+    out = {
+        "value": value, "value_eager": value_eager, "ms_per_step": ms_per_step,
+        "rays": total_rays, "rays_per_bounce": [x["n"] for x in waves],
+        "graph_replay_equals_eager": bool(graph_equal), "sorted_vs_unsorted_hits_equal": bool(same),
+        "stages": stages_roof, "stage_ms": stage_ms, "sort_log": sort_log,
+        "sorted_pipeline_mrays_s": round(total_rays / (sorted_ms / 1e3) / 1e6, 1),
+        "unsorted_trace_mrays_s": round(total_rays / (unsorted_ms / 1e3) / 1e6, 1),
+        "per_bounce_reorder_vs_unsorted": round(unsorted_ms / sorted_ms, 3),
+        "trace_speedup_sorted_vs_unsorted": round(unsorted_ms / stage_ms["trace"], 3),
+        "sort_largest_bounce": {"pairs": nbig, "ms": round(sort_big_ms, 4),
+                                "bytes_per_pair": bpe["sort"], "achieved_GBs": round(sort_gbs, 1),
+                                "frac_hbm": round(sort_gbs / HBM_PEAK_GBS, 4),
+                                "key_bits": p.kbits, "digit_bits": p.digit_bits},
+        "dominant_launch": {"kernel": "trace_kernel<reorder>" if p.fused else "trace_kernel",
+                            "elems": nbig, "ms": per_b[big]["trace"], "bytes_per_elem": bpe["trace"]},
+        "mode": mode, "cfg": cfg, "renderer": r, "scene": scene, "bounds": bounds, "waves": waves,
+    }
+    if ablations:
+        abl = {}
+        for name, km, db, period in (("every 2nd bounce", mode, args.digit_bits, 2),
+                                     ("30-bit origin keys", 1, args.digit_bits, 1),
+                                     ("60-bit concatenated keys", 2, args.digit_bits, 1),
+                                     ("11-bit digits", mode, 11, 1)):
+            p.set_keys(km, db)
+            pb = p.timed(stream, period=period)
+            ms = sum(sum(d.values()) for d in pb.values())
+            abl[name] = {"mrays_s": round(total_rays / (ms / 1e3) / 1e6, 1),
+                         "vs_unsorted": round(unsorted_ms / ms, 3),
+                         "ms_per_step": {k: round(sum(pb[b].get(k, 0.0) for b in pb), 4)
+                                         for k in p.names}}
+        p.set_keys(mode, args.digit_bits)
+        # sorting only the bounces whose own log shows a gain
+        pays = [s["sorting_pays"] for s in sort_log]
+        best_ms = sum(sum(per_b[b][k] for k in p.names) if pays[b] else per_b[b]["trace_unsorted"]
+                      for b in per_b)
+        abl["sort only where the per-bounce log shows a gain"] = {
+            "mrays_s": round(total_rays / (best_ms / 1e3) / 1e6, 1),
+            "vs_unsorted": round(unsorted_ms / best_ms, 3), "sorted_bounces": pays}
+        out["ablations"] = abl
+    return out
+
+
+def reference_frame(r, frames):
+    """Reference-style full frames of the C++ frame loop (Benchmark::run):
+    path Mrays/s includes shade; the renderer's own SortLog (with
+    Benchmark.printSortLogs, an unsorted trace of every sorted wavefront)."""
+    import torch
+
+    walls, stats = [], None
+    for _ in range(frames):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r.render()
+        walls.append(time.perf_counter() - t0)
+        stats = r.stats()
+    if stats is None:
+        return None
+    path_rays = stats["path_rays"]
+    return {
+        "path_rays": path_rays,
+        "path_mrays_s": round(path_rays / (stats["path_ms"] / 1e3) / 1e6, 1) if stats["path_ms"] > 0 else None,
+        "frame_ms_wall": round(1e3 * min(walls), 2),
+        "rays_per_frame": stats["primary_rays"] + stats["shadow_rays"] + path_rays,
+        "stage_ms": {k: round(v, 3) for k, v in stats["stage_ms"].items()},
+        "renderer_sort_log_path": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in x.items()}
+                                   for x in r.sort_logs("path")],
+    }
+
+
+def profile_figures():
+    """Counter figures of the dominant kernel from the committed rocprofv3
+    summaries (profiles/*.txt lines 'figure <name> <value>'), labelled with
+    the file they come from; counters cannot be read inside this process."""
+    out = {}
+    for f in sorted(PROFILE_DIR.glob("*.txt")):
+        for line in f.read_text().splitlines():
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "figure":
+                try:
+                    out[parts[1]] = {"value": float(parts[2]), "source": f"profiles/{f.name}"}
+                except ValueError:
+                    pass
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--env", default="config1.json")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--env", default="config2.json")
     ap.add_argument("--digit-bits", type=int, default=8)
-    ap.add_argument("--frames", type=int, default=3, help="reference-style full frames timed")
+    ap.add_argument("--frames", type=int, default=2, help="reference-style full frames timed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 30,
+                    help="rays per bounce of the CPU baseline (default: all)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable workload")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
-    ap.add_argument("--no-large", action="store_true", help="skip the large-scene reorder ablation")
+    ap.add_argument("--no-interior", action="store_true", help="skip the medium interior workload")
     ap.add_argument("--no-fuse", action="store_true",
-                    help="reorder and trace as separate kernels (wf_reorder + wf_trace) instead of "
-                         "the fused wf_reorder_trace")
+                    help="reorder and trace as separate kernels (wf_reorder + wf_trace)")
     ap.add_argument("--no-ablations", action="store_true",
-                    help="skip the unsorted-trace and key-layout ablations of the main workload "
-                         "(profiling runs: every trace launch is then one of the measured steps)")
+                    help="skip the sort ablations (profiling runs)")
     ap.add_argument("--batch-views", type=int, default=64)
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="wf_set_tuning knob applied before the scene is built (repeatable)")
@@ -72,21 +413,19 @@ def main():
                     help="first run one plain device copy of MIB MiB (scales HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the wavefront loop runs on a single device and has no "
-              "multi-GPU path; run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (the wavefront loop "
+              "runs on a single device); run with --gpus 1", file=sys.stderr)
         return 2
 
-    import numpy as np
     import torch
 
     from vulkanwavefrontpathtracer_amd import stages
-    from vulkanwavefrontpathtracer_amd.renderer import Renderer, env_path
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     for kv in args.tune:
         k, v = kv.split("=")
-        stages.call("wf_set_tuning", k.encode(), int(v))
+        stages.set_tuning(k, int(v))
     if args.copy_calibration:
         nb = args.copy_calibration << 20
         src = torch.ones(nb // 4, dtype=torch.float32, device=dev)
@@ -96,291 +435,97 @@ def main():
         torch.cuda.synchronize()
         log(f"copy calibration: {nb} bytes read + {nb} bytes written")
         del src, dst
-    env = env_path(args.env)
-    cfg = json.loads(env.read_text())
-    log(f"building scene for {args.env}")
-    r = Renderer(env, {"Renderer.digitBits": args.digit_bits})
-    r.build_scene()
-    scene_h = r.scene_handle()
-    r.set_capture(True)
-    r.render()                       # untimed: captures the bounce wavefronts
-    frame_stats = r.stats()
-    mode = KEY_MODES[cfg["Renderer"]["keyMode"]]
-    kbytes = 4 if mode < 2 else 8
-    kbits = stages.key_bits(mode)
-    smin, smax = (C.c_float * 3)(), (C.c_float * 3)()
-    stages.call("wf_scene_bounds", scene_h, smin, smax)
 
-    # Resident per-bounce buffers (captured rays are copied so the renderer can be reused).
-    bounces = []
-    b = 0
-    while True:
-        rp, sp, n = r.captured(b)
-        if n == 0:
-            break
-        rays = torch.empty((n, 8), dtype=torch.float32, device=dev)
-        s2i = torch.empty(n, dtype=torch.int32, device=dev)
-        torch.cuda.synchronize()
-        d2d(rays.data_ptr(), rp, 32 * n)
-        d2d(s2i.data_ptr(), sp, 4 * n)
-        bounces.append(dict(
-            n=n, rays=rays, s2i=s2i,
-            keys=torch.empty(n, dtype=torch.int64 if kbytes == 8 else torch.int32, device=dev),
-            idx=torch.empty(n, dtype=torch.int32, device=dev),
-            out_rays=torch.empty_like(rays), out_s2i=torch.empty_like(s2i),
-            out_i2s=torch.empty(r.size[0] * r.size[1], dtype=torch.int32, device=dev),
-            hits=torch.empty((n, 4), dtype=torch.int32, device=dev),
-            hits_u=torch.empty((n, 4), dtype=torch.int32, device=dev),
-            sorter=stages.Sorter()))
-        b += 1
-    total_rays = sum(x["n"] for x in bounces)
-    log(f"captured {len(bounces)} bounces, {total_rays} path rays: {[x['n'] for x in bounces]}")
+    head = run_workload(dev, args, args.env, ablations=not args.no_ablations)
+    cfg = head["cfg"]
+    R = cfg["Renderer"]
+    dl = head["dominant_launch"]
+    achieved = dl["elems"] * dl["bytes_per_elem"] / (dl["ms"] / 1e3) / 1e9
+    figs = profile_figures()
+    valu = figs.get("trace_valu_busy")
+    roofline = {
+        "bound": "hbm", "kernel": dl["kernel"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": figs.get("trace_hbm_bytes_per_launch", {}).get("value"),
+        "traffic_source": figs.get("trace_hbm_bytes_per_launch", {}).get("source"),
+        "launch_ms": round(dl["ms"], 4), "launch_elems": dl["elems"],
+        "bytes_per_elem": dl["bytes_per_elem"],
+        "note": "traversal is bound by dependent-fetch latency and lane divergence, not HBM: "
+                "see valu_issue and l2_hit_rate",
+        "valu_issue": ({"busy_frac": valu["value"], "source": valu["source"],
+                        "definition": "SQ_ACTIVE_INST_VALU * 4 / (SIMDs * GRBM_GUI_ACTIVE), "
+                                      "1024 SIMDs, one wave64 VALU op per SIMD per 4 cycles"}
+                       if valu else None),
+        "lane_utilization": figs.get("trace_valu_lane_util", {}).get("value"),
+        "l2_hit_rate": figs.get("trace_l2_hit_rate", {}).get("value"),
+    }
 
-    class SceneRef:
-        handle = scene_h
+    interior = None
+    if not args.no_interior:
+        it = run_workload(dev, args, "config1.json", ablations=False)
+        interior = {k: it[k] for k in ("value", "ms_per_step", "rays", "rays_per_bounce",
+                                       "per_bounce_reorder_vs_unsorted",
+                                       "trace_speedup_sorted_vs_unsorted", "sorted_pipeline_mrays_s",
+                                       "unsorted_trace_mrays_s", "stages", "sort_largest_bounce")}
+        interior["value"] = round(interior["value"], 2)
+        interior["workload"] = ("medium plain: interior 263,250 tris, 1920x1080, 1 spp, 8 bounces, "
+                                "RR after bounce 3, 60-bit 5-D Morton keys")
+        del it
 
-    sc = SceneRef()
-    stream = torch.cuda.current_stream()
-    s_ptr = C.c_void_p(stream.cuda_stream)
-    lib = stages._capi.load()
-    mn3, mx3 = (C.c_float * 3)(*smin), (C.c_float * 3)(*smax)
-    # pre-size the sort scratch
-    for x in bounces:
-        need = lib.wf_sort_scratch_bytes(x["n"], kbytes, kbits, args.digit_bits)
-        x["sorter"].scratch = torch.empty(need, dtype=torch.uint8, device=dev)
+    r = head["renderer"]
+    r_sortlogs = None
+    from vulkanwavefrontpathtracer_amd.renderer import Renderer, env_path
 
-    names = ["keys", "sort", "reorder", "trace"]
-    ev = {k: [] for k in names}
+    fr = Renderer(env_path(args.env), {"Benchmark.printSortLogs": True})
+    fr.set_scene(type("S", (), {"handle": head["scene"]})(), list(head["bounds"][0]),
+                 list(head["bounds"][1]))
+    ref_style = reference_frame(fr, args.frames)
+    del fr
 
-    fused = not args.no_fuse
-
-    def step(record=False, sp=None, key_mode=None):
-        s_ptr = sp if sp is not None else C.c_void_p(stream.cuda_stream)
-        km = mode if key_mode is None else key_mode
-        for x in bounces:
-            n = x["n"]
-            marks = []
-            if record:
-                marks.append(torch.cuda.Event(enable_timing=True))
-                marks[-1].record(stream)
-            stages._capi.check("wf_morton_keys", lib.wf_morton_keys(
-                C.c_void_p(x["rays"].data_ptr()), C.c_void_p(x["keys"].data_ptr()),
-                C.c_void_p(x["idx"].data_ptr()), n, km, mn3, mx3, s_ptr, None))
-            if record:
-                marks.append(torch.cuda.Event(enable_timing=True))
-                marks[-1].record(stream)
-            stages._capi.check("wf_sort_pairs", lib.wf_sort_pairs(
-                C.c_void_p(x["keys"].data_ptr()), C.c_void_p(x["idx"].data_ptr()), n, kbytes, kbits,
-                args.digit_bits, C.c_void_p(x["sorter"].scratch.data_ptr()),
-                x["sorter"].scratch.numel(), s_ptr, None))
-            if record:
-                marks.append(torch.cuda.Event(enable_timing=True))
-                marks[-1].record(stream)
-            if not fused:
-                stages._capi.check("wf_reorder", lib.wf_reorder(
-                    C.c_void_p(x["idx"].data_ptr()), C.c_void_p(x["rays"].data_ptr()),
-                    C.c_void_p(x["out_rays"].data_ptr()), C.c_void_p(x["s2i"].data_ptr()),
-                    C.c_void_p(x["out_s2i"].data_ptr()), C.c_void_p(x["out_i2s"].data_ptr()), n,
-                    s_ptr, None))
-            if record:
-                marks.append(torch.cuda.Event(enable_timing=True))
-                marks[-1].record(stream)
-            if fused:
-                # reorder + trace in one kernel: the same outputs as the two calls
-                stages._capi.check("wf_reorder_trace", lib.wf_reorder_trace(
-                    sc.handle, C.c_void_p(x["idx"].data_ptr()), C.c_void_p(x["rays"].data_ptr()),
-                    C.c_void_p(x["out_rays"].data_ptr()), C.c_void_p(x["s2i"].data_ptr()),
-                    C.c_void_p(x["out_s2i"].data_ptr()), C.c_void_p(x["out_i2s"].data_ptr()),
-                    C.c_void_p(x["hits"].data_ptr()), n, 1, s_ptr, None))
-            else:
-                stages._capi.check("wf_trace", lib.wf_trace(
-                    sc.handle, C.c_void_p(x["out_rays"].data_ptr()), None,
-                    C.c_void_p(x["hits"].data_ptr()), n, 1, s_ptr, None))
-            if record:
-                marks.append(torch.cuda.Event(enable_timing=True))
-                marks[-1].record(stream)
-                for k, name in enumerate(names):
-                    ev[name].append((marks[k], marks[k + 1], n))
-
-    log(f"warmup {args.warmup} steps")
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    log(f"timing {args.steps} eager steps")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    wall_eager = time.perf_counter() - t0
-    value_eager = total_rays * args.steps / wall_eager / 1e6
-    log(f"eager: {value_eager:.1f} Mrays/s ({wall_eager * 1e3 / args.steps:.3f} ms/step)")
-
-    # The same step captured once in a HIP graph (every stage call is
-    # stream-ordered with no host synchronisation) and replayed.
-    graph = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(graph):
-        step(sp=C.c_void_p(torch.cuda.current_stream().cuda_stream))
-    for _ in range(args.warmup):
-        graph.replay()
-    torch.cuda.synchronize()
-    log(f"timing {args.steps} graph-replayed steps")
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        graph.replay()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = wall * 1e3 / args.steps
-    value = total_rays * args.steps / wall / 1e6
-    log(f"graph: {value:.1f} Mrays/s ({ms_per_step:.3f} ms/step)")
-    # the graph's last replay must equal an eager step's hits
-    graph_hits = [x["hits"].clone() for x in bounces]
-    step()
-    torch.cuda.synchronize()
-    graph_equal = all(torch.equal(h, x["hits"]) for h, x in zip(graph_hits, bounces))
-    del graph
-
-    # Per-stage times on the kernels' stream (separate, event-bracketed steps).
-    for _ in range(3):
-        step(record=True)
-    torch.cuda.synchronize()
-    stage_ms = {k: sum(a.elapsed_time(b_) for a, b_, _ in ev[k]) / 3 for k in names}
-    # Algorithmic bytes per element: keys 32+K+4, sort P*2*(K+4)+K,
-    # reorder 80, trace 32 read + 16 write.
-    passes = -(-kbits // args.digit_bits)
-    # Fused reorder + trace: 4 (index) + 32 (ray gathered once) + 32 (reordered
-    # ray) + 4 + 4 + 4 (LUTs) + 16 (hit) = 96 B, all under "trace".
-    bpe = {"keys": 32 + kbytes + 4, "sort": passes * 2 * (kbytes + 4) + kbytes, "reorder": 80,
-           "trace": 96 if fused else 48}
-    stages_roof = {}
-    for k in names:
-        t = stage_ms[k] / 1e3
-        if fused and k == "reorder":
-            stages_roof[k] = {"ms_per_frame": 0.0, "fused_into": "trace"}
-            continue
-        gbs = total_rays * bpe[k] / t / 1e9 if t > 0 else 0.0
-        stages_roof[k] = {"ms_per_frame": round(stage_ms[k], 4), "bytes_per_ray": bpe[k],
-                          "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
-    if fused:
-        stages_roof["trace"]["fused"] = "reorder + trace (wf_reorder_trace)"
-    dominant = max(names, key=lambda k: stage_ms[k])
-    dom = stages_roof[dominant]
-    # per-launch average duration of the dominant kernel at the largest bounce
-    big = max(range(len(bounces)), key=lambda i: bounces[i]["n"])
-    recs = [e for e in ev[dominant]]
-    launch_ms = [a.elapsed_time(b_) for (a, b_, n) in recs if n == bounces[big]["n"]]
-    avg_launch = sum(launch_ms) / len(launch_ms)
-    launch_bytes = bounces[big]["n"] * bpe[dominant]
-    achieved = launch_bytes / (avg_launch / 1e3) / 1e9
-
-    traffic = measured_traffic(dominant, bounces[big]["n"])
-
-    # Unsorted trace of the same wavefronts (sort ablation).
-    ts = []
-    for _ in range(0 if args.no_ablations else 3):
-        for x in bounces:
-            a = torch.cuda.Event(enable_timing=True)
-            b_ = torch.cuda.Event(enable_timing=True)
-            a.record(stream)
-            stages._capi.check("wf_trace", lib.wf_trace(sc.handle, C.c_void_p(x["rays"].data_ptr()), None,
-                                                  C.c_void_p(x["hits_u"].data_ptr()), x["n"], 1,
-                                                  s_ptr, None))
-            b_.record(stream)
-            ts.append((a, b_))
-    torch.cuda.synchronize()
-    unsorted_ms = sum(a.elapsed_time(b_) for a, b_ in ts) / 3 if ts else None
-    unsorted_mrays = total_rays / (unsorted_ms / 1e3) / 1e6 if ts else None
-    sorted_ms = sum(stage_ms.values())
-    sorted_mrays = total_rays / (sorted_ms / 1e3) / 1e6
-    trace_only_sorted = total_rays / (stage_ms["trace"] / 1e3) / 1e6
-    # parity of the sorted trace with the unsorted one (same hits per ray)
-    same = all(torch.equal(x["hits"][torch.argsort(x["out_s2i"])], x["hits_u"][torch.argsort(x["s2i"])])
-               for x in bounces) if ts else None
-
-    # Key layout ablation: the same 60 origin + direction bits concatenated
-    # (origin30 << 30 | oct30) or interleaved as a 5-D Morton code.
-    key_layout = None
-    if mode in (2, 4) and not args.no_ablations:
-        key_layout = {}
-        for km, name in ((2, "origin30_oct30"), (4, "origin_dir_morton60")):
-            for k in names:
-                ev[k].clear()
-            for _ in range(3):
-                step(record=True, key_mode=km)
-            torch.cuda.synchronize()
-            kms = {k: sum(a.elapsed_time(b_) for a, b_, _ in ev[k]) / 3 for k in names}
-            key_layout[name] = {
-                "stage_ms": {k: round(v, 4) for k, v in kms.items()},
-                "mrays_s": round(total_rays / (sum(kms.values()) / 1e3) / 1e6, 1),
-                "trace_speedup_vs_unsorted": round(unsorted_ms / kms["trace"], 3)}
-
-    # Reference-style full frames (Benchmark::run): path Mrays/s includes shade.
-    frame_walls, fstats = [], []
-    for _ in range(args.frames):
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        r.render()
-        frame_walls.append(time.perf_counter() - t1)
-        fstats.append(r.stats())
-    ref_style = None
-    if fstats:
-        fs = fstats[-1]
-        path_rays = fs["path_rays"]
-        ref_style = {
-            "path_rays": path_rays,
-            "path_mrays_s": round(path_rays / (fs["path_ms"] / 1e3) / 1e6, 1) if fs["path_ms"] > 0 else None,
-            "frame_ms_wall": round(1e3 * sum(frame_walls) / len(frame_walls), 2),
-            "rays_per_frame": fs["primary_rays"] + fs["shadow_rays"] + path_rays,
-            "stage_ms": {k: round(v, 3) for k, v in fs["stage_ms"].items()},
-        }
-
-    large = None if args.no_large else large_section(dev)
     adjoint = None if args.no_adjoint else adjoint_section(dev)
     batched = None if args.no_batch else batch_section(dev, args.batch_views)
-
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(bounces[0], scene_h, smin, smax, mode, kbits, args.cpu_sample, r)
+        cpu = cpu_baseline(head, args.cpu_sample)
 
     out = {
         "metric": "Mrays/s (trace+sort+reorder), plain vs batched & adjoint, at RMSE≤1e-3 vs FP64",
-        "value": round(value, 2),
+        "value": round(head["value"], 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(head["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded procedural interior scene, captured bounce wavefronts)",
+        "data": "synthetic (seeded procedural city scene, captured bounce wavefronts)",
         "config": {
-            "workload": "medium plain: interior 263,250 tris, 1920x1080, 1 spp, 8 bounces, RR after "
-                        "bounce 3, 60-bit keys (30 origin + 30 octahedral bits, " + cfg["Renderer"]["keyMode"]
-                        + "), u64 key sort + u32 payload",
-            "rays_per_step": total_rays, "bounces": len(bounces),
-            "rays_per_bounce": [x["n"] for x in bounces], "digit_bits": args.digit_bits,
+            "workload": "large plain: city 2,821,186 tris, 1920x1080 x 4 spp in flight, "
+                        f"{R['recursionDepth']} bounces, 60-bit keys (30 origin + 30 octahedral "
+                        f"bits, {R['keyMode']}), u64 key sort + u32 payload, reorder every bounce",
+            "rays_per_step": head["rays"], "bounces": len(head["rays_per_bounce"]),
+            "rays_per_bounce": head["rays_per_bounce"], "digit_bits": args.digit_bits,
             "parallelism": "1 GPU",
         },
-        "tuning": tuning_state(),
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "launch_ms": round(avg_launch, 4),
-                     "launch_elems": bounces[big]["n"], "bytes_per_elem": bpe[dominant]},
-        "stages": stages_roof,
-        "value_eager_launches": round(value_eager, 2),
-        "graph_replay_equals_eager": bool(graph_equal),
-        "sorted_pipeline_mrays_s": round(sorted_mrays, 1),
-        "trace_only_sorted_mrays_s": round(trace_only_sorted, 1),
-        "unsorted_trace_mrays_s": round(unsorted_mrays, 1) if ts else None,
-        "sort_speedup_trace": round(unsorted_ms / stage_ms["trace"], 3) if ts else None,
-        "sorted_vs_unsorted_hits_equal": same,
-        "key_layout_ablation": key_layout,
+        "roofline": roofline,
+        "stages": head["stages"],
+        "value_eager_launches": round(head["value_eager"], 2),
+        "graph_replay_equals_eager": head["graph_replay_equals_eager"],
+        "sorted_vs_unsorted_hits_equal": head["sorted_vs_unsorted_hits_equal"],
+        "sorted_pipeline_mrays_s": head["sorted_pipeline_mrays_s"],
+        "unsorted_trace_mrays_s": head["unsorted_trace_mrays_s"],
+        "per_bounce_reorder_vs_unsorted": head["per_bounce_reorder_vs_unsorted"],
+        "trace_speedup_sorted_vs_unsorted": head["trace_speedup_sorted_vs_unsorted"],
+        "sort_largest_bounce": head["sort_largest_bounce"],
+        "sort_log": head["sort_log"],
+        "ablations": head.get("ablations"),
         "reference_style_frame": ref_style,
-        "untimed_capture_frame": {"path_rays": frame_stats["path_rays"]},
-        "large_ablation": large,
+        "interior": interior,
         "adjoint": adjoint,
         "batched": batched,
+        "tuning": tuning_state(),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -391,8 +536,9 @@ def adjoint_section(dev, reps=3):
     """Differentiable workload: the interior scene with 512 material groups,
     512x512, 16 spp, 6 bounces, L2 loss against a target rendered from
     parameters perturbed by N(0, 0.05).  Forward = the C++ frame loop (frame
-    1); adjoint = the path-replay kernel.  Reports both GPU times, their
-    ratio and the adjoint's replayed rays per second."""
+    1); adjoint = the path-replay kernel.  Both are timed on the GPU (the
+    forward as the sum of its stage events, the adjoint by its own events)
+    and on the wall clock; the ratio is GPU over GPU."""
     import numpy as np
     import torch
 
@@ -429,155 +575,17 @@ def adjoint_section(dev, reps=3):
     st = r.stats()
     rays = st["primary_rays"] + st["shadow_rays"] + st["path_rays"]
     loss = float(((img[..., :3] - target[..., :3]).double() ** 2).sum())
-    f_ms = min(fwd_wall) * 1e3
-    a_ms = min(adj_ms)
+    f_gpu, a_gpu = min(fwd_ms), min(adj_ms)
+    f_wall, a_wall = min(fwd_wall) * 1e3, min(adj_wall) * 1e3
     return {"workload": "interior 263,250 tris, 512 material groups, 512x512, 16 spp, 6 bounces, "
                         "L2 loss vs perturbed-parameter target",
-            "forward_ms_wall": round(f_ms, 3), "forward_stage_gpu_ms": round(min(fwd_ms), 3),
-            "adjoint_ms": round(a_ms, 3), "adjoint_ms_wall": round(min(adj_wall) * 1e3, 3),
-            "adjoint_over_forward": round(a_ms / f_ms, 3),
-            "rays_per_frame": rays, "forward_mrays_s": round(rays / (f_ms / 1e3) / 1e6, 1),
-            "adjoint_mrays_s": round(rays / (a_ms / 1e3) / 1e6, 1), "loss": loss,
+            "forward_gpu_ms": round(f_gpu, 3), "forward_wall_ms": round(f_wall, 3),
+            "adjoint_gpu_ms": round(a_gpu, 3), "adjoint_wall_ms": round(a_wall, 3),
+            "adjoint_over_forward": round(a_gpu / f_gpu, 3),
+            "adjoint_over_forward_wall": round(a_wall / f_wall, 3),
+            "rays_per_frame": rays, "forward_mrays_s": round(rays / (f_gpu / 1e3) / 1e6, 1),
+            "adjoint_mrays_s": round(rays / (a_gpu / 1e3) / 1e6, 1), "loss": loss,
             "grad_norm": float(torch.sqrt((ga.double() ** 2).sum() + (ge.double() ** 2).sum()))}
-
-
-def large_section(dev, reps=3):
-    """Large scene reorder ablation: the city (~2.8M triangles), 1920x1080,
-    4 spp, 8 bounces.  For each key (30-bit origin keys in u32, 60-bit
-    origin + octahedral keys in u64, concatenated or interleaved as a 5-D
-    Morton code) and sort period (every bounce, every 2nd,
-    never) a frame of the C++ frame loop with those settings captures every
-    pass's path wavefronts; the 4 passes of each bounce are concatenated into
-    one wavefront (8.3M rays, all samples in flight) and the timed step runs
-    keys -> sort -> reorder + closest-hit trace (fused, wf_reorder_trace) on
-    the bounces the period sorts and the trace alone on the others."""
-    import torch
-
-    from vulkanwavefrontpathtracer_amd import stages
-    from vulkanwavefrontpathtracer_amd.renderer import Renderer, env_path
-
-    env = env_path("config2.json")
-    cfg = json.loads(env.read_text())
-    depth, spp = cfg["Renderer"]["recursionDepth"], cfg["Renderer"]["numberOfPrimarySamples"]
-    log("large ablation: building the city scene")
-    base = Renderer(env)
-    base.build_scene()
-
-    class SceneRef:
-        handle = base.scene_handle()
-
-    smin, smax = (C.c_float * 3)(), (C.c_float * 3)()
-    stages.call("wf_scene_bounds", SceneRef.handle, smin, smax)
-    lib = stages._capi.load()
-    stream = torch.cuda.current_stream()
-    s_ptr = C.c_void_p(stream.cuda_stream)
-    modes = [("32-bit", "origin30", 1, 4, 30), ("60-bit", "origin30_oct30", 2, 8, 60),
-             ("60-bit 5-D", "origin_dir_morton60", 4, 8, 60)]
-    rows, sort_rates = [], {}
-    for name, key_mode, mode, kbytes, kbits in modes:
-        for period in (1, 2, 0):
-            r = Renderer(env, {"Renderer.keyMode": key_mode, "Renderer.reorderPeriod": period,
-                               "Renderer.sortPathRays": period != 0,
-                               "Renderer.reorderPathRays": period != 0})
-            r.set_scene(SceneRef, list(smin), list(smax))
-            r.set_capture(True)
-            r.render()
-            waves = []
-            for b in range(depth):
-                parts = [r.captured(p * depth + b) for p in range(spp)]
-                n = sum(x[2] for x in parts)
-                if n == 0:
-                    break
-                rays = torch.empty((n, 8), dtype=torch.float32, device=dev)
-                s2i = torch.empty(n, dtype=torch.int32, device=dev)
-                o = 0
-                torch.cuda.synchronize()
-                for rp, sp, k in parts:
-                    if k:
-                        d2d(rays[o:].data_ptr(), rp, 32 * k)
-                        d2d(s2i[o:].data_ptr(), sp, 4 * k)
-                        o += k
-                waves.append(dict(n=n, rays=rays, s2i=s2i))
-            del r
-            nmax = max(w["n"] for w in waves)
-            keys = torch.empty(nmax, dtype=torch.int64 if kbytes == 8 else torch.int32, device=dev)
-            idx = torch.empty(nmax, dtype=torch.int32, device=dev)
-            out_rays = torch.empty((nmax, 8), dtype=torch.float32, device=dev)
-            out_s2i = torch.empty(nmax, dtype=torch.int32, device=dev)
-            out_i2s = torch.empty(1920 * 1080, dtype=torch.int32, device=dev)
-            hits = torch.empty((nmax, 4), dtype=torch.int32, device=dev)
-            best = None
-            for digit_bits in ((11, 8) if period == 1 else (sort_rates[name]["digit_bits"],)):
-                need = lib.wf_sort_scratch_bytes(nmax, kbytes, kbits, digit_bits)
-                scratch = torch.empty(need, dtype=torch.uint8, device=dev)
-                ms = {"keys": 0.0, "sort": 0.0, "reorder": 0.0, "trace": 0.0}
-                sort_big = []
-                for rep in range(reps + 1):
-                    evs = []
-                    for b, w in enumerate(waves):
-                        n = w["n"]
-                        srt = period > 0 and b % period == 0
-                        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
-                        ev[0].record(stream)
-                        if srt:
-                            stages._capi.check("wf_morton_keys", lib.wf_morton_keys(
-                                C.c_void_p(w["rays"].data_ptr()), C.c_void_p(keys.data_ptr()),
-                                C.c_void_p(idx.data_ptr()), n, mode, smin, smax, s_ptr, None))
-                        ev[1].record(stream)
-                        if srt:
-                            stages._capi.check("wf_sort_pairs", lib.wf_sort_pairs(
-                                C.c_void_p(keys.data_ptr()), C.c_void_p(idx.data_ptr()), n, kbytes,
-                                kbits, digit_bits, C.c_void_p(scratch.data_ptr()), need, s_ptr, None))
-                        ev[2].record(stream)
-                        ev[3].record(stream)
-                        if srt:
-                            # reorder gather fused into the trace (its time counts as trace)
-                            stages._capi.check("wf_reorder_trace", lib.wf_reorder_trace(
-                                SceneRef.handle, C.c_void_p(idx.data_ptr()),
-                                C.c_void_p(w["rays"].data_ptr()), C.c_void_p(out_rays.data_ptr()),
-                                C.c_void_p(w["s2i"].data_ptr()), C.c_void_p(out_s2i.data_ptr()),
-                                C.c_void_p(out_i2s.data_ptr()), C.c_void_p(hits.data_ptr()), n, 1,
-                                s_ptr, None))
-                        else:
-                            stages._capi.check("wf_trace", lib.wf_trace(
-                                SceneRef.handle, C.c_void_p(w["rays"].data_ptr()), None,
-                                C.c_void_p(hits.data_ptr()), n, 1, s_ptr, None))
-                        ev[4].record(stream)
-                        evs.append((ev, n, srt))
-                    torch.cuda.synchronize()
-                    if rep == 0:
-                        continue                     # warm-up
-                    for ev, n, srt in evs:
-                        for k, nm in enumerate(["keys", "sort", "reorder", "trace"]):
-                            ms[nm] += ev[k].elapsed_time(ev[k + 1]) / reps
-                        if srt and n == nmax:
-                            sort_big.append(ev[1].elapsed_time(ev[2]))
-                total_rays = sum(w["n"] for w in waves)
-                t = sum(ms.values())
-                res = {"keys": name, "period": period, "digit_bits": digit_bits,
-                       "mrays_s": round(total_rays / (t / 1e3) / 1e6, 1),
-                       "ms_per_frame": {k: round(v, 3) for k, v in ms.items()},
-                       "rays": total_rays, "rays_per_bounce": [w["n"] for w in waves]}
-                if sort_big:
-                    passes = -(-kbits // digit_bits)
-                    bpe = passes * 2 * (kbytes + 4) + kbytes
-                    gbs = nmax * bpe / (min(sort_big) / 1e3) / 1e9
-                    res["sort_GBs"] = round(gbs, 1)
-                    res["sort_frac_hbm"] = round(gbs / HBM_PEAK_GBS, 4)
-                if best is None or res["mrays_s"] > best["mrays_s"]:
-                    best = res
-                del scratch
-            if period == 1:
-                sort_rates[name] = best
-            rows.append(best)
-            log(f"large ablation {name} period {period}: {best['mrays_s']} Mrays/s")
-    out = {"workload": "city ~2.8M tris, 1920x1080, 4 spp in flight (8.3M rays/bounce), 8 bounces",
-           "rows": rows}
-    for name, *_ in modes:
-        every = next(x for x in rows if x["keys"] == name and x["period"] == 1)
-        never = next(x for x in rows if x["keys"] == name and x["period"] == 0)
-        out[f"speedup_every_vs_never_{name}"] = round(every["mrays_s"] / never["mrays_s"], 3)
-    return out
 
 
 def batch_section(dev, num_views=64, reps=2):
@@ -586,8 +594,10 @@ def batch_section(dev, num_views=64, reps=2):
     {128, 256, 512, 1024}^2 and spp in 1..16, 6 bounces, 2 shadow samples.
     The batch (one queue, scene-prefixed keys, graph-replayed passes,
     autotuned) against the same views rendered one after another by the
-    single-view frame loop (one renderer per view, buffers allocated by an
-    untimed warm-up frame)."""
+    single-view frame loop with the batch's tuned knobs (digit width, trace
+    block size, reorder period; one renderer per view, buffers allocated by
+    an untimed warm-up frame)."""
+    import numpy as np
     import torch
 
     from vulkanwavefrontpathtracer_amd import stages
@@ -618,29 +628,35 @@ def batch_section(dev, num_views=64, reps=2):
     rays = st["primary_rays"] + st["shadow_rays"] + st["path_rays"]
     batch_s = min(walls)
 
-    log("batched workload: sequential single-view frames")
+    log("batched workload: sequential single-view frames with the batch's knobs")
     cfg = json.loads(env.read_text())
-    rs = []
-    for v in views:
-        c = json.loads(json.dumps(cfg))
-        c["Resolution"] = {"width": v.width, "height": v.height}
-        c["Renderer"].update({"numberOfPrimarySamples": v.spp, "numberOfShadowSamples": S,
-                              "recursionDepth": depth, "rayType": "path", "russianRoulette": False,
-                              "sortPathRays": True, "reorderPathRays": True,
-                              "keyMode": "origin30_oct30"})
-        r = Renderer(c)
-        r.set_scene(scenes[v.scene], hosts[v.scene].bmin, hosts[v.scene].bmax)
-        r.render()                               # allocation + warm-up
-        rs.append(r)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for r in rs:
-        r.reset_frame_index()
-        r.render()
-    torch.cuda.synchronize()
-    seq_s = time.perf_counter() - t0
-    # spot parity: the largest and the smallest view against the batch
-    import numpy as np
+    saved_block = stages.get_tuning("trace.block")
+    stages.set_tuning("trace.block", tuning["block"])
+    try:
+        rs = []
+        for v in views:
+            c = json.loads(json.dumps(cfg))
+            c["Resolution"] = {"width": v.width, "height": v.height}
+            c["Renderer"].update({"numberOfPrimarySamples": v.spp, "numberOfShadowSamples": S,
+                                  "recursionDepth": depth, "rayType": "path",
+                                  "russianRoulette": False,
+                                  "sortPathRays": tuning["reorder_period"] > 0,
+                                  "reorderPathRays": tuning["reorder_period"] > 0,
+                                  "reorderPeriod": tuning["reorder_period"],
+                                  "digitBits": tuning["digit_bits"], "keyMode": "origin30_oct30"})
+            r = Renderer(c)
+            r.set_scene(scenes[v.scene], hosts[v.scene].bmin, hosts[v.scene].bmax)
+            r.render()                               # allocation + warm-up
+            rs.append(r)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for r in rs:
+            r.reset_frame_index()
+            r.render()
+        torch.cuda.synchronize()
+        seq_s = time.perf_counter() - t0
+    finally:
+        stages.set_tuning("trace.block", saved_block)
     sizes = [v.width * v.height * v.spp for v in views]
     check = sorted({int(np.argmax(sizes)), int(np.argmin(sizes))})
     equal = all(np.array_equal(b.pixels(k)[:, :3], rs[k].pixels()[:, :3]) for k in check)
@@ -650,6 +666,7 @@ def batch_section(dev, num_views=64, reps=2):
             "rays_per_frame": rays, "batched_s": round(batch_s, 4),
             "batched_gpu_ms": round(gpu_ms, 2), "batched_mrays_s": round(rays / batch_s / 1e6, 1),
             "sequential_s": round(seq_s, 4), "sequential_mrays_s": round(rays / seq_s / 1e6, 1),
+            "sequential_knobs": "the batch's digit width, trace block and reorder period",
             "speedup_vs_sequential": round(seq_s / batch_s, 3), "tuning": tuning,
             "autotune_s": round(tune_s, 2), "autotune_best_pass_ms": round(best_pass_ms, 3),
             "views_checked_equal": equal}
@@ -659,65 +676,54 @@ def tuning_state():
     from vulkanwavefrontpathtracer_amd import stages
 
     out = {}
-    for k in ("bvh.presplit", "bvh.split_depth", "bvh.leaf_max", "bvh.builder", "bvh.collapse",
-              "trace.lds_stack", "trace.block", "trace.persistent", "trace.refill", "trace.width",
-              "trace.loop", "trace.group64", "trace.group16", "trace.group4", "sort.onesweep_tiles"):
-        v = C.c_int()
-        stages.call("wf_get_tuning", k.encode(), C.byref(v))
-        out[k] = v.value
+    for k in ("bvh.presplit", "bvh.split_depth", "bvh.leaf_max", "bvh.collapse", "trace.block",
+              "trace.group64", "trace.group16", "trace.group4", "sort.onesweep_tiles",
+              "sort.rank_max"):
+        out[k] = stages.get_tuning(k)
     return out
 
 
-KEY_MODES = {"aila32": 0, "origin30": 1, "origin30_oct30": 2, "origin_dir_morton60": 4}
-TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_interior_1080p.txt"
-STAGE_KERNEL = {"trace": "trace_kernel", "sort": "sort_bin_kernel", "reorder": "reorder_kernel",
-                "keys": "keys_kernel"}
-
-
-def measured_traffic(stage, elems):
-    """HBM bytes per launch of the stage's kernel at this launch size, as
-    measured by the two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE, scaled
-    by a plain 1 GiB copy) summarised in profiles/; None when not measured."""
-    if not TRAFFIC_FILE.exists():
-        return None
-    for line in TRAFFIC_FILE.read_text().splitlines():
-        f = line.split()
-        if len(f) == 4 and f[0] == "traffic" and f[1] == STAGE_KERNEL.get(stage) and int(f[2]) == elems:
-            return int(f[3])
-    return None
-
-
-def cpu_baseline(b0, scene_h, smin, smax, mode, kbits, sample, renderer):
-    """The oracle's port of keys -> sort -> reorder -> trace on the host cores,
-    over the first `sample` rays of the largest bounce (bounded CPU work)."""
+def cpu_baseline(head, sample):
+    """The oracle's port of keys -> sort -> reorder -> trace on the host
+    cores, in the headline's key mode, over the headline step's bounce
+    wavefronts (all of them unless `sample` caps the rays per bounce); the
+    oracle's BVH is built before the clock starts, and the trace is its fp32
+    watertight one (the same records as the GPU's)."""
     import numpy as np
 
     from oracle import oracle
     from vulkanwavefrontpathtracer_amd import stages
 
-    n = min(sample, b0["n"])
-    rays = b0["rays"][:n].cpu().numpy()
-    s2i = b0["s2i"][:n].cpu().numpy()
-    env = json.loads((ROOT / "vulkanwavefrontpathtracer_amd" / "env" / "config1.json").read_text())
-    hs = stages.HostScene(1, env["Scene"]["seed"])
-    tv = hs.tri_verts()
+    cfg = head["cfg"]
+    hs = stages.HostScene(stages.SCENE_CITY, cfg["Scene"]["seed"])
+    mn, mx = list(head["bounds"][0]), list(head["bounds"][1])
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    log(f"cpu baseline: {n} rays, {threads} threads")
-    t0 = time.perf_counter()
-    keys = oracle.keys_oct60(rays, list(smin), list(smax)) if mode == 2 else \
-        oracle.keys_aila32(rays, list(smin), list(smax))
-    t1 = time.perf_counter()
-    k, v = oracle.sort_pairs(keys, np.arange(n, dtype=np.uint32), kbits)
-    t2 = time.perf_counter()
-    out_r, out_s, out_i = oracle.reorder(v, rays, s2i, int(s2i.max()) + 1)
-    t3 = time.perf_counter()
-    t_, tri = oracle.trace_bvh_f64(out_r, tv, closest=True)
-    t4 = time.perf_counter()
-    total = t4 - t0
+    waves = [(x["rays"][:sample].cpu().numpy(), x["s2i"][:sample].cpu().numpy()) for x in head["waves"]]
+    n = sum(len(r) for r, _ in waves)
+    log(f"cpu baseline: building the oracle BVH, then {n} rays on {threads} threads")
+    bvh = oracle.Bvh(hs.tri_verts())
+    t = {"keys": 0.0, "sort": 0.0, "reorder": 0.0, "trace": 0.0}
+    hits = 0
+    for rays, s2i in waves:
+        t0 = time.perf_counter()
+        keys = oracle.keys_morton5d60(rays, mn, mx)
+        t1 = time.perf_counter()
+        k, v = oracle.sort_pairs(keys, np.arange(len(rays), dtype=np.uint32), 60)
+        t2 = time.perf_counter()
+        out_r, out_s, out_i = oracle.reorder(v, rays, s2i, int(s2i.max()) + 1)
+        t3 = time.perf_counter()
+        h = bvh.trace_f32(out_r, closest=True)
+        t4 = time.perf_counter()
+        hits += int((h[:, 0] >= 0).sum())
+        for name, a, b in (("keys", t0, t1), ("sort", t1, t2), ("reorder", t2, t3), ("trace", t3, t4)):
+            t[name] += b - a
+    total = sum(t.values())
     return {"value": round(n / total / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"first {n} path rays of bounce 0 of the same frame; oracle keys/sort/reorder "
-                      f"single-threaded ({t1 - t0:.2f}+{t2 - t1:.2f}+{t3 - t2:.2f} s), fp64 BVH "
-                      f"trace on {threads} OpenMP threads ({t4 - t3:.2f} s, BVH build included)"}
+            "sample": f"{n} path rays: {'the whole headline step' if n == head['rays'] else f'the first {sample} rays of each bounce of the headline step'}; "
+                      f"oracle 60-bit 5-D Morton keys, stable sort and reorder single-threaded "
+                      f"({t['keys']:.2f}+{t['sort']:.2f}+{t['reorder']:.2f} s), fp32 watertight BVH trace "
+                      f"on {threads} OpenMP threads ({t['trace']:.2f} s, BVH built before timing); "
+                      f"{hits} hits"}
 
 
 if __name__ == "__main__":

@@ -345,6 +345,27 @@ def trace_bvh_f32(rays, tri_verts, closest=True):
     return out
 
 
+class Bvh:
+    """The fp32 trace's BVH built once (or_bvh_create) for repeated traces."""
+
+    def __init__(self, tri_verts):
+        self.tv = np.ascontiguousarray(tri_verts, np.float32).reshape(-1, 9)
+        lib = load()
+        lib.or_bvh_create.restype = C.c_void_p
+        self.h = C.c_void_p(lib.or_bvh_create(_p(self.tv), C.c_int64(len(self.tv))))
+
+    def trace_f32(self, rays, closest=True):
+        r = _rays(rays)
+        out = np.zeros((len(r), 4), np.int32)
+        load().or_bvh_trace_f32(self.h, _p(r), C.c_int64(len(r)), C.c_int(int(closest)), _p(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            load().or_bvh_destroy(self.h)
+            self.h = None
+
+
 def lcg(seed, n):
     """n draws of raygenPath's LCG from seed: (states uint32, floats)."""
     st = np.zeros(n, np.uint32)

@@ -1409,15 +1409,33 @@ OR_API void or_trace_bvh_f64(const float* rays, int64_t n, const float* tri_vert
  * ulps, ~5e-7 relative) and the slab interval is tested in fp64 with slack.
  * Any hit: the smallest triangle index hit within the interval, which is
  * what the brute-force scan returns.  Output layout as or_trace_f32. */
-OR_API void or_trace_bvh_f32(const float* rays, int64_t n, const float* tri_verts, int64_t ntri,
-                             int closest, int32_t* out) {
-    cbvh b;
-    b.tv = tri_verts;
-    b.nodes = (bnode*)malloc(sizeof(bnode) * (2 * ntri + 1));
-    b.order = (int*)malloc(sizeof(int) * (ntri + 1));
-    b.nn = 0;
-    for (int64_t i = 0; i < ntri; ++i) b.order[i] = (int)i;
-    if (ntri > 0) cb_build(&b, 0, (int)ntri);
+typedef struct { cbvh b; int64_t ntri; } or_bvh;
+
+/* Build once, trace many times (the CPU baseline times the trace alone). */
+OR_API or_bvh* or_bvh_create(const float* tri_verts, int64_t ntri) {
+    or_bvh* h = (or_bvh*)malloc(sizeof(or_bvh));
+    h->ntri = ntri;
+    h->b.tv = tri_verts;
+    h->b.nodes = (bnode*)malloc(sizeof(bnode) * (2 * ntri + 1));
+    h->b.order = (int*)malloc(sizeof(int) * (ntri + 1));
+    h->b.nn = 0;
+    for (int64_t i = 0; i < ntri; ++i) h->b.order[i] = (int)i;
+    if (ntri > 0) cb_build(&h->b, 0, (int)ntri);
+    return h;
+}
+
+OR_API void or_bvh_destroy(or_bvh* h) {
+    if (!h) return;
+    free(h->b.nodes);
+    free(h->b.order);
+    free(h);
+}
+
+OR_API void or_bvh_trace_f32(const or_bvh* h, const float* rays, int64_t n, int closest,
+                             int32_t* out) {
+    const cbvh b = h->b;
+    const float* tri_verts = b.tv;
+    const int64_t ntri = h->ntri;
 #pragma omp parallel for schedule(dynamic, 64)
     for (int64_t i = 0; i < n; ++i) {
         const float* ry = rays + 8 * i;
@@ -1485,6 +1503,11 @@ OR_API void or_trace_bvh_f32(const float* rays, int64_t n, const float* tri_vert
         }
         q[3] = 0;
     }
-    free(b.nodes);
-    free(b.order);
+}
+
+OR_API void or_trace_bvh_f32(const float* rays, int64_t n, const float* tri_verts, int64_t ntri,
+                             int closest, int32_t* out) {
+    or_bvh* h = or_bvh_create(tri_verts, ntri);
+    or_bvh_trace_f32(h, rays, n, closest, out);
+    or_bvh_destroy(h);
 }
