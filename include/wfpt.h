@@ -304,22 +304,10 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
  *                           diagonal / this (default 64; 32-128 measure
  *                           alike on the interior scene, 256+ slower)
  *   "bvh.leaf_max"     1-8  triangles per collapsed leaf (default 1)
- *   "bvh.builder"      0/1  binary hierarchy: 0 Karras LBVH over centroid Morton
- *                           codes (default), 1 PLOC clustering of the
- *                           Morton-sorted references (Meister & Bittner 2018);
- *                           a PLOC tree too deep for the traversal stack is
- *                           rebuilt as the LBVH
- *   "bvh.ploc_radius"  1-32 PLOC nearest-neighbour search radius (default 16)
  *   "bvh.collapse"     0/1  4-wide nodes: 0 collapse every second level,
  *                           1 open the largest-area child until 4 children
  *                           (default 1)
- *   "trace.lds_stack"  0/1  keep the top of the traversal stack in LDS (default 1)
  *   "trace.block"      64/128/256  threads per traversal workgroup (default 64)
- *   "trace.persistent" 0/1  persistent waves refilling idle lanes from a ray
- *                           counter (default 0)
- *   "trace.refill"     1-64 idle lanes of a wave that trigger a refill (default 24)
- *   "trace.width"      2/4  traverse the 2-wide or the 4-wide quantized hierarchy
- *                           (both are built; default 4)
  *   "sort.onesweep_tiles" >= 0  sorts of more than 8192 pairs in at most this
  *                           many 4096-pair tiles, with digits of at most 8
  *                           bits, run one kernel per pass with decoupled
@@ -344,23 +332,7 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
  *                           read when a pass is enqueued or captured)
  *   "trace.fuse_reorder" 0/1 wf_reorder_trace as one fused kernel (1, default)
  *                           or as the reorder and trace kernels (0)
- *   "trace.loop"       0-3  4-wide traversal loop: 0 branch on node/leaf, then
- *                           load; 1 one load per step for node or leaf
- *                           (default); 2 leaves postponed until every lane of
- *                           the wave holds one (Aila & Laine 2009); 3 lanes at a
- *                           leaf wait until "trace.leaf_gate" lanes (1-64,
- *                           default 16) wait, then test their triangles
- *   "trace.compact"    0-64 tail hand-over of the loop 1 walk (wavefronts of at
- *                           least "trace.compact_min" rays, default 262144):
- *                           a wave with fewer walking lanes writes their
- *                           traversal state to a per-scene queue and a tail
- *                           pass finishes them ("trace.tail_refill" idle
- *                           lanes, default 16, take more; queue of
- *                           "trace.tail_cap" records at the scene's first
- *                           use, default max(2^20, n/2)); same results; 0 off
- *                           (default).  Traces of one scene on different
- *                           streams must not overlap while it is on.
- *   "trace.xcd"        >= 0 loop 1 walk: the 8 XCDs take runs of this many
+ *   "trace.xcd"        >= 0 the 8 XCDs take runs of this many
  *                           consecutive workgroups instead of one each
  *                           (0 off, default)
  * Returns WF_ERR_INVALID_ARG for an unknown key or value out of range. */

@@ -1,6 +1,6 @@
-"""GPU checks of the hierarchy builders.  Both binary builders (the Karras
-LBVH over centroid Morton codes and PLOC clustering) and both 4-wide
-collapses (every second level, greedy largest-area) must give a valid tree:
+"""GPU checks of the hierarchy builder.  The binary LBVH (Karras, over
+centroid Morton codes) and both 4-wide collapses (every second level,
+greedy largest-area) must give a valid tree:
 every leaf slot reached once and every node once, child boxes that contain
 the triangles below them, a traversal stack bound that holds on every
 root-to-leaf path, and closest-hit records equal to the oracle's bit for
@@ -15,7 +15,7 @@ import torch
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [(0, 0), (0, 1), (1, 0), (1, 1)]   # (bvh.builder, bvh.collapse)
+COLLAPSE = [0, 1]   # bvh.collapse: every second level, greedy largest-area
 
 
 @contextlib.contextmanager
@@ -157,11 +157,11 @@ def check_wide(tri_verts, sc):
     return max_need, st4.value
 
 
-@pytest.mark.parametrize("builder,collapse", VARIANTS)
-def test_builders_structure_boxes(dev, builder, collapse):
+@pytest.mark.parametrize("collapse", COLLAPSE)
+def test_builders_structure_boxes(dev, collapse):
     from vulkanwavefrontpathtracer_amd import stages
 
-    with knobs(bvh__builder=builder, bvh__collapse=collapse):
+    with knobs(bvh__collapse=collapse):
         hs = stages.HostScene(stages.SCENE_BOXES, 1000)
         sc = stages.Scene.from_host(hs, dev)
         depth = check_binary(hs.num_tris, sc)
@@ -175,11 +175,10 @@ _INTERIOR = {}
 
 
 @pytest.mark.parametrize("leaf", [1, 4])
-@pytest.mark.parametrize("builder,collapse", VARIANTS)
-def test_builders_bit_exact_interior(dev, orc, builder, collapse, leaf):
-    """Interior scene (walls split into many references): every builder and
-    collapse returns the oracle's closest-hit records, any-hit agrees on
-    visibility, in both the 4-wide and 2-wide traversal."""
+@pytest.mark.parametrize("collapse", COLLAPSE)
+def test_builders_bit_exact_interior(dev, orc, collapse, leaf):
+    """Interior scene (walls split into many references): both collapses
+    return the oracle's closest-hit records, any-hit agrees on visibility."""
     from vulkanwavefrontpathtracer_amd import stages
 
     if leaf not in _INTERIOR:
@@ -188,26 +187,23 @@ def test_builders_bit_exact_interior(dev, orc, builder, collapse, leaf):
         rays[::101, 7] = -1.0
         _INTERIOR[leaf] = (hs, rays, expected_hits(orc, hs.tri_verts(), hs.geom_first, rays))
     hs, rays, exp = _INTERIOR[leaf]
-    with knobs(bvh__builder=builder, bvh__collapse=collapse, bvh__leaf_max=leaf):
+    with knobs(bvh__collapse=collapse, bvh__leaf_max=leaf):
         sc = stages.Scene.from_host(hs, dev)
         tr = torch.from_numpy(rays).to(dev)
-        for width in (4, 2):
-            with knobs(trace__width=width):
-                got = stages.trace(sc, tr).cpu().numpy()
-                bad = np.nonzero(np.any(got != exp, axis=1))[0]
-                assert len(bad) == 0, f"width {width}: {len(bad)} mismatches, e.g. {bad[:3]}"
-                anyh = stages.trace(sc, tr, closest=False).cpu().numpy()
-                assert np.array_equal(anyh[:, 0] >= 0, got[:, 0] >= 0)
+        got = stages.trace(sc, tr).cpu().numpy()
+        bad = np.nonzero(np.any(got != exp, axis=1))[0]
+        assert len(bad) == 0, f"{len(bad)} mismatches, e.g. {bad[:3]}"
+        anyh = stages.trace(sc, tr, closest=False).cpu().numpy()
+        assert np.array_equal(anyh[:, 0] >= 0, got[:, 0] >= 0)
     assert np.mean(exp[:, 0] >= 0) > 0.5
 
 
 @pytest.mark.parametrize("presplit", [0, 1])
-@pytest.mark.parametrize("builder,collapse", VARIANTS)
-def test_builders_degenerate_inputs(dev, orc, builder, collapse, presplit):
+@pytest.mark.parametrize("collapse", COLLAPSE)
+def test_builders_degenerate_inputs(dev, orc, collapse, presplit):
     """Coincident triangles (identical boxes, tied hits: the smaller original
-    index wins) and two- and three-triangle scenes.  With split references
-    the large overlapping pieces can make a clustering tree too deep for the
-    traversal stack; the build then falls back to the Karras tree."""
+    index wins) and two- and three-triangle scenes, with and without split
+    references."""
     from vulkanwavefrontpathtracer_amd import stages
 
     rng = np.random.default_rng(5)
@@ -219,7 +215,7 @@ def test_builders_degenerate_inputs(dev, orc, builder, collapse, presplit):
     verts[:, :3] = tris.reshape(-1, 3)
     verts[:, 5] = 1.0
     idx = np.arange(len(tris) * 3, dtype=np.uint32).reshape(-1, 3)
-    with knobs(bvh__builder=builder, bvh__collapse=collapse, bvh__presplit=presplit):
+    with knobs(bvh__collapse=collapse, bvh__presplit=presplit):
         for nt in (2, 3, len(tris)):
             gf = np.array([0, nt], np.int32)
             sc = stages.Scene(verts[:3 * nt], idx[:nt], gf, device=dev)
@@ -234,16 +230,17 @@ def test_builders_degenerate_inputs(dev, orc, builder, collapse, presplit):
 
 
 def test_builder_node_visits(dev):
-    """Traversal work per ray on the interior scene for each builder and
-    collapse (printed); PLOC must not visit more nodes than the LBVH."""
+    """Traversal work per ray on the interior scene for each collapse
+    (printed); the greedy largest-area collapse must not visit more nodes
+    than collapsing every second level."""
     from vulkanwavefrontpathtracer_amd import _capi, stages
 
     hs = stages.HostScene(stages.SCENE_INTERIOR, 1001)
     rays = torch.from_numpy(rays_in_box(hs.bmin, hs.bmax, 20000, 3)).to(dev)
     hits = torch.empty((len(rays), 4), dtype=torch.int32, device=dev)
     visits = {}
-    for builder, collapse in VARIANTS:
-        with knobs(bvh__builder=builder, bvh__collapse=collapse):
+    for collapse in COLLAPSE:
+        with knobs(bvh__collapse=collapse):
             sc = stages.Scene.from_host(hs, dev)
             nv = torch.zeros(len(rays), dtype=torch.int32, device=dev)
             nt = torch.zeros_like(nv)
@@ -251,6 +248,6 @@ def test_builder_node_visits(dev):
                        C.c_void_p(hits.data_ptr()), len(rays), 1, C.c_void_p(nv.data_ptr()),
                        C.c_void_p(nt.data_ptr()), None)
             torch.cuda.synchronize()
-            visits[(builder, collapse)] = (nv.float().mean().item(), nt.float().mean().item())
-    print("node visits / triangle tests per ray (builder, collapse):", visits)
-    assert visits[(1, 0)][0] <= visits[(0, 0)][0] * 1.05
+            visits[collapse] = (nv.float().mean().item(), nt.float().mean().item())
+    print("node visits / triangle tests per ray by collapse:", visits)
+    assert visits[1][0] <= visits[0][0] * 1.05

@@ -298,20 +298,16 @@ def test_trace_sort_composite(dev, orc, boxes):
         assert all(t >= 0 for t in times) and times[1] > 0
 
 
-@pytest.mark.parametrize("presplit,leaf,lds,block,persistent,refill,width", [
-    (0, 4, 0, 256, 0, 16, 2), (1, 1, 1, 64, 0, 16, 2), (1, 8, 1, 128, 0, 16, 2), (1, 4, 0, 256, 0, 16, 4),
-    (1, 2, 1, 256, 0, 16, 4), (1, 8, 1, 64, 1, 16, 2), (1, 8, 1, 64, 1, 1, 4), (1, 8, 0, 128, 1, 64, 4),
-    (1, 4, 1, 256, 1, 32, 2), (1, 8, 1, 64, 1, 16, 4), (0, 1, 1, 64, 0, 16, 4)])
-def test_tuning_variants_bit_exact(dev, orc, presplit, leaf, lds, block, persistent, refill, width):
-    """Every build/traversal variant the autotuner may pick (including the
-    persistent kernel with its refill threshold) returns the same records
-    (interior scene: walls are split into many references)."""
+@pytest.mark.parametrize("presplit,leaf,block", [
+    (0, 4, 256), (1, 1, 64), (1, 8, 128), (1, 4, 256), (1, 2, 256), (1, 8, 64), (0, 1, 64)])
+def test_tuning_variants_bit_exact(dev, orc, presplit, leaf, block):
+    """Every build/traversal setting the tuner may pick (reference splitting,
+    leaf size, workgroup size) returns the same records (interior scene:
+    walls are split into many references)."""
     from vulkanwavefrontpathtracer_amd import _capi, stages
 
     saved = {}
-    knobs = {"bvh.presplit": presplit, "bvh.leaf_max": leaf, "trace.lds_stack": lds,
-             "trace.block": block, "trace.persistent": persistent, "trace.refill": refill,
-             "trace.width": width}
+    knobs = {"bvh.presplit": presplit, "bvh.leaf_max": leaf, "trace.block": block}
     import ctypes as C
     for k, v in knobs.items():
         cur = C.c_int()
@@ -407,9 +403,8 @@ def _fused_oracle(orc, hs, n, closest, idx, rays, s2i):
     return _FUSED_ORACLE[key]
 
 
-@pytest.mark.parametrize("knobs", [{}, {"trace.loop": 0}, {"trace.loop": 2}, {"trace.loop": 3},
-                                   {"trace.block": 128}, {"trace.xcd": 8},
-                                   {"trace.fuse_reorder": 0}, {"trace.width": 2}])
+@pytest.mark.parametrize("knobs", [{}, {"trace.block": 128}, {"trace.block": 64}, {"trace.xcd": 8},
+                                   {"trace.fuse_reorder": 0}])
 def test_reorder_trace_fused_bit_exact(dev, orc, knobs):
     """wf_reorder_trace writes exactly what wf_reorder followed by wf_trace on
     the reordered rays writes (rays, both LUTs, hit records), on wavefronts
@@ -466,116 +461,6 @@ def test_reorder_trace_fused_bit_exact(dev, orc, knobs):
     finally:
         for k, v in saved.items():
             _capi.call("wf_set_tuning", k.encode(), v)
-
-
-@pytest.mark.parametrize("gate", [1, 16, 64])
-def test_gated_leaf_loop_bit_exact(dev, orc, gate):
-    """trace.loop 3 (lanes at a leaf wait for `gate` waiting lanes) returns the
-    oracle's closest-hit records and any-hit visibility."""
-    import ctypes as C
-
-    from vulkanwavefrontpathtracer_amd import _capi, stages
-
-    saved = {}
-    for k, v in {"trace.loop": 3, "trace.leaf_gate": gate}.items():
-        cur = C.c_int()
-        _capi.call("wf_get_tuning", k.encode(), C.byref(cur))
-        saved[k] = cur.value
-        _capi.call("wf_set_tuning", k.encode(), v)
-    try:
-        hs = stages.HostScene(stages.SCENE_INTERIOR, 1001)
-        sc = stages.Scene.from_host(hs, dev)
-        rays = scene_rays(hs, 70_001, 300 + gate, tmin=1e-4)
-        rays[::61, 7] = -1.0
-        tr = torch.from_numpy(rays).to(dev)
-        got = stages.trace(sc, tr).cpu().numpy()
-        sub = np.arange(0, len(rays), 23)
-        exp = expected_hits(orc, hs, rays[sub])
-        assert np.array_equal(got[sub], exp)
-        anyh = stages.trace(sc, tr, closest=False).cpu().numpy()
-        assert np.array_equal(anyh[:, 0] >= 0, got[:, 0] >= 0)
-    finally:
-        for k, v in saved.items():
-            _capi.call("wf_set_tuning", k.encode(), v)
-
-
-@pytest.mark.parametrize("knobs", [
-    {"trace.compact": 16, "trace.compact_min": 0},
-    {"trace.compact": 64, "trace.compact_min": 0},
-    {"trace.compact": 48, "trace.compact_min": 0, "trace.tail_refill": 1},
-    {"trace.compact": 64, "trace.compact_min": 0, "trace.tail_cap": 1000},
-    {"trace.compact": 32, "trace.compact_min": 0, "trace.block": 256},
-    {},
-])
-def test_tail_handover_bit_exact(dev, orc, knobs):
-    """Waves that hand their last rays to the tail pass (trace.compact) give
-    the same hit records as walks without hand-over, for direct, indirect and
-    fused reorder traces, closest and any hit (the tail pass continues the
-    same step sequence), including a queue too small for the reservations
-    (trace.tail_cap, voided records) and the default knobs at a wavefront
-    above trace.compact_min; sampled rays also equal the oracle."""
-    import ctypes as C
-
-    from vulkanwavefrontpathtracer_amd import _capi, stages
-
-    def get(k):
-        cur = C.c_int()
-        _capi.call("wf_get_tuning", k.encode(), C.byref(cur))
-        return cur.value
-
-    keys_all = set(knobs) | {"trace.compact"}
-    saved = {k: get(k) for k in keys_all}
-    on = {k: v for k, v in knobs.items()}
-    on.setdefault("trace.compact", saved["trace.compact"])
-
-    def apply(d):
-        for k, v in d.items():
-            _capi.call("wf_set_tuning", k.encode(), v)
-
-    try:
-        apply(on)  # the scene's queue is allocated with these knobs
-        hs = stages.HostScene(stages.SCENE_INTERIOR, 1001)
-        sc = stages.Scene.from_host(hs, dev)
-        mn, mx = sc.bounds()
-        sizes = ((300_000, 1),) if not knobs else ((70_001, 1), (150_000, 0))
-        for n, closest in sizes:
-            rays = torch.from_numpy(scene_rays(hs, n, 7 * n + closest, tmin=1e-4)).to(dev)
-            rays[::97, 7] = -1.0
-            perm = torch.randperm(n, device=dev).to(torch.int32)
-            keys, idx = stages.morton_keys(rays, stages.KEY_ORIGIN_DIR_MORTON60, mn, mx)
-            stages.sort_pairs(keys, idx, 60, 8)
-            s2i = torch.arange(n, dtype=torch.int32, device=dev)
-            res = []
-            for setting in (on, {"trace.compact": 0}):
-                apply(setting)
-                direct = torch.full((n, 4), -9, dtype=torch.int32, device=dev)
-                _capi.call("wf_trace", sc.handle, C.c_void_p(rays.data_ptr()), None,
-                           C.c_void_p(direct.data_ptr()), n, closest, None, None)
-                indirect = torch.full((n, 4), -9, dtype=torch.int32, device=dev)
-                _capi.call("wf_trace", sc.handle, C.c_void_p(rays.data_ptr()),
-                           C.c_void_p(perm.data_ptr()), C.c_void_p(indirect.data_ptr()), n,
-                           closest, None, None)
-                out_rays = torch.empty_like(rays)
-                out_s2i = torch.empty_like(s2i)
-                out_i2s = torch.empty(n, dtype=torch.int32, device=dev)
-                fused = torch.full((n, 4), -9, dtype=torch.int32, device=dev)
-                ptrs = [C.c_void_p(t.data_ptr()) for t in (idx, rays, out_rays, s2i, out_s2i, out_i2s)]
-                _capi.call("wf_reorder_trace", sc.handle, *ptrs, C.c_void_p(fused.data_ptr()), n,
-                           closest, None, None)
-                torch.cuda.synchronize()
-                res.append((direct, indirect, fused, out_rays))
-            for a, b in zip(*res):
-                assert torch.equal(a, b), (n, closest, knobs)
-            got = res[0][0].cpu().numpy()
-            sub = np.arange(0, n, 211)
-            exp = expected_hits(orc, hs, rays.cpu().numpy()[sub], closest=bool(closest))
-            if closest:
-                assert np.array_equal(got[sub], exp)
-            else:
-                assert np.array_equal(got[sub, 0] >= 0, exp[:, 0] >= 0)
-            apply(on)
-    finally:
-        apply(saved)
 
 
 @pytest.mark.parametrize("n,k", [(64 * 8 * 500 + 3 * 64 + 5, 1), (64 * 8 * 500 + 3 * 64 + 5, 16),

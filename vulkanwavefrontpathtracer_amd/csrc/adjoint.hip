@@ -33,9 +33,9 @@ namespace wfpt {
 
 constexpr int ADJ_BLOCK = 64;
 
-template <bool LDS_GRAD, bool WIDE>
+template <bool LDS_GRAD>
 __global__ __launch_bounds__(ADJ_BLOCK) void adjoint_kernel(const AdjointParams p) {
-    const float4* nodes = WIDE ? p.nodes4 : p.nodes;
+    const float4* nodes = p.nodes4;
     __shared__ int s_stack[LDS_DEPTH * ADJ_BLOCK];
     extern __shared__ float s_grad[];  // [3 * nmat] albedo, then [3 * nmat] emission
     const int nm3 = 3 * p.nmat;
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adjoint_kernel(const AdjointParams 
                 WtRay wr;
                 const bool live = (rd.w >= ro.w) && wt_setup(ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, wr);
                 const TraceOut tr =
-                    traverse<true, ADJ_BLOCK, true, false, WIDE>(nodes, p.tris, ro, rd, wr, live, s_stack);
+                    traverse<true, ADJ_BLOCK, false>(nodes, p.tris, ro, rd, wr, live, s_stack);
                 int tri;
                 const int4 hit = hit_record(p.tris, wr, tr, &tri);
                 float Y[3];
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void adjoint_kernel(const AdjointParams 
                     WtRay swr;
                     const bool slive = (sb.w >= sa.w) && wt_setup(sa.x, sa.y, sa.z, sb.x, sb.y, sb.z, swr);
                     const TraceOut st =
-                        traverse<false, ADJ_BLOCK, true, false, WIDE>(nodes, p.tris, sa, sb, swr, slive, s_stack);
+                        traverse<false, ADJ_BLOCK, false>(nodes, p.tris, sa, sb, swr, slive, s_stack);
                     if (st.slot < 0) vis += 1.0f;
                 }
                 vis /= (float)p.S;
@@ -186,9 +186,9 @@ int adjoint_launch(const AdjointParams& p, hipStream_t s) {
     // pass traversed.
     const unsigned g = grid_for(np, ADJ_BLOCK);
     if (lds <= ADJ_LDS_GRAD_BYTES)
-        adjoint_kernel<true, true><<<g, ADJ_BLOCK, lds, s>>>(p);
+        adjoint_kernel<true><<<g, ADJ_BLOCK, lds, s>>>(p);
     else
-        adjoint_kernel<false, true><<<g, ADJ_BLOCK, 0, s>>>(p);
+        adjoint_kernel<false><<<g, ADJ_BLOCK, 0, s>>>(p);
     return launch_check("adjoint_kernel");
 }
 

@@ -72,7 +72,6 @@ struct BatchBuffers {
 
 struct BatchTuning {
     int digit_bits = 8, block = 64, reorder_period = 1;
-    int width = 4;  // hierarchy traversed (the process-wide trace.width at render time)
 };
 
 // Enqueue one pass (primary rays of every view with spp > pass, then

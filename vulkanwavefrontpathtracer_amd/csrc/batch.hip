@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void batch_primary_kernel(const BatchTables t,
 
 // ----------------------------------------------------------------- trace
 // Ray i belongs to the path owner_s2i[i / mult] (mult = S for shadow rays).
-template <bool CLOSEST, int BLOCK, bool WIDE>
+template <bool CLOSEST, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void batch_trace_kernel(const BatchTables t,
                                                             const uint32_t* __restrict__ q, int b,
                                                             int mult,
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(BLOCK) void batch_trace_kernel(const BatchTables t,
     WtRay wr;
     const bool live = (rd.w >= ro.w) && wt_setup(ro.x, ro.y, ro.z, rd.x, rd.y, rd.z, wr);
     const TraceOut tr =
-        traverse<CLOSEST, BLOCK, true, false, WIDE>(WIDE ? sc.nodes4 : sc.nodes, sc.tris, ro, rd, wr, live, s_stack);
+        traverse<CLOSEST, BLOCK, false>(sc.nodes4, sc.tris, ro, rd, wr, live, s_stack);
     *reinterpret_cast<int4*>(hits + i) = hit_record(sc.tris, wr, tr);
 }
 
@@ -330,29 +330,21 @@ __global__ __launch_bounds__(256) void batch_reorder_kernel(const uint32_t* __re
 }
 
 // ------------------------------------------------------------------ host
-template <bool CLOSEST, bool WIDE>
-static void launch_batch_trace_w(int block, const BatchTables& t, const uint32_t* q, int b, int mult,
-                                 const wf_ray* rays, const int32_t* owner, wf_hit* hits,
-                                 int64_t cap, hipStream_t s) {
-    switch (block) {
-        case 128:
-            batch_trace_kernel<CLOSEST, 128, WIDE><<<grid_for(cap, 128), 128, 0, s>>>(t, q, b, mult, rays, owner, hits);
-            break;
-        case 256:
-            batch_trace_kernel<CLOSEST, 256, WIDE><<<grid_for(cap, 256), 256, 0, s>>>(t, q, b, mult, rays, owner, hits);
-            break;
-        default:
-            batch_trace_kernel<CLOSEST, 64, WIDE><<<grid_for(cap, 64), 64, 0, s>>>(t, q, b, mult, rays, owner, hits);
-            break;
-    }
-}
-
 template <bool CLOSEST>
 static void launch_batch_trace(const BatchTuning& tu, const BatchTables& t, const uint32_t* q, int b,
                                int mult, const wf_ray* rays, const int32_t* owner, wf_hit* hits,
                                int64_t cap, hipStream_t s) {
-    if (tu.width == 4) launch_batch_trace_w<CLOSEST, true>(tu.block, t, q, b, mult, rays, owner, hits, cap, s);
-    else launch_batch_trace_w<CLOSEST, false>(tu.block, t, q, b, mult, rays, owner, hits, cap, s);
+    switch (tu.block) {
+        case 128:
+            batch_trace_kernel<CLOSEST, 128><<<grid_for(cap, 128), 128, 0, s>>>(t, q, b, mult, rays, owner, hits);
+            break;
+        case 256:
+            batch_trace_kernel<CLOSEST, 256><<<grid_for(cap, 256), 256, 0, s>>>(t, q, b, mult, rays, owner, hits);
+            break;
+        default:
+            batch_trace_kernel<CLOSEST, 64><<<grid_for(cap, 64), 64, 0, s>>>(t, q, b, mult, rays, owner, hits);
+            break;
+    }
 }
 
 int batch_init_frame(const BatchTables& t, const BatchBuffers& bb, int64_t total_pixels, int frame,

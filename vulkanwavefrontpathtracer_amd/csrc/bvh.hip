@@ -399,197 +399,6 @@ __global__ __launch_bounds__(256) void emit_tris_kernel(const char* __restrict__
     }
 }
 
-// ------------------------------------------------------------------- PLOC
-// Parallel locally-ordered clustering (Meister & Bittner, "Parallel
-// Locally-Ordered Clustering for Bounding Volume Hierarchy Construction",
-// IEEE TVCG 24(3), 2018).  The clusters start as the Morton-sorted
-// references.  Each round every cluster picks the cluster within `radius`
-// positions whose merged box has the smallest surface area (ties broken so
-// that all pairs are ordered totally, hence the smallest pair is always
-// mutual), mutual pairs merge into a new node at the lower position, and the
-// survivors are compacted in order.  Internal nodes take ids from m - 2
-// downwards, so the last merge, the root, is node 0 as in the Karras tree.
-// Leaves are m - 1 + Morton position until ploc_remap_kernel renumbers them
-// in depth-first order, which makes every subtree a contiguous slot range
-// (the [first, last] ranges the node emitters expect).
-constexpr int PLOC_BLOCK = 256;
-constexpr int PLOC_MAX_RADIUS = 32;
-// Deepest PLOC tree kept: the 4-wide every-second-level collapse needs
-// 3 * (depth / 2 + 1) <= TRACE_STACK4 stack entries.
-constexpr int PLOC_MAX_DEPTH = 2 * (TRACE_STACK4 / 3) - 2;
-
-__device__ __forceinline__ float half_area(float4 lo, float4 hi) {
-    const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
-    return dx * dy + dy * dz + dz * dx;
-}
-
-__device__ __forceinline__ float4 min4(float4 a, float4 b) {
-    return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.f);
-}
-__device__ __forceinline__ float4 max4(float4 a, float4 b) {
-    return make_float4(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z), 0.f);
-}
-
-__global__ __launch_bounds__(256) void ploc_init_kernel(const float4* __restrict__ item_box,
-                                                        const uint32_t* __restrict__ sorted_item,
-                                                        int m, int32_t* __restrict__ cid,
-                                                        float4* __restrict__ cbox) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= m) return;
-    const int64_t t = sorted_item[p];
-    cid[p] = m - 1 + p;
-    cbox[2 * (int64_t)p] = item_box[2 * t];
-    cbox[2 * (int64_t)p + 1] = item_box[2 * t + 1];
-}
-
-// Order of the pairs (i, j) and (i, jb) of equal merged area: nearer
-// position first, then a pair whose lower position is even, then the lower
-// position.  Together with the area this orders all pairs totally (so the
-// smallest pair is mutual), and coincident boxes pair up as (2k, 2k + 1)
-// instead of chaining onto position 0.
-__device__ __forceinline__ bool ploc_tie_before(int i, int j, int jb) {
-    const int dj = abs(i - j), db = abs(i - jb);
-    if (dj != db) return dj < db;
-    const int mj = min(i, j), mb = min(i, jb);
-    if ((mj & 1) != (mb & 1)) return (mj & 1) == 0;
-    return mj < mb;
-}
-
-// Nearest neighbour of every cluster within `radius` positions (the block's
-// boxes plus a halo of `radius` on each side staged in LDS).  pair_adjacent
-// is the fallback round for inputs where no pair is mutual (NaN boxes): it
-// pairs positions 2k and 2k + 1.
-__global__ __launch_bounds__(PLOC_BLOCK) void ploc_nn_kernel(const float4* __restrict__ cbox, int n,
-                                                             int radius, int pair_adjacent,
-                                                             int32_t* __restrict__ nn) {
-    __shared__ float4 s_lo[PLOC_BLOCK + 2 * PLOC_MAX_RADIUS];
-    __shared__ float4 s_hi[PLOC_BLOCK + 2 * PLOC_MAX_RADIUS];
-    const int base = blockIdx.x * PLOC_BLOCK;
-    for (int k = threadIdx.x; k < PLOC_BLOCK + 2 * radius; k += PLOC_BLOCK) {
-        const int j = base - radius + k;
-        if (j >= 0 && j < n) {
-            s_lo[k] = cbox[2 * (int64_t)j];
-            s_hi[k] = cbox[2 * (int64_t)j + 1];
-        }
-    }
-    __syncthreads();
-    const int i = base + (int)threadIdx.x;
-    if (i >= n) return;
-    if (pair_adjacent) {
-        const int j = i ^ 1;
-        nn[i] = j < n ? j : i - 1;
-        return;
-    }
-    const float4 lo = s_lo[threadIdx.x + radius], hi = s_hi[threadIdx.x + radius];
-    const int j0 = max(0, i - radius), j1 = min(n - 1, i + radius);
-    float best = 0.f;
-    int bj = -1;
-    for (int j = j0; j <= j1; ++j) {
-        if (j == i) continue;
-        const int k = j - base + radius;
-        const float a = half_area(min4(lo, s_lo[k]), max4(hi, s_hi[k]));
-        if (bj < 0 || a < best || (a == best && ploc_tie_before(i, j, bj))) {
-            best = a;
-            bj = j;
-        }
-    }
-    nn[i] = bj;
-}
-
-// mflag: cluster i merges with its (higher) mutual neighbour; kflag: cluster
-// i survives the round (it is not the higher half of a mutual pair).
-__global__ __launch_bounds__(256) void ploc_flag_kernel(const int32_t* __restrict__ nn, int n,
-                                                        uint32_t* __restrict__ mflag,
-                                                        uint32_t* __restrict__ kflag) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int j = nn[i];
-    const bool mutual = j >= 0 && j < n && nn[j] == i;
-    mflag[i] = (mutual && i < j) ? 1u : 0u;
-    kflag[i] = (mutual && j < i) ? 0u : 1u;
-}
-
-__global__ __launch_bounds__(256) void ploc_merge_kernel(
-    const int32_t* __restrict__ nn, int n, int m, int next_id, const uint32_t* __restrict__ mflag,
-    const uint32_t* __restrict__ mrank, const uint32_t* __restrict__ kflag,
-    const uint32_t* __restrict__ krank, const int32_t* __restrict__ cid,
-    const float4* __restrict__ cbox, int32_t* __restrict__ cid_out, float4* __restrict__ cbox_out,
-    int2* __restrict__ child, float4* __restrict__ box, int32_t* __restrict__ parent,
-    uint32_t* __restrict__ nsize) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !kflag[i]) return;
-    const int64_t p = krank[i];
-    float4 lo = cbox[2 * (int64_t)i], hi = cbox[2 * (int64_t)i + 1];
-    int id = cid[i];
-    if (mflag[i]) {
-        const int j = nn[i];
-        const int a = id, b = cid[j];
-        id = next_id - (int)mrank[i];
-        lo = min4(lo, cbox[2 * (int64_t)j]);
-        hi = max4(hi, cbox[2 * (int64_t)j + 1]);
-        child[id] = make_int2(a, b);
-        box[2 * (int64_t)id] = lo;
-        box[2 * (int64_t)id + 1] = hi;
-        parent[a] = id;
-        parent[b] = id;
-        nsize[id] = (a >= m - 1 ? 1u : nsize[a]) + (b >= m - 1 ? 1u : nsize[b]);
-    }
-    cid_out[p] = id;
-    cbox_out[2 * p] = lo;
-    cbox_out[2 * p + 1] = hi;
-}
-
-// Depth-first position of every node: the leaves of the left siblings met on
-// the way up to the root.  Leaves (x >= m - 1) get their slot and depth,
-// internal nodes their inclusive slot range.
-__global__ __launch_bounds__(256) void ploc_slot_kernel(int m, const int2* __restrict__ child,
-                                                        const int32_t* __restrict__ parent,
-                                                        const uint32_t* __restrict__ nsize,
-                                                        uint32_t* __restrict__ slot,
-                                                        int2* __restrict__ range,
-                                                        uint32_t* __restrict__ max_depth) {
-    const int x0 = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x0 >= 2 * m - 1) return;
-    uint32_t off = 0, depth = 0;
-    for (int x = x0, q; (q = parent[x]) >= 0; x = q) {
-        const int2 c = child[q];
-        if (c.y == x) off += c.x >= m - 1 ? 1u : nsize[c.x];
-        ++depth;
-    }
-    if (x0 >= m - 1) {
-        slot[x0 - (m - 1)] = off;
-        atomicMax(max_depth, depth);
-    } else {
-        range[x0] = make_int2((int)off, (int)(off + nsize[x0]) - 1);
-    }
-}
-
-// Leaf p (Morton position) moves to slot[p]: child links, parents, leaf
-// boxes and the slot -> reference table follow.
-__global__ __launch_bounds__(256) void ploc_remap_kernel(int m, const uint32_t* __restrict__ slot,
-                                                         int2* __restrict__ child,
-                                                         const int32_t* __restrict__ parent,
-                                                         int32_t* __restrict__ parent_out,
-                                                         const float4* __restrict__ item_box,
-                                                         const uint32_t* __restrict__ sorted_item,
-                                                         uint32_t* __restrict__ slot_item,
-                                                         float4* __restrict__ box) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    const int64_t s = slot[i], t = sorted_item[i];
-    slot_item[s] = (uint32_t)t;
-    box[2 * (m - 1 + s)] = item_box[2 * t];
-    box[2 * (m - 1 + s) + 1] = item_box[2 * t + 1];
-    parent_out[m - 1 + s] = parent[m - 1 + i];
-    if (i < m - 1) {
-        int2 c = child[i];
-        if (c.x >= m - 1) c.x = m - 1 + (int)slot[c.x - (m - 1)];
-        if (c.y >= m - 1) c.y = m - 1 + (int)slot[c.y - (m - 1)];
-        child[i] = c;
-        parent_out[i] = parent[i];
-    }
-}
-
 // ------------------------------------------------------- 4-wide hierarchy
 // Every interior node of the 2-wide hierarchy at even depth becomes a 4-wide
 // node whose children are its grandchildren (a leaf child is kept as it is).
@@ -889,88 +698,7 @@ static void free_scene(wf_scene_s* s) {
     if (s->geom_first) (void)hipFree(s->geom_first);
     if (s->albedo) (void)hipFree(s->albedo);
     if (s->emission) (void)hipFree(s->emission);
-    if (s->tailq) (void)hipFree(s->tailq);
-    if (s->tailq_ctr) (void)hipFree(s->tailq_ctr);
     delete s;
-}
-
-// PLOC rounds (see ploc_nn_kernel) over the m >= 2 Morton-sorted references
-// `sorted_item`; writes the binary hierarchy in the Karras layout (child,
-// range, parent over internal 0..m-2 and leaves m-1+slot, node boxes) and
-// the slot -> reference table.  One host round trip per round for the
-// surviving count.
-static int build_ploc(int64_t m64, int radius, const float4* refs, const uint32_t* sorted_item,
-                      int2* child, int2* range, int32_t* parent_out, float4* box,
-                      uint32_t* slot_item, uint32_t* max_depth, hipStream_t s) {
-    const int m = (int)m64;
-    DevBuf cid[2], cbox[2], nn, mflag, kflag, mrank, krank, state, nsize, tot, slot, parent;
-    const int64_t tiles = (m64 + 255) / 256;
-    for (int k = 0; k < 2; ++k) {
-        WF_TRY(dev_alloc(cid[k], sizeof(int32_t) * m64));
-        WF_TRY(dev_alloc(cbox[k], sizeof(float4) * 2 * m64));
-    }
-    WF_TRY(dev_alloc(nn, sizeof(int32_t) * m64));
-    WF_TRY(dev_alloc(mflag, sizeof(uint32_t) * m64));
-    WF_TRY(dev_alloc(kflag, sizeof(uint32_t) * m64));
-    WF_TRY(dev_alloc(mrank, sizeof(uint32_t) * m64));
-    WF_TRY(dev_alloc(krank, sizeof(uint32_t) * m64));
-    WF_TRY(dev_alloc(state, sizeof(uint32_t) * 2 * (1 + tiles)));
-    WF_TRY(dev_alloc(nsize, sizeof(uint32_t) * (m64 - 1)));
-    WF_TRY(dev_alloc(tot, sizeof(uint32_t) * 2));
-    WF_TRY(dev_alloc(slot, sizeof(uint32_t) * m64));
-    WF_TRY(dev_alloc(parent, sizeof(int32_t) * (2 * m64 - 1)));
-    WF_TRY(fill_u32((uint32_t*)parent.p, 2 * m64 - 1, 0xffffffffu, s));
-    ploc_init_kernel<<<grid_for(m64, 256), 256, 0, s>>>(refs, sorted_item, m, (int32_t*)cid[0].p,
-                                                        (float4*)cbox[0].p);
-    WF_TRY(launch_check("ploc_init_kernel"));
-    radius = std::min(std::max(radius, 1), PLOC_MAX_RADIUS);
-    int count = m, next_id = m - 2, cur = 0;
-    uint32_t* st = (uint32_t*)state.p;
-    uint32_t* tt = (uint32_t*)tot.p;
-    while (count > 1) {
-        uint32_t ht[2] = {0, 0};
-        const int64_t nt = (count + 255) / 256;
-        for (int pair_adjacent = 0; pair_adjacent < 2; ++pair_adjacent) {
-            ploc_nn_kernel<<<grid_for(count, PLOC_BLOCK), PLOC_BLOCK, 0, s>>>(
-                (const float4*)cbox[cur].p, count, radius, pair_adjacent, (int32_t*)nn.p);
-            WF_TRY(launch_check("ploc_nn_kernel"));
-            ploc_flag_kernel<<<grid_for(count, 256), 256, 0, s>>>(
-                (const int32_t*)nn.p, count, (uint32_t*)mflag.p, (uint32_t*)kflag.p);
-            WF_TRY(launch_check("ploc_flag_kernel"));
-            WF_TRY(fill_u32(st, 2 * (1 + nt), 0u, s));
-            flag_scan_kernel<<<(unsigned)nt, 256, 0, s>>>((const uint32_t*)mflag.p, count,
-                                                          (uint32_t*)mrank.p, st, st + 1, tt);
-            flag_scan_kernel<<<(unsigned)nt, 256, 0, s>>>((const uint32_t*)kflag.p, count,
-                                                          (uint32_t*)krank.p, st + 1 + nt,
-                                                          st + 2 + nt, tt + 1);
-            WF_TRY(launch_check("flag_scan_kernel"));
-            WF_HIP(hipMemcpyAsync(ht, tt, sizeof(ht), hipMemcpyDeviceToHost, s));
-            WF_HIP(hipStreamSynchronize(s));
-            if (ht[0] > 0) break;
-        }
-        if (ht[0] == 0 || (int64_t)ht[1] != (int64_t)count - ht[0])
-            return fail(WF_ERR_HIP, "wf_scene_create: PLOC round made no progress");
-        ploc_merge_kernel<<<grid_for(count, 256), 256, 0, s>>>(
-            (const int32_t*)nn.p, count, m, next_id, (const uint32_t*)mflag.p,
-            (const uint32_t*)mrank.p, (const uint32_t*)kflag.p, (const uint32_t*)krank.p,
-            (const int32_t*)cid[cur].p, (const float4*)cbox[cur].p, (int32_t*)cid[cur ^ 1].p,
-            (float4*)cbox[cur ^ 1].p, child, box, (int32_t*)parent.p, (uint32_t*)nsize.p);
-        WF_TRY(launch_check("ploc_merge_kernel"));
-        next_id -= (int)ht[0];
-        count = (int)ht[1];
-        cur ^= 1;
-    }
-    if (next_id != -1) return fail(WF_ERR_HIP, "wf_scene_create: PLOC node count mismatch");
-    ploc_slot_kernel<<<grid_for(2 * m64 - 1, 256), 256, 0, s>>>(
-        m, child, (const int32_t*)parent.p, (const uint32_t*)nsize.p, (uint32_t*)slot.p, range,
-        max_depth);
-    WF_TRY(launch_check("ploc_slot_kernel"));
-    ploc_remap_kernel<<<grid_for(m64, 256), 256, 0, s>>>(m, (const uint32_t*)slot.p, child,
-                                                         (const int32_t*)parent.p, parent_out,
-                                                         refs, sorted_item, slot_item, box);
-    WF_TRY(launch_check("ploc_remap_kernel"));
-    WF_HIP(hipStreamSynchronize(s));  // the temporaries leave scope
-    return WF_OK;
 }
 
 }  // namespace wfpt
@@ -1184,43 +912,14 @@ extern "C" int wf_scene_create(const void* vertices, int64_t num_vertices, int v
                                                               (uint32_t*)vals.p);
         WF_TRY(launch_check("centroid_keys_kernel"));
         WF_TRY(wf_sort_pairs(keys.p, (uint32_t*)vals.p, m, 8, 63, 8, scratch.p, sb, stream, nullptr));
-        bool karras = tu.bvh_builder != 1;
-        if (!karras) {
-            // PLOC reorders the slots depth first: vals becomes slot -> reference
-            DevBuf morton_item;
-            WF_TRY(dev_alloc(morton_item, sizeof(uint32_t) * m));
-            WF_HIP(hipMemcpyAsync(morton_item.p, vals.p, sizeof(uint32_t) * m,
-                                  hipMemcpyDeviceToDevice, s));
-            WF_TRY(build_ploc(m, tu.bvh_ploc_radius, refs, (const uint32_t*)morton_item.p,
-                              (int2*)child.p, (int2*)range.p, (int32_t*)parent.p, (float4*)box.p,
-                              (uint32_t*)vals.p, (uint32_t*)misc.p + 1, s));
-            // Clustering has no depth bound (a large box can absorb its
-            // neighbours one per round); a tree deeper than the traversal
-            // stacks allow is rebuilt as the Karras tree, whose depth the
-            // key bits bound.
-            uint32_t depth = 0;
-            WF_HIP(hipMemcpyAsync(&depth, (uint32_t*)misc.p + 1, sizeof(depth),
-                                  hipMemcpyDeviceToHost, s));
-            WF_HIP(hipStreamSynchronize(s));
-            if (depth > (uint32_t)PLOC_MAX_DEPTH) {
-                WF_HIP(hipMemcpyAsync(vals.p, morton_item.p, sizeof(uint32_t) * m,
-                                      hipMemcpyDeviceToDevice, s));
-                WF_TRY(fill_u32((uint32_t*)misc.p + 1, 1, 0u, s));
-                karras = true;
-            }
-            WF_HIP(hipStreamSynchronize(s));  // morton_item leaves scope
-        }
-        if (karras) {
-            karras_kernel<<<grid_for(m - 1, 256), 256, 0, s>>>((const uint64_t*)keys.p, (int)m,
-                                                               (int2*)child.p, (int2*)range.p,
-                                                               (int32_t*)parent.p);
-            WF_TRY(launch_check("karras_kernel"));
-            refit_kernel<<<grid_for(m, 256), 256, 0, s>>>(
-                refs, (const uint32_t*)vals.p, (int)m, (const int2*)child.p,
-                (const int32_t*)parent.p, (float4*)box.p, (uint32_t*)flags.p,
-                (uint32_t*)misc.p + 1);
-            WF_TRY(launch_check("refit_kernel"));
-        }
+        karras_kernel<<<grid_for(m - 1, 256), 256, 0, s>>>((const uint64_t*)keys.p, (int)m,
+                                                           (int2*)child.p, (int2*)range.p,
+                                                           (int32_t*)parent.p);
+        WF_TRY(launch_check("karras_kernel"));
+        refit_kernel<<<grid_for(m, 256), 256, 0, s>>>(
+            refs, (const uint32_t*)vals.p, (int)m, (const int2*)child.p, (const int32_t*)parent.p,
+            (float4*)box.p, (uint32_t*)flags.p, (uint32_t*)misc.p + 1);
+        WF_TRY(launch_check("refit_kernel"));
         emit_nodes_kernel<<<grid_for(m - 1, 256), 256, 0, s>>>(
             (int)m, tu.bvh_leaf_max, (const int2*)child.p, (const int2*)range.p,
             (const float4*)box.p, b, sc->nodes, (uint32_t*)misc.p + 2);

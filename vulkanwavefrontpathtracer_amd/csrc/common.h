@@ -166,14 +166,8 @@ struct Tuning {
     int bvh_split_depth = 6;
     int bvh_split_denom = 64;   // split triangles whose box exceeds diagonal / this
     int bvh_leaf_max = 1;
-    int bvh_builder = 0;        // 0: Karras LBVH over Morton codes; 1: PLOC clustering
-    int bvh_ploc_radius = 16;   // PLOC nearest-neighbour search radius (positions)
     int bvh_collapse = 1;       // 4-wide nodes: 0 every second level; 1 greedy largest-area
-    int trace_lds_stack = 1;
     int trace_block = 64;
-    int trace_persistent = 0;   // persistent waves that refill idle lanes with new rays
-    int trace_refill = 24;      // idle lanes of a wave that trigger a refill
-    int trace_width = 4;        // traverse the 2-wide or the 4-wide hierarchy
     int sort_onesweep_tiles = 64;  // sorts of at most this many 4096-element tiles use onesweep
     int sort_rank_max = 28000;  // sorts of at most this many pairs x passes / 8 count each rank directly (0 off)
     int trace_group64 = 5000;   // wavefronts of at most this many rays: 64 lanes per ray
@@ -181,12 +175,6 @@ struct Tuning {
     int trace_group4 = 64000;   // ... 4 lanes per ray
     int batch_key_layout = 0;   // batched queue key below the scene prefix: 0 concatenated, 1 5-D Morton
     int trace_fuse_reorder = 1;  // wf_reorder_trace: one fused kernel where available
-    int trace_leaf_gate = 16;   // trace.loop 3: waiting lanes that trigger the triangle test
-    int trace_loop = 1;         // 4-wide loop: 0 branch then load, 1 one load per step, 2 postponed leaves
-    int trace_compact = 0;      // waves with fewer active lanes hand their rays to the tail pass (0 off)
-    int trace_compact_min = 262144;  // smallest wavefront traced with hand-over
-    int trace_tail_refill = 16;  // tail pass: idle lanes that trigger taking more handed-over rays
-    int trace_tail_cap = 0;     // tail queue records at a scene's first use (0: max(2^20, n/2))
     int trace_xcd = 0;          // loop 1 walk: runs of this many workgroups per XCD (0 off)
 };
 Tuning& tuning();

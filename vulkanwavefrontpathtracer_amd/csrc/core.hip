@@ -83,20 +83,8 @@ extern "C" int wf_set_tuning(const char* key, int value) {
     else if (k == "bvh.split_depth" && value >= 0 && value <= 8) t.bvh_split_depth = value;
     else if (k == "bvh.split_denom" && value >= 1 && value <= 4096) t.bvh_split_denom = value;
     else if (k == "bvh.leaf_max" && value >= 1 && value <= 8) t.bvh_leaf_max = value;
-    else if (k == "bvh.builder" && (value == 0 || value == 1)) t.bvh_builder = value;
-    else if (k == "bvh.ploc_radius" && value >= 1 && value <= 32) t.bvh_ploc_radius = value;
     else if (k == "bvh.collapse" && (value == 0 || value == 1)) t.bvh_collapse = value;
-    else if (k == "trace.lds_stack" && (value == 0 || value == 1)) t.trace_lds_stack = value;
     else if (k == "trace.block" && (value == 64 || value == 128 || value == 256)) t.trace_block = value;
-    else if (k == "trace.persistent" && (value == 0 || value == 1)) t.trace_persistent = value;
-    else if (k == "trace.refill" && value >= 1 && value <= 64) t.trace_refill = value;
-    else if (k == "trace.width" && (value == 2 || value == 4)) t.trace_width = value;
-    else if (k == "trace.loop" && value >= 0 && value <= 3) t.trace_loop = value;
-    else if (k == "trace.leaf_gate" && value >= 1 && value <= 64) t.trace_leaf_gate = value;
-    else if (k == "trace.compact" && value >= 0 && value <= 64) t.trace_compact = value;
-    else if (k == "trace.compact_min" && value >= 0) t.trace_compact_min = value;
-    else if (k == "trace.tail_refill" && value >= 1 && value <= 64) t.trace_tail_refill = value;
-    else if (k == "trace.tail_cap" && value >= 0) t.trace_tail_cap = value;
     else if (k == "trace.xcd" && value >= 0 && value <= 65536) t.trace_xcd = value;
     else if (k == "trace.fuse_reorder" && (value == 0 || value == 1)) t.trace_fuse_reorder = value;
     else if (k == "batch.key_layout" && (value == 0 || value == 1)) t.batch_key_layout = value;
@@ -118,20 +106,8 @@ extern "C" int wf_get_tuning(const char* key, int* value) {
     else if (k == "bvh.split_depth") *value = t.bvh_split_depth;
     else if (k == "bvh.split_denom") *value = t.bvh_split_denom;
     else if (k == "bvh.leaf_max") *value = t.bvh_leaf_max;
-    else if (k == "bvh.builder") *value = t.bvh_builder;
-    else if (k == "bvh.ploc_radius") *value = t.bvh_ploc_radius;
     else if (k == "bvh.collapse") *value = t.bvh_collapse;
-    else if (k == "trace.lds_stack") *value = t.trace_lds_stack;
     else if (k == "trace.block") *value = t.trace_block;
-    else if (k == "trace.persistent") *value = t.trace_persistent;
-    else if (k == "trace.refill") *value = t.trace_refill;
-    else if (k == "trace.width") *value = t.trace_width;
-    else if (k == "trace.loop") *value = t.trace_loop;
-    else if (k == "trace.leaf_gate") *value = t.trace_leaf_gate;
-    else if (k == "trace.compact") *value = t.trace_compact;
-    else if (k == "trace.compact_min") *value = t.trace_compact_min;
-    else if (k == "trace.tail_refill") *value = t.trace_tail_refill;
-    else if (k == "trace.tail_cap") *value = t.trace_tail_cap;
     else if (k == "trace.xcd") *value = t.trace_xcd;
     else if (k == "trace.fuse_reorder") *value = t.trace_fuse_reorder;
     else if (k == "batch.key_layout") *value = t.batch_key_layout;

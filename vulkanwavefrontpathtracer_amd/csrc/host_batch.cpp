@@ -127,12 +127,11 @@ static int capture_pass(wf_batch_s* b) {
 
 static bool same_tuning(const BatchTuning& a, const BatchTuning& c) {
     return a.digit_bits == c.digit_bits && a.block == c.block &&
-           a.reorder_period == c.reorder_period && a.width == c.width;
+           a.reorder_period == c.reorder_period;
 }
 
 static int run_pass(wf_batch_s* b, int frame, int pass) {
     const BatchBuffers bb = buffers(b);
-    b->tuning.width = tuning().trace_width;
     WF_TRY(batch_set_pass(bb, frame, pass, b->pass_count[pass], b->depth, b->stream));
     if (b->use_graph) {
         if (!b->graph || !same_tuning(b->graph_tuning, b->tuning) ||

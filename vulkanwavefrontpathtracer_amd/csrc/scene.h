@@ -46,16 +46,11 @@ struct wf_scene_s {
     float* albedo = nullptr;
     float* emission = nullptr;
     int device = 0;
-    // Tail queue of the compacting trace (rays handed over by waves with few
-    // lanes left; allocated on first use, stream-ordered with the traces).
-    int4* tailq = nullptr;
-    uint32_t* tailq_ctr = nullptr;
-    int64_t tailq_cap = 0;
 };
 
 namespace wfpt {
 
-constexpr int TRACE_STACK = 96;        // traversal stack entries per ray (2-wide)
+constexpr int TRACE_STACK = 96;        // stack bound of the binary hierarchy's depth checks
 constexpr int TRACE_STACK4 = 96;       // traversal stack entries per ray (4-wide)
 
 __device__ __forceinline__ int leaf_ref(int first, int count) { return ~((first << 3) | (count - 1)); }

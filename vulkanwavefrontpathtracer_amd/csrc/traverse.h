@@ -114,69 +114,6 @@ __device__ __forceinline__ bool trav_tri(TravState<LDS, STACK>& st, int slot, fl
     return !CLOSEST;
 }
 
-// Leaf `leaf` (reference of up to 8 triangle slots); returns true when an
-// any-hit query is done.
-template <bool CLOSEST, bool LDS, int STACK, bool STATS, bool RECORD = false>
-__device__ __forceinline__ bool trav_leaf(TravState<LDS, STACK>& st, int leaf,
-                                          const float4* __restrict__ tris, const WtRay& wr) {
-    const int first = leaf_first(leaf), cnt = leaf_count(leaf);
-    for (int k = 0; k < cnt; ++k) {
-        const float4* tp = tris + 3 * (int64_t)(first + k);
-        if (trav_tri<CLOSEST, LDS, STACK, STATS, RECORD>(st, first + k, tp[0], tp[1], tp[2], wr))
-            return true;
-    }
-    return false;
-}
-
-// One traversal step of the 2-wide hierarchy: an interior node (both child
-// boxes) or a whole leaf.  Returns true when the ray is finished (stack
-// empty, or any-hit found).
-template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, int STACK, bool RECORD = false>
-__device__ __forceinline__ bool trav_step(TravState<LDS, STACK>& st,
-                                          int (&spill)[TravState<LDS, STACK>::SPILL],
-                                          const float4* __restrict__ nodes,
-                                          const float4* __restrict__ tris, const WtRay& wr,
-                                          int* s_stack) {
-    const float ix = st.ix, iy = st.iy, iz = st.iz, oix = st.oix, oiy = st.oiy, oiz = st.oiz;
-    const float tmin = st.tmin;
-    if (st.node >= 0) {
-        if (STATS) ++st.nodes_visited;
-        const float4* nd = nodes + 4 * (int64_t)st.node;
-        const float4 a = nd[0], b = nd[1], c = nd[2], d = nd[3];
-        const float best_t = st.best_t;
-        // child 0: lo (a.x a.y a.z) hi (a.w b.x b.y); child 1: lo (b.z b.w c.x) hi (c.y c.z c.w)
-        float t0x = fmaf(a.x, ix, -oix), t1x = fmaf(a.w, ix, -oix);
-        float t0y = fmaf(a.y, iy, -oiy), t1y = fmaf(b.x, iy, -oiy);
-        float t0z = fmaf(a.z, iz, -oiz), t1z = fmaf(b.y, iz, -oiz);
-        float n0 = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-        float f0 = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), best_t));
-        float u0x = fmaf(b.z, ix, -oix), u1x = fmaf(c.y, ix, -oix);
-        float u0y = fmaf(b.w, iy, -oiy), u1y = fmaf(c.z, iy, -oiy);
-        float u0z = fmaf(c.x, iz, -oiz), u1z = fmaf(c.w, iz, -oiz);
-        float n1 = fmaxf(fmaxf(fminf(u0x, u1x), fminf(u0y, u1y)), fmaxf(fminf(u0z, u1z), tmin));
-        float f1 = fminf(fminf(fmaxf(u0x, u1x), fmaxf(u0y, u1y)), fminf(fmaxf(u0z, u1z), best_t));
-        const bool h0 = n0 <= f0, h1 = n1 <= f1;
-        const int r0 = __float_as_int(d.x), r1 = __float_as_int(d.y);
-        if (h0 && h1) {
-            const bool second_first = n1 < n0;
-            st.node = second_first ? r1 : r0;
-            stack_push<LDS, BLOCK, STACK>(st, spill, s_stack, second_first ? r0 : r1);
-        } else if (h0) {
-            st.node = r0;
-        } else if (h1) {
-            st.node = r1;
-        } else {
-            if (st.sp == 0) return true;
-            st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
-        }
-        return false;
-    }
-    if (trav_leaf<CLOSEST, LDS, STACK, STATS, RECORD>(st, st.node, tris, wr)) return true;
-    if (st.sp == 0) return true;
-    st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
-    return false;
-}
-
 // Compare-exchange of (distance, reference) pairs, written as selects so the
 // sorting network stays branch-free (one compare and four selects).
 __device__ __forceinline__ void cswap(float& ta, int& ra, float& tb, int& rb) {
@@ -283,14 +220,6 @@ __device__ __forceinline__ bool trav_node4_data(TravState<LDS, STACK>& st,
     return true;
 }
 
-template <int BLOCK, bool LDS, bool STATS, int STACK>
-__device__ __forceinline__ bool trav_node4(TravState<LDS, STACK>& st,
-                                           int (&spill)[TravState<LDS, STACK>::SPILL],
-                                           const float4* __restrict__ nodes4, int* s_stack) {
-    const float4* nd = nodes4 + 4 * (int64_t)st.node;
-    return trav_node4_data<BLOCK, LDS, STATS>(st, spill, nd[0], nd[1], nd[2], nd[3], s_stack);
-}
-
 // One traversal step of the 4-wide hierarchy that issues a single load per
 // lane whatever the lane visits: the node's 64 bytes, or the first triangle
 // of a leaf (48 bytes), so a wave whose lanes are split between nodes and
@@ -330,203 +259,22 @@ __device__ __forceinline__ bool trav_step4_fused(TravState<LDS, STACK>& st,
     return false;
 }
 
-// One traversal step of the 4-wide hierarchy: an interior node or a whole
-// leaf.  Returns true when the ray is finished (stack empty, or any-hit
-// found).
-template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, int STACK, bool RECORD = false>
-__device__ __forceinline__ bool trav_step4(TravState<LDS, STACK>& st,
-                                           int (&spill)[TravState<LDS, STACK>::SPILL],
-                                           const float4* __restrict__ nodes4,
-                                           const float4* __restrict__ tris, const WtRay& wr,
-                                           int* s_stack) {
-    if (st.node >= 0) return !trav_node4<BLOCK, LDS, STATS>(st, spill, nodes4, s_stack);
-    if (trav_leaf<CLOSEST, LDS, STACK, STATS, RECORD>(st, st.node, tris, wr)) return true;
-    if (st.sp == 0) return true;
-    st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
-    return false;
-}
 
-// 4-wide traversal with postponed leaves (Aila & Laine, "Understanding the
-// Efficiency of Ray Traversal on GPUs", HPG 2009, speculative traversal): a
-// lane that reaches a leaf parks it and keeps descending interior nodes; the
-// wave tests triangles only once every lane still descending has a parked
-// leaf, so the triangle test is not issued for one or two lanes on most
-// steps.  Order-independent result: the closest hit (ties to the smaller
-// original triangle index) or any hit.
-template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, int STACK>
-__device__ __forceinline__ void trav4_postponed(TravState<LDS, STACK>& st,
-                                                int (&spill)[TravState<LDS, STACK>::SPILL],
-                                                const float4* __restrict__ nodes4,
-                                                const float4* __restrict__ tris, const WtRay& wr,
-                                                int* s_stack, bool live) {
-    bool have = live;  // st.node holds a node (interior or leaf) still to visit
-    int leaf = 0;      // parked leaf reference (< 0), 0 for none
-    while (have || leaf != 0) {
-        while (have && (st.node >= 0 || leaf == 0)) {
-            if (st.node < 0) {
-                leaf = st.node;
-                have = st.sp > 0;
-                if (have) st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
-            } else {
-                have = trav_node4<BLOCK, LDS, STATS>(st, spill, nodes4, s_stack);
-            }
-            if (__all(!have || leaf != 0)) break;
-        }
-        while (leaf != 0) {
-            if (trav_leaf<CLOSEST, LDS, STACK, STATS>(st, leaf, tris, wr)) {
-                have = false;
-                leaf = 0;
-                break;
-            }
-            if (have && st.node < 0) {
-                leaf = st.node;
-                have = st.sp > 0;
-                if (have) st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
-            } else {
-                leaf = 0;
-            }
-        }
-    }
-}
-
-// 4-wide traversal with gated leaves: a lane whose next entry is a leaf
-// waits (it neither tests nor moves) while the wave steps its interior
-// nodes, and the wave runs the triangle test only once `gate` or more lanes
-// wait, or when no lane has an interior node to step.  With one-triangle
-// leaves some lane sits at a leaf on most steps, so without the gate the
-// triangle test is issued nearly every step for one or two lanes; a waiting
-// lane keeps its place, so no traversal is done before its closest hit is
-// known (unlike postponed leaves).  gate = 1 is the plain loop.
-template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, int STACK>
-__device__ __forceinline__ void trav4_gated(TravState<LDS, STACK>& st,
-                                            int (&spill)[TravState<LDS, STACK>::SPILL],
-                                            const float4* __restrict__ nodes4,
-                                            const float4* __restrict__ tris, const WtRay& wr,
-                                            int* s_stack, bool live, int gate) {
-    bool active = live;
-    while (true) {
-        const bool inner = active && st.node >= 0;
-        const uint64_t at_inner = ballot64(inner);
-        const uint64_t at_leaf = ballot64(active && st.node < 0);
-        if ((at_inner | at_leaf) == 0) break;
-        if (inner) active = trav_node4<BLOCK, LDS, STATS>(st, spill, nodes4, s_stack);
-        if (at_leaf != 0 && (__popcll(at_leaf) >= gate || at_inner == 0)) {
-            if (active && st.node < 0 && ((at_leaf >> lane_id()) & 1ull)) {
-                if (trav_leaf<CLOSEST, LDS, STACK, STATS>(st, st.node, tris, wr)) {
-                    active = false;
-                } else if (st.sp == 0) {
-                    active = false;
-                } else {
-                    st.node = stack_pop<LDS, BLOCK, STACK>(st, spill, s_stack);
-                }
-            }
-        }
-    }
-}
-
-// Loop structures of the 4-wide traversal (the trace.loop knob).
-enum TraceLoop { LOOP_BRANCH = 0, LOOP_FUSED = 1, LOOP_POSTPONED = 2, LOOP_GATED = 3 };
-constexpr int TRACE_LOOP_DEFAULT = LOOP_FUSED;
-
-// s_stack: this block's LDS stack (LDS_DEPTH * BLOCK ints) when LDS is set.
-// WIDE selects the 4-wide quantized hierarchy (nodes = scene nodes4) and
-// LOOP its loop structure.
-template <bool CLOSEST, int BLOCK, bool LDS, bool STATS, bool WIDE = false,
-          int LOOP = TRACE_LOOP_DEFAULT>
-__device__ __forceinline__ TraceOut traverse(const float4* __restrict__ nodes,
+// Closest-hit or any-hit walk of one ray through the 4-wide quantized
+// hierarchy (nodes4) with the single-load step; s_stack is this block's LDS
+// stack (LDS_DEPTH * BLOCK ints).
+template <bool CLOSEST, int BLOCK, bool STATS>
+__device__ __forceinline__ TraceOut traverse(const float4* __restrict__ nodes4,
                                              const float4* __restrict__ tris, float4 ro, float4 rd,
-                                             const WtRay& wr, bool live, int* s_stack,
-                                             int leaf_gate = 1) {
-    constexpr int STACK = WIDE ? TRACE_STACK4 : TRACE_STACK;
-    TravState<LDS, STACK> st;
-    int spill[TravState<LDS, STACK>::SPILL];
+                                             const WtRay& wr, bool live, int* s_stack) {
+    TravState<true, TRACE_STACK4> st;
+    int spill[TravState<true, TRACE_STACK4>::SPILL];
     trav_begin(st, ro, rd);
-    if (WIDE && LOOP == LOOP_GATED) {
-        trav4_gated<CLOSEST, BLOCK, LDS, STATS>(st, spill, nodes, tris, wr, s_stack, live, leaf_gate);
-    } else if (WIDE && LOOP == LOOP_POSTPONED) {
-        trav4_postponed<CLOSEST, BLOCK, LDS, STATS>(st, spill, nodes, tris, wr, s_stack, live);
-    } else if (live) {
-        if (WIDE && LOOP == LOOP_FUSED) {
-            while (!trav_step4_fused<CLOSEST, BLOCK, LDS, STATS>(st, spill, nodes, tris, wr, s_stack)) {
-            }
-        } else if (WIDE) {
-            while (!trav_step4<CLOSEST, BLOCK, LDS, STATS>(st, spill, nodes, tris, wr, s_stack)) {
-            }
-        } else {
-            while (!trav_step<CLOSEST, BLOCK, LDS, STATS>(st, spill, nodes, tris, wr, s_stack)) {
-            }
+    if (live) {
+        while (!trav_step4_fused<CLOSEST, BLOCK, true, STATS>(st, spill, nodes4, tris, wr, s_stack)) {
         }
     }
     return TraceOut{st.best_slot, st.best_t, st.nodes_visited, st.tris_tested};
-}
-
-// Tail hand-over.  A wave holds its SIMD slot until its slowest ray is done,
-// and on sorted path-ray wavefronts the last ~15 of 64 rays take about a
-// quarter of the wave's steps.  The compacting loop below therefore stops a
-// wave once fewer than `compact` of its lanes are still walking, writes each
-// walking ray's whole traversal state to the tail queue, and lets a tail
-// pass (trace_tail_kernel) finish those rays in full waves.  The state is
-// the ray slot, the current node, the stack (at most LDS_DEPTH entries, all
-// in LDS; a wave with a deeper stack keeps walking) and the closest hit so
-// far, so the tail pass continues exactly the same step sequence and the
-// results do not change.  Record: 2 + LDS_DEPTH / 4 int4 = {slot, node, sp,
-// best slot}, {best original index, best t, 0, 0}, then the LDS_DEPTH stack
-// entries.
-static_assert(LDS_DEPTH % 4 == 0, "tail record holds the LDS stack as int4");
-constexpr int TAIL_INT4 = 2 + LDS_DEPTH / 4;
-
-struct TailQueue {
-    int4* q = nullptr;          // cap records
-    uint32_t* ctr = nullptr;    // [0]: records reserved (may exceed cap)
-    int64_t cap = 0;
-    int compact = 0;            // hand over below this many walking lanes
-};
-
-// Closest-hit or any-hit walk of the 4-wide hierarchy with the single-load
-// loop, handing the wave's walking rays over to `tq` once fewer than
-// tq.compact lanes walk (one attempt per wave).  Returns true when this
-// lane's ray was handed over (its record is then written by the tail pass).
-template <bool CLOSEST, int BLOCK, int STACK>
-__device__ __forceinline__ bool traverse4_handover(TravState<true, STACK>& st,
-                                                   int (&spill)[TravState<true, STACK>::SPILL],
-                                                   const float4* __restrict__ nodes4,
-                                                   const float4* __restrict__ tris,
-                                                   const WtRay& wr, bool live, int* s_stack,
-                                                   int64_t slot, const TailQueue& tq) {
-    bool active = live;
-    bool tried = false;
-    while (true) {
-        const uint64_t act = ballot64(active);
-        if (act == 0) return false;
-        const int walking = __popcll(act);
-        if (!tried && walking < tq.compact) {
-            tried = true;
-            if (__all(!active || st.sp <= LDS_DEPTH)) {
-                const int leader = __ffsll((unsigned long long)act) - 1;
-                uint32_t base = 0;
-                if ((int)lane_id() == leader) base = atomicAdd(tq.ctr, (uint32_t)walking);
-                base = __shfl(base, leader, 64);
-                const int64_t rec = (int64_t)base + popc_below(act);
-                if ((int64_t)base + walking <= tq.cap) {
-                    if (!active) return false;
-                    const lds_int* col = (const lds_int*)s_stack + threadIdx.x;
-                    int4* o = tq.q + rec * TAIL_INT4;
-                    o[0] = make_int4((int)slot, st.node, st.sp, st.best_slot);
-                    o[1] = make_int4(st.best_orig, __float_as_int(st.best_t), 0, 0);
-#pragma unroll
-                    for (int k = 0; k < LDS_DEPTH / 4; ++k)
-                        o[2 + k] = make_int4(col[(4 * k) * BLOCK], col[(4 * k + 1) * BLOCK],
-                                             col[(4 * k + 2) * BLOCK], col[(4 * k + 3) * BLOCK]);
-                    return true;
-                }
-                // The queue is full: void the reserved records below the
-                // capacity and keep walking.
-                if (active && rec < tq.cap) tq.q[rec * TAIL_INT4] = make_int4(-1, 0, 0, 0);
-            }
-        }
-        if (active)
-            active = !trav_step4_fused<CLOSEST, BLOCK, true, false>(st, spill, nodes4, tris, wr, s_stack);
-    }
 }
 
 // Hit record {prim in geometry, geometry, t, packUnorm2x16(bary)} of an
