@@ -2,12 +2,12 @@
 
 Headline workload (one GPU): the large plain scene, a procedural city of
 2,821,186 triangles, 1920x1080 x 4 spp with all samples in flight (8.3M
-primary rays, 5.5M path rays at the first bounce), 8 bounces, 60-bit keys
-(30 origin + 30 octahedral direction bits as a 5-D Morton code) sorted as
-u64 keys with a u32 payload, 8-bit digits, reorder every bounce.  One untimed
-frame of the C++ frame loop captures every pass's path wavefronts in HBM;
-the 4 passes of each bounce are concatenated into one wavefront.  A timed
-step is one pass of the hot path over all captured bounces of that frame:
+primary rays, 5.5M path rays at the first bounce), 8 bounces, the env file's
+ray key (30-bit origin Morton keys sorted as u32 keys with a u32 payload,
+8-bit digits), reorder every bounce.  One untimed frame of the C++ frame loop
+captures every pass's path wavefronts in HBM; the 4 passes of each bounce are
+concatenated into one wavefront.  A timed step is one pass of the hot path
+over all captured bounces of that frame:
     keys -> sort -> reorder + closest-hit trace    (Tracer::traceSort)
 with the reorder gather fused into the trace kernel (wf_reorder_trace).
 value = rays through the step / wall time of the timed steps, the step
@@ -18,11 +18,12 @@ Also reported: per-stage GPU times (hipEvents on the stream the kernels run
 on); the reference's per-bounce sort log (SortLog: rays, key, sort, sorted
 trace and unsorted trace times of each bounce, Renderer.cpp:280-289); the
 sort ablation (the same wavefronts traced unsorted, sorted every 2nd bounce,
-with 32-bit origin keys, with the concatenated 60-bit key); the dominant
-kernel's roofline; the medium interior workload by the same method; the
-reference-style full-frame figures of the C++ frame loop; the batched and
-differentiable workloads; and a CPU baseline (the oracle's port of the same
-stages on a bounded sample of the headline's first bounce).
+with the other key layouts and with 11-bit digits); the dominant launch's
+roofline, with hardware counters read in this run by rocprofv3 child passes;
+the medium interior workload by the same method; the reference-style
+full-frame figures of the C++ frame loop; the batched and differentiable
+workloads; and a CPU baseline (the oracle's port of the same stages, in the
+headline's key mode, over the headline step's wavefronts).
 """
 from __future__ import annotations
 
@@ -39,11 +40,19 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
 KEY_MODES = {"aila32": 0, "origin30": 1, "origin30_oct30": 2, "origin_dir_morton60": 4}
+KEY_DESC = {0: "32-bit Aila keys (7 origin + 4 direction bits per axis), u32 key sort + u32 payload",
+            1: "30-bit origin Morton keys (10 bits per axis), u32 key sort + u32 payload",
+            2: "60-bit keys (30 origin bits then 30 octahedral direction bits), u64 key sort + u32 payload",
+            4: "60-bit 5-D Morton keys (30 origin + 30 octahedral direction bits interleaved), "
+               "u64 key sort + u32 payload"}
 PROFILE_DIR = ROOT / "profiles"
 
 
+_T0 = time.perf_counter()
+
+
 def log(msg):
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+    print(f"[bench +{time.perf_counter() - _T0:.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
 def d2d(dst: int, src: int, nbytes: int) -> None:
@@ -221,11 +230,11 @@ class Pipeline:
         return out
 
 
-def run_workload(dev, args, env_name, ablations=True, overrides=None):
+def run_workload(dev, args, env_name, ablations=True, overrides=None, key_mode=None):
     import torch
 
     r, scene, bounds, waves, index_cap, cfg = capture_waves(dev, env_name, overrides)
-    mode = KEY_MODES[cfg["Renderer"]["keyMode"]]
+    mode = KEY_MODES[key_mode or cfg["Renderer"]["keyMode"]]
     p = Pipeline(dev, scene, bounds, waves, index_cap, mode, args.digit_bits, fused=not args.no_fuse)
     total_rays = sum(x["n"] for x in waves)
     stream = torch.cuda.current_stream()
@@ -301,6 +310,17 @@ def run_workload(dev, args, env_name, ablations=True, overrides=None):
     sorted_ms = sum(stage_ms.values())
     big = max(range(len(waves)), key=lambda i: waves[i]["n"])
     nbig = waves[big]["n"]
+    # traversal work of the largest bounce (per-ray node visits and triangle
+    # tests do not depend on the order the rays are traced in)
+    nv = torch.zeros(nbig, dtype=torch.int32, device=dev)
+    nt = torch.zeros_like(nv)
+    scratch_hits = torch.empty((nbig, 4), dtype=torch.int32, device=dev)
+    p.stages._capi.check("wf_trace_stats", p.lib.wf_trace_stats(
+        scene, C.c_void_p(waves[big]["rays"].data_ptr()), C.c_void_p(scratch_hits.data_ptr()), nbig, 1,
+        C.c_void_p(nv.data_ptr()), C.c_void_p(nt.data_ptr()), None))
+    torch.cuda.synchronize()
+    nodes_per_ray, tris_per_ray = nv.double().mean().item(), nt.double().mean().item()
+    del nv, nt, scratch_hits
     sort_big_ms = per_b[big]["sort"]
     sort_gbs = nbig * bpe["sort"] / (sort_big_ms / 1e3) / 1e9
     out = {
@@ -317,15 +337,22 @@ def run_workload(dev, args, env_name, ablations=True, overrides=None):
                                 "frac_hbm": round(sort_gbs / HBM_PEAK_GBS, 4),
                                 "key_bits": p.kbits, "digit_bits": p.digit_bits},
         "dominant_launch": {"kernel": "trace_kernel<reorder>" if p.fused else "trace_kernel",
-                            "elems": nbig, "ms": per_b[big]["trace"], "bytes_per_elem": bpe["trace"]},
+                            "elems": nbig, "ms": per_b[big]["trace"], "bytes_per_elem": bpe["trace"],
+                            "unsorted_ms": per_b[big]["trace_unsorted"],
+                            "nodes_per_ray": nodes_per_ray, "tris_per_ray": tris_per_ray},
         "mode": mode, "cfg": cfg, "renderer": r, "scene": scene, "bounds": bounds, "waves": waves,
     }
     if ablations:
         abl = {}
-        for name, km, db, period in (("every 2nd bounce", mode, args.digit_bits, 2),
-                                     ("30-bit origin keys", 1, args.digit_bits, 1),
-                                     ("60-bit concatenated keys", 2, args.digit_bits, 1),
-                                     ("11-bit digits", mode, 11, 1)):
+        variants = [("every 2nd bounce", mode, args.digit_bits, 2),
+                    ("32-bit Aila keys", 0, args.digit_bits, 1),
+                    ("30-bit origin keys", 1, args.digit_bits, 1),
+                    ("60-bit 5-D Morton keys", 4, args.digit_bits, 1),
+                    ("60-bit concatenated keys", 2, args.digit_bits, 1),
+                    ("11-bit digits", mode, 11, 1)]
+        for name, km, db, period in variants:
+            if km == mode and db == args.digit_bits and period == 1:
+                continue                          # the headline itself
             p.set_keys(km, db)
             pb = p.timed(stream, period=period)
             ms = sum(sum(d.values()) for d in pb.values())
@@ -372,19 +399,206 @@ def reference_frame(r, frames):
     }
 
 
-def profile_figures():
-    """Counter figures of the dominant kernel from the committed rocprofv3
-    summaries (profiles/*.txt lines 'figure <name> <value>'), labelled with
-    the file they come from; counters cannot be read inside this process."""
-    out = {}
-    for f in sorted(PROFILE_DIR.glob("*.txt")):
-        for line in f.read_text().splitlines():
-            parts = line.split()
-            if len(parts) == 3 and parts[0] == "figure":
-                try:
-                    out[parts[1]] = {"value": float(parts[2]), "source": f"profiles/{f.name}"}
-                except ValueError:
-                    pass
+# In-run hardware counters.  rocprofv3 cannot attach to this process, so
+# each counter set is read by a child process (this script with
+# --counter-pass) under `rocprofv3 --pmc`: it rebuilds the headline
+# wavefronts, runs one plain 1 GiB device copy (the scale for FETCH_SIZE and
+# WRITE_SIZE), then the fused reorder + trace of the largest bounce and the
+# unsorted trace of the same bounce, in that order.
+COUNTER_SETS = (
+    ("SQ_WAVES", "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "GRBM_GUI_ACTIVE",
+     "TCC_HIT_sum", "TCC_MISS_sum"),
+    ("FETCH_SIZE",),
+    ("WRITE_SIZE",),
+)
+SIMDS = 1024                  # 256 CUs x 4 SIMDs
+ENGINE_GHZ = 2.4
+# wave64 VALU instructions per SIMD per clock behind the published 157.3
+# TFLOP/s fp32 (1024 SIMDs x 2.4 GHz x 64 lanes x 2 flops x 0.5)
+VALU_ISSUE_PEAK = 0.5
+FP32_PEAK_TFLOPS = 157.3
+# fp32 operations (an FMA counts 2) of one 4-wide node test and one
+# triangle test as traverse.h and intersect.h write them: node = 3 scale
+# multiplies + 3 origin FMAs + per child (6 plane FMAs, 6 min/max, 1
+# compare) + 5 compares of the ordering network; triangle = 9 vertex
+# subtractions + 6 shear FMAs + 3 edge functions (multiply + FMA) + 6 sign
+# compares + 3 zero tests (the miss path; a hit adds the determinant and t).
+FLOPS_PER_NODE = 3 + 6 + 4 * (12 + 6 + 1) + 5
+FLOPS_PER_TRI = 9 + 12 + 9 + 6 + 3
+
+
+def counter_child(args):
+    """Child of one counter pass (runs under rocprofv3 --pmc)."""
+    import torch
+
+    from vulkanwavefrontpathtracer_amd import stages
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    for kv in args.tune:
+        k, v = kv.split("=")
+        stages.set_tuning(k, int(v))
+    r, scene, bounds, waves, index_cap, cfg = capture_waves(dev, args.env)
+    mode = KEY_MODES[args.key_mode or cfg["Renderer"]["keyMode"]]
+    p = Pipeline(dev, scene, bounds, waves, index_cap, mode, args.digit_bits)
+    big = max(range(len(waves)), key=lambda i: waves[i]["n"])
+    p.waves = [waves[big]]
+    sp = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p.step(sp)                                   # keys + sort (+ a first trace)
+    torch.cuda.synchronize()
+    nb = 1 << 30
+    src = torch.ones(nb // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    x = waves[big]
+    V = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+    stages._capi.check("wf_reorder_trace", p.lib.wf_reorder_trace(
+        scene, V(x["idx"]), V(x["rays"]), V(x["out_rays"]), V(x["s2i"]), V(x["out_s2i"]),
+        V(x["out_i2s"]), V(x["hits"]), x["n"], 1, sp, None))
+    torch.cuda.synchronize()
+    stages._capi.check("wf_trace", p.lib.wf_trace(scene, V(x["rays"]), None, V(x["hits_u"]), x["n"], 1,
+                                                  sp, None))
+    torch.cuda.synchronize()
+    log(f"counter pass: {x['n']} rays traced sorted and unsorted")
+    return 0
+
+
+def read_counter_csv(path):
+    """Dispatches of one rocprofv3 --pmc CSV in dispatch order: (name,
+    grid, {counter: value})."""
+    import csv
+
+    per = {}
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            d = per.setdefault(int(row["Dispatch_Id"]), [row["Kernel_Name"], int(row["Grid_Size"]), {}])
+            d[2][row["Counter_Name"]] = d[2].get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+    return [per[k] for k in sorted(per)]
+
+
+def in_run_counters(args, timeout_s=240):
+    """Counters of the largest bounce's fused reorder + trace launch and of
+    the unsorted trace of the same rays, each pass a rocprofv3 child; None
+    (with the reason logged) when rocprofv3 is missing or a pass fails."""
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        log("counters: rocprofv3 not found")
+        return None
+    tmp = tempfile.mkdtemp(prefix="wfpt_counters_", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+    child = [sys.executable, str(ROOT / "bench.py"), "--counter-pass", "--env", args.env,
+             "--digit-bits", str(args.digit_bits)]
+    if args.key_mode:
+        child += ["--key-mode", args.key_mode]
+    for kv in args.tune:
+        child += ["--tune", kv]
+    found = {}
+    for i, counters in enumerate(COUNTER_SETS):
+        out = os.path.join(tmp, f"p{i}")
+        cmd = [prof, "--kernel-include-regex", "trace_kernel|copyBuffer", "--pmc", *counters,
+               "-d", out, "-o", "run", "--output-format", "csv", "--", *child]
+        log(f"counters: pass {i + 1}/{len(COUNTER_SETS)} ({' '.join(counters)})")
+        with open(os.path.join(tmp, f"p{i}.log"), "w") as lf:
+            pr = subprocess.Popen(cmd, cwd=tmp, env=env, stdout=lf, stderr=subprocess.STDOUT,
+                                  start_new_session=True)
+            try:
+                rc = pr.wait(timeout=timeout_s)
+            except subprocess.TimeoutExpired:
+                os.killpg(pr.pid, signal.SIGKILL)
+                pr.wait()
+                log(f"counters: pass {i + 1} timed out")
+                return None
+        csv_path = os.path.join(out, "run_counter_collection.csv")
+        if not os.path.exists(csv_path):
+            log(f"counters: pass {i + 1} wrote no results (exit {rc})")
+            return None
+        disp = read_counter_csv(csv_path)
+        copies = [k for k, d in enumerate(disp) if "copyBuffer" in d[0]]
+        if not copies:
+            log("counters: no copy dispatch found")
+            return None
+        cal = disp[copies[-1]]
+        traces = [d for d in disp[copies[-1] + 1:] if "trace_kernel" in d[0]]
+        if len(traces) < 2:
+            log("counters: the two trace launches were not found")
+            return None
+        found.setdefault("calibration", {}).update(cal[2])
+        found.setdefault("sorted", {"kernel": traces[0][0].split("(")[0], "grid": traces[0][1]})
+        found["sorted"].update(traces[0][2])
+        found.setdefault("unsorted", {"kernel": traces[1][0].split("(")[0], "grid": traces[1][1]})
+        found["unsorted"].update(traces[1][2])
+    log("counters: done")
+    return found
+
+
+def counter_figures(c, launch_ms):
+    """Issue fraction, lane utilisation, L2 hit rate and HBM bytes of one
+    launch from its counters and its live duration."""
+    if c is None:
+        return None
+    waves = c["SQ_WAVES"]
+    issue = c["SQ_INSTS_VALU"] / SIMDS / (launch_ms * 1e-3 * ENGINE_GHZ * 1e9)
+    return {
+        "valu_insts_per_wave": round(c["SQ_INSTS_VALU"] / waves, 1),
+        "valu_issue_per_simd_clk": round(issue, 4),
+        "valu_issue_frac": round(issue / VALU_ISSUE_PEAK, 4),
+        "lane_utilization": round(c["SQ_THREAD_CYCLES_VALU"] / (64.0 * c["SQ_ACTIVE_INST_VALU"]), 4),
+        "l2_hit_rate": round(c["TCC_HIT_sum"] / (c["TCC_HIT_sum"] + c["TCC_MISS_sum"]), 4),
+    }
+
+
+def dominant_roofline(dl, counters):
+    """Roofline of the dominant launch (the fused reorder + trace of the
+    largest bounce).  Traversal is bound by VALU instruction issue, not by
+    HBM: achieved = algorithmic fp32 work (node and triangle tests the rays
+    make) over the live launch time against the published fp32 peak; beside
+    it the hardware's VALU issue rate, lane utilisation and L2 hit rate for
+    the sorted launch and for the unsorted trace of the same rays, and the
+    HBM bytes of the sorted launch."""
+    n, ms = dl["elems"], dl["ms"]
+    flops = n * (dl["nodes_per_ray"] * FLOPS_PER_NODE + dl["tris_per_ray"] * FLOPS_PER_TRI)
+    achieved = flops / (ms * 1e-3) / 1e12
+    out = {
+        "bound": "valu", "kernel": dl["kernel"], "achieved": round(achieved, 2),
+        "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+        "traffic": None, "launch_ms": round(ms, 4), "launch_elems": n,
+        "algorithmic": {"nodes_per_ray": round(dl["nodes_per_ray"], 3),
+                        "tris_per_ray": round(dl["tris_per_ray"], 3),
+                        "flops_per_node": FLOPS_PER_NODE, "flops_per_tri": FLOPS_PER_TRI,
+                        "streamed_bytes_per_ray": dl["bytes_per_elem"]},
+        "valu_issue_peak_per_simd_clk": VALU_ISSUE_PEAK,
+        "counters": None,
+    }
+    if counters:
+        cal = counters["calibration"]
+        fetch_scale = (1 << 30) / (cal["FETCH_SIZE"] * 1024.0)
+        write_scale = (1 << 30) / (cal["WRITE_SIZE"] * 1024.0)
+        srt, uns = counters["sorted"], counters["unsorted"]
+        rd = srt["FETCH_SIZE"] * 1024.0 * fetch_scale
+        wr = srt["WRITE_SIZE"] * 1024.0 * write_scale
+        out["traffic"] = int(rd + wr)
+        sf = counter_figures(srt, ms)
+        out.update({"valu_issue_frac": sf["valu_issue_frac"], "lane_utilization": sf["lane_utilization"],
+                    "l2_hit_rate": sf["l2_hit_rate"]})
+        uf = counter_figures(uns, dl["unsorted_ms"])
+        uf["traffic"] = int(uns["FETCH_SIZE"] * 1024.0 * fetch_scale + uns["WRITE_SIZE"] * 1024.0 * write_scale)
+        sf["traffic"] = out["traffic"]
+        out["counters"] = {
+            "source": "rocprofv3 --pmc child passes of this run (one counter set each)",
+            "fetch_scale": round(fetch_scale, 4), "write_scale": round(write_scale, 4),
+            "sorted_launch": dict(sf, kernel=srt["kernel"], grid=srt["grid"], ms=round(ms, 4)),
+            "unsorted_launch": dict(uf, kernel=uns["kernel"], grid=uns["grid"],
+                                    ms=round(dl["unsorted_ms"], 4)),
+            "traffic_over_streamed_bytes": round(out["traffic"] / (n * dl["bytes_per_elem"]), 2),
+        }
     return out
 
 
@@ -411,7 +625,14 @@ def main():
                     help="wf_set_tuning knob applied before the scene is built (repeatable)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (scales HBM counters)")
+    ap.add_argument("--key-mode", choices=sorted(KEY_MODES), default=None,
+                    help="ray key of the timed step (default: the env file's Renderer.keyMode)")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the rocprofv3 counter passes of the dominant launch")
+    ap.add_argument("--counter-pass", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.counter_pass:
+        return counter_child(args)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the wavefront loop "
               "runs on a single device); run with --gpus 1", file=sys.stderr)
@@ -436,29 +657,12 @@ def main():
         log(f"copy calibration: {nb} bytes read + {nb} bytes written")
         del src, dst
 
-    head = run_workload(dev, args, args.env, ablations=not args.no_ablations)
+    head = run_workload(dev, args, args.env, ablations=not args.no_ablations, key_mode=args.key_mode)
     cfg = head["cfg"]
     R = cfg["Renderer"]
     dl = head["dominant_launch"]
-    achieved = dl["elems"] * dl["bytes_per_elem"] / (dl["ms"] / 1e3) / 1e9
-    figs = profile_figures()
-    valu = figs.get("trace_valu_busy")
-    roofline = {
-        "bound": "hbm", "kernel": dl["kernel"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": figs.get("trace_hbm_bytes_per_launch", {}).get("value"),
-        "traffic_source": figs.get("trace_hbm_bytes_per_launch", {}).get("source"),
-        "launch_ms": round(dl["ms"], 4), "launch_elems": dl["elems"],
-        "bytes_per_elem": dl["bytes_per_elem"],
-        "note": "traversal is bound by dependent-fetch latency and lane divergence, not HBM: "
-                "see valu_issue and l2_hit_rate",
-        "valu_issue": ({"busy_frac": valu["value"], "source": valu["source"],
-                        "definition": "SQ_ACTIVE_INST_VALU * 4 / (SIMDs * GRBM_GUI_ACTIVE), "
-                                      "1024 SIMDs, one wave64 VALU op per SIMD per 4 cycles"}
-                       if valu else None),
-        "lane_utilization": figs.get("trace_valu_lane_util", {}).get("value"),
-        "l2_hit_rate": figs.get("trace_l2_hit_rate", {}).get("value"),
-    }
+    counters = None if args.no_counters else in_run_counters(args)
+    roofline = dominant_roofline(dl, counters)
 
     interior = None
     if not args.no_interior:
@@ -503,8 +707,8 @@ def main():
         "data": "synthetic (seeded procedural city scene, captured bounce wavefronts)",
         "config": {
             "workload": "large plain: city 2,821,186 tris, 1920x1080 x 4 spp in flight, "
-                        f"{R['recursionDepth']} bounces, 60-bit keys (30 origin + 30 octahedral "
-                        f"bits, {R['keyMode']}), u64 key sort + u32 payload, reorder every bounce",
+                        f"{R['recursionDepth']} bounces, {KEY_DESC[head['mode']]}, reorder every bounce",
+            "key_mode": {v: k for k, v in KEY_MODES.items()}[head["mode"]],
             "rays_per_step": head["rays"], "bounces": len(head["rays_per_bounce"]),
             "rays_per_bounce": head["rays_per_bounce"], "digit_bits": args.digit_bits,
             "parallelism": "1 GPU",
@@ -704,11 +908,14 @@ def cpu_baseline(head, sample):
     bvh = oracle.Bvh(hs.tri_verts())
     t = {"keys": 0.0, "sort": 0.0, "reorder": 0.0, "trace": 0.0}
     hits = 0
+    mode = head["mode"]
+    keyfn, kbits = {0: (oracle.keys_aila32, 32), 1: (oracle.keys_origin30, 30),
+                    2: (oracle.keys_oct60, 60), 4: (oracle.keys_morton5d60, 60)}[mode]
     for rays, s2i in waves:
         t0 = time.perf_counter()
-        keys = oracle.keys_morton5d60(rays, mn, mx)
+        keys = keyfn(rays, mn, mx)
         t1 = time.perf_counter()
-        k, v = oracle.sort_pairs(keys, np.arange(len(rays), dtype=np.uint32), 60)
+        k, v = oracle.sort_pairs(keys, np.arange(len(rays), dtype=np.uint32), kbits)
         t2 = time.perf_counter()
         out_r, out_s, out_i = oracle.reorder(v, rays, s2i, int(s2i.max()) + 1)
         t3 = time.perf_counter()
@@ -720,7 +927,7 @@ def cpu_baseline(head, sample):
     total = sum(t.values())
     return {"value": round(n / total / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"{n} path rays: {'the whole headline step' if n == head['rays'] else f'the first {sample} rays of each bounce of the headline step'}; "
-                      f"oracle 60-bit 5-D Morton keys, stable sort and reorder single-threaded "
+                      f"oracle {KEY_DESC[mode].split(',')[0]}, stable sort and reorder single-threaded "
                       f"({t['keys']:.2f}+{t['sort']:.2f}+{t['reorder']:.2f} s), fp32 watertight BVH trace "
                       f"on {threads} OpenMP threads ({t['trace']:.2f} s, BVH built before timing); "
                       f"{hits} hits"}

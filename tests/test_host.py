@@ -173,6 +173,40 @@ def test_pixel_table_small_known_answer():
     assert np.array_equal(p2i[i2p], np.arange(128))
 
 
+def _pixel_order_by_sort(w, h):
+    """The slot order as a lexicographic sort of every pixel by (region,
+    primary, secondary): region 0 is the 8-aligned part, ordered by the
+    Morton code of its 8x8 block and then by the Morton code inside the
+    block; region 1 the bottom stripe ordered by (x, y); region 2 the right
+    stripe with the corner ordered by (y, x)."""
+    def morton(x, y):
+        code = np.zeros_like(x)
+        for bit in range(16):
+            code |= ((x >> bit) & 1) << (2 * bit) | ((y >> bit) & 1) << (2 * bit + 1)
+        return code
+
+    y, x = np.divmod(np.arange(w * h, dtype=np.int64), w)
+    bw, bh = w & ~7, h & ~7
+    region = np.where((x < bw) & (y < bh), 0, np.where(x < bw, 1, 2))
+    primary = np.select([region == 0, region == 1], [morton(x >> 3, y >> 3), x], y)
+    secondary = np.select([region == 0, region == 1], [morton(x & 7, y & 7), y], x)
+    return np.lexsort((secondary, primary, region))
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (8, 8), (16, 8), (7, 5), (13, 21), (100, 37), (256, 256),
+                                 (203, 117), (1920, 1080), (72, 1000)])
+def test_pixel_table_equals_sorted_definition(w, h):
+    """Every slot of the table equals the ordering's definition restated as a
+    numpy sort (sizes with and without stripes, wide and tall images whose
+    block grid is not a power of two)."""
+    from vulkanwavefrontpathtracer_amd import stages
+
+    i2p, p2i = stages.pixel_table(w, h)
+    exp = _pixel_order_by_sort(w, h)
+    assert np.array_equal(i2p, exp)
+    assert np.array_equal(p2i[exp], np.arange(w * h))
+
+
 @pytest.mark.parametrize("w,h", [(1, 1), (7, 5), (13, 21), (256, 256), (1920, 1080), (100, 37)])
 def test_pixel_table_bijection_with_stripes(w, h):
     from vulkanwavefrontpathtracer_amd import stages

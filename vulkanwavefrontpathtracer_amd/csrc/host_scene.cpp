@@ -106,52 +106,61 @@ void camera_matrices(const float pos[3], const float rot_deg[3], float fov_deg, 
     std::memcpy(out.screen_to_world, s2w.m, sizeof(s2w.m));
 }
 
-// PixelTable::recalculate (Ray/PixelTable.cpp:21-126): 8x8 blocks in Morton
-// order over the largest power-of-two square, pixels Z-swizzled inside each
-// block, then the bottom stripe and the right stripe with the corner.
+// PixelTable::recalculate (Ray/PixelTable.cpp:21-126).  The slot order:
+//   1. the 8x8 blocks of the 8-aligned part of the image (bw x bh), in
+//      ascending 2-D Morton code of the block coordinates (x in the even
+//      bits), and inside each block its 64 pixels in ascending Morton code
+//      of the in-block coordinates;
+//   2. the bottom stripe (rows bh..height-1 under the aligned part), column
+//      by column;
+//   3. the right stripe (columns bw..width-1, with the corner), row by row.
+// i2p[slot] = pixel (y * width + x) and p2i is its inverse.
+namespace {
+uint32_t spread_even_bits(uint32_t v) {   // bit k of v -> bit 2k
+    v &= 0xffffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+uint32_t gather_even_bits(uint32_t v) {   // bit 2k of v -> bit k
+    v &= 0x55555555u;
+    v = (v | (v >> 1)) & 0x33333333u;
+    v = (v | (v >> 2)) & 0x0f0f0f0fu;
+    v = (v | (v >> 4)) & 0x00ff00ffu;
+    v = (v | (v >> 8)) & 0x0000ffffu;
+    return v;
+}
+}  // namespace
+
 void pixel_table(int width, int height, std::vector<int32_t>& i2p, std::vector<int32_t>& p2i) {
-    i2p.assign((size_t)width * height, 0);
-    p2i.assign((size_t)width * height, 0);
-    int idx = 0;
-    int bh = height & ~7, bw = width & ~7;
-    int maxdim = bw > bh ? bw : bh;
-    maxdim |= maxdim >> 1;
-    maxdim |= maxdim >> 2;
-    maxdim |= maxdim >> 4;
-    maxdim |= maxdim >> 8;
-    maxdim |= maxdim >> 16;
-    maxdim = (maxdim + 1) >> 1;
-    int w8 = bw >> 3, h8 = bh >> 3;
-    for (int i = 0; i < maxdim * maxdim; ++i) {
-        int tx = 0, ty = 0, val = i, bit = 1;
-        while (val) {
-            if (val & 1) tx |= bit;
-            if (val & 2) ty |= bit;
-            bit += bit;
-            val >>= 2;
-        }
-        if (tx < w8 && ty < h8) {
-            for (int in = 0; in < 64; ++in) {
-                int ix = ((in & 1) >> 0) | ((in & 4) >> 1) | ((in & 16) >> 2);
-                int iy = ((in & 2) >> 1) | ((in & 8) >> 2) | ((in & 32) >> 3);
-                int pos = (ty * 8 + iy) * width + (tx * 8 + ix);
-                p2i[pos] = idx;
-                i2p[idx++] = pos;
-            }
-        }
+    const size_t npx = (size_t)width * height;
+    i2p.assign(npx, 0);
+    p2i.assign(npx, 0);
+    int32_t slot = 0;
+    auto emit = [&](int x, int y) {
+        const int32_t pos = y * width + x;
+        i2p[slot] = pos;
+        p2i[pos] = slot++;
+    };
+    const int bw = width & ~7, bh = height & ~7;
+    std::vector<uint64_t> blocks;              // Morton code << 32 | block x << 16 | block y
+    blocks.reserve((size_t)(bw / 8) * (bh / 8));
+    for (uint32_t by = 0; by < (uint32_t)bh / 8; ++by)
+        for (uint32_t bx = 0; bx < (uint32_t)bw / 8; ++bx)
+            blocks.push_back((uint64_t)(spread_even_bits(bx) | spread_even_bits(by) << 1) << 32 |
+                             bx << 16 | by);
+    std::sort(blocks.begin(), blocks.end());
+    for (uint64_t b : blocks) {
+        const int x0 = 8 * (int)((b >> 16) & 0xffffu), y0 = 8 * (int)(b & 0xffffu);
+        for (uint32_t k = 0; k < 64; ++k)
+            emit(x0 + (int)gather_even_bits(k), y0 + (int)gather_even_bits(k >> 1));
     }
-    for (int px = 0; px < bw; ++px)
-        for (int py = bh; py < height; ++py) {
-            int pos = px + py * width;
-            p2i[pos] = idx;
-            i2p[idx++] = pos;
-        }
-    for (int py = 0; py < height; ++py)
-        for (int px = bw; px < width; ++px) {
-            int pos = px + py * width;
-            p2i[pos] = idx;
-            i2p[idx++] = pos;
-        }
+    for (int x = 0; x < bw; ++x)
+        for (int y = bh; y < height; ++y) emit(x, y);
+    for (int y = 0; y < height; ++y)
+        for (int x = bw; x < width; ++x) emit(x, y);
 }
 
 // ------------------------------------------------------- procedural scene
