@@ -170,8 +170,8 @@ int wf_scene_set_materials(wf_scene scene, const float* albedo_rgb,
 int wf_scene_get_materials(wf_scene scene, int* num_materials, float* albedo_rgb,
                            float* emission_rgb);
 /* The 4-wide quantized hierarchy: node count, traversal stack bound and,
- * for a non-NULL host array (num_nodes4 * 16 floats), a copy of the nodes
- * (layout in csrc/scene.h). */
+ * for a non-NULL host array (num_nodes4 * 20 floats), a copy of the nodes
+ * (layout in csrc/scene.h: a header, fp16 child-box planes, references). */
 int wf_scene_wide_info(wf_scene scene, int64_t* num_nodes4, int* max_stack4, float* nodes4_host);
 /* Scene AABB (getSceneDimensions, base/VulkanglTFModel.cpp:1478-1488). */
 int wf_scene_bounds(wf_scene scene, float out_min[3], float out_max[3]);

@@ -83,7 +83,7 @@ def test_city_hierarchy_structure(dev, city):
     contains every whole (unsplit) triangle under it, and the stack bound
     covers every root-to-node path."""
     from vulkanwavefrontpathtracer_amd import _capi
-    from test_bvh_build_gpu import binary_tree, decode_wide
+    from test_bvh_build_gpu import binary_tree, decode_wide, download_wide
 
     hs, sc = city
     info = sc.info()
@@ -102,12 +102,9 @@ def test_city_hierarchy_structure(dev, city):
     np.add.at(cover, first + cnt, -1)
     assert np.all(np.cumsum(cover)[:slots] == 1)
 
-    n4, st4 = C.c_int64(), C.c_int()
-    _capi.call("wf_scene_wide_info", sc.handle, C.byref(n4), C.byref(st4), None)
-    nodes4 = np.zeros((n4.value, 16), np.float32)
-    _capi.call("wf_scene_wide_info", sc.handle, None, None, nodes4.ctypes.data_as(_capi.pf))
+    nodes4, st4 = download_wide(sc)
     lo, hi, wrefs, nc = decode_wide(nodes4)
-    N = n4.value
+    N = len(nodes4)
     valid = np.arange(4)[None, :] < nc[:, None]
     assert np.all((nc >= 2) & (nc <= 4))
     is_inner = valid & (wrefs >= 0)
@@ -132,7 +129,7 @@ def test_city_hierarchy_structure(dev, city):
         depth[ch] = depth[par] + 1
         need[ch] = need[par] + nc[ch] - 1
         frontier = ch
-    assert depth.min() >= 0 and need.max() <= st4.value <= 96
+    assert depth.min() >= 0 and need.max() <= st4 <= 96
     # every child box contains the whole (unsplit) triangles below it: leaf
     # slot ranges reduced directly, subtrees accumulated bottom-up
     whole = np.bincount(order, minlength=hs.num_tris)[order] == 1
@@ -156,7 +153,7 @@ def test_city_hierarchy_structure(dev, city):
         np.maximum.at(sub_hi, nd_i[sel], sub_hi[child[sel]])
     assert np.all(sub_lo[child] >= lo[nd_i, k_i]) and np.all(sub_hi[child] <= hi[nd_i, k_i])
     assert np.all(np.isfinite(sub_lo[0]))
-    print(f"[city] {info['nodes']} binary nodes, {N} wide nodes, {slots} slots, stack bound {st4.value}")
+    print(f"[city] {info['nodes']} binary nodes, {N} wide nodes, {slots} slots, stack bound {st4}")
 
 
 def test_city_trace_matches_oracle(dev, orc, city, captured):

@@ -468,7 +468,9 @@ __global__ __launch_bounds__(256) void flag_scan_kernel(const uint32_t* __restri
 
 __device__ __forceinline__ int wide_exponent(float extent, int e_min) {
     int e = e_min;
-    const float x = extent / 252.0f;
+    // extent / 2^e <= QMAX - 3: the origin sits one quantum below the low
+    // plane and the planes round outwards by one more
+    const float x = extent / (float)(NODE4_QMAX - 3);
     if (x > 0.0f) {
         int ex = 0;
         (void)frexpf(x, &ex);  // x = m * 2^ex, m in [0.5, 1): 2^ex >= x
@@ -495,31 +497,68 @@ __device__ __forceinline__ int wide_min_exponent(const uint32_t* __restrict__ bo
     return ex - 18;  // quantum >= 2^-18 * 2^ex > 2^-18 * size... >= 2^-19 * size
 }
 
+// Two plane values (integers 0..NODE4_QMAX, exact in fp16) as one word of
+// fp16 halves, the first in the low half.
+__device__ __forceinline__ float half_pair(float a, float b) {
+    const uint32_t lo = __builtin_bit_cast(unsigned short, (_Float16)a);
+    const uint32_t hi = __builtin_bit_cast(unsigned short, (_Float16)b);
+    return __uint_as_float(lo | hi << 16);
+}
+
+// Writes one 4-wide node (layout in scene.h) from per-child plane values.
+__device__ __forceinline__ void write_node4(float4* __restrict__ o, const float (&origin)[3],
+                                            uint32_t meta, const float (&ql)[4][3],
+                                            const float (&qh)[4][3], const int (&ref)[4]) {
+    o[0] = make_float4(origin[0], origin[1], origin[2], __uint_as_float(meta));
+    for (int a = 0; a < 3; ++a)
+        o[1 + a] = make_float4(half_pair(ql[0][a], ql[1][a]), half_pair(ql[2][a], ql[3][a]),
+                               half_pair(qh[0][a], qh[1][a]), half_pair(qh[2][a], qh[3][a]));
+    o[4] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]),
+                       __int_as_float(ref[3]));
+}
+
+// Quantized planes of one child box in the node frame (origin, scale):
+// rounded outwards by one quantum and clamped to the grid.
+__device__ __forceinline__ void quantize_child(const float* lo, const float* hi, const float* origin,
+                                               const float* scale, float* ql, float* qh) {
+    for (int a = 0; a < 3; ++a) {
+        ql[a] = fminf(fmaxf(floorf((lo[a] - origin[a]) / scale[a]) - 1.0f, 0.0f), (float)NODE4_QMAX);
+        qh[a] = fminf(fmaxf(ceilf((hi[a] - origin[a]) / scale[a]) + 1.0f, 0.0f), (float)NODE4_QMAX);
+    }
+}
+
+// Empty child slot k: an inverted box (low plane QMAX above high plane 0 on
+// every axis), which no slab test with a nonzero scale meets, and a reference
+// that is harmless if a degenerate ray meets it anyway (leaf slot 0: one
+// extra test of a scene triangle changes no closest hit), so the traversal
+// needs no child count.
+__device__ __forceinline__ void empty_child(float (&ql)[4][3], float (&qh)[4][3], int (&ref)[4], int k) {
+    for (int a = 0; a < 3; ++a) {
+        ql[k][a] = (float)NODE4_QMAX;
+        qh[k][a] = 0.0f;
+    }
+    ref[k] = leaf_ref(0, 1);
+}
+
 // Single-reference scene: one 4-wide node with the leaf as its only child.
 __global__ void wide_single_kernel(const float4* __restrict__ nodes, const uint32_t* __restrict__ bounds,
                                    float4* __restrict__ nodes4) {
     const float4 a = nodes[0], b = nodes[1];
     const float lo[3] = {a.x, a.y, a.z}, hi[3] = {a.w, b.x, b.y};
     const int e_min = wide_min_exponent(bounds);
-    float origin[3];
-    uint32_t meta = 1u << 24, qlo[3], qhi[3];
+    float origin[3], scale[3], ql[4][3], qh[4][3];
+    int ref[4];
+    uint32_t meta = 1u << 24;
     for (int k = 0; k < 3; ++k) {
         const int e = wide_exponent(hi[k] - lo[k], e_min);
-        const float scale = ldexpf(1.0f, e);
-        origin[k] = lo[k] - scale;
+        scale[k] = ldexpf(1.0f, e);
+        origin[k] = lo[k] - scale[k];
         meta |= (uint32_t)(e + 127) << (8 * k);
-        qlo[k] = (uint32_t)fminf(fmaxf(floorf((lo[k] - origin[k]) / scale) - 1.0f, 0.0f), 255.0f);
-        qhi[k] = (uint32_t)fminf(fmaxf(ceilf((hi[k] - origin[k]) / scale) + 1.0f, 0.0f), 255.0f);
     }
-    nodes4[0] = make_float4(origin[0], origin[1], origin[2], __uint_as_float(meta));
-    // children 1-3 empty: inverted boxes and the harmless leaf slot 0, as
-    // wide_emit_kernel writes them
-    for (int ax = 0; ax < 3; ++ax) qlo[ax] |= 0xffffff00u;
-    nodes4[1] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qlo[1]), __uint_as_float(qlo[2]),
-                            __uint_as_float(qhi[0]));
-    nodes4[2] = make_float4(__uint_as_float(qhi[1]), __uint_as_float(qhi[2]),
-                            __int_as_float(leaf_ref(0, 1)), __int_as_float(leaf_ref(0, 1)));
-    nodes4[3] = make_float4(__int_as_float(leaf_ref(0, 1)), __int_as_float(leaf_ref(0, 1)), 0.f, 0.f);
+    quantize_child(lo, hi, origin, scale, ql[0], qh[0]);
+    ref[0] = leaf_ref(0, 1);
+    for (int k = 1; k < 4; ++k) empty_child(ql, qh, ref, k);
+    write_node4(nodes4, origin, meta, ql, qh, ref);
 }
 
 // Child c (0/1) of 2-wide node `node`: its box and reference.
@@ -646,32 +685,10 @@ __global__ __launch_bounds__(256) void wide_emit_kernel(int n, const uint32_t* _
         origin[a] = ulo[a] - scale[a];
         meta |= (uint32_t)(e + 127) << (8 * a);
     }
-    uint32_t qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0};
-    for (int k = 0; k < nc; ++k)
-        for (int a = 0; a < 3; ++a) {
-            const float ql = floorf((lo[k][a] - origin[a]) / scale[a]) - 1.0f;
-            const float qh = ceilf((hi[k][a] - origin[a]) / scale[a]) + 1.0f;
-            const uint32_t l = (uint32_t)fminf(fmaxf(ql, 0.0f), 255.0f);
-            const uint32_t h = (uint32_t)fminf(fmaxf(qh, 0.0f), 255.0f);
-            qlo[a] |= l << (8 * k);
-            qhi[a] |= h << (8 * k);
-        }
-    // Empty child slots: an inverted box (low plane 255 above high plane 0 on
-    // every axis), which no slab test with a nonzero scale meets, and a
-    // reference that is harmless if a degenerate ray meets it anyway (leaf
-    // slot 0: one extra test of a scene triangle changes no closest hit), so
-    // the traversal needs no child count.
-    for (int k = nc; k < 4; ++k) {
-        for (int a = 0; a < 3; ++a) qlo[a] |= 255u << (8 * k);
-        ref[k] = leaf_ref(0, 1);
-    }
-    float4* o = nodes4 + 4 * (int64_t)idx4[i];
-    o[0] = make_float4(origin[0], origin[1], origin[2], __uint_as_float(meta));
-    o[1] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qlo[1]), __uint_as_float(qlo[2]),
-                       __uint_as_float(qhi[0]));
-    o[2] = make_float4(__uint_as_float(qhi[1]), __uint_as_float(qhi[2]), __int_as_float(ref[0]),
-                       __int_as_float(ref[1]));
-    o[3] = make_float4(__int_as_float(ref[2]), __int_as_float(ref[3]), 0.f, 0.f);
+    float ql[4][3], qh[4][3];
+    for (int k = 0; k < nc; ++k) quantize_child(lo[k], hi[k], origin, scale, ql[k], qh[k]);
+    for (int k = nc; k < 4; ++k) empty_child(ql, qh, ref, k);
+    write_node4(nodes4 + NODE4_F4 * (int64_t)idx4[i], origin, meta, ql, qh, ref);
 }
 
 // ------------------------------------------------------------------ host
@@ -796,12 +813,16 @@ extern "C" int wf_scene_create(const void* vertices, int64_t num_vertices, int v
     if (n == 0) {
         WF_HIP(hipMalloc((void**)&sc->tris, sizeof(float4) * 3));
         WF_HIP(hipMalloc((void**)&sc->nodes, sizeof(float4) * 4));
-        WF_HIP(hipMalloc((void**)&sc->nodes4, sizeof(float4) * 4));
+        WF_HIP(hipMalloc((void**)&sc->nodes4, sizeof(float4) * NODE4_F4));
         // empty nodes: inverted boxes / no children, so any traversal ends at the root
         const float inf = INFINITY;
         const float empty2[16] = {inf, inf, inf, -inf, -inf, -inf, inf, inf,
                                   inf, -inf, -inf, -inf, 0.f, 0.f, 0.f, 0.f};
-        const float empty4[16] = {0.f};
+        // 4-wide root: every child slot empty (planes QMAX above 0, fp16
+        // 2047 = 0x67ff), every reference leaf slot 0
+        uint32_t empty4[4 * NODE4_F4] = {0u};
+        for (int a = 0; a < 3; ++a) empty4[4 + 4 * a] = empty4[5 + 4 * a] = 0x67ff67ffu;
+        for (int k = 0; k < 4; ++k) empty4[16 + k] = (uint32_t)leaf_ref(0, 1);
         WF_HIP(hipMemcpy(sc->nodes, empty2, sizeof(empty2), hipMemcpyHostToDevice));
         WF_HIP(hipMemcpy(sc->nodes4, empty4, sizeof(empty4), hipMemcpyHostToDevice));
         sc->num_nodes = 1;
@@ -993,7 +1014,7 @@ extern "C" int wf_scene_create(const void* vertices, int64_t num_vertices, int v
         sc->max_stack4 = greedy && m > 1 ? (int)hw[2] : 3 * ((int)hw[0] / 2 + 1);
         if (sc->max_stack4 > TRACE_STACK4)
             return fail(WF_ERR_TOO_LARGE, "wf_scene_create: 4-wide BVH deeper than the traversal stack");
-        WF_HIP(hipMalloc((void**)&sc->nodes4, sizeof(float4) * 4 * sc->num_nodes4));
+        WF_HIP(hipMalloc((void**)&sc->nodes4, sizeof(float4) * NODE4_F4 * sc->num_nodes4));
         if (m > 1) {
             if (greedy)
                 wide_emit_kernel<true><<<grid_for(nin, 256), 256, 0, s>>>(
@@ -1062,7 +1083,7 @@ extern "C" int wf_scene_wide_info(wf_scene scene, int64_t* num_nodes4, int* max_
     if (num_nodes4) *num_nodes4 = scene->num_nodes4;
     if (max_stack4) *max_stack4 = scene->max_stack4;
     if (nodes4_host)
-        WF_HIP(hipMemcpy(nodes4_host, scene->nodes4, sizeof(float4) * 4 * scene->num_nodes4,
+        WF_HIP(hipMemcpy(nodes4_host, scene->nodes4, sizeof(float4) * NODE4_F4 * scene->num_nodes4,
                          hipMemcpyDeviceToHost));
     return WF_OK;
 }

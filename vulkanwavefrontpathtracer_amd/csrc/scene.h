@@ -9,12 +9,15 @@
 //             {c0.lo.xyz, c0.hi.x}, {c0.hi.yz, c1.lo.xy}, {c1.lo.z, c1.hi.xyz},
 //             {ref0, ref1, 0, 0} (int bits).  ref >= 0: internal node index;
 //             ref < 0: leaf ~(first_slot << 3 | (count - 1)), count <= 8.
-//   nodes4  [num_nodes4] x 4 float4 (64 bytes), the same hierarchy with
-//           every second level collapsed (2-4 children per node) and the child
-//           boxes quantized to 8 bits per plane against the node's own frame:
+//   nodes4  [num_nodes4] x NODE4_F4 float4 (80 bytes), the same hierarchy
+//           collapsed to 2-4 children per node, the child boxes quantized to
+//           integers 0..2047 per plane against the node's own frame and held
+//           as fp16 (every such integer is exact in fp16):
 //             {origin.xyz, exponents ex | ey << 8 | ez << 16 | nchild << 24},
-//             {qlo.x, qlo.y, qlo.z, qhi.x}, {qhi.y, qhi.z, ref0, ref1},
-//             {ref2, ref3, 0, 0}; byte k of each q word belongs to child k;
+//             {lo.x c0|c1, lo.x c2|c3, hi.x c0|c1, hi.x c2|c3},
+//             the same for y, the same for z   (fp16 pairs, child 0/2 in the
+//                                               low half),
+//             {ref0, ref1, ref2, ref3};
 //           child box plane = origin + q * 2^(e - 127), rounded outwards by
 //           at least one quantum (conservative); refs as in nodes, interior
 //           refs index nodes4.
@@ -25,6 +28,10 @@
 #pragma once
 
 #include "common.h"
+
+// float4 per 4-wide node and the largest plane value of its quantized boxes.
+constexpr int NODE4_F4 = 5;
+constexpr int NODE4_QMAX = 2047;
 
 struct wf_scene_s {
     int64_t num_tris = 0;
@@ -53,7 +60,7 @@ namespace wfpt {
 constexpr int TRACE_STACK = 96;        // stack bound of the binary hierarchy's depth checks
 constexpr int TRACE_STACK4 = 96;       // traversal stack entries per ray (4-wide)
 
-__device__ __forceinline__ int leaf_ref(int first, int count) { return ~((first << 3) | (count - 1)); }
+__host__ __device__ __forceinline__ int leaf_ref(int first, int count) { return ~((first << 3) | (count - 1)); }
 __device__ __forceinline__ int leaf_first(int ref) { return (~ref) >> 3; }
 __device__ __forceinline__ int leaf_count(int ref) { return ((~ref) & 7) + 1; }
 

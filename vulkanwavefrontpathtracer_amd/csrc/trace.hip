@@ -129,9 +129,9 @@ __global__ __launch_bounds__(64) void trace_group_kernel(const float4* __restric
         if (gl < fsize) {
             const int node = s_front[cur][fbase + gl];
             if (node >= 0) {
-                const float4* nd = nodes4 + 4 * (int64_t)node;
+                const float4* nd = nodes4 + NODE4_F4 * (int64_t)node;
                 float tn[4];
-                node4_hits(st, nd[0], nd[1], nd[2], nd[3], tn, ch);
+                node4_hits(st, nd[0], nd[1], nd[2], nd[3], nd[4], tn, ch);
                 c = (tn[0] < INFINITY) + (tn[1] < INFINITY) + (tn[2] < INFINITY) + (tn[3] < INFINITY);
             } else {
                 ch[0] = node;  // a leaf stays in the frontier

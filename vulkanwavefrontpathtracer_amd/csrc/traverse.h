@@ -126,15 +126,25 @@ __device__ __forceinline__ void cswap(float& ta, int& ra, float& tb, int& rb) {
     rb = rh;
 }
 
-// Interior node st.node of the 4-wide quantized hierarchy: up to four child
-// boxes decoded from 8-bit planes (t = q * 2^e / d + (origin - o) / d, one
-// fused multiply-add per plane), the hit children visited nearest first.
-// Returns false when the lane has no node left (no child hit, stack empty).
-// Child boxes of a 4-wide node (a..d) hit within [tmin, best_t], nearest
-// first: tn[k] the entry distance (INFINITY for a miss), ref[k] the child.
+// Plane value of child k from a pair of fp16 words (children 0/1 in w01,
+// 2/3 in w23, the first of each pair in the low half).  Converted inside the
+// fused multiply-add that uses it (v_fma_mix_f32 takes fp16 halves).
+__device__ __forceinline__ float plane_of(uint32_t w01, uint32_t w23, int k) {
+    const uint32_t w = k < 2 ? w01 : w23;
+    const unsigned short h = (k & 1) ? (unsigned short)(w >> 16) : (unsigned short)(w & 0xffffu);
+    return (float)__builtin_bit_cast(_Float16, h);
+}
+
+// Interior node of the 4-wide quantized hierarchy (header a, x/y/z plane
+// words px/py/pz, references r; layout in scene.h): the four child boxes
+// decoded from their planes (t = q * 2^e / d + (origin - o) / d, one fused
+// multiply-add per plane) and tested against [tmin, best_t].  tn[k] is the
+// entry distance of the k-th nearest hit child (INFINITY for a miss),
+// ref[k] its reference.
 template <bool LDS, int STACK>
-__device__ __forceinline__ void node4_hits(const TravState<LDS, STACK>& st, float4 a, float4 b,
-                                           float4 c, float4 d, float (&tn)[4], int (&ref)[4]) {
+__device__ __forceinline__ void node4_hits(const TravState<LDS, STACK>& st, float4 a, float4 px,
+                                           float4 py, float4 pz, float4 r, float (&tn)[4],
+                                           int (&ref)[4]) {
     // empty child slots hold inverted boxes and a harmless reference (see
     // wide_emit_kernel), so every slot is tested without the child count
     const uint32_t meta = __float_as_uint(a.w);
@@ -147,26 +157,25 @@ __device__ __forceinline__ void node4_hits(const TravState<LDS, STACK>& st, floa
     // positive one meets the low plane first): the same floats as the min
     // and max of the two plane distances, chosen once per node instead of
     // per child.
-    const uint32_t qlx = __float_as_uint(b.x), qly = __float_as_uint(b.y),
-                   qlz = __float_as_uint(b.z), qhx = __float_as_uint(b.w),
-                   qhy = __float_as_uint(c.x), qhz = __float_as_uint(c.y);
     const bool fx = st.ix < 0.0f, fy = st.iy < 0.0f, fz = st.iz < 0.0f;
-    const uint32_t qnx = fx ? qhx : qlx, qfx = fx ? qlx : qhx;
-    const uint32_t qny = fy ? qhy : qly, qfy = fy ? qly : qhy;
-    const uint32_t qnz = fz ? qhz : qlz, qfz = fz ? qlz : qhz;
-    ref[0] = __float_as_int(c.z);
-    ref[1] = __float_as_int(c.w);
-    ref[2] = __float_as_int(d.x);
-    ref[3] = __float_as_int(d.y);
+    const uint32_t nx01 = __float_as_uint(fx ? px.z : px.x), nx23 = __float_as_uint(fx ? px.w : px.y);
+    const uint32_t fx01 = __float_as_uint(fx ? px.x : px.z), fx23 = __float_as_uint(fx ? px.y : px.w);
+    const uint32_t ny01 = __float_as_uint(fy ? py.z : py.x), ny23 = __float_as_uint(fy ? py.w : py.y);
+    const uint32_t fy01 = __float_as_uint(fy ? py.x : py.z), fy23 = __float_as_uint(fy ? py.y : py.w);
+    const uint32_t nz01 = __float_as_uint(fz ? pz.z : pz.x), nz23 = __float_as_uint(fz ? pz.w : pz.y);
+    const uint32_t fz01 = __float_as_uint(fz ? pz.x : pz.z), fz23 = __float_as_uint(fz ? pz.y : pz.w);
+    ref[0] = __float_as_int(r.x);
+    ref[1] = __float_as_int(r.y);
+    ref[2] = __float_as_int(r.z);
+    ref[3] = __float_as_int(r.w);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const int sh = 8 * k;
-        const float nx = fmaf((float)((qnx >> sh) & 0xffu), sax, bx);
-        const float fxv = fmaf((float)((qfx >> sh) & 0xffu), sax, bx);
-        const float ny = fmaf((float)((qny >> sh) & 0xffu), say, by);
-        const float fyv = fmaf((float)((qfy >> sh) & 0xffu), say, by);
-        const float nz = fmaf((float)((qnz >> sh) & 0xffu), saz, bz);
-        const float fzv = fmaf((float)((qfz >> sh) & 0xffu), saz, bz);
+        const float nx = fmaf(plane_of(nx01, nx23, k), sax, bx);
+        const float fxv = fmaf(plane_of(fx01, fx23, k), sax, bx);
+        const float ny = fmaf(plane_of(ny01, ny23, k), say, by);
+        const float fyv = fmaf(plane_of(fy01, fy23, k), say, by);
+        const float nz = fmaf(plane_of(nz01, nz23, k), saz, bz);
+        const float fzv = fmaf(plane_of(fz01, fz23, k), saz, bz);
         const float nn = fmaxf(fmaxf(nx, ny), fmaxf(nz, st.tmin));
         const float ff = fminf(fminf(fxv, fyv), fminf(fzv, st.best_t));
         tn[k] = nn <= ff ? nn : INFINITY;
@@ -184,12 +193,12 @@ __device__ __forceinline__ void node4_hits(const TravState<LDS, STACK>& st, floa
 template <int BLOCK, bool LDS, bool STATS, int STACK, bool TOP = false>
 __device__ __forceinline__ bool trav_node4_data(TravState<LDS, STACK>& st,
                                                 int (&spill)[TravState<LDS, STACK>::SPILL],
-                                                float4 a, float4 b, float4 c, float4 d, int* s_stack,
-                                                int top = 0) {
+                                                float4 a, float4 px, float4 py, float4 pz, float4 r,
+                                                int* s_stack, int top = 0) {
     if (STATS) ++st.nodes_visited;
     float tn[4];
     int ref[4];
-    node4_hits(st, a, b, c, d, tn, ref);
+    node4_hits(st, a, px, py, pz, r, tn, ref);
     const int h1 = tn[1] < INFINITY, h2 = tn[2] < INFINITY, h3 = tn[3] < INFINITY;
     if (LDS && __all(st.sp <= LDS_DEPTH - 3)) {
         // The hit children come first (nearest first), so the pushes can be
@@ -221,7 +230,7 @@ __device__ __forceinline__ bool trav_node4_data(TravState<LDS, STACK>& st,
 }
 
 // One traversal step of the 4-wide hierarchy that issues a single load per
-// lane whatever the lane visits: the node's 64 bytes, or the first triangle
+// lane whatever the lane visits: the node's 80 bytes, or the first triangle
 // of a leaf (48 bytes), so a wave whose lanes are split between nodes and
 // leaves waits for memory once per step instead of once per branch.
 // Returns true when the ray is finished.
@@ -232,16 +241,20 @@ __device__ __forceinline__ bool trav_step4_fused(TravState<LDS, STACK>& st,
                                                  const float4* __restrict__ tris, const WtRay& wr,
                                                  int* s_stack) {
     const bool inner = st.node >= 0;
-    const float4* p = inner ? nodes4 + 4 * (int64_t)st.node : tris + 3 * (int64_t)leaf_first(st.node);
+    const float4* p = inner ? nodes4 + NODE4_F4 * (int64_t)st.node
+                            : tris + 3 * (int64_t)leaf_first(st.node);
     const float4 a = p[0], b = p[1], c = p[2];
-    float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (inner) d = p[3];
+    float4 d = make_float4(0.f, 0.f, 0.f, 0.f), e = d;
+    if (inner) {
+        d = p[3];
+        e = p[4];
+    }
     // The stack's top entry, read while the fetch is in flight: the next node
     // of a step that pushes nothing (a leaf, or a node with no child hit),
     // which then waits for neither LDS nor the fetch's dependents to pop.
     int top = 0;
     if (LDS && st.sp > 0) top = ((const lds_int*)s_stack)[min(st.sp - 1, LDS_DEPTH - 1) * BLOCK + threadIdx.x];
-    if (inner) return !trav_node4_data<BLOCK, LDS, STATS, STACK, true>(st, spill, a, b, c, d, s_stack, top);
+    if (inner) return !trav_node4_data<BLOCK, LDS, STATS, STACK, true>(st, spill, a, b, c, d, e, s_stack, top);
     const int first = leaf_first(st.node), cnt = leaf_count(st.node);
     if (trav_tri<CLOSEST, LDS, STACK, STATS, RECORD>(st, first, a, b, c, wr)) return true;
     for (int k = 1; k < cnt; ++k) {
