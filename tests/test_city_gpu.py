@@ -189,22 +189,30 @@ def test_city_trace_matches_oracle(dev, orc, city, captured):
     assert 0.3 < np.mean(exp[:, 0] >= 0) < 1.0
 
 
-def keys_and_sort(orc, stages, rays, mn, mx):
-    keys, idx = stages.morton_keys(rays, stages.KEY_ORIGIN_DIR_MORTON60, mn, mx)
+# ray keys of the large workload: the headline's 32-bit Aila key and the
+# 60-bit 5-D Morton key of its ablation (key mode, oracle, key bits, dtype)
+KEYS = {"aila32": (0, "keys_aila32", 32, np.uint32),
+        "origin_dir_morton60": (4, "keys_morton5d60", 60, np.uint64)}
+
+
+def keys_and_sort(orc, stages, rays, mn, mx, key="origin_dir_morton60"):
+    mode, ref_fn, bits, dt = KEYS[key]
+    keys, idx = stages.morton_keys(rays, mode, mn, mx)
     rays_h = rays.cpu().numpy()
-    k_ref = orc.keys_morton5d60(rays_h, mn, mx)
-    assert np.array_equal(keys.cpu().numpy().view(np.uint64), k_ref)
-    stages.sort_pairs(keys, idx, 60, 8)
+    k_ref = getattr(orc, ref_fn)(rays_h, mn, mx)
+    assert np.array_equal(keys.cpu().numpy().view(dt), k_ref)
+    stages.sort_pairs(keys, idx, bits, 8)
     order = np.argsort(k_ref, kind="stable")
     assert np.array_equal(idx.cpu().numpy().view(np.uint32), order.astype(np.uint32))
-    assert np.array_equal(keys.cpu().numpy().view(np.uint64), k_ref[order])
+    assert np.array_equal(keys.cpu().numpy().view(dt), k_ref[order])
     return idx, order, rays_h
 
 
-def test_city_primary_keys_sort_8m(dev, orc, city):
+@pytest.mark.parametrize("key", sorted(KEYS))
+def test_city_primary_keys_sort_8m(dev, orc, city, key):
     """The 4 passes' primary rays (raygenPrimary, sample indices 0..3) make
     the 8,294,400-ray wavefront: keys bit-exact and the sort permutation
-    equal to numpy's stable argsort."""
+    equal to numpy's stable argsort, for the 32-bit and the 60-bit key."""
     import json
 
     from vulkanwavefrontpathtracer_amd import stages
@@ -223,14 +231,16 @@ def test_city_primary_keys_sort_8m(dev, orc, city):
                       for p in range(SPP)])
     assert rays.shape[0] == 8_294_400
     mn, mx = [float(x) for x in hs.bmin], [float(x) for x in hs.bmax]
-    keys_and_sort(orc, stages, rays, mn, mx)
+    keys_and_sort(orc, stages, rays, mn, mx, key)
 
 
-def test_city_bounce_sort_reorder_trace(dev, orc, city, captured):
+@pytest.mark.parametrize("key", sorted(KEYS))
+def test_city_bounce_sort_reorder_trace(dev, orc, city, captured, key):
     """The whole first path bounce through Tracer::traceSort's stages at full
-    size: keys, sort, then wf_reorder_trace, whose reordered rays and LUTs
-    equal the oracle's reorder and whose hits equal the oracle's trace of the
-    reordered rays for 20k slots."""
+    size, with the headline's key and the 60-bit one: keys, sort, then
+    wf_reorder_trace, whose reordered rays and LUTs equal the oracle's
+    reorder and whose hits equal the oracle's trace of the reordered rays
+    for 20k slots."""
     from vulkanwavefrontpathtracer_amd import stages
 
     hs, sc = city
@@ -238,7 +248,7 @@ def test_city_bounce_sort_reorder_trace(dev, orc, city, captured):
     n = rays.shape[0]
     assert n > 4_000_000
     mn, mx = [float(x) for x in hs.bmin], [float(x) for x in hs.bmax]
-    idx, order, rays_h = keys_and_sort(orc, stages, rays, mn, mx)
+    idx, order, rays_h = keys_and_sort(orc, stages, rays, mn, mx, key)
     npx = 1920 * 1080
     out_rays = torch.empty_like(rays)
     out_s2i = torch.empty_like(s2i)
