@@ -3,8 +3,8 @@
 Headline workload (one GPU): the large plain scene, a procedural city of
 2,821,186 triangles, 1920x1080 x 4 spp with all samples in flight (8.3M
 primary rays, 5.5M path rays at the first bounce), 8 bounces, the env file's
-ray key (30-bit origin Morton keys sorted as u32 keys with a u32 payload,
-8-bit digits), reorder every bounce.  One untimed frame of the C++ frame loop
+ray key (the reference's 32-bit Aila key sorted as u32 keys with a u32
+payload, 8-bit digits), reorder every bounce.  One untimed frame of the C++ frame loop
 captures every pass's path wavefronts in HBM; the 4 passes of each bounce are
 concatenated into one wavefront.  A timed step is one pass of the hot path
 over all captured bounces of that frame:
