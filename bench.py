@@ -881,7 +881,7 @@ def tuning_state():
 
     out = {}
     for k in ("bvh.presplit", "bvh.split_depth", "bvh.leaf_max", "bvh.collapse", "trace.block",
-              "trace.group64", "trace.group16", "trace.group4", "sort.onesweep_tiles",
+              "trace.xcd", "trace.group64", "trace.group16", "trace.group4", "sort.onesweep_tiles",
               "sort.rank_max"):
         out[k] = stages.get_tuning(k)
     return out

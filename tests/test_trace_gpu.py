@@ -448,7 +448,7 @@ def test_reorder_trace_fused_bit_exact(dev, orc, knobs):
 
 
 @pytest.mark.parametrize("n,k", [(64 * 8 * 500 + 3 * 64 + 5, 1), (64 * 8 * 500 + 3 * 64 + 5, 16),
-                                 (300_000, 4), (300_000, 4096)])
+                                 (300_000, 4), (300_000, 4096), (64 * 8 * 256 * 2 + 3 * 64 + 5, 256)])
 def test_xcd_block_order_bit_exact(dev, n, k):
     """The XCD run order of workgroups (trace.xcd = k) traces every ray exactly
     once: direct and fused reorder traces equal the default order, for grids

@@ -175,7 +175,7 @@ struct Tuning {
     int trace_group4 = 64000;   // ... 4 lanes per ray
     int batch_key_layout = 0;   // batched queue key below the scene prefix: 0 concatenated, 1 5-D Morton
     int trace_fuse_reorder = 1;  // wf_reorder_trace: one fused kernel where available
-    int trace_xcd = 0;          // runs of this many consecutive workgroups per XCD (0 off)
+    int trace_xcd = 64;         // runs of this many consecutive workgroups per XCD (0 off)
 };
 Tuning& tuning();
 // Incremented by every successful wf_set_tuning: a captured graph records the
