@@ -890,7 +890,8 @@ def tuning_state():
 def cpu_baseline(head, sample):
     """The oracle's port of keys -> sort -> reorder -> trace on the host
     cores, in the headline's key mode, over the headline step's bounce
-    wavefronts (all of them unless `sample` caps the rays per bounce); the
+    wavefronts (all of them unless `sample` caps the rays per bounce): keys,
+    reorder and trace on OMP_NUM_THREADS threads, the stable sort on one; the
     oracle's BVH is built before the clock starts, and the trace is its fp32
     watertight one (the same records as the GPU's)."""
     import numpy as np
@@ -927,10 +928,10 @@ def cpu_baseline(head, sample):
     total = sum(t.values())
     return {"value": round(n / total / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"{n} path rays: {'the whole headline step' if n == head['rays'] else f'the first {sample} rays of each bounce of the headline step'}; "
-                      f"oracle {KEY_DESC[mode].split(',')[0]}, stable sort and reorder single-threaded "
-                      f"({t['keys']:.2f}+{t['sort']:.2f}+{t['reorder']:.2f} s), fp32 watertight BVH trace "
-                      f"on {threads} OpenMP threads ({t['trace']:.2f} s, BVH built before timing); "
-                      f"{hits} hits"}
+                      f"oracle {KEY_DESC[mode].split(',')[0]} and reorder on {threads} OpenMP threads, "
+                      f"stable LSD sort on one thread ({t['keys']:.2f}+{t['sort']:.2f}+{t['reorder']:.2f} s), "
+                      f"fp32 watertight BVH trace on {threads} OpenMP threads ({t['trace']:.2f} s, BVH "
+                      f"built before timing); {hits} hits"}
 
 
 if __name__ == "__main__":

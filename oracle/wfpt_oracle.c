@@ -56,6 +56,7 @@ static const int AILA_TABLE[32][4] = {
 OR_API void or_keys_aila32(const float* rays, int64_t n, const float mn[3], const float mx[3],
                            uint32_t* keys) {
     float sz[3] = {mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
         uint32_t ia[3], ib[3];
         aila_quantise(rays + 8 * i, mn, sz, ia, ib);
@@ -114,12 +115,14 @@ static uint32_t oct_code(const float* ray, int bits) {
 OR_API void or_keys_origin30(const float* rays, int64_t n, const float mn[3], const float mx[3],
                              uint32_t* keys) {
     float sz[3] = {mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) keys[i] = origin30(rays + 8 * i, mn, sz);
 }
 
 OR_API void or_keys_oct60(const float* rays, int64_t n, const float mn[3], const float mx[3],
                           uint64_t* keys) {
     float sz[3] = {mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i)
         keys[i] = ((uint64_t)origin30(rays + 8 * i, mn, sz) << 30) | oct_code(rays + 8 * i, 15);
 }
@@ -131,6 +134,7 @@ OR_API void or_keys_oct60(const float* rays, int64_t n, const float mn[3], const
 OR_API void or_keys_morton5d60(const float* rays, int64_t n, const float mn[3], const float mx[3],
                                uint64_t* keys) {
     float sz[3] = {mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
         uint32_t q[3], qu, qv;
         origin_q10(rays + 8 * i, mn, sz, q);
@@ -204,6 +208,7 @@ OR_API void or_sort_pairs(void* keys, uint32_t* vals, int64_t n, int key_bytes, 
 /* reorderRays.comp:37-44. */
 OR_API void or_reorder(const uint32_t* idx, const float* in_rays, float* out_rays,
                        const int32_t* in_s2i, int32_t* out_s2i, int32_t* out_i2s, int64_t n) {
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
         uint32_t src = idx[i];
         memcpy(out_rays + 8 * i, in_rays + 8 * (int64_t)src, 32);
