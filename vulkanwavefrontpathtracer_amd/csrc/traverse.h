@@ -244,7 +244,9 @@ __device__ __forceinline__ bool trav_step4_fused(TravState<LDS, STACK>& st,
     const float4* p = inner ? nodes4 + NODE4_F4 * (int64_t)st.node
                             : tris + 3 * (int64_t)leaf_first(st.node);
     const float4 a = p[0], b = p[1], c = p[2];
-    float4 d = make_float4(0.f, 0.f, 0.f, 0.f), e = d;
+    // the node's last two quads; a leaf step never reads them, so they are
+    // left unset there (no register moves on the leaf lanes)
+    float4 d, e;
     if (inner) {
         d = p[3];
         e = p[4];
