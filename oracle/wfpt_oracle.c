@@ -14,7 +14,19 @@
  *             radix sort) checked against numpy's stable argsort.
  *   reorder - permutation invariants.
  *   trace   - FP64 brute force; FP32 watertight test checked against the
- *             FP64 one on random and constructed edge cases.
+ *             FP64 one on random and constructed edge cases; the FP32 BVH
+ *             trace equal to the FP32 brute force record for record.
+ *   RNGs    - TEA, Sobol, Halton and the LCG (the C standard's sample rand())
+ *             against published values; the LCG's full low-24-bit period; the
+ *             Jenkins mix against its inverse derived from the shader's shifts;
+ *             shadow sample points one per stratum.
+ *   shading - the Duff et al. basis orthonormal and right-handed; cosine-lobe
+ *             moments and distribution over LCG draws; the tone map's
+ *             chromaticity and Reinhard curve.
+ *   frame   - FP64 frame: empty scene, headlit plane and closed tessellated
+ *             sphere closed forms; agreement with the FP32 stage composition
+ *             once the replay takes its discrete choices; the FP64 adjoint
+ *             against central differences.
  * Functions without such a check say "parity unpinned" in their comment.
  */
 #include <math.h>
