@@ -234,19 +234,19 @@ def test_city_primary_keys_sort_8m(dev, orc, city, key):
     keys_and_sort(orc, stages, rays, mn, mx, key)
 
 
-@pytest.mark.parametrize("key", sorted(KEYS))
-def test_city_bounce_sort_reorder_trace(dev, orc, city, captured, key):
-    """The whole first path bounce through Tracer::traceSort's stages at full
-    size, with the headline's key and the 60-bit one: keys, sort, then
-    wf_reorder_trace, whose reordered rays and LUTs equal the oracle's
-    reorder and whose hits equal the oracle's trace of the reordered rays
-    for 20k slots."""
+@pytest.mark.parametrize("key,bounce", [("aila32", 0), ("origin_dir_morton60", 0), ("aila32", 2)])
+def test_city_bounce_sort_reorder_trace(dev, orc, city, captured, key, bounce):
+    """A whole path bounce (the first, 5.5M rays, and the third) through
+    Tracer::traceSort's stages at full size, with the headline's key and the
+    60-bit one: keys, sort, then wf_reorder_trace, whose reordered rays and
+    LUTs equal the oracle's reorder and whose hits equal the oracle's trace
+    of the reordered rays for 20k slots."""
     from vulkanwavefrontpathtracer_amd import stages
 
     hs, sc = city
-    rays, s2i, sizes = captured[0]
+    rays, s2i, sizes = captured[bounce]
     n = rays.shape[0]
-    assert n > 4_000_000
+    assert n > (4_000_000 if bounce == 0 else 1_000_000)
     mn, mx = [float(x) for x in hs.bmin], [float(x) for x in hs.bmax]
     idx, order, rays_h = keys_and_sort(orc, stages, rays, mn, mx, key)
     npx = 1920 * 1080
@@ -269,7 +269,7 @@ def test_city_bounce_sort_reorder_trace(dev, orc, city, captured, key):
     assert np.array_equal(out_rays.cpu().numpy().view(np.uint32), r_rays.view(np.uint32))
     assert np.array_equal(out_s2i.cpu().numpy(), r_s2i)
     assert np.array_equal(out_i2s.cpu().numpy(), r_i2s)
-    rng = np.random.default_rng(22)
+    rng = np.random.default_rng(22 + bounce)
     pick = np.sort(rng.choice(n, 20000, replace=False))
     exp = expected(orc, hs, r_rays[pick])
     assert np.array_equal(hits.cpu().numpy()[pick], exp)
