@@ -270,12 +270,13 @@ class FrameCfg(C.Structure):
     _fields_ = [("w", C.c_int), ("h", C.c_int), ("spp", C.c_int), ("depth", C.c_int), ("S", C.c_int),
                 ("frame", C.c_int), ("rr", C.c_int), ("rr_start", C.c_int),
                 ("light", C.c_float * 3), ("bg", C.c_float * 3), ("origin", C.c_float * 3),
-                ("radius", C.c_float), ("max_dist", C.c_float), ("s2w", C.c_float * 16)]
+                ("radius", C.c_float), ("max_dist", C.c_float), ("s2w", C.c_float * 16),
+                ("pixel_stride", C.c_int)]
 
 
 def render_frame_f64(tri_verts, tri_n, tri_mat, albedo, emission, *, w, h, spp, depth, S, frame,
                      light, bg, origin, radius, max_dist, s2w_colmajor, rr=False, rr_start=3,
-                     d_pixels=None, hints=None, first_disagreement=None):
+                     d_pixels=None, hints=None, first_disagreement=None, pixel_stride=1):
     """Frame 1 of Renderer::renderPath, primary rays in fp32 (as raygenPrimary
     makes them) and the paths followed in fp64.  Returns ([w*h, 4] float32
     linear RGBA, [w*h] bool ambiguous): a pixel is ambiguous when some vertex
@@ -299,10 +300,14 @@ def render_frame_f64(tri_verts, tri_n, tri_mat, albedo, emission, *, w, h, spp, 
 
     With d_pixels ([w*h, 4] float32, dLoss/dpixel), also returns the fp64
     adjoint: (image, ambiguous, grad_albedo [nmat, 3], grad_emission [nmat, 3])
-    with grad = sum over pixels of d_pixels . dpixel/dparameter."""
+    with grad = sum over pixels of d_pixels . dpixel/dparameter.
+
+    pixel_stride k > 1 renders only pixels 0, k, 2k, ... (a sample of a
+    large frame); the other entries of every output stay as initialised."""
     cfg = FrameCfg(w, h, spp, depth, S, frame, int(rr), rr_start, (C.c_float * 3)(*light),
                    (C.c_float * 3)(*bg), (C.c_float * 3)(*origin), radius, max_dist,
-                   (C.c_float * 16)(*np.asarray(s2w_colmajor, np.float32).reshape(16)))
+                   (C.c_float * 16)(*np.asarray(s2w_colmajor, np.float32).reshape(16)),
+                   int(pixel_stride))
     tv = np.ascontiguousarray(tri_verts, np.float32)
     tn = np.ascontiguousarray(tri_n, np.float32)
     tm = np.ascontiguousarray(tri_mat, np.int32)

@@ -1009,6 +1009,7 @@ done:
 typedef struct {
     int w, h, spp, depth, S, frame, rr, rr_start;
     float light[3], bg[3], origin[3], radius, max_dist, s2w[16];
+    int pixel_stride;   /* > 1: render only pixels 0, k, 2k, ... (the others are left as given) */
 } or_frame_cfg;
 
 static void dnorm(double v[3]) {
@@ -1245,8 +1246,11 @@ OR_API void or_render_frame_f64(const float* tv, const float* tri_n, const int32
     {
         double* ga = want_grad ? (double*)calloc(6 * (size_t)nmat, sizeof(double)) : NULL;
         double* ge = ga ? ga + 3 * nmat : NULL;
+        const int64_t stride = c->pixel_stride > 1 ? c->pixel_stride : 1;
+        const int64_t nsel = (np + stride - 1) / stride;
 #pragma omp for schedule(dynamic, 256)
-        for (int64_t pixel = 0; pixel < np; ++pixel) {
+        for (int64_t sel = 0; sel < nsel; ++sel) {
+            const int64_t pixel = sel * stride;
             uint32_t seed = tea((uint32_t)pixel, (uint32_t)c->frame, 16);
             double acc[3] = {0, 0, 0};
             double g[3] = {0, 0, 0};

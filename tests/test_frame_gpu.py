@@ -197,7 +197,7 @@ def test_frame_deterministic_and_sort_invariant(dev):
         assert np.array_equal(img, base), (sort, reorder, mode)
 
 
-def oracle_frame(orc, env_name, overrides=None, hints=None, first=None):
+def oracle_frame(orc, env_name, overrides=None, hints=None, first=None, pixel_stride=1):
     from vulkanwavefrontpathtracer_amd import stages
     from vulkanwavefrontpathtracer_amd.renderer import env_path
 
@@ -220,7 +220,7 @@ def oracle_frame(orc, env_name, overrides=None, hints=None, first=None):
         bg=vec(cfg["Scene"]["backgroundcolor"]), origin=vec(cam["position"]),
         radius=R["shadowRadius"], max_dist=cam["farPlane"], s2w_colmajor=S2W.T.reshape(-1),
         rr=R.get("russianRoulette", False), rr_start=R.get("russianRouletteStart", 3), hints=hints,
-        first_disagreement=first)
+        first_disagreement=first, pixel_stride=pixel_stride)
 
 
 def frame_rmse(gpu, ref, amb):
@@ -306,6 +306,35 @@ def test_frame_rmse_vs_fp64(dev, orc, env, overrides):
     if env in ALL_PIXELS:
         assert rmse_all <= RMSE_TOL, rmse_all
     assert float(np.mean(ref[:, :3])) > 0.05      # a lit image, not a trivial match
+
+
+def test_frame_rmse_city_full_resolution(dev, orc):
+    """The headline workload's own frame (the city, 1920x1080, 4 spp, 8
+    bounces, its sort and key settings) against the fp64 oracle over every
+    pixel, the replay taking the GPU's logged choices as in
+    test_frame_rmse_vs_fp64: RMSE <= 1e-3 over all pixels, no pixel off by
+    more than 1e-3, and disagreements beyond the fp32 bound on at most 0.1%
+    of them."""
+    stride = 1
+    r = render({}, "config2.json", decisions=True)
+    gpu = r.pixels()
+    hints = r.decisions()
+    n = gpu.shape[0]
+    assert n == 1920 * 1080
+    first = np.full((n, 4), -1, np.int32)
+    ref, amb, follow = oracle_frame(orc, "config2.json", hints=hints, first=first, pixel_stride=stride)
+    sel = np.arange(0, n, stride)
+    err = gpu[sel, :3].astype(np.float64) - ref[sel, :3]
+    rmse = float(np.sqrt(np.mean(err ** 2)))
+    outside = first[sel, 0] >= 0
+    print(f"city 1920x1080 x 4 spp: RMSE {rmse:.3e} over {len(sel)} pixels (max "
+          f"{np.abs(err).max():.3e}); GPU choices taken at {int(np.count_nonzero(follow[sel] & 1))} "
+          f"pixels within the fp32 bound, {int(outside.sum())} beyond it")
+    assert rmse <= RMSE_TOL, rmse
+    assert np.abs(err).max() <= RMSE_TOL
+    assert outside.mean() <= 0.001
+    assert not (follow[sel] & 4).any()
+    assert float(np.mean(ref[sel, :3])) > 0.05
 
 
 def test_frame_closed_sphere(dev):

@@ -558,6 +558,19 @@ def test_frame_f64_closed_sphere_closed_form(orc, depth):
     assert np.mean(k == 0) > 0.97
 
 
+def test_frame_f64_pixel_sample_equals_full_frame(orc):
+    """pixel_stride k renders exactly the pixels 0, k, 2k, ... of the full
+    frame (same values and flags) and leaves the others as given."""
+    W, H = 40, 24
+    hs, geo, kw = _boxes_frame_args(orc, W, H, depth=3, spp=2)
+    full, amb = orc.render_frame_f64(*geo, hs.albedo, hs.emission, **kw)
+    part, amb_p = orc.render_frame_f64(*geo, hs.albedo, hs.emission, **kw, pixel_stride=7)
+    sel = np.arange(0, W * H, 7)
+    rest = np.setdiff1d(np.arange(W * H), sel)
+    assert np.array_equal(part[sel], full[sel]) and np.array_equal(amb_p[sel], amb[sel])
+    assert not part[rest].any() and not amb_p[rest].any()
+
+
 def test_frame_f64_ambiguity_flags_grazing_paths(orc):
     """A ray that passes exactly through the shared vertex/edge of two
     triangles is flagged; one through a triangle's interior is not."""
