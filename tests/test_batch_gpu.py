@@ -150,11 +150,35 @@ def test_batch_views_match_fp64_oracle(dev, orc, small_batch):
         assert float(ref[:, :3].mean()) > 0.05
 
 
-def test_batch_full_workload_equals_single_view_frames(dev):
+def replay_rmse(orc, r, hs, v, depth, S, stride=7):
+    """RMSE of a rendered single-view frame (decision log on) against the
+    fp64 oracle's replay of every `stride`-th pixel, and that pixel count."""
+    from vulkanwavefrontpathtracer_amd import stages
+
+    _, _, s2w = stages.camera_matrices(list(v.position), list(v.rotation), v.fov,
+                                       v.width / v.height, v.znear, v.zfar)
+    n = v.width * v.height
+    first = np.full((n, 4), -1, np.int32)
+    ref, _, follow = orc.render_frame_f64(
+        hs.tri_verts(), orc.tri_normals(hs.vertices, hs.indices), hs.tri_material, hs.albedo,
+        hs.emission, w=v.width, h=v.height, spp=v.spp, depth=depth, S=S, frame=1,
+        light=list(v.light), bg=list(v.background), origin=list(v.position),
+        radius=v.shadow_radius, max_dist=v.zfar, s2w_colmajor=s2w.T.reshape(-1),
+        hints=r.decisions(), first_disagreement=first, pixel_stride=stride)
+    sel = np.arange(0, n, stride)
+    err = r.pixels()[sel, :3].astype(np.float64) - ref[sel, :3]
+    assert (first[sel, 0] >= 0).mean() <= 0.001
+    return float(np.sqrt(np.mean(err ** 2))), len(sel)
+
+
+def test_batch_full_workload_equals_single_view_frames(dev, orc):
     """The whole batched workload (64 boxes scenes with 1k-500k triangles,
     views of 128^2-1024^2 at 1-16 spp, 6 bounces, 2 shadow samples) in one
     queue with the graph on: every view equals the single-view frame loop's
-    frame of its scene and camera bit for bit, and the ray counts add up."""
+    frame of its scene and camera bit for bit, and the ray counts add up.
+    The largest view and the view with the most samples per pixel are also
+    replayed by the fp64 oracle (every 7th pixel, taking the frame loop's
+    logged choices): RMSE <= 1e-3 over all of those pixels."""
     import json
 
     from vulkanwavefrontpathtracer_amd import stages
@@ -177,6 +201,8 @@ def test_batch_full_workload_equals_single_view_frames(dev):
     assert st["pixels"] == sum(v.width * v.height for v in views)
     assert st["primary_rays"] == sum(v.width * v.height * v.spp for v in views)
     cfg = json.loads(env.read_text())
+    sizes = [v.width * v.height for v in views]
+    replay = {int(np.argmax(sizes)), int(np.argmax([v.spp * 4096 + v.width for v in views]))}
     for k, v in enumerate(views):
         c = json.loads(json.dumps(cfg))
         c["Resolution"] = {"width": v.width, "height": v.height}
@@ -184,6 +210,13 @@ def test_batch_full_workload_equals_single_view_frames(dev):
                               "recursionDepth": depth, "rayType": "path", "russianRoulette": False})
         r = Renderer(c)
         r.set_scene(scenes[v.scene], hosts[v.scene].bmin, hosts[v.scene].bmax)
+        checked = k in replay
+        r.set_decision_log(checked)
         r.render()
         assert np.array_equal(b.pixels(k)[:, :3], r.pixels()[:, :3]), (k, v.width, v.spp)
+        if checked:
+            rmse, npx = replay_rmse(orc, r, hosts[v.scene], v, depth, s)
+            print(f"[batch] view {k} ({v.width}x{v.height}, {v.spp} spp): fp64 replay RMSE "
+                  f"{rmse:.3e} over {npx} pixels", flush=True)
+            assert rmse <= 1e-3, (k, rmse)
         del r
