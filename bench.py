@@ -321,6 +321,22 @@ def run_workload(dev, args, env_name, ablations=True, overrides=None, key_mode=N
     torch.cuda.synchronize()
     nodes_per_ray, tris_per_ray = nv.double().mean().item(), nt.double().mean().item()
     del nv, nt, scratch_hits
+    # a library reference point for the sort of the largest bounce: torch's
+    # stable sort of the same keys (rocPRIM's radix sort on this image),
+    # event-timed like the stages
+    lib_keys, _ = p.stages.morton_keys(waves[big]["rays"], mode, list(p.mn3), list(p.mx3))
+    torch.sort(lib_keys, stable=True)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    lib_ms = []
+    for _ in range(5):
+        ev0.record(stream)
+        torch.sort(lib_keys, stable=True)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        lib_ms.append(ev0.elapsed_time(ev1))
+    lib_sort_ms = min(lib_ms)
+    del lib_keys
     sort_big_ms = per_b[big]["sort"]
     sort_gbs = nbig * bpe["sort"] / (sort_big_ms / 1e3) / 1e9
     out = {
@@ -335,7 +351,12 @@ def run_workload(dev, args, env_name, ablations=True, overrides=None, key_mode=N
         "sort_largest_bounce": {"pairs": nbig, "ms": round(sort_big_ms, 4),
                                 "bytes_per_pair": bpe["sort"], "achieved_GBs": round(sort_gbs, 1),
                                 "frac_hbm": round(sort_gbs / HBM_PEAK_GBS, 4),
-                                "key_bits": p.kbits, "digit_bits": p.digit_bits},
+                                "key_bits": p.kbits, "digit_bits": p.digit_bits,
+                                "library_reference": {
+                                    "what": "torch.sort(keys, stable=True) of the same keys (values and "
+                                            "indices out; rocPRIM radix sort)",
+                                    "ms": round(lib_sort_ms, 4),
+                                    "achieved_GBs_same_basis": round(nbig * bpe["sort"] / (lib_sort_ms / 1e3) / 1e9, 1)}},
         "dominant_launch": {"kernel": "trace_kernel<reorder>" if p.fused else "trace_kernel",
                             "elems": nbig, "ms": per_b[big]["trace"], "bytes_per_elem": bpe["trace"],
                             "unsorted_ms": per_b[big]["trace_unsorted"],
