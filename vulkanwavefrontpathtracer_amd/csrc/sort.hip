@@ -30,6 +30,17 @@ constexpr int SORT_WAVES = SORT_BLOCK / 64;
 constexpr int SORT_ITEMS = 8;
 constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
 constexpr int SORT_MAX_PASSES = 16;
+// Tile counts are stored in chunks of SORT_CHUNK consecutive tiles: within a
+// chunk digit-major, so a digit's counts of the chunk's tiles share one
+// 64-byte line.  The upsweep and the downsweep hand each XCD runs of
+// SORT_CHUNK consecutive tiles (xcd_block), so the lines of a chunk, and the
+// neighbouring digit runs its tiles write, stay in one XCD's L2.
+constexpr int SORT_CHUNK = 16;
+
+template <int RADIX>
+__host__ __device__ __forceinline__ int64_t count_index(int d, int64_t tile) {
+    return (tile / SORT_CHUNK) * (int64_t)(RADIX * SORT_CHUNK) + (int64_t)d * SORT_CHUNK + tile % SORT_CHUNK;
+}
 
 inline int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
@@ -56,8 +67,8 @@ static bool make_plan(int64_t n, int key_bytes, int key_bits, int digit_bits, So
     p.radix = 1 << digit_bits;
     p.tiles = (n + SORT_TILE - 1) / SORT_TILE;
     int64_t o = 0;
-    p.off_counts = o;  // tile counts of one pass, digit-major, scanned in place
-    o += align256(sizeof(uint32_t) * p.tiles * p.radix);
+    p.off_counts = o;  // tile counts of one pass, in chunks of SORT_CHUNK tiles, scanned in place
+    o += align256(sizeof(uint32_t) * ((p.tiles + SORT_CHUNK - 1) / SORT_CHUNK) * SORT_CHUNK * p.radix);
     p.off_totals = o;  // digit totals of one pass
     o += align256(sizeof(uint32_t) * p.radix);
     p.off_keys_a = o;
@@ -131,23 +142,26 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, bool ok) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Upsweep: digit counts of one tile, written digit-major (counts[d * tiles
-// + tile]) so that the spine scans contiguous rows.  Tiles past the element
-// count write zeros.
+// Upsweep: digit counts of one tile, written to count_index(d, tile).
+// Tiles past the element count write zeros.  Lanes count into COPIES
+// histograms per wave (lane & (COPIES - 1)), so a digit shared by many keys
+// (skewed high digits) costs at most 64 / COPIES same-address LDS atomics per
+// instruction.
 template <typename K, int DBITS>
 __global__ __launch_bounds__(SORT_BLOCK) void sort_upsweep_kernel(const K* __restrict__ keys,
                                                                   int64_t n, int shift, int width,
                                                                   uint32_t* __restrict__ counts,
-                                                                  int64_t tiles,
                                                                   const uint32_t* __restrict__ d_n) {
     constexpr int RADIX = 1 << DBITS;
-    __shared__ uint32_t s_cnt[SORT_WAVES][RADIX];
+    constexpr int COPIES = DBITS <= 8 ? 4 : 1;
+    __shared__ uint32_t s_cnt[SORT_WAVES * COPIES][RADIX];
     const unsigned tid = threadIdx.x, wave = tid >> 6;
-    for (int i = tid; i < SORT_WAVES * RADIX; i += SORT_BLOCK) (&s_cnt[0][0])[i] = 0;
+    const unsigned hsel = wave * COPIES + (lane_id() & (COPIES - 1));
+    for (int i = tid; i < SORT_WAVES * COPIES * RADIX; i += SORT_BLOCK) (&s_cnt[0][0])[i] = 0;
     __syncthreads();
     if (d_n) n = min(n, (int64_t)*d_n);
     const uint32_t mask = (1u << width) - 1u;
-    const int64_t tile = blockIdx.x;
+    const int64_t tile = xcd_block(blockIdx.x, gridDim.x, SORT_CHUNK);
     const int64_t base = tile * SORT_TILE;
     if (base < n) {
         K kk[SORT_ITEMS];  // all loads in flight before the counting
@@ -159,36 +173,39 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_upsweep_kernel(const K* __res
 #pragma unroll
         for (int it = 0; it < SORT_ITEMS; ++it) {
             const int64_t idx = base + (int64_t)it * SORT_BLOCK + tid;
-            if (idx < n) atomicAdd(&s_cnt[wave][digit_of(kk[it], shift, mask)], 1u);
+            if (idx < n) atomicAdd(&s_cnt[hsel][digit_of(kk[it], shift, mask)], 1u);
         }
     }
     __syncthreads();
     for (int d = tid; d < RADIX; d += SORT_BLOCK) {
         uint32_t c = 0;
 #pragma unroll
-        for (int w = 0; w < SORT_WAVES; ++w) c += s_cnt[w][d];
-        counts[(int64_t)d * tiles + tile] = c;
+        for (int w = 0; w < SORT_WAVES * COPIES; ++w) c += s_cnt[w][d];
+        counts[count_index<RADIX>(d, tile)] = c;
     }
 }
 
-// Spine: one workgroup per digit scans its row of tile counts in place
-// (exclusive) and writes the row total, the digit's count in the pass.
+// Spine: one workgroup per digit scans its tile counts in tile order, in
+// place (exclusive), and writes the total, the digit's count in the pass.
 constexpr int SPINE_BLOCK = 512;
+static_assert(SPINE_BLOCK % SORT_CHUNK == 0, "spine rows cover whole chunks");
+template <int RADIX>
 __global__ __launch_bounds__(SPINE_BLOCK) void sort_spine_kernel(uint32_t* __restrict__ counts,
                                                                  int64_t tiles,
                                                                  uint32_t* __restrict__ totals) {
     __shared__ uint32_t s_tmp[SPINE_BLOCK / 64];
-    uint32_t* row = counts + (int64_t)blockIdx.x * tiles;
+    const int d = blockIdx.x;
     uint32_t carry = 0;
     for (int64_t base = 0; base < tiles; base += SPINE_BLOCK) {
         const int64_t i = base + threadIdx.x;
-        const uint32_t v = i < tiles ? row[i] : 0u;
+        uint32_t* slot = counts + count_index<RADIX>(d, i);
+        const uint32_t v = i < tiles ? *slot : 0u;
         uint32_t total;
         const uint32_t ex = block_excl_scan<SPINE_BLOCK / 64>(v, s_tmp, &total);
-        if (i < tiles) row[i] = carry + ex;
+        if (i < tiles) *slot = carry + ex;
         carry += total;
     }
-    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+    if (threadIdx.x == 0) totals[d] = carry;
 }
 
 // Downsweep: stable ranking of a tile, staged in LDS in digit order, each
@@ -197,7 +214,7 @@ template <typename K, int DBITS>
 __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
     const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     K* __restrict__ keys_out, uint32_t* __restrict__ vals_out, int64_t n, int shift, int width,
-    const uint32_t* __restrict__ tile_offsets, int64_t tiles, const uint32_t* __restrict__ totals,
+    const uint32_t* __restrict__ tile_offsets, const uint32_t* __restrict__ totals,
     const uint32_t* __restrict__ d_n) {
     constexpr int RADIX = 1 << DBITS;
     constexpr int DPT = RADIX >= SORT_BLOCK ? RADIX / SORT_BLOCK : 1;  // digits per thread
@@ -212,7 +229,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
     const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const uint32_t mask = (1u << width) - 1u;
     if (d_n) n = min(n, (int64_t)*d_n);
-    const int64_t tile = blockIdx.x;
+    const int64_t tile = xcd_block(blockIdx.x, gridDim.x, SORT_CHUNK);
     const int64_t base = tile * SORT_TILE;
     if (base >= n) return;  // uniform: surplus tiles of a capacity-sized grid
     for (int i = tid; i < SORT_WAVES * RADIX; i += SORT_BLOCK) (&s_wave[0][0])[i] = 0;
@@ -223,7 +240,7 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
     for (int k = 0; k < DPT; ++k) {
         const int d = tid * DPT + k;
         tot[k] = d < RADIX ? totals[d] : 0u;
-        toff[k] = d < RADIX ? tile_offsets[(int64_t)d * tiles + tile] : 0u;
+        toff[k] = d < RADIX ? tile_offsets[count_index<RADIX>(d, tile)] : 0u;
     }
     __syncthreads();
 
@@ -754,13 +771,13 @@ static int launch_pass(const void* ki, const uint32_t* vi, void* ko, uint32_t* v
                        const uint32_t* d_n, hipStream_t s) {
     constexpr int RADIX = 1 << DBITS;
     sort_upsweep_kernel<K, DBITS><<<(unsigned)tiles, SORT_BLOCK, 0, s>>>(
-        static_cast<const K*>(ki), n, shift, width, counts, tiles, d_n);
+        static_cast<const K*>(ki), n, shift, width, counts, d_n);
     WF_TRY(launch_check("sort_upsweep_kernel"));
-    sort_spine_kernel<<<RADIX, SPINE_BLOCK, 0, s>>>(counts, tiles, totals);
+    sort_spine_kernel<RADIX><<<RADIX, SPINE_BLOCK, 0, s>>>(counts, tiles, totals);
     WF_TRY(launch_check("sort_spine_kernel"));
     sort_bin_kernel<K, DBITS><<<(unsigned)tiles, SORT_BLOCK, 0, s>>>(
-        static_cast<const K*>(ki), vi, static_cast<K*>(ko), vo, n, shift, width, counts, tiles,
-        totals, d_n);
+        static_cast<const K*>(ki), vi, static_cast<K*>(ko), vo, n, shift, width, counts, totals,
+        d_n);
     return launch_check("sort_bin_kernel");
 }
 
