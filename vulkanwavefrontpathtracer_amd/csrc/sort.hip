@@ -52,8 +52,11 @@ struct SortPlan {
     int64_t off_counts = 0, off_totals = 0;
     int64_t off_keys_a = 0, off_vals_a = 0, off_keys_b = 0, off_vals_b = 0;
     // onesweep: digit histograms of all passes, per-pass tile counters and
-    // two look-back status arrays (tiles x radix), cleared as one region
-    int64_t off_hist = 0, off_ctr = 0, off_status = 0, status_words = 0, clear_words = 0;
+    // two look-back regions, each a status array (tiles x radix) followed by
+    // the chunk sums (chunks x radix); the histograms, the counters and the
+    // first region are cleared as one span
+    int64_t off_hist = 0, off_ctr = 0, off_status = 0, status_words = 0, csum_words = 0,
+            region_words = 0, clear_words = 0;
     int64_t total = 0;
 };
 
@@ -83,8 +86,10 @@ static bool make_plan(int64_t n, int key_bytes, int key_bits, int digit_bits, So
     p.off_ctr = o + sizeof(uint32_t) * (int64_t)p.passes * p.radix;
     p.off_status = p.off_ctr + sizeof(uint32_t) * SORT_MAX_PASSES;
     p.status_words = p.tiles * p.radix;
-    p.clear_words = ((int64_t)p.passes * p.radix + SORT_MAX_PASSES) + p.status_words;
-    o = align256(p.off_status + sizeof(uint32_t) * 2 * p.status_words);
+    p.csum_words = (p.tiles + SORT_CHUNK - 1) / SORT_CHUNK * p.radix;
+    p.region_words = p.status_words + p.csum_words;
+    p.clear_words = ((int64_t)p.passes * p.radix + SORT_MAX_PASSES) + p.region_words;
+    o = align256(p.off_status + sizeof(uint32_t) * 2 * p.region_words);
     p.total = o;
     return true;
 }
@@ -363,6 +368,11 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_bin_kernel(
 // (few tiles).
 
 // Digit histograms of every pass (hist[pass * RADIX + digit]) in one read.
+// A workgroup counts tiles blockIdx.x, + gridDim.x, ... into COPIES LDS
+// histograms (lane & (COPIES - 1): a digit shared by many keys meets fewer
+// same-address atomics) and adds them into hist once, so the global atomics
+// per address are the grid size, not the tile count.
+constexpr int HIST_GRID_MAX = 512;
 template <typename K, int DBITS>
 __global__ __launch_bounds__(SORT_BLOCK) void sort_hist_all_kernel(const K* __restrict__ keys,
                                                                    int64_t n, int key_bits,
@@ -370,51 +380,123 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_hist_all_kernel(const K* __re
                                                                    uint32_t* __restrict__ hist,
                                                                    const uint32_t* __restrict__ d_n) {
     constexpr int RADIX = 1 << DBITS;
-    constexpr int MAXP = (64 + DBITS - 1) / DBITS;
-    __shared__ uint32_t s_h[MAXP * RADIX];
+    constexpr int MAXP = (8 * (int)sizeof(K) + DBITS - 1) / DBITS;
+    constexpr int ONE = MAXP * RADIX * 4;  // bytes of one histogram copy
+    constexpr int COPIES = ONE * 8 <= 32768 ? 8 : ONE * 4 <= 32768 ? 4 : ONE * 2 <= 32768 ? 2 : 1;
+    __shared__ uint32_t s_h[COPIES][MAXP * RADIX];
     const unsigned tid = threadIdx.x;
-    for (int i = tid; i < passes * RADIX; i += SORT_BLOCK) s_h[i] = 0;
+    uint32_t* mine = s_h[lane_id() & (COPIES - 1)];
+    for (int i = tid; i < COPIES * MAXP * RADIX; i += SORT_BLOCK) (&s_h[0][0])[i] = 0;
     __syncthreads();
     if (d_n) n = min(n, (int64_t)*d_n);
-    const int64_t base = (int64_t)blockIdx.x * SORT_TILE;
-#pragma unroll 4
-    for (int it = 0; it < SORT_ITEMS; ++it) {
-        const int64_t idx = base + (int64_t)it * SORT_BLOCK + tid;
-        if (idx < n) {
-            const K k = keys[idx];
+    for (int64_t base = (int64_t)blockIdx.x * SORT_TILE; base < n; base += (int64_t)gridDim.x * SORT_TILE) {
+        K kk[SORT_ITEMS];  // all loads of the tile in flight before the counting
+#pragma unroll
+        for (int it = 0; it < SORT_ITEMS; ++it) {
+            const int64_t idx = base + (int64_t)it * SORT_BLOCK + tid;
+            kk[it] = idx < n ? keys[idx] : K(0);
+        }
+#pragma unroll
+        for (int it = 0; it < SORT_ITEMS; ++it) {
+            if (base + (int64_t)it * SORT_BLOCK + tid >= n) continue;
             for (int p = 0; p < passes; ++p) {
                 const int shift = p * DBITS;
                 const int width = min(DBITS, key_bits - shift);
-                atomicAdd(&s_h[p * RADIX + digit_of(k, shift, (1u << width) - 1u)], 1u);
+                atomicAdd(&mine[p * RADIX + digit_of(kk[it], shift, (1u << width) - 1u)], 1u);
             }
         }
     }
     __syncthreads();
-    for (int i = tid; i < passes * RADIX; i += SORT_BLOCK)
-        if (s_h[i]) atomicAdd(&hist[i], s_h[i]);
+    for (int i = tid; i < passes * RADIX; i += SORT_BLOCK) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k < COPIES; ++k) c += s_h[k][i];
+        if (c) atomicAdd(&hist[i], c);
+    }
 }
 
-// Exclusive count of digit d over the tiles before `tile`: walks back over
-// the status words (AGG: that tile's count, keep walking; INC: the inclusive
-// count up to that tile, stop), four words per round, waiting on a word not
-// yet published.
-__device__ __forceinline__ uint32_t onesweep_lookback(const uint32_t* __restrict__ status,
-                                                      int64_t tile, int radix, int d) {
-    uint32_t excl = 0;
-    int64_t k = tile - 1;
-    while (true) {
-        uint32_t v[4];
+// Exclusive count of every digit over the tiles before `tile`, without a
+// chain through the earlier tiles' prefixes: a tile publishes its digit
+// counts (status word: LB_FLAG_AGG | count) and adds them into its chunk's
+// sums (csum word: tiles added << 24 | count sum), so the prefix is the sum
+// of the earlier whole chunks (each complete once its SORT_CHUNK tiles have
+// added) plus the counts of the earlier tiles of the tile's own chunk.  The
+// workgroup reads those rows in batches of LB_BATCH rows per thread, every
+// load of a batch in flight before any is used; a word not yet complete is
+// read again (bounded, so the kernel always ends).  Thread tid takes digit
+// tid % RADIX and rows tid / RADIX, + SORT_BLOCK / RADIX, ...; the result is
+// valid on threads tid < RADIX.  Tiles take their index from a counter, so
+// every earlier tile's workgroup is already running or done.
+constexpr int LB_BATCH = 8;
+constexpr uint32_t CSUM_ONE = 1u << 24;
+constexpr uint32_t CSUM_MASK = CSUM_ONE - 1u;
+constexpr int LB_SPIN_MAX = 1 << 18;
+
+template <int RADIX>
+__device__ __forceinline__ uint32_t chunk_lookback(const uint32_t* __restrict__ status,
+                                                   int64_t status_words, int64_t tile,
+                                                   uint32_t* s_part) {
+    static_assert(RADIX <= SORT_BLOCK && SORT_BLOCK % RADIX == 0, "digits per thread <= 1");
+    constexpr int H = SORT_BLOCK / RADIX;
+    const int tid = (int)threadIdx.x, d = tid % RADIX, h = tid / RADIX;
+    const int c = (int)(tile / SORT_CHUNK);
+    const int rows = c + (int)(tile % SORT_CHUNK);
+    // word offsets from `status` (the region is below 2^32 bytes): chunk row
+    // r < c at the chunk sums, else tile SORT_CHUNK * c + (r - c)
+    const uint32_t csum0 = (uint32_t)status_words + (uint32_t)d;
+    const uint32_t own0 = (uint32_t)(c * SORT_CHUNK - c) * RADIX + (uint32_t)d;
+    uint32_t sum = 0;
+    for (int r0 = h; r0 < rows; r0 += LB_BATCH * H) {
+        uint32_t w[LB_BATCH];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            v[j] = (k - j >= 0) ? lb_load(&status[(k - j) * radix + d]) : LB_FLAG_INC;
-        int j = 0;
-        for (; j < 4; ++j) {
-            if ((v[j] & ~LB_MASK) == 0u) break;
-            excl += v[j] & LB_MASK;
-            if (v[j] & LB_FLAG_INC) return excl;
+        for (int b = 0; b < LB_BATCH; ++b) {
+            const int r = r0 + b * H;
+            w[b] = r < rows ? lb_load(status + (r < c ? csum0 : own0) + (uint32_t)r * RADIX) : 0u;
         }
-        k -= j;
+        bool done = true;
+#pragma unroll
+        for (int b = 0; b < LB_BATCH; ++b) {
+            const int r = r0 + b * H;
+            if (r < c) {
+                done &= (w[b] >> 24) == (uint32_t)SORT_CHUNK;
+                sum += w[b] & CSUM_MASK;
+            } else if (r < rows) {
+                done &= (w[b] & LB_FLAG_AGG) != 0u;
+                sum += w[b] & LB_MASK;
+            }
+        }
+        if (!done) {
+            // some earlier tile has not published yet: take the incomplete
+            // words again, one at a time, until they are complete
+#pragma unroll
+            for (int b = 0; b < LB_BATCH; ++b) {
+                const int r = r0 + b * H;
+                if (r >= rows) continue;
+                const uint32_t* q = status + (r < c ? csum0 : own0) + (uint32_t)r * RADIX;
+                uint32_t v = w[b];
+                if (r < c) {
+                    sum -= v & CSUM_MASK;
+                    for (int spin = 0; (v >> 24) != (uint32_t)SORT_CHUNK && spin < LB_SPIN_MAX; ++spin)
+                        v = lb_load(q);
+                    sum += v & CSUM_MASK;
+                } else {
+                    sum -= v & LB_MASK;
+                    for (int spin = 0; (v & LB_FLAG_AGG) == 0u && spin < LB_SPIN_MAX; ++spin)
+                        v = lb_load(q);
+                    sum += v & LB_MASK;
+                }
+            }
+        }
     }
+    s_part[tid] = sum;
+    __syncthreads();
+    uint32_t excl = 0;
+    if (tid < RADIX) {
+#pragma unroll
+        for (int k = 0; k < H; ++k) excl += s_part[k * RADIX + tid];
+    }
+    __syncthreads();
+    return excl;
 }
 
 template <typename K, int DBITS>
@@ -422,16 +504,18 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_onesweep_kernel(
     const K* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, K* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out, int64_t n, int shift, int width,
     const uint32_t* __restrict__ hist, uint32_t* __restrict__ status,
-    uint32_t* __restrict__ status_next, uint32_t* __restrict__ tile_counter,
+    uint32_t* __restrict__ status_next, int64_t status_words, uint32_t* __restrict__ tile_counter,
     const uint32_t* __restrict__ d_n) {
     constexpr int RADIX = 1 << DBITS;
-    constexpr int DPT = RADIX >= SORT_BLOCK ? RADIX / SORT_BLOCK : 1;  // digits per thread
+    static_assert(RADIX <= SORT_BLOCK, "one digit per thread");
+    constexpr int DPT = 1;  // digits per thread
     __shared__ uint32_t s_wave[SORT_WAVES][RADIX];
     __shared__ uint32_t s_lstart[RADIX];
     __shared__ uint32_t s_goff[RADIX];
     __shared__ K s_stage[SORT_TILE];
     __shared__ uint32_t s_tmp[SORT_WAVES];
     __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_part[SORT_BLOCK];  // look-back partial sums
 
     const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const uint32_t mask = (1u << width) - 1u;
@@ -439,9 +523,16 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_onesweep_kernel(
     for (int i = tid; i < SORT_WAVES * RADIX; i += SORT_BLOCK) (&s_wave[0][0])[i] = 0;
     __syncthreads();
     const int64_t tile = s_tile;
-    // the next pass's status words of this tile index start at zero
-    if (status_next)
+    // each look-back region: status words, then the chunk sums
+    uint32_t* csum = status + status_words;
+    // the next pass's status words of this tile index, and its chunk's sums
+    // (cleared by the chunk's first tile), start at zero
+    if (status_next) {
         for (int d = tid; d < RADIX; d += SORT_BLOCK) status_next[tile * RADIX + d] = 0u;
+        if (tile % SORT_CHUNK == 0)
+            for (int d = tid; d < RADIX; d += SORT_BLOCK)
+                status_next[status_words + tile / SORT_CHUNK * RADIX + d] = 0u;
+    }
     if (d_n) n = min(n, (int64_t)*d_n);
     const int64_t base = tile * SORT_TILE;
     if (base >= n) return;  // uniform: tiles past the count
@@ -491,8 +582,11 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_onesweep_kernel(
                 s_wave[w][d] = sum;
                 sum += c;
             }
-            // publish this tile's count early so later tiles can walk past it
-            lb_store(&status[tile * RADIX + d], (tile == 0 ? LB_FLAG_INC : LB_FLAG_AGG) | sum);
+            // publish this tile's counts for the later tiles of its chunk and
+            // add them into the chunk's sums for the later chunks
+            lb_store(&status[tile * RADIX + d], LB_FLAG_AGG | sum);
+            __hip_atomic_fetch_add(&csum[tile / SORT_CHUNK * RADIX + d], CSUM_ONE + sum,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         cnt[k] = sum;
         my_total += sum;
@@ -506,23 +600,14 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_onesweep_kernel(
     uint32_t all_total;
     uint32_t dstart = block_excl_scan<SORT_WAVES>(my_tot, s_tmp, &all_total);
     (void)all_total;
+    // tile-local digit starts; the keys are staged in digit order before the
+    // look-back, so their registers are free while it waits
 #pragma unroll
     for (int k = 0; k < DPT; ++k) {
         const int d = tid * DPT + k;
-        if (d < RADIX) {
-            uint32_t excl = 0;
-            if (tile > 0) {
-                excl = onesweep_lookback(status, tile, RADIX, d);
-                lb_store(&status[tile * RADIX + d], LB_FLAG_INC | (excl + cnt[k]));
-            }
-            s_lstart[d] = lstart;
-            s_goff[d] = dstart + excl - lstart;
-            lstart += cnt[k];
-            dstart += tot[k];
-        }
+        if (d < RADIX) s_lstart[d] = lstart;
     }
     __syncthreads();
-
     uint32_t pos[SORT_ITEMS];
 #pragma unroll
     for (int it = 0; it < SORT_ITEMS; ++it) {
@@ -533,6 +618,16 @@ __global__ __launch_bounds__(SORT_BLOCK) void sort_onesweep_kernel(
 #pragma unroll
     for (int it = 0; it < SORT_ITEMS; ++it)
         if (wbase + it * 64 + lane < n) s_stage[pos[it]] = key[it];
+    const uint32_t excl = chunk_lookback<RADIX>(status, status_words, tile, s_part);
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        const int d = tid * DPT + k;
+        if (d < RADIX) {
+            s_goff[d] = dstart + excl - lstart;
+            lstart += cnt[k];
+            dstart += tot[k];
+        }
+    }
     __syncthreads();
     uint32_t gpos[SORT_ITEMS];
 #pragma unroll
@@ -813,19 +908,19 @@ static int sort_impl(void* keys, uint32_t* vals, int64_t n, int key_bits, int di
         for (int i = 3; i < p.passes; ++i) seq[len++] = (i % 2 == 1) ? A : C;
     }
 
-    // onesweep where the passes are latency-bound: few tiles, and digits of
-    // at most 8 bits (an 11-bit look-back walks 8 digits per thread)
+    // onesweep up to sort.onesweep_tiles tiles, with digits of at most 8 bits
+    // (its look-back takes one digit per thread)
     const bool onesweep = p.tiles <= tuning().sort_onesweep_tiles && digit_bits <= 8;
     uint32_t* hist = reinterpret_cast<uint32_t*>(scratch + p.off_hist);
     uint32_t* ctr = reinterpret_cast<uint32_t*>(scratch + p.off_ctr);
     uint32_t* status = reinterpret_cast<uint32_t*>(scratch + p.off_status);
     if (onesweep) {
         WF_TRY(fill_u32(hist, p.clear_words, 0u, s));
-        switch (digit_bits) {
-            case 4: sort_hist_all_kernel<K, 4><<<(unsigned)p.tiles, SORT_BLOCK, 0, s>>>(static_cast<const K*>(keys), n, key_bits, p.passes, hist, d_n); break;
-            case 8: sort_hist_all_kernel<K, 8><<<(unsigned)p.tiles, SORT_BLOCK, 0, s>>>(static_cast<const K*>(keys), n, key_bits, p.passes, hist, d_n); break;
-            default: sort_hist_all_kernel<K, 11><<<(unsigned)p.tiles, SORT_BLOCK, 0, s>>>(static_cast<const K*>(keys), n, key_bits, p.passes, hist, d_n); break;
-        }
+        const unsigned hg = (unsigned)std::min<int64_t>(p.tiles, HIST_GRID_MAX);
+        if (digit_bits == 4)
+            sort_hist_all_kernel<K, 4><<<hg, SORT_BLOCK, 0, s>>>(static_cast<const K*>(keys), n, key_bits, p.passes, hist, d_n);
+        else
+            sort_hist_all_kernel<K, 8><<<hg, SORT_BLOCK, 0, s>>>(static_cast<const K*>(keys), n, key_bits, p.passes, hist, d_n);
         WF_TRY(launch_check("sort_hist_all_kernel"));
     }
     for (int pass = 0; pass < p.passes; ++pass) {
@@ -834,15 +929,14 @@ static int sort_impl(void* keys, uint32_t* vals, int64_t n, int key_bits, int di
         const Buf& in = seq[pass];
         const Buf& out = seq[pass + 1];
         if (onesweep) {
-            uint32_t* st = status + (pass & 1) * p.status_words;
-            uint32_t* st_next = pass + 1 < p.passes ? status + ((pass + 1) & 1) * p.status_words : nullptr;
+            uint32_t* st = status + (pass & 1) * p.region_words;
+            uint32_t* st_next = pass + 1 < p.passes ? status + ((pass + 1) & 1) * p.region_words : nullptr;
             const uint32_t* h = hist + (int64_t)pass * p.radix;
             const unsigned g = (unsigned)p.tiles;
-            switch (digit_bits) {
-                case 4: sort_onesweep_kernel<K, 4><<<g, SORT_BLOCK, 0, s>>>(static_cast<const K*>(in.k), in.v, static_cast<K*>(out.k), out.v, n, shift, width, h, st, st_next, ctr + pass, d_n); break;
-                case 8: sort_onesweep_kernel<K, 8><<<g, SORT_BLOCK, 0, s>>>(static_cast<const K*>(in.k), in.v, static_cast<K*>(out.k), out.v, n, shift, width, h, st, st_next, ctr + pass, d_n); break;
-                default: sort_onesweep_kernel<K, 11><<<g, SORT_BLOCK, 0, s>>>(static_cast<const K*>(in.k), in.v, static_cast<K*>(out.k), out.v, n, shift, width, h, st, st_next, ctr + pass, d_n); break;
-            }
+            if (digit_bits == 4)
+                sort_onesweep_kernel<K, 4><<<g, SORT_BLOCK, 0, s>>>(static_cast<const K*>(in.k), in.v, static_cast<K*>(out.k), out.v, n, shift, width, h, st, st_next, p.status_words, ctr + pass, d_n);
+            else
+                sort_onesweep_kernel<K, 8><<<g, SORT_BLOCK, 0, s>>>(static_cast<const K*>(in.k), in.v, static_cast<K*>(out.k), out.v, n, shift, width, h, st, st_next, p.status_words, ctr + pass, d_n);
             WF_TRY(launch_check("sort_onesweep_kernel"));
             continue;
         }
