@@ -1048,7 +1048,7 @@ extern "C" int wf_sort_pairs_count(void* keys, uint32_t* vals, int64_t capacity,
 // loops).  Every call on a GPU shares that GPU's copy of the array.
 namespace wfpt {
 constexpr int64_t NOALLOC_SORT_MAX_PAIRS = int64_t(1) << 20;
-// wf_sort_scratch_bytes at 2^20 pairs peaks at 31,514,880 bytes (u64 keys,
+// wf_sort_scratch_bytes at 2^20 pairs peaks at 31,777,024 bytes (u64 keys,
 // 11-bit digits); the array rounds that up to 32 MiB.
 constexpr int64_t NOALLOC_SORT_SCRATCH_BYTES = int64_t(32) << 20;
 }  // namespace wfpt
