@@ -222,6 +222,50 @@ def test_sort_count_on_device_bit_exact(dev, orc, rank_max, rank, dtype, bits, d
     assert np.array_equal(gk[count:], keys[count:]) and np.array_equal(gv[count:], vals[count:])
 
 
+@pytest.mark.parametrize("tiles", [0, 1 << 20])
+@pytest.mark.parametrize("dtype,bits,digit", [(np.uint32, 32, 8), (np.uint64, 60, 8),
+                                              (np.uint64, 64, 4)])
+@pytest.mark.parametrize("count", [0, 100_000, 300_017])
+def test_sort_count_on_device_passes_bit_exact(dev, orc, onesweep_tiles, rank_max, tiles, dtype,
+                                               bits, digit, count):
+    """wf_sort_pairs_count through reduce-then-scan passes (tiles = 0) and
+    through onesweep passes (tiles = 2^20) on a 300,017-pair capacity (74
+    tiles in 5 chunks of 16): the grid covers the capacity, the tiles past
+    the device count return early but still clear their share of the next
+    pass's look-back words, and the first count pairs get the oracle's stable
+    order.  Two calls on one scratch buffer, which starts out filled with a
+    non-zero byte, so the second finds what the first left."""
+    import ctypes as C
+
+    from vulkanwavefrontpathtracer_amd import _capi
+
+    onesweep_tiles(tiles)
+    rank_max(0)
+    capacity = 300_017
+    rng = np.random.default_rng(count + bits + digit)
+    keys = rng.integers(0, np.iinfo(dtype).max, size=capacity, dtype=dtype, endpoint=True)
+    keys[: count // 3] = keys[0]
+    vals = rng.permutation(capacity).astype(np.uint32)
+    kb = np.dtype(dtype).itemsize
+    tdt = np.int32 if kb == 4 else np.int64
+    tk = torch.from_numpy(keys.view(tdt)).to(dev)
+    tv = torch.from_numpy(vals.view(np.int32)).to(dev)
+    d_count = torch.tensor([count], dtype=torch.int32, device=dev)
+    need = _capi.load().wf_sort_scratch_bytes(capacity, kb, bits, digit)
+    scratch = torch.full((need,), 0x5A, dtype=torch.uint8, device=dev)
+    rk, rv = orc.sort_pairs(keys[:count], vals[:count], bits)
+    for _ in range(2):
+        tk.copy_(torch.from_numpy(keys.view(tdt)))
+        tv.copy_(torch.from_numpy(vals.view(np.int32)))
+        _capi.call("wf_sort_pairs_count", C.c_void_p(tk.data_ptr()), C.c_void_p(tv.data_ptr()),
+                   capacity, C.c_void_p(d_count.data_ptr()), kb, bits, digit,
+                   C.c_void_p(scratch.data_ptr()), need, None)
+        torch.cuda.synchronize()
+        gk, gv = tk.cpu().numpy().view(dtype), tv.cpu().numpy().view(np.uint32)
+        assert np.array_equal(gk[:count], rk) and np.array_equal(gv[:count], rv)
+        assert np.array_equal(gk[count:], keys[count:]) and np.array_equal(gv[count:], vals[count:])
+
+
 def test_sort_rejects_bad_args(dev):
     from vulkanwavefrontpathtracer_amd import _capi, stages
 
