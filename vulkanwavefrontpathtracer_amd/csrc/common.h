@@ -163,7 +163,7 @@ namespace wfpt {
 // Process-wide tuning knobs (wf_set_tuning).
 struct Tuning {
     int bvh_presplit = 1;
-    int bvh_split_depth = 6;
+    int bvh_split_depth = 8;
     int bvh_split_denom = 64;   // split triangles whose box exceeds diagonal / this
     int bvh_leaf_max = 1;
     int bvh_collapse = 1;       // 4-wide nodes: 0 every second level; 1 greedy largest-area
