@@ -299,7 +299,7 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
 /* ----------------------------------------------------------------- tuning */
 /* Process-wide knobs read by later calls (the autotuner sets them):
  *   "bvh.presplit"     0/1  split references of large triangles (default 1)
- *   "bvh.split_depth"  0-8  max binary splits per triangle (default 8)
+ *   "bvh.split_depth"  0-12 max binary splits per triangle (default 8)
  *   "bvh.split_denom"  1-4096 split triangles whose box exceeds the scene
  *                           diagonal / this (default 64; 32-128 measure
  *                           alike on the interior scene, 256+ slower)

@@ -82,10 +82,12 @@ def test_city_hierarchy_structure(dev, city):
     every interior node once.  4-wide nodes: the same, each decoded child box
     contains every whole (unsplit) triangle under it, and the stack bound
     covers every root-to-node path."""
-    from vulkanwavefrontpathtracer_amd import _capi
+    check_hierarchy(*city)
+
+
+def check_hierarchy(hs, sc):
     from test_bvh_build_gpu import binary_tree, decode_wide, download_wide
 
-    hs, sc = city
     info = sc.info()
     assert info["tris"] == hs.num_tris == 2_821_186
     nodes, order = binary_tree(sc)
@@ -160,9 +162,65 @@ def test_city_trace_matches_oracle(dev, orc, city, captured):
     """Closest-hit records bit-exact and any-hit visibility, on 20k random
     rays in the scene box and 20k rays sampled from captured bounces 0, 1
     and 3 of the 8.3M-ray wavefronts."""
+    check_trace(dev, orc, *city, captured)
+
+
+@pytest.fixture(scope="module")
+def city12(dev, city):
+    """The city's hierarchy built with presplit depth 12 (the city env file's
+    Scene.splitDepth; the process default is 8)."""
     from vulkanwavefrontpathtracer_amd import stages
 
-    hs, sc = city
+    hs, _ = city
+    old = stages.get_tuning("bvh.split_depth")
+    stages.set_tuning("bvh.split_depth", 12)
+    try:
+        print("[city] building the depth-12 hierarchy", flush=True)
+        sc = stages.Scene.from_host(hs, dev)
+    finally:
+        stages.set_tuning("bvh.split_depth", old)
+    return hs, sc
+
+
+def test_city_split12_hierarchy_and_trace(dev, orc, city, city12, captured):
+    """The depth-12 presplit hierarchy of the headline workload: the same
+    structure checks, more leaf slots than the default build, and the same
+    oracle-exact closest-hit and any-hit results."""
+    hs, sc12 = city12
+    check_hierarchy(hs, sc12)
+    assert sc12.info()["leaves"] > city[1].info()["leaves"]
+    check_trace(dev, orc, hs, sc12, captured)
+
+
+def test_city_env_split_depth_frame(dev, city):
+    """Scene.splitDepth reaches the renderer's scene build, leaves the
+    process-wide knob as it was, and does not change the frame: one
+    160x90 x 4 spp, 8-bounce city frame with the env file's depth 12 equals
+    the frame with depth 8 bit for bit."""
+    from vulkanwavefrontpathtracer_amd import stages
+    from vulkanwavefrontpathtracer_amd.renderer import Renderer, env_path
+
+    before = stages.get_tuning("bvh.split_depth")
+    frames, leaves = [], []
+    for depth in (12, 8):
+        r = Renderer(env_path("config2.json"), {"Resolution.width": 160, "Resolution.height": 90,
+                                                "Scene.splitDepth": depth})
+        r.build_scene()
+        nt, nn, nl, ng = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int()
+        stages.call("wf_scene_info", r.scene_handle(), C.byref(nt), C.byref(nn), C.byref(nl),
+                    C.byref(ng))
+        leaves.append(nl.value)
+        r.render()
+        frames.append(r.pixels())
+        del r
+    assert stages.get_tuning("bvh.split_depth") == before
+    assert leaves[0] > leaves[1]
+    assert np.array_equal(frames[0], frames[1])
+
+
+def check_trace(dev, orc, hs, sc, captured):
+    from vulkanwavefrontpathtracer_amd import stages
+
     rng = np.random.default_rng(21)
     n = 20000
     r = np.zeros((n, 8), np.float32)

@@ -117,6 +117,7 @@ __global__ __launch_bounds__(256) void tri_prep_kernel(
 // Sutherland-Hodgman clip of polygon `in` (n vertices) to the half space
 // x[axis] <= mid (below) or >= mid (above); returns the vertex count.
 constexpr int SPLIT_MAXV = 12;
+constexpr int SPLIT_MAX_DEPTH = 12;  // largest bvh.split_depth
 __device__ int clip_half(const float (*in)[3], int n, float (*out)[3], int axis, float mid,
                          bool below) {
     int m = 0;
@@ -169,8 +170,10 @@ __global__ __launch_bounds__(128) void split_kernel(
         }
         return;
     }
-    float poly[9][SPLIT_MAXV][3];  // depth-first stack of pieces
-    int nv[9], dep[9];
+    // depth-first stack of pieces: a split replaces the top piece by its two
+    // halves, so at most one piece per depth level plus one is pending
+    float poly[SPLIT_MAX_DEPTH + 1][SPLIT_MAXV][3];
+    int nv[SPLIT_MAX_DEPTH + 1], dep[SPLIT_MAX_DEPTH + 1];
     int sp = 0;
     for (int k = 0; k < 3; ++k) {
         const float* v = reinterpret_cast<const float*>(verts + (int64_t)idx[3 * t + k] * stride);
@@ -194,7 +197,7 @@ __global__ __launch_bounds__(128) void split_kernel(
         int axis = 0;
         for (int k = 1; k < 3; ++k)
             if (hi[k] - lo[k] > hi[axis] - lo[axis]) axis = k;
-        if (hi[axis] - lo[axis] <= limit || d >= maxd || sp + 2 > 9) {
+        if (hi[axis] - lo[axis] <= limit || d >= maxd || sp + 2 > SPLIT_MAX_DEPTH + 1) {
             if (EMIT) {
                 item_box[2 * out] = make_float4(lo[0], lo[1], lo[2], 0.f);
                 item_box[2 * out + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);

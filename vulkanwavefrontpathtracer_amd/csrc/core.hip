@@ -80,7 +80,7 @@ extern "C" int wf_set_tuning(const char* key, int value) {
     std::string k(key);
     Tuning& t = tuning();
     if (k == "bvh.presplit" && (value == 0 || value == 1)) t.bvh_presplit = value;
-    else if (k == "bvh.split_depth" && value >= 0 && value <= 8) t.bvh_split_depth = value;
+    else if (k == "bvh.split_depth" && value >= 0 && value <= 12) t.bvh_split_depth = value;
     else if (k == "bvh.split_denom" && value >= 1 && value <= 4096) t.bvh_split_denom = value;
     else if (k == "bvh.leaf_max" && value >= 1 && value <= 8) t.bvh_leaf_max = value;
     else if (k == "bvh.collapse" && (value == 0 || value == 1)) t.bvh_collapse = value;

@@ -646,8 +646,18 @@ extern "C" int wf_renderer_build_scene(wf_renderer r) {
     WF_HIP(hipMemcpy(mat.p, hs.tri_material.data(), sizeof(int32_t) * hs.tri_material.size(), hipMemcpyHostToDevice));
     wf_scene sc = nullptr;
     wf_stream_t ws = reinterpret_cast<wf_stream_t>(r->stream);
-    WF_TRY(wf_scene_create(v.p, nv, 24, idx.as<uint32_t>(), nt, hs.geom_first.data(),
-                           (int)hs.geom_first.size() - 1, mat.as<int32_t>(), ws, &sc));
+    // Scene.splitDepth: the presplit depth of this scene's hierarchy (the
+    // process-wide bvh.split_depth is restored after the build)
+    int split = -1, split_saved = 0;
+    r->env.get_int("Scene.splitDepth", split);
+    if (split >= 0) {
+        WF_TRY(wf_get_tuning("bvh.split_depth", &split_saved));
+        WF_TRY(wf_set_tuning("bvh.split_depth", split));
+    }
+    const int built = wf_scene_create(v.p, nv, 24, idx.as<uint32_t>(), nt, hs.geom_first.data(),
+                                      (int)hs.geom_first.size() - 1, mat.as<int32_t>(), ws, &sc);
+    if (split >= 0) (void)wf_set_tuning("bvh.split_depth", split_saved);  // a value read back
+    WF_TRY(built);
     int st = wf_scene_set_materials(sc, hs.albedo.data(), hs.emission.data(),
                                     (int)hs.albedo.size() / 3, ws);
     if (st != WF_OK) {
