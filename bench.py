@@ -39,12 +39,14 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-KEY_MODES = {"aila32": 0, "origin30": 1, "origin30_oct30": 2, "origin_dir_morton60": 4}
+KEY_MODES = {"aila32": 0, "origin30": 1, "origin30_oct30": 2, "origin_dir_morton60": 4,
+             "origin_dir_morton30": 5}
 KEY_DESC = {0: "32-bit Aila keys (7 origin + 4 direction bits per axis), u32 key sort + u32 payload",
             1: "30-bit origin Morton keys (10 bits per axis), u32 key sort + u32 payload",
             2: "60-bit keys (30 origin bits then 30 octahedral direction bits), u64 key sort + u32 payload",
             4: "60-bit 5-D Morton keys (30 origin + 30 octahedral direction bits interleaved), "
-               "u64 key sort + u32 payload"}
+               "u64 key sort + u32 payload",
+            5: "30-bit 5-D Morton keys (the top 30 bits of the 60-bit one), u32 key sort + u32 payload"}
 PROFILE_DIR = ROOT / "profiles"
 
 
@@ -141,7 +143,7 @@ class Pipeline:
 
     def set_keys(self, key_mode, digit_bits):
         self.mode = key_mode
-        self.kbytes = 4 if key_mode < 2 else 8
+        self.kbytes = self.lib.wf_key_bytes(key_mode)
         self.kbits = self.stages.key_bits(key_mode)
         self.digit_bits = digit_bits
 
@@ -365,12 +367,14 @@ def run_workload(dev, args, env_name, ablations=True, overrides=None, key_mode=N
     }
     if ablations:
         abl = {}
-        variants = [("every 2nd bounce", mode, args.digit_bits, 2),
-                    ("32-bit Aila keys", 0, args.digit_bits, 1),
-                    ("30-bit origin keys", 1, args.digit_bits, 1),
-                    ("60-bit 5-D Morton keys", 4, args.digit_bits, 1),
-                    ("60-bit concatenated keys", 2, args.digit_bits, 1),
-                    ("11-bit digits", mode, 11, 1)]
+        variants = ablations if isinstance(ablations, list) else [
+            ("every 2nd bounce", mode, args.digit_bits, 2),
+            ("32-bit Aila keys", 0, args.digit_bits, 1),
+            ("30-bit origin keys", 1, args.digit_bits, 1),
+            ("60-bit 5-D Morton keys", 4, args.digit_bits, 1),
+            ("30-bit 5-D Morton keys", 5, args.digit_bits, 1),
+            ("60-bit concatenated keys", 2, args.digit_bits, 1),
+            ("11-bit digits", mode, 11, 1)]
         for name, km, db, period in variants:
             if km == mode and db == args.digit_bits and period == 1:
                 continue                          # the headline itself
@@ -687,11 +691,15 @@ def main():
 
     interior = None
     if not args.no_interior:
-        it = run_workload(dev, args, "config1.json", ablations=False)
+        it = run_workload(dev, args, "config1.json",
+                          ablations=[] if args.no_ablations else
+                          [("30-bit 5-D Morton keys", 5, args.digit_bits, 1)])
         interior = {k: it[k] for k in ("value", "ms_per_step", "rays", "rays_per_bounce",
                                        "per_bounce_reorder_vs_unsorted",
                                        "trace_speedup_sorted_vs_unsorted", "sorted_pipeline_mrays_s",
                                        "unsorted_trace_mrays_s", "stages", "sort_largest_bounce")}
+        if it.get("ablations"):
+            interior["ablations"] = it["ablations"]
         interior["value"] = round(interior["value"], 2)
         interior["workload"] = ("medium plain: interior 263,250 tris, 1920x1080, 1 spp, 8 bounces, "
                                 "RR after bounce 3, 60-bit 5-D Morton keys")

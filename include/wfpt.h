@@ -81,16 +81,23 @@ const char* wf_version(void);
  *                            direction's low 5 levels as (u, v) pairs, u64;
  *                            in the spirit of the reference's interleaved
  *                            Aila32 key (origin and direction bits alternate).
- * keys receive u32 (AILA32, ORIGIN30) or u64 (others); indices[i] = i. */
+ *   WF_KEY_ORIGIN_DIR_MORTON30 : the top 30 bits of ORIGIN_DIR_MORTON60 (6
+ *                            levels of (u, v, x, y, z): direction bits 14..9
+ *                            with origin bits 9..4), u32: the same coherence
+ *                            at half the sort passes.  No counterpart in the
+ *                            reference.
+ * keys receive u32 (AILA32, ORIGIN30, ORIGIN_DIR_MORTON30) or u64 (others);
+ * indices[i] = i. */
 enum {
     WF_KEY_AILA32 = 0,
     WF_KEY_ORIGIN30 = 1,
     WF_KEY_ORIGIN30_OCT30 = 2,
     WF_KEY_SCENE6_ORIGIN30_OCT28 = 3,
-    WF_KEY_ORIGIN_DIR_MORTON60 = 4
+    WF_KEY_ORIGIN_DIR_MORTON60 = 4,
+    WF_KEY_ORIGIN_DIR_MORTON30 = 5
 };
 int wf_key_bytes(int mode);  /* 4 or 8, or WF_ERR_INVALID_ARG */
-int wf_key_bits(int mode);   /* significant bits: 32, 30, 60, 64 */
+int wf_key_bits(int mode);   /* significant bits: 32, 30, 60, 64, 60, 30 */
 int wf_morton_keys(const wf_ray* rays, void* keys, uint32_t* indices, int64_t n,
                    int mode, const float scene_min[3], const float scene_max[3],
                    wf_stream_t stream, float* ms);

@@ -67,6 +67,12 @@ def keys_morton5d60(rays, mn, mx):
     return out
 
 
+def keys_morton5d30(rays, mn, mx):
+    """WF_KEY_ORIGIN_DIR_MORTON30: the top 30 bits of the 60-bit 5-D Morton
+    key (keys.h morton5d30), u32."""
+    return (keys_morton5d60(rays, mn, mx) >> np.uint64(30)).astype(np.uint32)
+
+
 def keys_scene64(rays, scene, mins, maxs):
     r = _rays(rays)
     s = np.ascontiguousarray(scene, np.uint8)

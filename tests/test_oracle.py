@@ -127,6 +127,42 @@ def test_morton5d60_same_bits_as_oct60(orc):
     assert len(np.unique(c[3])) > 1000 and len(np.unique(c[0])) > 500
 
 
+def test_morton5d30_top_levels(orc):
+    """The 30-bit 5-D key holds 6 levels of (u, v, x, y, z): decoding it gives
+    the top 6 bits of each origin coordinate of the concatenated 60-bit key
+    (10 bits per axis) and of each octahedral direction coordinate (15 bits),
+    and it orders rays like the 60-bit 5-D key up to ties."""
+    rng = np.random.default_rng(19)
+    n = 5000
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, :3] = rng.uniform(-0.2, 1.2, (n, 3))
+    d = rng.standard_normal((n, 3))
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    k60 = orc.keys_oct60(rays, UNIT_MIN, UNIT_MAX).astype(np.uint64)
+    k30 = orc.keys_morton5d30(rays, UNIT_MIN, UNIT_MAX).astype(np.uint64)
+    assert k30.max() < (1 << 30)
+
+    def bits(k, positions):
+        out = np.zeros(n, np.uint64)
+        for i, p in enumerate(positions):
+            out |= ((k >> np.uint64(p)) & np.uint64(1)) << np.uint64(i)
+        return out
+
+    # concatenated 60-bit layout: origin x at bit 30 + 3i + 2, direction u at 2i + 1
+    origin = [bits(k60, [30 + 3 * i + 2 - a for i in range(10)]) >> np.uint64(4) for a in range(3)]
+    direc = [bits(k60, [2 * i + 1 - a for i in range(15)]) >> np.uint64(9) for a in range(2)]
+    # 30-bit 5-D layout: levels of (u, v, x, y, z) from the top, x at 5i + 2, u at 5i + 4
+    got = [bits(k30, [5 * i + 2 - a for i in range(6)]) for a in range(3)]
+    got += [bits(k30, [5 * i + 4 - a for i in range(6)]) for a in range(2)]
+    for a in range(3):
+        assert np.array_equal(got[a], origin[a]), a
+    for a in range(2):
+        assert np.array_equal(got[3 + a], direc[a]), a
+    k5 = orc.keys_morton5d60(rays, UNIT_MIN, UNIT_MAX).astype(np.uint64)
+    o = np.argsort(k5, kind="stable")
+    assert np.all(np.diff(k30[o].astype(np.int64)) >= 0)
+
+
 def test_scene64_prefix(orc):
     rays = np.array([ray((0, 0, 0), (0, 0, 1))] * 3, np.float32)
     scene = np.array([0, 5, 63], np.uint8)

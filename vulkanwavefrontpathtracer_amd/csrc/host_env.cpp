@@ -99,7 +99,7 @@ Environment::Environment() {
     reg("Scene.seed", "1000", OPT_INT);
     reg("Scene.materialGroups", "0", OPT_INT);     // > 0: contiguous material groups
     reg("Scene.splitDepth", "-1", OPT_INT);        // >= 0: bvh.split_depth for this scene's build
-    reg("Renderer.keyMode", "aila32", OPT_STRING);  // aila32 | origin30 | origin30_oct30 | origin_dir_morton60
+    reg("Renderer.keyMode", "aila32", OPT_STRING);  // aila32 | origin30 | origin30_oct30 | origin_dir_morton60 | origin_dir_morton30
     reg("Renderer.digitBits", "8", OPT_INT);        // 4 | 8 | 11
     reg("Renderer.reorderPeriod", "1", OPT_INT);    // sort every k-th bounce
     reg("Renderer.russianRouletteStart", "3", OPT_INT);

@@ -231,8 +231,10 @@ static int load_settings(wf_renderer_s* r) {
         else if (km == "origin30") r->key_mode = WF_KEY_ORIGIN30;
         else if (km == "origin30_oct30") r->key_mode = WF_KEY_ORIGIN30_OCT30;
         else if (km == "origin_dir_morton60") r->key_mode = WF_KEY_ORIGIN_DIR_MORTON60;
+        else if (km == "origin_dir_morton30") r->key_mode = WF_KEY_ORIGIN_DIR_MORTON30;
         else return fail(WF_ERR_INVALID_ARG,
-                         "Renderer.keyMode must be aila32, origin30, origin30_oct30 or origin_dir_morton60");
+                         "Renderer.keyMode must be aila32, origin30, origin30_oct30, origin_dir_morton60 "
+                         "or origin_dir_morton30");
     }
     if (r->width <= 0 || r->height <= 0 || (int64_t)r->width * r->height > (int64_t)WF_MAX_ELEMENTS)
         return fail(WF_ERR_INVALID_ARG, "Resolution out of range");

@@ -118,6 +118,12 @@ __device__ __forceinline__ uint64_t morton5d(const KeyFrame& f, float4 o, float4
 __device__ __forceinline__ uint64_t morton5d60(const KeyFrame& f, float4 o, float4 d) {
     return morton5d<15>(f, o, d);
 }
+// Its top 30 bits as a u32 key: 6 levels of (u, v, x, y, z), direction bits
+// 14..9 beside origin bits 9..4.  Sorted in 4 passes of 8-bit digits instead
+// of 8; rays equal in these bits keep their order (the sort is stable).
+__device__ __forceinline__ uint32_t morton5d30(const KeyFrame& f, float4 o, float4 d) {
+    return (uint32_t)(morton5d<15>(f, o, d) >> 30);
+}
 
 __device__ __forceinline__ uint32_t aila32(const KeyFrame& f, float4 o, float4 d) {
     float ax, ay, az;

@@ -30,6 +30,8 @@ __global__ __launch_bounds__(256) void keys_kernel(const wf_ray* __restrict__ ra
         static_cast<uint32_t*>(keys)[i] = origin30(f, o);
     } else if (MODE == WF_KEY_ORIGIN_DIR_MORTON60) {
         static_cast<uint64_t*>(keys)[i] = morton5d60(f, o, d);
+    } else if (MODE == WF_KEY_ORIGIN_DIR_MORTON30) {
+        static_cast<uint32_t*>(keys)[i] = morton5d30(f, o, d);
     } else {
         uint64_t k = ((uint64_t)origin30(f, o) << 30) | (uint64_t)oct_code(d, 15);
         static_cast<uint64_t*>(keys)[i] = k;
@@ -65,7 +67,8 @@ using namespace wfpt;
 extern "C" int wf_key_bytes(int mode) {
     switch (mode) {
         case WF_KEY_AILA32:
-        case WF_KEY_ORIGIN30: return 4;
+        case WF_KEY_ORIGIN30:
+        case WF_KEY_ORIGIN_DIR_MORTON30: return 4;
         case WF_KEY_ORIGIN30_OCT30:
         case WF_KEY_SCENE6_ORIGIN30_OCT28:
         case WF_KEY_ORIGIN_DIR_MORTON60: return 8;
@@ -76,7 +79,8 @@ extern "C" int wf_key_bytes(int mode) {
 extern "C" int wf_key_bits(int mode) {
     switch (mode) {
         case WF_KEY_AILA32: return 32;
-        case WF_KEY_ORIGIN30: return 30;
+        case WF_KEY_ORIGIN30:
+        case WF_KEY_ORIGIN_DIR_MORTON30: return 30;
         case WF_KEY_ORIGIN30_OCT30:
         case WF_KEY_ORIGIN_DIR_MORTON60: return 60;
         case WF_KEY_SCENE6_ORIGIN30_OCT28: return 64;
@@ -89,7 +93,7 @@ extern "C" int wf_morton_keys(const wf_ray* rays, void* keys, uint32_t* indices,
                               wf_stream_t stream, float* ms) {
     WF_CHECK_ARG(n >= 0, "wf_morton_keys: n < 0");
     WF_CHECK_ARG((mode >= WF_KEY_AILA32 && mode <= WF_KEY_ORIGIN30_OCT30) ||
-                     mode == WF_KEY_ORIGIN_DIR_MORTON60,
+                     mode == WF_KEY_ORIGIN_DIR_MORTON60 || mode == WF_KEY_ORIGIN_DIR_MORTON30,
                  "wf_morton_keys: unknown key mode (use wf_morton_keys_batched for scene keys)");
     WF_CHECK_ARG(scene_min && scene_max, "wf_morton_keys: null scene bounds");
     if (n == 0) {
@@ -113,6 +117,9 @@ extern "C" int wf_morton_keys(const wf_ray* rays, void* keys, uint32_t* indices,
             break;
         case WF_KEY_ORIGIN_DIR_MORTON60:
             keys_kernel<WF_KEY_ORIGIN_DIR_MORTON60><<<g, 256, 0, s>>>(rays, keys, indices, n, f);
+            break;
+        case WF_KEY_ORIGIN_DIR_MORTON30:
+            keys_kernel<WF_KEY_ORIGIN_DIR_MORTON30><<<g, 256, 0, s>>>(rays, keys, indices, n, f);
             break;
         default:
             keys_kernel<WF_KEY_ORIGIN30_OCT30><<<g, 256, 0, s>>>(rays, keys, indices, n, f);

@@ -20,6 +20,7 @@ KEY_ORIGIN30 = 1
 KEY_ORIGIN30_OCT30 = 2
 KEY_SCENE6_ORIGIN30_OCT28 = 3
 KEY_ORIGIN_DIR_MORTON60 = 4
+KEY_ORIGIN_DIR_MORTON30 = 5
 
 
 def _ptr(t: torch.Tensor | None) -> C.c_void_p:
