@@ -287,6 +287,54 @@ def test_adjoint_argument_checks_without_gpu():
         _capi.call("wf_renderer_adjoint", r.handle, 1, 64, 64, 64, None, None)
 
 
+def test_stage_entry_points_check_arguments_without_gpu():
+    """Every stage entry point of the C ABI, the way the reference's stage
+    calls get their counts: n = 0 (an empty wavefront) returns WF_OK and
+    touches nothing, n < 0 and null buffers with n > 0 return
+    WF_ERR_INVALID_ARG.  All of it is decided before any device call, so it
+    runs on the CPU (buffer and scene arguments are never dereferenced on
+    these paths; a non-null stand-in marks "present")."""
+    from vulkanwavefrontpathtracer_amd import _capi
+
+    lib = _capi.load()
+    OK, BAD = 0, -1
+    P = C.c_void_p(0x1000)          # present (never dereferenced on these paths)
+    N = None                        # absent
+    f3 = (C.c_float * 3)(0.0, 0.0, 0.0)
+    calls = {
+        "wf_morton_keys": lambda n, b: lib.wf_morton_keys(b, b, b, n, 0, f3, f3, None, None),
+        "wf_sort_pairs": lambda n, b: lib.wf_sort_pairs(b, b, n, 4, 32, 8, b, 1 << 40, None, None),
+        "wf_reorder": lambda n, b: lib.wf_reorder(b, b, P, b, b, b, n, None, None),
+        "wf_trace": lambda n, b: lib.wf_trace(P, b, None, b, n, 1, None, None),
+        "wf_reorder_trace": lambda n, b: lib.wf_reorder_trace(P, b, b, P, b, b, b, b, n, 1, None, None),
+        "wf_trace_stats": lambda n, b: lib.wf_trace_stats(P, b, b, n, 1, b, b, None),
+        "wf_init_seeds": lambda n, b: lib.wf_init_seeds(b, n, 1, None, None),
+        "wf_fill_f32": lambda n, b: lib.wf_fill_f32(b, n, 1.0, None, None),
+        "wf_count_hits": lambda n, b: lib.wf_count_hits(b, n, P, None, None),
+        "wf_interpolate_colors": lambda n, b: lib.wf_interpolate_colors(b, b, n, 1, 0.6, 2.0, None, None),
+        "wf_reconstruct_smooth": lambda n, b: lib.wf_reconstruct_smooth(P, b, b, b, b, b, n, f3, f3, 1,
+                                                                          None, None),
+        "wf_reconstruct_shadow": lambda n, b: lib.wf_reconstruct_shadow(b, b, b, b, b, 0, n, 2, 0,
+                                                                          None, None),
+        "wf_raygen_shadow": lambda n, b: lib.wf_raygen_shadow(b, b, b, b, b, b, b, 0, n, 2, f3, 1.0,
+                                                                None, None),
+    }
+    for name, fn in calls.items():
+        assert fn(0, N) == OK, f"{name}: empty input"
+        assert fn(-1, P) == BAD, f"{name}: negative count"
+        assert fn(5, N) == BAD, f"{name}: null buffers"
+    # in-place reorders are refused
+    assert lib.wf_reorder(P, P, P, P, P, P, 4, None, None) == BAD
+    assert lib.wf_reorder_trace(P, P, P, P, P, P, P, P, 4, 1, None, None) == BAD
+    # unknown key modes and digit widths
+    assert lib.wf_morton_keys(P, P, P, 4, 7, f3, f3, None, None) == BAD
+    assert lib.wf_sort_pairs(P, P, 4, 4, 32, 5, P, 1 << 40, None, None) == BAD
+    assert lib.wf_sort_pairs(P, P, 4, 4, 33, 8, P, 1 << 40, None, None) == BAD
+    # a sort scratch smaller than wf_sort_scratch_bytes
+    need = lib.wf_sort_scratch_bytes(1000, 4, 32, 8)
+    assert lib.wf_sort_pairs(P, P, 1000, 4, 32, 8, P, need - 1, None, None) != OK
+
+
 def test_sized_boxes_and_batched_workload_spec():
     from vulkanwavefrontpathtracer_amd import stages
     from vulkanwavefrontpathtracer_amd.batch import workload
