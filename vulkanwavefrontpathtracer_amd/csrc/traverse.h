@@ -140,8 +140,10 @@ __device__ __forceinline__ float plane_of(uint32_t w01, uint32_t w23, int k) {
 // decoded from their planes (t = q * 2^e / d + (origin - o) / d, one fused
 // multiply-add per plane) and tested against [tmin, best_t].  tn[k] is the
 // entry distance of the k-th nearest hit child (INFINITY for a miss),
-// ref[k] its reference.
-template <bool LDS, int STACK>
+// ref[k] its reference.  Without FULL_ORDER (any-hit queries, which end at
+// the first hit found whatever the order) only slot 0 is sorted: it holds the
+// nearest hit child, or a miss when no child is hit.
+template <bool LDS, int STACK, bool FULL_ORDER = true>
 __device__ __forceinline__ void node4_hits(const TravState<LDS, STACK>& st, float4 a, float4 px,
                                            float4 py, float4 pz, float4 r, float (&tn)[4],
                                            int (&ref)[4]) {
@@ -183,14 +185,16 @@ __device__ __forceinline__ void node4_hits(const TravState<LDS, STACK>& st, floa
     cswap(tn[0], ref[0], tn[1], ref[1]);
     cswap(tn[2], ref[2], tn[3], ref[3]);
     cswap(tn[0], ref[0], tn[2], ref[2]);
-    cswap(tn[1], ref[1], tn[3], ref[3]);
-    cswap(tn[1], ref[1], tn[2], ref[2]);
+    if (FULL_ORDER) {
+        cswap(tn[1], ref[1], tn[3], ref[3]);
+        cswap(tn[1], ref[1], tn[2], ref[2]);
+    }
 }
 
 // TOP: `top` holds the stack's top entry read before this node's pushes (the
 // fused step reads it beside the node fetch); a node that pushes nothing and
 // pops takes it instead of reading LDS again.
-template <int BLOCK, bool LDS, bool STATS, int STACK, bool TOP = false>
+template <int BLOCK, bool LDS, bool STATS, int STACK, bool TOP = false, bool FULL_ORDER = true>
 __device__ __forceinline__ bool trav_node4_data(TravState<LDS, STACK>& st,
                                                 int (&spill)[TravState<LDS, STACK>::SPILL],
                                                 float4 a, float4 px, float4 py, float4 pz, float4 r,
@@ -198,12 +202,13 @@ __device__ __forceinline__ bool trav_node4_data(TravState<LDS, STACK>& st,
     if (STATS) ++st.nodes_visited;
     float tn[4];
     int ref[4];
-    node4_hits(st, a, px, py, pz, r, tn, ref);
+    node4_hits<LDS, STACK, FULL_ORDER>(st, a, px, py, pz, r, tn, ref);
     const int h1 = tn[1] < INFINITY, h2 = tn[2] < INFINITY, h3 = tn[3] < INFINITY;
     if (LDS && __all(st.sp <= LDS_DEPTH - 3)) {
-        // The hit children come first (nearest first), so the pushes can be
-        // three unconditional LDS writes: a missed child's slot is
-        // overwritten by the next write or left above the new top.
+        // Three unconditional LDS writes, each at the top plus the hits
+        // written before it, so the hit children land contiguously whatever
+        // their order: a missed child's slot is overwritten by the next write
+        // or left above the new top.
         lds_int* col = (lds_int*)s_stack + threadIdx.x;
         col[st.sp * BLOCK] = ref[3];
         col[(st.sp + h3) * BLOCK] = ref[2];
@@ -256,7 +261,9 @@ __device__ __forceinline__ bool trav_step4_fused(TravState<LDS, STACK>& st,
     // which then waits for neither LDS nor the fetch's dependents to pop.
     int top = 0;
     if (LDS && st.sp > 0) top = ((const lds_int*)s_stack)[min(st.sp - 1, LDS_DEPTH - 1) * BLOCK + threadIdx.x];
-    if (inner) return !trav_node4_data<BLOCK, LDS, STATS, STACK, true>(st, spill, a, b, c, d, e, s_stack, top);
+    if (inner)
+        return !trav_node4_data<BLOCK, LDS, STATS, STACK, true, CLOSEST>(st, spill, a, b, c, d, e, s_stack,
+                                                                          top);
     const int first = leaf_first(st.node), cnt = leaf_count(st.node);
     if (trav_tri<CLOSEST, LDS, STACK, STATS, RECORD>(st, first, a, b, c, wr)) return true;
     for (int k = 1; k < cnt; ++k) {
