@@ -721,6 +721,9 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(head, args.cpu_sample)
 
+    split_depth = cfg.get("Scene", {}).get("splitDepth", -1)
+    if split_depth < 0:
+        split_depth = stages.get_tuning("bvh.split_depth")
     out = {
         "metric": "Mrays/s (trace+sort+reorder), plain vs batched & adjoint, at RMSE≤1e-3 vs FP64",
         "value": round(head["value"], 2),
@@ -740,6 +743,7 @@ def main():
             "key_mode": {v: k for k, v in KEY_MODES.items()}[head["mode"]],
             "rays_per_step": head["rays"], "bounces": len(head["rays_per_bounce"]),
             "rays_per_bounce": head["rays_per_bounce"], "digit_bits": args.digit_bits,
+            "bvh_split_depth": split_depth,
             "parallelism": "1 GPU",
         },
         "roofline": roofline,
