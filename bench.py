@@ -913,7 +913,7 @@ def tuning_state():
     from vulkanwavefrontpathtracer_amd import stages
 
     out = {}
-    for k in ("bvh.presplit", "bvh.split_depth", "bvh.leaf_max", "bvh.collapse", "trace.block",
+    for k in ("bvh.builder", "bvh.presplit", "bvh.split_depth", "bvh.leaf_max", "bvh.collapse", "trace.block",
               "trace.xcd", "trace.group64", "trace.group16", "trace.group4", "sort.onesweep_tiles",
               "sort.rank_max"):
         out[k] = stages.get_tuning(k)

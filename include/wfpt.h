@@ -311,6 +311,11 @@ int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, wf_stream_t 
  *                           diagonal / this (default 64; 32-128 measure
  *                           alike on the interior scene, 256+ slower)
  *   "bvh.leaf_max"     1-8  triangles per collapsed leaf (default 1)
+ *   "bvh.builder"      0/1  binary hierarchy: 0 radix tree over centroid
+ *                           Morton codes (Karras), 1 top-down binned SAH
+ *                           (32 bins per axis) (default 1: on the city the
+ *                           SAH cost falls from 107 to 72 and the trace
+ *                           takes 20% less time, on the interior 117 to 106)
  *   "bvh.collapse"     0/1  4-wide nodes: 0 collapse every second level,
  *                           1 open the largest-area child until 4 children
  *                           (default 1)

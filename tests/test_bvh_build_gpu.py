@@ -1,5 +1,6 @@
-"""GPU checks of the hierarchy builder.  The binary LBVH (Karras, over
-centroid Morton codes) and both 4-wide collapses (every second level,
+"""GPU checks of the hierarchy builders.  Both binary builders (the radix
+tree over centroid Morton codes, Karras; the top-down binned SAH) and both
+4-wide collapses (every second level,
 greedy largest-area) must give a valid tree:
 every leaf slot reached once and every node once, child boxes that contain
 the triangles below them, a traversal stack bound that holds on every
@@ -16,6 +17,7 @@ import torch
 pytestmark = pytest.mark.gpu
 
 COLLAPSE = [0, 1]   # bvh.collapse: every second level, greedy largest-area
+BUILDER = [0, 1]    # bvh.builder: radix tree, binned SAH
 
 
 @contextlib.contextmanager
@@ -179,11 +181,12 @@ def check_wide(tri_verts, sc):
     return max_need, st4
 
 
+@pytest.mark.parametrize("builder", BUILDER)
 @pytest.mark.parametrize("collapse", COLLAPSE)
-def test_builders_structure_boxes(dev, collapse):
+def test_builders_structure_boxes(dev, collapse, builder):
     from vulkanwavefrontpathtracer_amd import stages
 
-    with knobs(bvh__collapse=collapse):
+    with knobs(bvh__collapse=collapse, bvh__builder=builder):
         hs = stages.HostScene(stages.SCENE_BOXES, 1000)
         sc = stages.Scene.from_host(hs, dev)
         depth = check_binary(hs.num_tris, sc)
@@ -196,9 +199,10 @@ def test_builders_structure_boxes(dev, collapse):
 _INTERIOR = {}
 
 
+@pytest.mark.parametrize("builder", BUILDER)
 @pytest.mark.parametrize("leaf", [1, 4])
 @pytest.mark.parametrize("collapse", COLLAPSE)
-def test_builders_bit_exact_interior(dev, orc, collapse, leaf):
+def test_builders_bit_exact_interior(dev, orc, collapse, leaf, builder):
     """Interior scene (walls split into many references): both collapses
     return the oracle's closest-hit records, any-hit agrees on visibility."""
     from vulkanwavefrontpathtracer_amd import stages
@@ -209,8 +213,9 @@ def test_builders_bit_exact_interior(dev, orc, collapse, leaf):
         rays[::101, 7] = -1.0
         _INTERIOR[leaf] = (hs, rays, expected_hits(orc, hs.tri_verts(), hs.geom_first, rays))
     hs, rays, exp = _INTERIOR[leaf]
-    with knobs(bvh__collapse=collapse, bvh__leaf_max=leaf):
+    with knobs(bvh__collapse=collapse, bvh__leaf_max=leaf, bvh__builder=builder):
         sc = stages.Scene.from_host(hs, dev)
+        check_binary(hs.num_tris, sc)
         tr = torch.from_numpy(rays).to(dev)
         got = stages.trace(sc, tr).cpu().numpy()
         bad = np.nonzero(np.any(got != exp, axis=1))[0]
@@ -220,9 +225,10 @@ def test_builders_bit_exact_interior(dev, orc, collapse, leaf):
     assert np.mean(exp[:, 0] >= 0) > 0.5
 
 
+@pytest.mark.parametrize("builder", BUILDER)
 @pytest.mark.parametrize("presplit", [0, 1])
 @pytest.mark.parametrize("collapse", COLLAPSE)
-def test_builders_degenerate_inputs(dev, orc, collapse, presplit):
+def test_builders_degenerate_inputs(dev, orc, collapse, presplit, builder):
     """Coincident triangles (identical boxes, tied hits: the smaller original
     index wins) and two- and three-triangle scenes, with and without split
     references."""
@@ -237,7 +243,7 @@ def test_builders_degenerate_inputs(dev, orc, collapse, presplit):
     verts[:, :3] = tris.reshape(-1, 3)
     verts[:, 5] = 1.0
     idx = np.arange(len(tris) * 3, dtype=np.uint32).reshape(-1, 3)
-    with knobs(bvh__collapse=collapse, bvh__presplit=presplit):
+    with knobs(bvh__collapse=collapse, bvh__presplit=presplit, bvh__builder=builder):
         for nt in (2, 3, len(tris)):
             gf = np.array([0, nt], np.int32)
             sc = stages.Scene(verts[:3 * nt], idx[:nt], gf, device=dev)
@@ -273,3 +279,39 @@ def test_builder_node_visits(dev):
             visits[collapse] = (nv.float().mean().item(), nt.float().mean().item())
     print("node visits / triangle tests per ray by collapse:", visits)
     assert visits[1][0] <= visits[0][0] * 1.05
+
+
+def sah_cost(nodes):
+    """Surface-area cost of a downloaded binary hierarchy relative to its
+    root: sum over interior nodes and leaf references of their box area
+    (traversal and intersection weighted alike)."""
+    boxes = nodes[:, :12].astype(np.float64).reshape(-1, 2, 2, 3)   # node, child, lo/hi, xyz
+    refs = nodes[:, 12:14].view(np.int32)
+    ext = np.maximum(boxes[:, :, 1] - boxes[:, :, 0], 0.0)
+    area = ext[..., 0] * ext[..., 1] + ext[..., 1] * ext[..., 2] + ext[..., 2] * ext[..., 0]
+    count = np.where(refs >= 0, 1, (~np.minimum(refs, -1) & 7) + 1)
+    root_lo = boxes[0, :, 0].min(axis=0)
+    root_hi = boxes[0, :, 1].max(axis=0)
+    re = root_hi - root_lo
+    root = re[0] * re[1] + re[1] * re[2] + re[2] * re[0]
+    return (root + np.sum(area * count)) / root
+
+
+def test_sah_builder_cost_and_determinism(dev):
+    """The binned SAH builder lowers the hierarchy's surface-area cost against
+    the radix tree on the interior scene, and two builds of one scene give
+    the same nodes and leaf order (the node numbering is a function of the
+    tree, not of the order the GPU processes nodes in)."""
+    from vulkanwavefrontpathtracer_amd import stages
+
+    hs = stages.HostScene(stages.SCENE_INTERIOR, 1001)
+    cost = {}
+    for builder in BUILDER:
+        with knobs(bvh__builder=builder):
+            a = binary_tree(stages.Scene.from_host(hs, dev))
+            b = binary_tree(stages.Scene.from_host(hs, dev))
+        assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
+        assert np.array_equal(a[1], b[1])
+        cost[builder] = sah_cost(a[0])
+    print("surface-area cost by builder (radix tree, binned SAH):", cost)
+    assert cost[1] < 0.95 * cost[0]

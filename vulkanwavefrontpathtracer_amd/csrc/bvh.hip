@@ -15,8 +15,10 @@
 //      the root; small triangles are one reference each.
 //   3. centroid Morton codes of the references, 21 bits per axis (63-bit
 //      keys), sorted with this library's radix sort (reference payload).
-//   4. Karras 2012 binary radix tree over the sorted codes (equal codes are
-//      split by index), parent links.
+//   4. the binary tree over the sorted references (bvh.builder): top-down
+//      binned SAH (default; the Morton order only seeds the reference
+//      order), or Karras 2012's radix tree over the sorted codes (equal codes
+//      split by index); parent links.
 //   5. bottom-up AABB refit (second child to arrive at a node merges).
 //   6. emit 64-byte BVH2 nodes holding both child boxes; subtrees of at most
 //      BVH_LEAF_MAX triangles collapse into one leaf (contiguous slots);
@@ -297,6 +299,505 @@ __global__ __launch_bounds__(256) void karras_kernel(const uint64_t* __restrict_
     parent[left] = i;
     parent[right] = i;
     if (i == 0) parent[0] = -1;
+}
+
+// ------------------------------------------------------------ binned SAH
+// Top-down binned SAH build (Wald 2007's binning of reference centroids,
+// 32 bins per axis, cost = area(left) * count(left) + area(right) *
+// count(right)), the alternative to the radix tree (bvh.builder = 1). Nodes
+// of more than SAH_SMALL references are split a level at a time, every node
+// of the level at once: their references are cut into chunks of SAH_CHUNK
+// positions, one workgroup per chunk, for the centroid bounds, the bins and
+// the stable partition; one wave per node picks the split. Smaller nodes
+// finish their whole subtree in one wave, the references in LDS. Nodes at
+// depth SAH_MEDIAN_DEPTH and below, and nodes whose centroids share one bin
+// on every axis, split at the median position. The references start in
+// Morton order and every partition is stable, so each node's references stay
+// in Morton order.
+// The build writes the radix tree's child / range / parent arrays and leaves
+// the final reference order in the sorted-item array, so refit and emission
+// are shared. Numbering: every gap between consecutive positions g, g+1 is
+// split by exactly one internal node, which takes index g, except that the
+// root and the node of gap 0 trade indices so that the root is 0. The
+// numbering is therefore a function of the tree, whatever order the levels'
+// nodes are processed in.
+constexpr int SAH_BINS = 32;
+constexpr int SAH_SMALL = 512;
+constexpr int SAH_CHUNK = 4096;
+constexpr int SAH_MEDIAN_DEPTH = 40;
+constexpr int SAH_BIN_WORDS = 3 * SAH_BINS * 7;  // per axis and bin: count, box lo xyz, box hi xyz
+constexpr int SAH_WAVES = 4;                      // small-node waves per workgroup
+
+struct SahNode {          // a node of more than SAH_SMALL references
+    int first, count;     // positions [first, first + count)
+    int parent, side;     // parent index and child slot (-1 for the root)
+    int depth, axis;      // axis of the split, -1: median position
+    int split, lc;        // last bin of the left side; references on the left
+    uint32_t cmin[3], cmax[3];  // centroid bounds (ordered keys)
+    int pad[2];
+};
+static_assert(sizeof(SahNode) == 64, "SahNode is 64 bytes");
+
+__device__ __forceinline__ float3 ref_centre(const float4* __restrict__ item_box, uint32_t it) {
+    const float4 lo = item_box[2 * (int64_t)it], hi = item_box[2 * (int64_t)it + 1];
+    return make_float3((lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f);
+}
+
+__device__ __forceinline__ float f3(const float3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+
+// Bin of a centroid coordinate between the node's centroid bounds (the same
+// arithmetic in every kernel, so binning and partition agree).
+__device__ __forceinline__ int sah_bin(float c, float mn, float mx) {
+    const float ext = mx - mn;
+    if (!(ext > 0.f)) return 0;
+    const int b = (int)((c - mn) / ext * (float)SAH_BINS);
+    return min(max(b, 0), SAH_BINS - 1);
+}
+
+__device__ __forceinline__ int sah_index(int g, int groot) { return g == groot ? 0 : (g == 0 ? groot : g); }
+
+__device__ __forceinline__ float half_area(float lx, float ly, float lz, float hx, float hy, float hz) {
+    const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
+    return dx * dy + dy * dz + dz * dx;
+}
+
+// The best split over the bins of one node (global or LDS words), evaluated by
+// a whole wave; lanes i and i + 32 mirror each other. Returns the axis (-1 if
+// no plane leaves references on both sides), the last bin of the left side
+// and the left count. Ties go to the lower axis, then the lower bin.
+__device__ void sah_best_split(const uint32_t* bins, int& axis_out, int& split_out, int& lc_out) {
+    const int k = threadIdx.x & 31;
+    float best = INFINITY;
+    int axis = -1, split = 0, lc = 0;
+    for (int a = 0; a < 3; ++a) {
+        const uint32_t* w = bins + (a * SAH_BINS + k) * 7;
+        uint32_t cnt = w[0];
+        float lo[3], hi[3];
+        for (int j = 0; j < 3; ++j) {
+            lo[j] = cnt ? funkey(w[1 + j]) : INFINITY;
+            hi[j] = cnt ? funkey(w[4 + j]) : -INFINITY;
+        }
+        // inclusive prefix (bins 0..k) and suffix (bins k..31)
+        uint32_t pc = cnt, sc = cnt;
+        float pl[3], ph[3], sl[3], sh[3];
+        for (int j = 0; j < 3; ++j) {
+            pl[j] = sl[j] = lo[j];
+            ph[j] = sh[j] = hi[j];
+        }
+        for (int o = 1; o < 32; o <<= 1) {
+            const uint32_t uc = __shfl_up(pc, o, 32), dc = __shfl_down(sc, o, 32);
+            float ul[3], uh[3], dl[3], dh[3];
+            for (int j = 0; j < 3; ++j) {
+                ul[j] = __shfl_up(pl[j], o, 32);
+                uh[j] = __shfl_up(ph[j], o, 32);
+                dl[j] = __shfl_down(sl[j], o, 32);
+                dh[j] = __shfl_down(sh[j], o, 32);
+            }
+            if (k >= o) {
+                pc += uc;
+                for (int j = 0; j < 3; ++j) {
+                    pl[j] = fminf(pl[j], ul[j]);
+                    ph[j] = fmaxf(ph[j], uh[j]);
+                }
+            }
+            if (k + o < 32) {
+                sc += dc;
+                for (int j = 0; j < 3; ++j) {
+                    sl[j] = fminf(sl[j], dl[j]);
+                    sh[j] = fmaxf(sh[j], dh[j]);
+                }
+            }
+        }
+        // split after bin k: left = prefix(k), right = suffix(k + 1)
+        const uint32_t rc = __shfl_down(sc, 1, 32);
+        float rl[3], rh[3];
+        for (int j = 0; j < 3; ++j) {
+            rl[j] = __shfl_down(sl[j], 1, 32);
+            rh[j] = __shfl_down(sh[j], 1, 32);
+        }
+        float cost = INFINITY;
+        if (k < 31 && pc > 0 && rc > 0)
+            cost = half_area(pl[0], pl[1], pl[2], ph[0], ph[1], ph[2]) * (float)pc +
+                   half_area(rl[0], rl[1], rl[2], rh[0], rh[1], rh[2]) * (float)rc;
+        // argmin over the 32 planes, lowest bin on ties
+        float c = cost;
+        int b = k;
+        uint32_t l = pc;
+        for (int o = 16; o >= 1; o >>= 1) {
+            const float oc = __shfl_xor(c, o, 32);
+            const int ob = __shfl_xor(b, o, 32);
+            const uint32_t ol = __shfl_xor(l, o, 32);
+            if (oc < c || (oc == c && ob < b)) {
+                c = oc;
+                b = ob;
+                l = ol;
+            }
+        }
+        if (c < best) {
+            best = c;
+            axis = a;
+            split = b;
+            lc = (int)l;
+        }
+    }
+    axis_out = axis;
+    split_out = split;
+    lc_out = lc;
+}
+
+__device__ __forceinline__ bool sah_goes_left(int axis, int split, int lc, int local, float3 c,
+                                              const float* cmn, const float* cmx) {
+    if (axis < 0) return local < lc;
+    return sah_bin(f3(c, axis), cmn[axis], cmx[axis]) <= split;
+}
+
+// Links of a node that has just been split at gap g: its own index, its slot
+// in its parent, its range, and its one-reference children (leaves).
+__device__ __forceinline__ int sah_link(int first, int count, int lc, int par, int side, int groot,
+                                        int n, int2* __restrict__ child, int2* __restrict__ range,
+                                        int32_t* __restrict__ parent) {
+    const int g = first + lc - 1;
+    const int id = par < 0 ? 0 : sah_index(g, groot);
+    if (par >= 0) reinterpret_cast<int*>(child)[2 * par + side] = id;
+    parent[id] = par;
+    range[id] = make_int2(first, first + count - 1);
+    if (lc == 1) {
+        reinterpret_cast<int*>(child)[2 * id] = n - 1 + first;
+        parent[n - 1 + first] = id;
+    }
+    if (count - lc == 1) {
+        reinterpret_cast<int*>(child)[2 * id + 1] = n - 1 + first + lc;
+        parent[n - 1 + first + lc] = id;
+    }
+    return id;
+}
+
+__global__ __launch_bounds__(256) void sah_bins_init_kernel(uint32_t* __restrict__ bins, int64_t words) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= words) return;
+    const int w = (int)(i % 7);
+    bins[i] = w == 0 ? 0u : (w <= 3 ? 0xffffffffu : 0u);
+}
+
+// chunk = {node, first position, count, 0}
+__global__ __launch_bounds__(256) void sah_cbounds_kernel(const int4* __restrict__ chunks,
+                                                          const uint32_t* __restrict__ perm,
+                                                          const float4* __restrict__ item_box,
+                                                          SahNode* __restrict__ nodes) {
+    const int4 ch = chunks[blockIdx.x];
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = threadIdx.x; i < ch.z; i += 256) {
+        const float3 c = ref_centre(item_box, perm[ch.y + i]);
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = fminf(mn[a], f3(c, a));
+            mx[a] = fmaxf(mx[a], f3(c, a));
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        mn[a] = wave_min(mn[a]);
+        mx[a] = wave_max(mx[a]);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&nodes[ch.x].cmin[a], fkey(mn[a]));
+            atomicMax(&nodes[ch.x].cmax[a], fkey(mx[a]));
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void sah_bin_kernel(const int4* __restrict__ chunks,
+                                                      const uint32_t* __restrict__ perm,
+                                                      const float4* __restrict__ item_box,
+                                                      const SahNode* __restrict__ nodes,
+                                                      uint32_t* __restrict__ bins) {
+    __shared__ uint32_t s[SAH_BIN_WORDS];
+    for (int k = threadIdx.x; k < SAH_BIN_WORDS; k += 256) {
+        const int w = k % 7;
+        s[k] = w == 0 ? 0u : (w <= 3 ? 0xffffffffu : 0u);
+    }
+    __syncthreads();
+    const int4 ch = chunks[blockIdx.x];
+    const SahNode& nd = nodes[ch.x];
+    float cmn[3], cmx[3];
+    for (int a = 0; a < 3; ++a) {
+        cmn[a] = funkey(nd.cmin[a]);
+        cmx[a] = funkey(nd.cmax[a]);
+    }
+    for (int i = threadIdx.x; i < ch.z; i += 256) {
+        const uint32_t it = perm[ch.y + i];
+        const float4 lo = item_box[2 * (int64_t)it], hi = item_box[2 * (int64_t)it + 1];
+        const float3 c = make_float3((lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f);
+        const uint32_t kl[3] = {fkey(lo.x), fkey(lo.y), fkey(lo.z)};
+        const uint32_t kh[3] = {fkey(hi.x), fkey(hi.y), fkey(hi.z)};
+        for (int a = 0; a < 3; ++a) {
+            uint32_t* w = s + (a * SAH_BINS + sah_bin(f3(c, a), cmn[a], cmx[a])) * 7;
+            atomicAdd(w, 1u);
+            for (int j = 0; j < 3; ++j) {
+                atomicMin(w + 1 + j, kl[j]);
+                atomicMax(w + 4 + j, kh[j]);
+            }
+        }
+    }
+    __syncthreads();
+    uint32_t* g = bins + (int64_t)ch.x * SAH_BIN_WORDS;
+    for (int k = threadIdx.x; k < 3 * SAH_BINS; k += 256) {
+        const uint32_t* w = s + 7 * k;
+        if (w[0] == 0) continue;
+        atomicAdd(g + 7 * k, w[0]);
+        for (int j = 1; j <= 3; ++j) atomicMin(g + 7 * k + j, w[j]);
+        for (int j = 4; j <= 6; ++j) atomicMax(g + 7 * k + j, w[j]);
+    }
+}
+
+// One wave per node of the level: the split, the links, the children.
+// counters: [0] nodes of the next level, [1] small nodes, [2] the root's gap.
+__global__ __launch_bounds__(256) void sah_split_kernel(SahNode* __restrict__ nodes, int nl,
+                                                        const uint32_t* __restrict__ bins, int n,
+                                                        SahNode* __restrict__ next,
+                                                        int4* __restrict__ small,
+                                                        uint32_t* __restrict__ counters,
+                                                        int2* __restrict__ child,
+                                                        int2* __restrict__ range,
+                                                        int32_t* __restrict__ parent) {
+    const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (j >= nl) return;
+    const SahNode nd = nodes[j];
+    int axis, split, lc;
+    sah_best_split(bins + (int64_t)j * SAH_BIN_WORDS, axis, split, lc);
+    if (axis < 0 || nd.depth >= SAH_MEDIAN_DEPTH) {
+        axis = -1;
+        split = 0;
+        lc = nd.count / 2;
+    }
+    if ((threadIdx.x & 63) != 0) return;
+    nodes[j].axis = axis;
+    nodes[j].split = split;
+    nodes[j].lc = lc;
+    const int groot = nd.parent < 0 ? nd.first + lc - 1 : (int)counters[2];
+    if (nd.parent < 0) counters[2] = (uint32_t)groot;
+    const int id = sah_link(nd.first, nd.count, lc, nd.parent, nd.side, groot, n, child, range, parent);
+    for (int sd = 0; sd < 2; ++sd) {
+        const int f = sd ? nd.first + lc : nd.first;
+        const int c = sd ? nd.count - lc : lc;
+        if (c > SAH_SMALL) {
+            SahNode ch;
+            ch.first = f;
+            ch.count = c;
+            ch.parent = id;
+            ch.side = sd;
+            ch.depth = nd.depth + 1;
+            ch.axis = -1;
+            ch.split = 0;
+            ch.lc = 0;
+            for (int a = 0; a < 3; ++a) {
+                ch.cmin[a] = 0xffffffffu;
+                ch.cmax[a] = 0u;
+            }
+            ch.pad[0] = ch.pad[1] = 0;
+            next[atomicAdd(&counters[0], 1u)] = ch;
+        } else if (c >= 2) {
+            small[atomicAdd(&counters[1], 1u)] = make_int4(f, c, id, sd | ((nd.depth + 1) << 1));
+        }
+    }
+}
+
+// Lefts per chunk (the same test as the scatter).
+__global__ __launch_bounds__(256) void sah_count_kernel(const int4* __restrict__ chunks,
+                                                        const uint32_t* __restrict__ perm,
+                                                        const float4* __restrict__ item_box,
+                                                        const SahNode* __restrict__ nodes,
+                                                        uint32_t* __restrict__ lefts) {
+    __shared__ uint32_t s_w[4];
+    const int4 ch = chunks[blockIdx.x];
+    const SahNode& nd = nodes[ch.x];
+    float cmn[3], cmx[3];
+    for (int a = 0; a < 3; ++a) {
+        cmn[a] = funkey(nd.cmin[a]);
+        cmx[a] = funkey(nd.cmax[a]);
+    }
+    uint32_t cnt = 0;
+    for (int i = threadIdx.x; i < ch.z; i += 256) {
+        const float3 c = ref_centre(item_box, perm[ch.y + i]);
+        cnt += sah_goes_left(nd.axis, nd.split, nd.lc, ch.y + i - nd.first, c, cmn, cmx) ? 1u : 0u;
+    }
+    for (int o = 32; o >= 1; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) lefts[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// Per node: left and right references before each of its chunks.
+__global__ __launch_bounds__(256) void sah_offsets_kernel(const int* __restrict__ chunk0, int nl,
+                                                          const uint32_t* __restrict__ lefts,
+                                                          int2* __restrict__ offs) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nl) return;
+    int l = 0, seen = 0;
+    for (int c = chunk0[j]; c < chunk0[j + 1]; ++c) {
+        offs[c] = make_int2(l, seen - l);
+        l += (int)lefts[c];
+        seen += SAH_CHUNK;
+    }
+}
+
+// Stable partition of each chunk's references into the node's two sides.
+__global__ __launch_bounds__(256) void sah_scatter_kernel(const int4* __restrict__ chunks,
+                                                          const uint32_t* __restrict__ perm,
+                                                          const float4* __restrict__ item_box,
+                                                          const SahNode* __restrict__ nodes,
+                                                          const int2* __restrict__ offs,
+                                                          uint32_t* __restrict__ out) {
+    __shared__ uint32_t s_w[4];
+    const int4 ch = chunks[blockIdx.x];
+    const int cnt = __builtin_amdgcn_readfirstlane(ch.z);  // uniform trip count for the barriers
+    const SahNode& nd = nodes[ch.x];
+    float cmn[3], cmx[3];
+    for (int a = 0; a < 3; ++a) {
+        cmn[a] = funkey(nd.cmin[a]);
+        cmx[a] = funkey(nd.cmax[a]);
+    }
+    const int2 o = offs[blockIdx.x];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int lbase = 0;
+    for (int r = 0; r < cnt; r += 256) {
+        const int i = r + threadIdx.x;
+        const bool valid = i < cnt;
+        uint32_t it = 0;
+        bool left = false;
+        if (valid) {
+            it = perm[ch.y + i];
+            left = sah_goes_left(nd.axis, nd.split, nd.lc, ch.y + i - nd.first,
+                                 ref_centre(item_box, it), cmn, cmx);
+        }
+        const uint64_t m = __ballot(left);
+        if (lane == 0) s_w[wv] = (uint32_t)__popcll(m);
+        __syncthreads();
+        int before = lbase;
+        for (int k = 0; k < wv; ++k) before += (int)s_w[k];
+        const int total = (int)(s_w[0] + s_w[1] + s_w[2] + s_w[3]);
+        __syncthreads();
+        if (valid) {
+            const int lb = before + (int)__popcll(m & below);  // lefts before i in this chunk
+            const int dst = left ? nd.first + o.x + lb : nd.first + nd.lc + o.y + (i - lb);
+            if (dst >= nd.first && dst < nd.first + nd.count) out[dst] = it;
+        }
+        lbase += total;
+    }
+}
+
+__global__ __launch_bounds__(256) void sah_copy_kernel(const int4* __restrict__ chunks,
+                                                       const uint32_t* __restrict__ src,
+                                                       uint32_t* __restrict__ dst) {
+    const int4 ch = chunks[blockIdx.x];
+    for (int i = threadIdx.x; i < ch.z; i += 256) dst[ch.y + i] = src[ch.y + i];
+}
+
+// One wave per small node (task = {first, count, parent, side | depth << 1}):
+// the node's whole subtree, depth first, with its references in LDS.
+__global__ __launch_bounds__(256) void sah_small_kernel(const int4* __restrict__ tasks, int ntask,
+                                                        const float4* __restrict__ item_box,
+                                                        uint32_t* __restrict__ perm, int n,
+                                                        const uint32_t* __restrict__ counters,
+                                                        int2* __restrict__ child,
+                                                        int2* __restrict__ range,
+                                                        int32_t* __restrict__ parent) {
+    __shared__ uint32_t s_perm[SAH_WAVES][SAH_SMALL];
+    __shared__ uint32_t s_tmp[SAH_WAVES][SAH_SMALL];
+    __shared__ uint32_t s_bins[SAH_WAVES][SAH_BIN_WORDS];
+    __shared__ int4 s_stack[SAH_WAVES][SAH_SMALL / 2];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int task = blockIdx.x * SAH_WAVES + wv;
+    if (task >= ntask) return;  // whole waves; no workgroup barriers below
+    const int4 t = tasks[task];
+    const int base = t.x;
+    uint32_t* sp_perm = s_perm[wv];
+    uint32_t* sp_tmp = s_tmp[wv];
+    uint32_t* sp_bins = s_bins[wv];
+    int4* sp_stack = s_stack[wv];
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int i = lane; i < t.y; i += 64) sp_perm[i] = perm[base + i];
+    int groot = t.z < 0 ? -1 : (int)counters[2];
+    int sp = 0;
+    if (lane == 0) sp_stack[0] = make_int4(0, t.y, t.z, t.w);
+    sp = 1;
+    while (sp > 0) {
+        __builtin_amdgcn_wave_barrier();
+        const int4 e = sp_stack[--sp];
+        const int f = e.x, c = e.y, par = e.z, side = e.w & 1, depth = e.w >> 1;
+        float cmn[3] = {INFINITY, INFINITY, INFINITY}, cmx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int i = lane; i < c; i += 64) {
+            const float3 cc = ref_centre(item_box, sp_perm[f + i]);
+            for (int a = 0; a < 3; ++a) {
+                cmn[a] = fminf(cmn[a], f3(cc, a));
+                cmx[a] = fmaxf(cmx[a], f3(cc, a));
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            cmn[a] = wave_min(cmn[a]);
+            cmx[a] = wave_max(cmx[a]);
+        }
+        for (int k = lane; k < SAH_BIN_WORDS; k += 64) {
+            const int w = k % 7;
+            sp_bins[k] = w == 0 ? 0u : (w <= 3 ? 0xffffffffu : 0u);
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (int i = lane; i < c; i += 64) {
+            const uint32_t it = sp_perm[f + i];
+            const float4 lo = item_box[2 * (int64_t)it], hi = item_box[2 * (int64_t)it + 1];
+            const float3 cc = make_float3((lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f);
+            const uint32_t kl[3] = {fkey(lo.x), fkey(lo.y), fkey(lo.z)};
+            const uint32_t kh[3] = {fkey(hi.x), fkey(hi.y), fkey(hi.z)};
+            for (int a = 0; a < 3; ++a) {
+                uint32_t* w = sp_bins + (a * SAH_BINS + sah_bin(f3(cc, a), cmn[a], cmx[a])) * 7;
+                atomicAdd(w, 1u);
+                for (int j = 0; j < 3; ++j) {
+                    atomicMin(w + 1 + j, kl[j]);
+                    atomicMax(w + 4 + j, kh[j]);
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        int axis, split, lc;
+        sah_best_split(sp_bins, axis, split, lc);
+        if (axis < 0 || depth >= SAH_MEDIAN_DEPTH) {
+            axis = -1;
+            split = 0;
+            lc = c / 2;
+        }
+        int lbase = 0, rbase = 0;
+        for (int i0 = 0; i0 < c; i0 += 64) {
+            const int i = i0 + lane;
+            const bool valid = i < c;
+            uint32_t it = 0;
+            bool left = false;
+            if (valid) {
+                it = sp_perm[f + i];
+                left = sah_goes_left(axis, split, lc, i, ref_centre(item_box, it), cmn, cmx);
+            }
+            const uint64_t ml = __ballot(left), mv = __ballot(valid);
+            if (valid) {
+                const int lb = (int)__popcll(ml & below), rb = (int)__popcll(mv & ~ml & below);
+                sp_tmp[f + (left ? lbase + lb : lc + rbase + rb)] = it;
+            }
+            lbase += (int)__popcll(ml);
+            rbase += (int)__popcll(mv & ~ml);
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (int i = lane; i < c; i += 64) sp_perm[f + i] = sp_tmp[f + i];
+        const int first = base + f;
+        if (par < 0) groot = first + lc - 1;
+        int id = 0;
+        if (lane == 0) id = sah_link(first, c, lc, par, side, groot, n, child, range, parent);
+        id = __shfl(id, 0, 64);
+        if (lane == 0) {
+            if (c - lc >= 2) sp_stack[sp] = make_int4(f + lc, c - lc, id, 1 | ((depth + 1) << 1));
+            if (lc >= 2) sp_stack[sp + (c - lc >= 2 ? 1 : 0)] = make_int4(f, lc, id, (depth + 1) << 1);
+        }
+        sp += (c - lc >= 2 ? 1 : 0) + (lc >= 2 ? 1 : 0);
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (int i = lane; i < t.y; i += 64) perm[base + i] = sp_perm[i];
 }
 
 __global__ __launch_bounds__(256) void refit_kernel(const float4* __restrict__ item_box,
@@ -709,6 +1210,107 @@ static int dev_alloc(DevBuf& b, size_t bytes) {
     return WF_OK;
 }
 
+// Binned SAH build over the m >= 2 Morton-sorted references: writes child,
+// range and parent like karras_kernel and leaves the final reference order in
+// `perm` (in place). The large-node levels are driven from the host, which
+// reads each level's node list back to cut it into chunks.
+static int build_sah(int64_t m64, const float4* refs, uint32_t* perm, int2* child, int2* range,
+                     int32_t* parent, hipStream_t s) {
+    const int m = (int)m64;
+    const int64_t nl_cap = m64 / (SAH_SMALL + 1) + 2;
+    const int64_t chunk_cap = m64 / SAH_CHUNK + nl_cap + 2;
+    DevBuf large[2], bins, small, chunks, chunk0, lefts, offs, tmp, counters;
+    for (int k = 0; k < 2; ++k) WF_TRY(dev_alloc(large[k], sizeof(SahNode) * nl_cap));
+    WF_TRY(dev_alloc(bins, sizeof(uint32_t) * SAH_BIN_WORDS * nl_cap));
+    WF_TRY(dev_alloc(small, sizeof(int4) * (m64 / 2 + 1)));
+    WF_TRY(dev_alloc(chunks, sizeof(int4) * chunk_cap));
+    WF_TRY(dev_alloc(chunk0, sizeof(int) * (nl_cap + 1)));
+    WF_TRY(dev_alloc(lefts, sizeof(uint32_t) * chunk_cap));
+    WF_TRY(dev_alloc(offs, sizeof(int2) * chunk_cap));
+    WF_TRY(dev_alloc(tmp, sizeof(uint32_t) * m64));
+    WF_TRY(dev_alloc(counters, sizeof(uint32_t) * 4));
+    WF_TRY(fill_u32((uint32_t*)counters.p, 4, 0u, s));
+    int nl = 0;
+    if (m <= SAH_SMALL) {
+        const int4 root = make_int4(0, m, -1, 0);
+        const uint32_t one = 1;
+        WF_HIP(hipMemcpyAsync(small.p, &root, sizeof(root), hipMemcpyHostToDevice, s));
+        WF_HIP(hipMemcpyAsync((uint32_t*)counters.p + 1, &one, sizeof(one), hipMemcpyHostToDevice, s));
+        WF_HIP(hipStreamSynchronize(s));
+    } else {
+        SahNode root{};
+        root.count = m;
+        root.parent = -1;
+        for (int a = 0; a < 3; ++a) root.cmin[a] = 0xffffffffu;
+        WF_HIP(hipMemcpyAsync(large[0].p, &root, sizeof(root), hipMemcpyHostToDevice, s));
+        WF_HIP(hipStreamSynchronize(s));
+        nl = 1;
+    }
+    std::vector<SahNode> hn;
+    std::vector<int4> hc;
+    std::vector<int> hc0;
+    int cur = 0;
+    while (nl > 0) {
+        if (nl > nl_cap) return fail(WF_ERR_HIP, "wf_scene_create: SAH level larger than its bound");
+        hn.resize(nl);
+        WF_HIP(hipMemcpyAsync(hn.data(), large[cur].p, sizeof(SahNode) * nl, hipMemcpyDeviceToHost, s));
+        WF_HIP(hipStreamSynchronize(s));
+        hc.clear();
+        hc0.resize(nl + 1);
+        for (int j = 0; j < nl; ++j) {
+            hc0[j] = (int)hc.size();
+            const int end = hn[j].first + hn[j].count;
+            for (int st = hn[j].first; st < end; st += SAH_CHUNK)
+                hc.push_back(make_int4(j, st, std::min(SAH_CHUNK, end - st), 0));
+        }
+        hc0[nl] = (int)hc.size();
+        const int nc = (int)hc.size();
+        if (nc > chunk_cap) return fail(WF_ERR_HIP, "wf_scene_create: SAH chunks beyond their bound");
+        WF_HIP(hipMemcpyAsync(chunks.p, hc.data(), sizeof(int4) * nc, hipMemcpyHostToDevice, s));
+        WF_HIP(hipMemcpyAsync(chunk0.p, hc0.data(), sizeof(int) * (nl + 1), hipMemcpyHostToDevice, s));
+        const int64_t words = (int64_t)nl * SAH_BIN_WORDS;
+        sah_bins_init_kernel<<<grid_for(words, 256), 256, 0, s>>>((uint32_t*)bins.p, words);
+        WF_TRY(launch_check("sah_bins_init_kernel"));
+        WF_TRY(fill_u32((uint32_t*)counters.p, 1, 0u, s));
+        SahNode* nodes = (SahNode*)large[cur].p;
+        const int4* ch = (const int4*)chunks.p;
+        sah_cbounds_kernel<<<nc, 256, 0, s>>>(ch, perm, refs, nodes);
+        WF_TRY(launch_check("sah_cbounds_kernel"));
+        sah_bin_kernel<<<nc, 256, 0, s>>>(ch, perm, refs, nodes, (uint32_t*)bins.p);
+        WF_TRY(launch_check("sah_bin_kernel"));
+        sah_split_kernel<<<grid_for(nl, 4), 256, 0, s>>>(nodes, nl, (const uint32_t*)bins.p, m,
+                                                        (SahNode*)large[cur ^ 1].p, (int4*)small.p,
+                                                        (uint32_t*)counters.p, child, range, parent);
+        WF_TRY(launch_check("sah_split_kernel"));
+        sah_count_kernel<<<nc, 256, 0, s>>>(ch, perm, refs, nodes, (uint32_t*)lefts.p);
+        WF_TRY(launch_check("sah_count_kernel"));
+        sah_offsets_kernel<<<grid_for(nl, 256), 256, 0, s>>>((const int*)chunk0.p, nl,
+                                                            (const uint32_t*)lefts.p, (int2*)offs.p);
+        WF_TRY(launch_check("sah_offsets_kernel"));
+        sah_scatter_kernel<<<nc, 256, 0, s>>>(ch, perm, refs, nodes, (const int2*)offs.p,
+                                              (uint32_t*)tmp.p);
+        WF_TRY(launch_check("sah_scatter_kernel"));
+        sah_copy_kernel<<<nc, 256, 0, s>>>(ch, (const uint32_t*)tmp.p, perm);
+        WF_TRY(launch_check("sah_copy_kernel"));
+        uint32_t hnext = 0;
+        WF_HIP(hipMemcpyAsync(&hnext, counters.p, sizeof(hnext), hipMemcpyDeviceToHost, s));
+        WF_HIP(hipStreamSynchronize(s));
+        nl = (int)hnext;
+        cur ^= 1;
+    }
+    uint32_t ns = 0;
+    WF_HIP(hipMemcpyAsync(&ns, (uint32_t*)counters.p + 1, sizeof(ns), hipMemcpyDeviceToHost, s));
+    WF_HIP(hipStreamSynchronize(s));
+    if (ns > 0) {
+        sah_small_kernel<<<grid_for(ns, SAH_WAVES), 64 * SAH_WAVES, 0, s>>>(
+            (const int4*)small.p, (int)ns, refs, perm, m, (const uint32_t*)counters.p, child, range,
+            parent);
+        WF_TRY(launch_check("sah_small_kernel"));
+    }
+    WF_HIP(hipStreamSynchronize(s));  // the temporaries leave scope
+    return WF_OK;
+}
+
 static void free_scene(wf_scene_s* s) {
     if (!s) return;
     if (s->tris) (void)hipFree(s->tris);
@@ -936,10 +1538,15 @@ extern "C" int wf_scene_create(const void* vertices, int64_t num_vertices, int v
                                                               (uint32_t*)vals.p);
         WF_TRY(launch_check("centroid_keys_kernel"));
         WF_TRY(wf_sort_pairs(keys.p, (uint32_t*)vals.p, m, 8, 63, 8, scratch.p, sb, stream, nullptr));
-        karras_kernel<<<grid_for(m - 1, 256), 256, 0, s>>>((const uint64_t*)keys.p, (int)m,
-                                                           (int2*)child.p, (int2*)range.p,
-                                                           (int32_t*)parent.p);
-        WF_TRY(launch_check("karras_kernel"));
+        if (tu.bvh_builder == 1) {
+            WF_TRY(build_sah(m, refs, (uint32_t*)vals.p, (int2*)child.p, (int2*)range.p,
+                             (int32_t*)parent.p, s));
+        } else {
+            karras_kernel<<<grid_for(m - 1, 256), 256, 0, s>>>((const uint64_t*)keys.p, (int)m,
+                                                               (int2*)child.p, (int2*)range.p,
+                                                               (int32_t*)parent.p);
+            WF_TRY(launch_check("karras_kernel"));
+        }
         refit_kernel<<<grid_for(m, 256), 256, 0, s>>>(
             refs, (const uint32_t*)vals.p, (int)m, (const int2*)child.p, (const int32_t*)parent.p,
             (float4*)box.p, (uint32_t*)flags.p, (uint32_t*)misc.p + 1);

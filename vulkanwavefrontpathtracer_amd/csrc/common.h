@@ -167,6 +167,7 @@ struct Tuning {
     int bvh_split_denom = 64;   // split triangles whose box exceeds diagonal / this
     int bvh_leaf_max = 1;
     int bvh_collapse = 1;       // 4-wide nodes: 0 every second level; 1 greedy largest-area
+    int bvh_builder = 1;        // binary hierarchy: 0 radix tree (Karras), 1 binned SAH
     int trace_block = 64;
     int sort_onesweep_tiles = 64;  // sorts of at most this many 4096-element tiles use onesweep
     int sort_rank_max = 28000;  // sorts of at most this many pairs x passes / 8 count each rank directly (0 off)

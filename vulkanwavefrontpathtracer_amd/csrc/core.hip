@@ -84,6 +84,7 @@ extern "C" int wf_set_tuning(const char* key, int value) {
     else if (k == "bvh.split_denom" && value >= 1 && value <= 4096) t.bvh_split_denom = value;
     else if (k == "bvh.leaf_max" && value >= 1 && value <= 8) t.bvh_leaf_max = value;
     else if (k == "bvh.collapse" && (value == 0 || value == 1)) t.bvh_collapse = value;
+    else if (k == "bvh.builder" && (value == 0 || value == 1)) t.bvh_builder = value;
     else if (k == "trace.block" && (value == 64 || value == 128 || value == 256)) t.trace_block = value;
     else if (k == "trace.xcd" && value >= 0 && value <= 65536) t.trace_xcd = value;
     else if (k == "trace.fuse_reorder" && (value == 0 || value == 1)) t.trace_fuse_reorder = value;
@@ -107,6 +108,7 @@ extern "C" int wf_get_tuning(const char* key, int* value) {
     else if (k == "bvh.split_denom") *value = t.bvh_split_denom;
     else if (k == "bvh.leaf_max") *value = t.bvh_leaf_max;
     else if (k == "bvh.collapse") *value = t.bvh_collapse;
+    else if (k == "bvh.builder") *value = t.bvh_builder;
     else if (k == "trace.block") *value = t.trace_block;
     else if (k == "trace.xcd") *value = t.trace_xcd;
     else if (k == "trace.fuse_reorder") *value = t.trace_fuse_reorder;
