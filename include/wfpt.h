@@ -149,9 +149,10 @@ int wf_reorder(const uint32_t* idx, const wf_ray* in_rays, wf_ray* out_rays,
 
 /* ------------------------------------------------------------------- scene */
 /* Replaces the BLAS/TLAS build (WaveFrontPathTracer.cpp:202-350, :355-457)
- * and Tracer::setAccererationStructure (Tracer/Tracer.cpp:166-183) with an
- * LBVH.  Input (device): vertex records of vertex_stride bytes holding
- * position float3 at byte 0 and normal float3 at byte 12 (vkglTF::Vertex,
+ * and Tracer::setAccererationStructure (Tracer/Tracer.cpp:166-183) with a
+ * hierarchy built on the GPU (binned SAH by default, bvh.builder).  Input
+ * (device): vertex records of vertex_stride bytes holding position float3
+ * at byte 0 and normal float3 at byte 12 (vkglTF::Vertex,
  * base/VulkanglTFModel.h:221-228 uses stride 96), and a u32 index buffer.
  * geom_first_tri (host, num_geoms+1 entries, ascending, geom_first_tri[0] = 0)
  * splits the triangles into geometries: a hit reports

@@ -167,8 +167,8 @@ def test_city_trace_matches_oracle(dev, orc, city, captured):
 
 @pytest.fixture(scope="module")
 def city12(dev, city):
-    """The city's hierarchy built with presplit depth 12 (the city env file's
-    Scene.splitDepth; the process default is 8)."""
+    """The city's hierarchy built with the deepest presplit, depth 12 (the
+    process default is 8)."""
     from vulkanwavefrontpathtracer_amd import stages
 
     hs, _ = city
@@ -183,7 +183,7 @@ def city12(dev, city):
 
 
 def test_city_split12_hierarchy_and_trace(dev, orc, city, city12, captured):
-    """The depth-12 presplit hierarchy of the headline workload: the same
+    """The depth-12 presplit hierarchy of the headline scene: the same
     structure checks, more leaf slots than the default build, and the same
     oracle-exact closest-hit and any-hit results."""
     hs, sc12 = city12
@@ -195,8 +195,8 @@ def test_city_split12_hierarchy_and_trace(dev, orc, city, city12, captured):
 def test_city_env_split_depth_frame(dev, city):
     """Scene.splitDepth reaches the renderer's scene build, leaves the
     process-wide knob as it was, and does not change the frame: one
-    160x90 x 4 spp, 8-bounce city frame with the env file's depth 12 equals
-    the frame with depth 8 bit for bit."""
+    160x90 x 4 spp, 8-bounce city frame with depth 12 equals the frame with
+    depth 8 bit for bit."""
     from vulkanwavefrontpathtracer_amd import stages
     from vulkanwavefrontpathtracer_amd.renderer import Renderer, env_path
 

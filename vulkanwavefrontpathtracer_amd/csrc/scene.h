@@ -1,4 +1,4 @@
-// Scene = triangles + LBVH + materials, owned by the library.
+// Scene = triangles + bounding volume hierarchy + materials, owned by the library.
 //
 // Device layout (all 16-byte aligned):
 //   tris    [3 * num_slots] float4, in BVH leaf order ("slots"; a split

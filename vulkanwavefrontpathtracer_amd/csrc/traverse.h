@@ -1,4 +1,4 @@
-// Stack-based LBVH traversal of one ray by one lane of a wave64, shared by
+// Stack-based BVH traversal of one ray by one lane of a wave64, shared by
 // the wavefront trace kernel and the path-replay adjoint kernel (which
 // re-traces every path vertex and must reproduce the trace stage bit for
 // bit).  Semantics follow trace.comp:27-88: opaque geometry, no face
