@@ -171,6 +171,19 @@ def test_reconstruct_stages_bit_exact(dev, orc, boxes):
     assert stages.count_hits(hits) == int((hits[:, 0] >= 0).sum())
 
 
+def test_count_hits_large_and_ragged(dev):
+    """The hit counter's workgroups stride over the records: sizes past one
+    stride (2048 x 256 records), ragged tails, all hits, no hits."""
+    from vulkanwavefrontpathtracer_amd import stages
+
+    rng = np.random.default_rng(17)
+    for n, p in ((0, 0.5), (1, 1.0), (300017, 0.3), (524288, 0.5), (524289, 1.0), (3000001, 0.6),
+                 (1048577, 0.0)):
+        hits = np.zeros((n, 4), np.int32)
+        hits[:, 0] = np.where(rng.uniform(size=n) < p, rng.integers(0, 1000, n), -1)
+        assert stages.count_hits(torch.from_numpy(hits).to(dev)) == int((hits[:, 0] >= 0).sum()), n
+
+
 # ------------------------------------------------------------------- frames
 def render(overrides, env="config0.json", decisions=False):
     from vulkanwavefrontpathtracer_amd.renderer import Renderer, env_path

@@ -116,13 +116,19 @@ __global__ __launch_bounds__(256) void interpolate_kernel(float4* __restrict__ f
     }
 }
 
-// countRayHits.comp:22-51: wave sum -> LDS -> one atomic per workgroup.
+// countRayHits.comp:22-51: wave sum -> LDS -> one atomic per workgroup. The
+// workgroups stride over the records (at most COUNT_HITS_GRID of them), so the
+// one counter takes a few thousand atomics instead of one per 256 records,
+// which serialised on its address.
+constexpr int COUNT_HITS_GRID = 2048;
 __global__ __launch_bounds__(256) void count_hits_kernel(const wf_hit* __restrict__ hits, int64_t n,
                                                          uint32_t* __restrict__ count) {
     __shared__ uint32_t s[4];
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool h = i < n && hits[i].prim >= 0;
-    uint32_t c = (uint32_t)__popcll(ballot64(h));
+    uint32_t c = 0;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        c += (uint32_t)__popcll(ballot64(i < n && hits[i].prim >= 0));
+    }
     if (lane_id() == 0) s[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -206,7 +212,8 @@ extern "C" int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, w
     WF_CHECK_ARG(hits, "wf_count_hits: null buffer");
     hipStream_t s = as_stream(stream);
     StageTimer t(s, ms);
-    count_hits_kernel<<<grid_for(n, 256), 256, 0, s>>>(hits, n, d_count);
+    count_hits_kernel<<<(unsigned)std::min<int64_t>(grid_for(n, 256), COUNT_HITS_GRID), 256, 0, s>>>(
+        hits, n, d_count);
     WF_TRY(launch_check("count_hits_kernel"));
     return t.finish();
 }
