@@ -192,6 +192,29 @@ def test_city_split12_hierarchy_and_trace(dev, orc, city, city12, captured):
     check_trace(dev, orc, hs, sc12, captured)
 
 
+def test_city_radix_tree_hierarchy_and_trace(dev, orc, city, captured):
+    """The radix-tree builder (bvh.builder = 0) on the headline scene: the
+    same structure checks and oracle-exact closest-hit and any-hit results as
+    the default binned SAH build, and a higher surface-area cost."""
+    from test_bvh_build_gpu import binary_tree, sah_cost
+
+    from vulkanwavefrontpathtracer_amd import stages
+
+    hs, sc_sah = city
+    old = stages.get_tuning("bvh.builder")
+    stages.set_tuning("bvh.builder", 0)
+    try:
+        print("[city] building the radix-tree hierarchy", flush=True)
+        sc = stages.Scene.from_host(hs, dev)
+    finally:
+        stages.set_tuning("bvh.builder", old)
+    check_hierarchy(hs, sc)
+    check_trace(dev, orc, hs, sc, captured)
+    radix, sah = sah_cost(binary_tree(sc)[0]), sah_cost(binary_tree(sc_sah)[0])
+    print(f"[city] surface-area cost: radix tree {radix:.2f}, binned SAH {sah:.2f}")
+    assert sah < 0.8 * radix
+
+
 def test_city_env_split_depth_frame(dev, city):
     """Scene.splitDepth reaches the renderer's scene build, leaves the
     process-wide knob as it was, and does not change the frame: one
