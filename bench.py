@@ -744,6 +744,7 @@ def main():
             "rays_per_step": head["rays"], "bounces": len(head["rays_per_bounce"]),
             "rays_per_bounce": head["rays_per_bounce"], "digit_bits": args.digit_bits,
             "bvh_split_depth": split_depth,
+            "bvh_builder": "binned SAH" if stages.get_tuning("bvh.builder") == 1 else "radix tree",
             "parallelism": "1 GPU",
         },
         "roofline": roofline,
