@@ -631,7 +631,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--env", default="config2.json")
     ap.add_argument("--digit-bits", type=int, default=8)
     ap.add_argument("--frames", type=int, default=2, help="reference-style full frames timed")
