@@ -321,7 +321,7 @@ __global__ __launch_bounds__(256) void karras_kernel(const uint64_t* __restrict_
 // root and the node of gap 0 trade indices so that the root is 0. The
 // numbering is therefore a function of the tree, whatever order the levels'
 // nodes are processed in.
-constexpr int SAH_BINS = 32;
+constexpr int SAH_BINS = 32;  // a power of two up to the wave width
 constexpr int SAH_SMALL = 512;
 constexpr int SAH_CHUNK = 4096;
 constexpr int SAH_MEDIAN_DEPTH = 40;
@@ -362,11 +362,12 @@ __device__ __forceinline__ float half_area(float lx, float ly, float lz, float h
 }
 
 // The best split over the bins of one node (global or LDS words), evaluated by
-// a whole wave; lanes i and i + 32 mirror each other. Returns the axis (-1 if
+// a whole wave, one lane per bin (lanes i and i + SAH_BINS mirror each other
+// when the bins are fewer than the lanes). Returns the axis (-1 if
 // no plane leaves references on both sides), the last bin of the left side
 // and the left count. Ties go to the lower axis, then the lower bin.
 __device__ void sah_best_split(const uint32_t* bins, int& axis_out, int& split_out, int& lc_out) {
-    const int k = threadIdx.x & 31;
+    const int k = threadIdx.x & (SAH_BINS - 1);
     float best = INFINITY;
     int axis = -1, split = 0, lc = 0;
     for (int a = 0; a < 3; ++a) {
@@ -377,21 +378,21 @@ __device__ void sah_best_split(const uint32_t* bins, int& axis_out, int& split_o
             lo[j] = cnt ? funkey(w[1 + j]) : INFINITY;
             hi[j] = cnt ? funkey(w[4 + j]) : -INFINITY;
         }
-        // inclusive prefix (bins 0..k) and suffix (bins k..31)
+        // inclusive prefix (bins 0..k) and suffix (bins k..SAH_BINS - 1)
         uint32_t pc = cnt, sc = cnt;
         float pl[3], ph[3], sl[3], sh[3];
         for (int j = 0; j < 3; ++j) {
             pl[j] = sl[j] = lo[j];
             ph[j] = sh[j] = hi[j];
         }
-        for (int o = 1; o < 32; o <<= 1) {
-            const uint32_t uc = __shfl_up(pc, o, 32), dc = __shfl_down(sc, o, 32);
+        for (int o = 1; o < SAH_BINS; o <<= 1) {
+            const uint32_t uc = __shfl_up(pc, o, SAH_BINS), dc = __shfl_down(sc, o, SAH_BINS);
             float ul[3], uh[3], dl[3], dh[3];
             for (int j = 0; j < 3; ++j) {
-                ul[j] = __shfl_up(pl[j], o, 32);
-                uh[j] = __shfl_up(ph[j], o, 32);
-                dl[j] = __shfl_down(sl[j], o, 32);
-                dh[j] = __shfl_down(sh[j], o, 32);
+                ul[j] = __shfl_up(pl[j], o, SAH_BINS);
+                uh[j] = __shfl_up(ph[j], o, SAH_BINS);
+                dl[j] = __shfl_down(sl[j], o, SAH_BINS);
+                dh[j] = __shfl_down(sh[j], o, SAH_BINS);
             }
             if (k >= o) {
                 pc += uc;
@@ -400,7 +401,7 @@ __device__ void sah_best_split(const uint32_t* bins, int& axis_out, int& split_o
                     ph[j] = fmaxf(ph[j], uh[j]);
                 }
             }
-            if (k + o < 32) {
+            if (k + o < SAH_BINS) {
                 sc += dc;
                 for (int j = 0; j < 3; ++j) {
                     sl[j] = fminf(sl[j], dl[j]);
@@ -409,24 +410,24 @@ __device__ void sah_best_split(const uint32_t* bins, int& axis_out, int& split_o
             }
         }
         // split after bin k: left = prefix(k), right = suffix(k + 1)
-        const uint32_t rc = __shfl_down(sc, 1, 32);
+        const uint32_t rc = __shfl_down(sc, 1, SAH_BINS);
         float rl[3], rh[3];
         for (int j = 0; j < 3; ++j) {
-            rl[j] = __shfl_down(sl[j], 1, 32);
-            rh[j] = __shfl_down(sh[j], 1, 32);
+            rl[j] = __shfl_down(sl[j], 1, SAH_BINS);
+            rh[j] = __shfl_down(sh[j], 1, SAH_BINS);
         }
         float cost = INFINITY;
-        if (k < 31 && pc > 0 && rc > 0)
+        if (k < SAH_BINS - 1 && pc > 0 && rc > 0)
             cost = half_area(pl[0], pl[1], pl[2], ph[0], ph[1], ph[2]) * (float)pc +
                    half_area(rl[0], rl[1], rl[2], rh[0], rh[1], rh[2]) * (float)rc;
-        // argmin over the 32 planes, lowest bin on ties
+        // argmin over the planes, lowest bin on ties
         float c = cost;
         int b = k;
         uint32_t l = pc;
-        for (int o = 16; o >= 1; o >>= 1) {
-            const float oc = __shfl_xor(c, o, 32);
-            const int ob = __shfl_xor(b, o, 32);
-            const uint32_t ol = __shfl_xor(l, o, 32);
+        for (int o = SAH_BINS / 2; o >= 1; o >>= 1) {
+            const float oc = __shfl_xor(c, o, SAH_BINS);
+            const int ob = __shfl_xor(b, o, SAH_BINS);
+            const uint32_t ol = __shfl_xor(l, o, SAH_BINS);
             if (oc < c || (oc == c && ob < b)) {
                 c = oc;
                 b = ob;
