@@ -59,10 +59,16 @@ __global__ __launch_bounds__(256) void raygen_shadow_kernel(
 
 // ----------------------------------------------------------- shade/extend
 constexpr int SHADE_BLOCK = 256;
+constexpr int SHADE_ITEMS = 4;                       // slots per thread
+constexpr int SHADE_TILE = SHADE_BLOCK * SHADE_ITEMS;
 
-// One thread per input slot.  Survivors are written in input order: each
+// raygenPath.comp:78-170. A workgroup takes a tile of SHADE_TILE input
+// slots, one per thread and item (slot tile * SHADE_TILE + item *
+// SHADE_BLOCK + thread). Survivors are written in input order: each
 // workgroup counts its survivors, resolves the count of earlier workgroups
 // by decoupled look-back (tiles taken in arrival order), then scatters.
+// Several slots per thread keep the look-back chain (one status word per
+// tile, published and read at agent scope) short against the shading work.
 __global__ __launch_bounds__(SHADE_BLOCK) void shade_extend_kernel(
     const float4* __restrict__ shade, const int32_t* __restrict__ geom_first,
     const float* __restrict__ albedo, int nmat, uint32_t* __restrict__ seeds,
@@ -72,73 +78,83 @@ __global__ __launch_bounds__(SHADE_BLOCK) void shade_extend_kernel(
     uint32_t* __restrict__ tile_counter, uint32_t* __restrict__ status, int rr, int64_t ntiles,
     uint8_t* __restrict__ sign_log) {
     __shared__ uint32_t s_tile, s_excl;
-    __shared__ uint32_t s_wave[SHADE_BLOCK / 64];
+    __shared__ uint32_t s_wave[SHADE_ITEMS][SHADE_BLOCK / 64];
     const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
     __syncthreads();
     const int64_t tile = s_tile;
-    const int64_t j = tile * SHADE_BLOCK + tid;
 
-    bool alive = false;
-    int pixel = 0;
-    float4 oa, ob;
-    float wr = 1.f, wg = 1.f, wb = 1.f;
-    if (j < n) {
-        pixel = in_s2i[j];
-        const int4 hit = *reinterpret_cast<const int4*>(in_hits + j);
-        if (hit.x >= 0) {
-            uint32_t seed = seeds[pixel];
-            float4 ro, rd;
-            load_ray(in_rays, j, ro, rd);
-            int64_t tri = (int64_t)geom_first[hit.y] + hit.x;
-            V3 nrm;
-            int m = clamp_material(shading_normal(shade, tri, (uint32_t)hit.w, nrm), nmat);
-            V3 ff = faceforward3(nrm, v3(rd.x, rd.y, rd.z));
-            // decision log: the normal flipped to face the ray (bit 0) and the
-            // basis branch of a normal with negative z (bit 1)
-            if (sign_log)
-                sign_log[pixel] = (uint8_t)((dot3(nrm, v3(rd.x, rd.y, rd.z)) < 0.0f ? 0 : 1) |
-                                            (ff.z >= 0.0f ? 0 : 2));
-            float r0 = randf(seed);
-            float r1 = randf(seed);
-            float t = __int_as_float(hit.z);
-            V3 p = v3(ro.x + rd.x * t, ro.y + rd.y * t, ro.z + rd.z * t);
-            const float eps = 1.0e-4f;
-            V3 d = cosine_direction(r0, r1, ff);
-            oa = make_float4(p.x + d.x * eps, p.y + d.y * eps, p.z + d.z * eps, eps);
-            ob = make_float4(d.x, d.y, d.z, FLT_MAX);
-            wr = albedo[3 * m];
-            wg = albedo[3 * m + 1];
-            wb = albedo[3 * m + 2];
-            alive = true;
-            if (rr) {
-                // Russian roulette on the throughput after this bounce.
-                float4 dec = decreases[pixel];
-                float q = fmaxf(fmaxf(dec.x * wr, dec.y * wg), dec.z * wb);
-                q = fminf(fmaxf(q, 0.05f), 1.0f);
-                float r2 = randf(seed);
-                if (r2 >= q) {
-                    alive = false;
-                } else {
-                    wr /= q;
-                    wg /= q;
-                    wb /= q;
-                }
-            }
-            seeds[pixel] = seed;
-        }
-    }
-    // block-local ranks
-    const uint64_t bal = ballot64(alive);
-    const uint32_t rank = popc_below(bal);
-    if (lane == 0) s_wave[wave] = (uint32_t)__popcll(bal);
-    __syncthreads();
-    uint32_t wbase = 0, total = 0;
+    bool alive[SHADE_ITEMS];
+    int pixel[SHADE_ITEMS];
+    float4 oa[SHADE_ITEMS], ob[SHADE_ITEMS];
+    float wr[SHADE_ITEMS], wg[SHADE_ITEMS], wb[SHADE_ITEMS];
+    uint32_t rank[SHADE_ITEMS];
 #pragma unroll
-    for (int w = 0; w < SHADE_BLOCK / 64; ++w) {
-        uint32_t c = s_wave[w];
-        if (w < (int)wave) wbase += c;
-        total += c;
+    for (int it = 0; it < SHADE_ITEMS; ++it) {
+        const int64_t j = tile * SHADE_TILE + (int64_t)it * SHADE_BLOCK + tid;
+        alive[it] = false;
+        pixel[it] = 0;
+        wr[it] = wg[it] = wb[it] = 1.f;
+        if (j < n) {
+            pixel[it] = in_s2i[j];
+            const int4 hit = *reinterpret_cast<const int4*>(in_hits + j);
+            if (hit.x >= 0) {
+                uint32_t seed = seeds[pixel[it]];
+                float4 ro, rd;
+                load_ray(in_rays, j, ro, rd);
+                int64_t tri = (int64_t)geom_first[hit.y] + hit.x;
+                V3 nrm;
+                int m = clamp_material(shading_normal(shade, tri, (uint32_t)hit.w, nrm), nmat);
+                V3 ff = faceforward3(nrm, v3(rd.x, rd.y, rd.z));
+                // decision log: the normal flipped to face the ray (bit 0) and the
+                // basis branch of a normal with negative z (bit 1)
+                if (sign_log)
+                    sign_log[pixel[it]] = (uint8_t)((dot3(nrm, v3(rd.x, rd.y, rd.z)) < 0.0f ? 0 : 1) |
+                                                    (ff.z >= 0.0f ? 0 : 2));
+                float r0 = randf(seed);
+                float r1 = randf(seed);
+                float t = __int_as_float(hit.z);
+                V3 p = v3(ro.x + rd.x * t, ro.y + rd.y * t, ro.z + rd.z * t);
+                const float eps = 1.0e-4f;
+                V3 d = cosine_direction(r0, r1, ff);
+                oa[it] = make_float4(p.x + d.x * eps, p.y + d.y * eps, p.z + d.z * eps, eps);
+                ob[it] = make_float4(d.x, d.y, d.z, FLT_MAX);
+                wr[it] = albedo[3 * m];
+                wg[it] = albedo[3 * m + 1];
+                wb[it] = albedo[3 * m + 2];
+                alive[it] = true;
+                if (rr) {
+                    // Russian roulette on the throughput after this bounce.
+                    float4 dec = decreases[pixel[it]];
+                    float q = fmaxf(fmaxf(dec.x * wr[it], dec.y * wg[it]), dec.z * wb[it]);
+                    q = fminf(fmaxf(q, 0.05f), 1.0f);
+                    float r2 = randf(seed);
+                    if (r2 >= q) {
+                        alive[it] = false;
+                    } else {
+                        wr[it] /= q;
+                        wg[it] /= q;
+                        wb[it] /= q;
+                    }
+                }
+                seeds[pixel[it]] = seed;
+            }
+        }
+        // rank among the item's survivors of this wave
+        const uint64_t bal = ballot64(alive[it]);
+        rank[it] = popc_below(bal);
+        if (lane == 0) s_wave[it][wave] = (uint32_t)__popcll(bal);
+    }
+    __syncthreads();
+    // survivors before (item, wave) in slot order, and the tile's total
+    uint32_t base[SHADE_ITEMS], total = 0;
+#pragma unroll
+    for (int it = 0; it < SHADE_ITEMS; ++it) {
+#pragma unroll
+        for (int w = 0; w < SHADE_BLOCK / 64; ++w) {
+            if (w == (int)wave) base[it] = total;
+            total += s_wave[it][w];
+        }
     }
     if (tid == 0) {
         uint32_t excl = 0;
@@ -162,12 +178,15 @@ __global__ __launch_bounds__(SHADE_BLOCK) void shade_extend_kernel(
         if (tile == ntiles - 1) *d_count = excl + total;
     }
     __syncthreads();
-    if (alive) {
-        const int64_t k = (int64_t)s_excl + wbase + rank;
-        store_ray(out_rays, k, oa, ob);
-        out_s2i[k] = pixel;
-        float4 dec = decreases[pixel];
-        decreases[pixel] = make_float4(dec.x * wr, dec.y * wg, dec.z * wb, dec.w * 1.0f);
+#pragma unroll
+    for (int it = 0; it < SHADE_ITEMS; ++it) {
+        if (alive[it]) {
+            const int64_t k = (int64_t)s_excl + base[it] + rank[it];
+            store_ray(out_rays, k, oa[it], ob[it]);
+            out_s2i[k] = pixel[it];
+            float4 dec = decreases[pixel[it]];
+            decreases[pixel[it]] = make_float4(dec.x * wr[it], dec.y * wg[it], dec.z * wb[it], dec.w * 1.0f);
+        }
     }
 }
 
@@ -248,7 +267,7 @@ extern "C" int wf_raygen_shadow(const uint32_t* seeds, const wf_ray* in_rays, co
 
 extern "C" int64_t wf_compact_scratch_bytes(int64_t n) {
     if (n < 0) return WF_ERR_INVALID_ARG;
-    int64_t tiles = (n + SHADE_BLOCK - 1) / SHADE_BLOCK;
+    int64_t tiles = (n + SHADE_TILE - 1) / SHADE_TILE;
     return 256 + 4 * (tiles + 1);
 }
 
@@ -294,7 +313,7 @@ int wfpt::shade_extend_logged(wf_scene scene, uint32_t* seeds, const wf_ray* in_
                      "wf_shade_extend: null buffer");
         if (scratch_bytes < wf_compact_scratch_bytes(n))
             return fail(WF_ERR_SCRATCH_TOO_SMALL, "wf_shade_extend: scratch < wf_compact_scratch_bytes");
-        int64_t tiles = (n + SHADE_BLOCK - 1) / SHADE_BLOCK;
+        int64_t tiles = (n + SHADE_TILE - 1) / SHADE_TILE;
         uint32_t* ctr = reinterpret_cast<uint32_t*>(scratch);
         uint32_t* status = ctr + 64;
         WF_TRY(fill_u32(ctr, 64 + tiles, 0u, s));
