@@ -194,8 +194,12 @@ __global__ __launch_bounds__(256) void batch_shadow_recon_kernel(
 
 // ----------------------------------------------------------- shade/extend
 // raygenPath.comp:78-170 (no roulette) with order-preserving look-back
-// compaction; the survivor count goes to q[count b + 1].
+// compaction; the survivor count goes to q[count b + 1]. A workgroup takes
+// BSHADE_TILE slots, BSHADE_ITEMS per thread (slot tile * BSHADE_TILE + item
+// * BSHADE_BLOCK + thread), which keeps the look-back chain short.
 constexpr int BSHADE_BLOCK = 256;
+constexpr int BSHADE_ITEMS = 4;
+constexpr int BSHADE_TILE = BSHADE_BLOCK * BSHADE_ITEMS;
 
 __global__ __launch_bounds__(BSHADE_BLOCK) void batch_shade_kernel(
     const BatchTables t, uint32_t* __restrict__ q, int b, uint32_t* __restrict__ seeds,
@@ -204,58 +208,67 @@ __global__ __launch_bounds__(BSHADE_BLOCK) void batch_shade_kernel(
     int32_t* __restrict__ out_s2i, float4* __restrict__ dec, uint32_t* __restrict__ tile_counter,
     uint32_t* __restrict__ status) {
     __shared__ uint32_t s_tile, s_excl;
-    __shared__ uint32_t s_wave[BSHADE_BLOCK / 64];
+    __shared__ uint32_t s_wave[BSHADE_ITEMS][BSHADE_BLOCK / 64];
     const unsigned tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const int64_t n = count_at(q, b);
-    const int64_t ntiles = (n + BSHADE_BLOCK - 1) / BSHADE_BLOCK;
+    const int64_t ntiles = (n + BSHADE_TILE - 1) / BSHADE_TILE;
     if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
     __syncthreads();
     const int64_t tile = s_tile;
     if (tile >= ntiles) return;  // surplus workgroups of the capacity-sized grid
-    const int64_t j = tile * BSHADE_BLOCK + tid;
 
-    bool alive = false;
-    int gp = 0;
-    float4 oa, ob;
-    float wr = 1.f, wg = 1.f, wb = 1.f;
-    if (j < n) {
-        gp = in_s2i[j];
-        const int4 hit = *reinterpret_cast<const int4*>(hits + j);
-        if (hit.x >= 0) {
-            const int v = view_of(t, gp);
-            const BatchScene& sc = t.scenes[t.views[v].scene];
-            uint32_t seed = seeds[gp];
-            float4 ro, rd;
-            load_ray(in_rays, j, ro, rd);
-            const int64_t tri = (int64_t)sc.geom_first[hit.y] + hit.x;
-            V3 nrm;
-            const int m = clamp_material(shading_normal(sc.shade, tri, (uint32_t)hit.w, nrm), sc.nmat);
-            const V3 ff = faceforward3(nrm, v3(rd.x, rd.y, rd.z));
-            const float r0 = randf(seed);
-            const float r1 = randf(seed);
-            const float th = __int_as_float(hit.z);
-            const V3 p = v3(ro.x + rd.x * th, ro.y + rd.y * th, ro.z + rd.z * th);
-            const float eps = 1.0e-4f;
-            const V3 d = cosine_direction(r0, r1, ff);
-            oa = make_float4(p.x + d.x * eps, p.y + d.y * eps, p.z + d.z * eps, eps);
-            ob = make_float4(d.x, d.y, d.z, FLT_MAX);
-            wr = sc.albedo[3 * m];
-            wg = sc.albedo[3 * m + 1];
-            wb = sc.albedo[3 * m + 2];
-            alive = true;
-            seeds[gp] = seed;
-        }
-    }
-    const uint64_t bal = ballot64(alive);
-    const uint32_t rank = popc_below(bal);
-    if (lane == 0) s_wave[wave] = (uint32_t)__popcll(bal);
-    __syncthreads();
-    uint32_t wbase = 0, total = 0;
+    bool alive[BSHADE_ITEMS];
+    int gp[BSHADE_ITEMS];
+    float4 oa[BSHADE_ITEMS], ob[BSHADE_ITEMS];
+    float wr[BSHADE_ITEMS], wg[BSHADE_ITEMS], wb[BSHADE_ITEMS];
+    uint32_t rank[BSHADE_ITEMS];
 #pragma unroll
-    for (int w = 0; w < BSHADE_BLOCK / 64; ++w) {
-        const uint32_t c = s_wave[w];
-        if (w < (int)wave) wbase += c;
-        total += c;
+    for (int it = 0; it < BSHADE_ITEMS; ++it) {
+        const int64_t j = tile * BSHADE_TILE + (int64_t)it * BSHADE_BLOCK + tid;
+        alive[it] = false;
+        gp[it] = 0;
+        wr[it] = wg[it] = wb[it] = 1.f;
+        if (j < n) {
+            gp[it] = in_s2i[j];
+            const int4 hit = *reinterpret_cast<const int4*>(hits + j);
+            if (hit.x >= 0) {
+                const int v = view_of(t, gp[it]);
+                const BatchScene& sc = t.scenes[t.views[v].scene];
+                uint32_t seed = seeds[gp[it]];
+                float4 ro, rd;
+                load_ray(in_rays, j, ro, rd);
+                const int64_t tri = (int64_t)sc.geom_first[hit.y] + hit.x;
+                V3 nrm;
+                const int m = clamp_material(shading_normal(sc.shade, tri, (uint32_t)hit.w, nrm), sc.nmat);
+                const V3 ff = faceforward3(nrm, v3(rd.x, rd.y, rd.z));
+                const float r0 = randf(seed);
+                const float r1 = randf(seed);
+                const float th = __int_as_float(hit.z);
+                const V3 p = v3(ro.x + rd.x * th, ro.y + rd.y * th, ro.z + rd.z * th);
+                const float eps = 1.0e-4f;
+                const V3 d = cosine_direction(r0, r1, ff);
+                oa[it] = make_float4(p.x + d.x * eps, p.y + d.y * eps, p.z + d.z * eps, eps);
+                ob[it] = make_float4(d.x, d.y, d.z, FLT_MAX);
+                wr[it] = sc.albedo[3 * m];
+                wg[it] = sc.albedo[3 * m + 1];
+                wb[it] = sc.albedo[3 * m + 2];
+                alive[it] = true;
+                seeds[gp[it]] = seed;
+            }
+        }
+        const uint64_t bal = ballot64(alive[it]);
+        rank[it] = popc_below(bal);
+        if (lane == 0) s_wave[it][wave] = (uint32_t)__popcll(bal);
+    }
+    __syncthreads();
+    uint32_t base[BSHADE_ITEMS], total = 0;
+#pragma unroll
+    for (int it = 0; it < BSHADE_ITEMS; ++it) {
+#pragma unroll
+        for (int w = 0; w < BSHADE_BLOCK / 64; ++w) {
+            if (w == (int)wave) base[it] = total;
+            total += s_wave[it][w];
+        }
     }
     if (tid == 0) {
         uint32_t excl = 0;
@@ -279,12 +292,15 @@ __global__ __launch_bounds__(BSHADE_BLOCK) void batch_shade_kernel(
         if (tile == ntiles - 1) q[BQ_COUNT0 + b + 1] = excl + total;
     }
     __syncthreads();
-    if (alive) {
-        const int64_t k = (int64_t)s_excl + wbase + rank;
-        store_ray(out_rays, k, oa, ob);
-        out_s2i[k] = gp;
-        const float4 dc = dec[gp];
-        dec[gp] = make_float4(dc.x * wr, dc.y * wg, dc.z * wb, dc.w * 1.0f);
+#pragma unroll
+    for (int it = 0; it < BSHADE_ITEMS; ++it) {
+        if (alive[it]) {
+            const int64_t k = (int64_t)s_excl + base[it] + rank[it];
+            store_ray(out_rays, k, oa[it], ob[it]);
+            out_s2i[k] = gp[it];
+            const float4 dc = dec[gp[it]];
+            dec[gp[it]] = make_float4(dc.x * wr[it], dc.y * wg[it], dc.z * wb[it], dc.w * 1.0f);
+        }
     }
 }
 
@@ -392,7 +408,7 @@ int batch_enqueue_pass(const BatchTables& t, const BatchBuffers& bb, const Batch
     batch_primary_kernel<<<grid_for(cap, 256), 256, 0, s>>>(t, bb.q, bb.rays[0], bb.s2i[0], bb.dec);
     WF_TRY(launch_check("batch_primary_kernel"));
     WF_TRY(batch_vertex(t, bb, tu, 0, cur, s));
-    const int64_t tiles = (cap + BSHADE_BLOCK - 1) / BSHADE_BLOCK;
+    const int64_t tiles = (cap + BSHADE_TILE - 1) / BSHADE_TILE;
     for (int b = 0; b < depth; ++b) {
         WF_TRY(fill_u32(bb.compact, 64 + tiles, 0u, s));
         batch_shade_kernel<<<(unsigned)tiles, BSHADE_BLOCK, 0, s>>>(
