@@ -173,11 +173,11 @@ def test_reconstruct_stages_bit_exact(dev, orc, boxes):
 
 def test_count_hits_large_and_ragged(dev):
     """The hit counter's workgroups stride over the records: sizes past one
-    stride (2048 x 256 records), ragged tails, all hits, no hits."""
+    stride (512 workgroups x 2048 records), ragged tails, all hits, no hits."""
     from vulkanwavefrontpathtracer_amd import stages
 
     rng = np.random.default_rng(17)
-    for n, p in ((0, 0.5), (1, 1.0), (300017, 0.3), (524288, 0.5), (524289, 1.0), (3000001, 0.6),
+    for n, p in ((0, 0.5), (1, 1.0), (2047, 0.5), (2049, 0.7), (300017, 0.3), (524288, 0.5), (524289, 1.0), (1048576, 0.2), (1048577, 0.9), (3000001, 0.6), (9000001, 0.4),
                  (1048577, 0.0)):
         hits = np.zeros((n, 4), np.int32)
         hits[:, 0] = np.where(rng.uniform(size=n) < p, rng.integers(0, 1000, n), -1)

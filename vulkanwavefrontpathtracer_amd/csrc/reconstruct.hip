@@ -116,19 +116,29 @@ __global__ __launch_bounds__(256) void interpolate_kernel(float4* __restrict__ f
     }
 }
 
-// countRayHits.comp:22-51: wave sum -> LDS -> one atomic per workgroup. The
-// workgroups stride over the records (at most COUNT_HITS_GRID of them), so the
-// one counter takes a few thousand atomics instead of one per 256 records,
-// which serialised on its address.
-constexpr int COUNT_HITS_GRID = 2048;
+// countRayHits.comp:22-51: wave sum -> LDS -> one atomic per workgroup. A
+// workgroup takes COUNT_HITS_ITEMS x 256 consecutive records per round (the
+// loads of a round all in flight together) and strides over the records
+// with at most COUNT_HITS_GRID workgroups, so the one counter takes a few
+// hundred atomics, which serialise on its address.
+constexpr int COUNT_HITS_GRID = 512;
+constexpr int COUNT_HITS_ITEMS = 8;
 __global__ __launch_bounds__(256) void count_hits_kernel(const wf_hit* __restrict__ hits, int64_t n,
                                                          uint32_t* __restrict__ count) {
     __shared__ uint32_t s[4];
     uint32_t c = 0;
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < n; base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = base + threadIdx.x;
-        c += (uint32_t)__popcll(ballot64(i < n && hits[i].prim >= 0));
+    const int64_t round = (int64_t)COUNT_HITS_ITEMS * 256;
+    for (int64_t base = (int64_t)blockIdx.x * round; base < n; base += (int64_t)gridDim.x * round) {
+        int prim[COUNT_HITS_ITEMS];
+#pragma unroll
+        for (int k = 0; k < COUNT_HITS_ITEMS; ++k) {
+            const int64_t i = base + (int64_t)k * 256 + threadIdx.x;
+            prim[k] = i < n ? hits[i].prim : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < COUNT_HITS_ITEMS; ++k) c += prim[k] >= 0 ? 1u : 0u;
     }
+    for (int o = 32; o >= 1; o >>= 1) c += __shfl_xor(c, o, 64);
     if (lane_id() == 0) s[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -212,8 +222,8 @@ extern "C" int wf_count_hits(const wf_hit* hits, int64_t n, uint32_t* d_count, w
     WF_CHECK_ARG(hits, "wf_count_hits: null buffer");
     hipStream_t s = as_stream(stream);
     StageTimer t(s, ms);
-    count_hits_kernel<<<(unsigned)std::min<int64_t>(grid_for(n, 256), COUNT_HITS_GRID), 256, 0, s>>>(
-        hits, n, d_count);
+    count_hits_kernel<<<(unsigned)std::min<int64_t>(grid_for(n, 256 * COUNT_HITS_ITEMS), COUNT_HITS_GRID),
+                        256, 0, s>>>(hits, n, d_count);
     WF_TRY(launch_check("count_hits_kernel"));
     return t.finish();
 }
